@@ -1,0 +1,191 @@
+/*
+ * rtsim.h -- C ABI of the MI355X (gfx950) sample-to-ping library.
+ *
+ * The library replaces the reference's sensor-instance pipeline (rtac_simulation
+ * L3): per-sample directivity x round-trip phase, sort by travel, Binner
+ * (keys, segments, windows) and the sparse complex-PSF convolution into 1D /
+ * 2D complex pings.  Every entry point below cites the reference interface it
+ * replaces (paths relative to the rtac_simulation source tree).
+ *
+ * Conventions
+ *   - Every function returns an int status (RTSIM_OK = 0).  On error,
+ *     rtsim_last_error() returns a thread-local message.
+ *   - Device pointers returned by the library are borrowed: valid until the
+ *     next set_* / compute call on the same handle or its destruction.
+ *   - Work is queued on the caller's HIP stream (hipStream_t passed as
+ *     void*; NULL = the default stream) and is asynchronous until
+ *     rtsim_stream_sync().  A handle is not thread-safe; distinct handles are.
+ *   - Sample record layout (16 B, both sensor kinds):
+ *       {float re, float im, float travel, float bearing}
+ *     SimSample2D is exactly this (Sample.h:101-161); a 1D sensor leaves
+ *     `bearing` at 0.  A null sample has travel = -1 and value 0.
+ *   - Ping layout: complex64, row-major [range][bearing] (1D: [range]),
+ *     as SensorInstance2D.cu:81 writes out(blockIdx.y, blockIdx.x).
+ */
+#ifndef RTSIM_H
+#define RTSIM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTSIM_OK 0
+#define RTSIM_ERR_INVALID 1     /* bad argument or handle              */
+#define RTSIM_ERR_HIP 2         /* HIP runtime error                   */
+#define RTSIM_ERR_UNSUPPORTED 3 /* valid request this form cannot do   */
+#define RTSIM_ERR_CONFIG 4      /* configuration file error            */
+#define RTSIM_ERR_IO 5          /* file not found / unreadable         */
+
+#define RTSIM_SENSOR_1D 1       /* SensorInstance1D_Complex            */
+#define RTSIM_SENSOR_2D 2       /* SensorInstance2D_2<Complex<float>>  */
+
+/* PSF convolution forms */
+#define RTSIM_FORM_PLAIN_FP32 0 /* reference sparse gather, FP32       */
+#define RTSIM_FORM_MFMA_BF16 1  /* grid form on MFMA, BF16 hi+lo split */
+#define RTSIM_FORM_MFMA_FP16 2  /* grid form on MFMA, FP16 hi+lo split */
+
+/* Coordinate in which the PSF bearing axis is uniform */
+#define RTSIM_AXIS_ANGLE 0      /* radians, as KernelView2D (reference) */
+#define RTSIM_AXIS_SINE 1       /* sin(bearing) (phased-array beams)    */
+
+#define RTSIM_SPREAD_NONE 0
+#define RTSIM_SPREAD_INV_R2 1   /* value /= r^2 (RayCaster.cu:70-71)    */
+
+#define RTSIM_KEY_OUT_OF_RANGE 0xFFFFFFFFu /* Binner::Key::OutOfRange */
+
+typedef struct rtsim_sensor_s* rtsim_sensor;
+
+/*
+ * Sensor description: what SensorInfo / SensorInfo2D (SensorInfo.h:18-105),
+ * Directivity (Directivity.h:43-83) and the PSF table of
+ * SensorInstance{1D,2D}::generate_psf_data (SensorInstance1D.cu:15-25,
+ * SensorInstance2D.cu:20-33) carry, with the PSF taps given explicitly.
+ */
+typedef struct {
+    int kind;                   /* RTSIM_SENSOR_1D / RTSIM_SENSOR_2D               */
+    /* ranges: Linspace<float>(range_min, range_max, range_count)                  */
+    float range_min, range_max;
+    uint32_t range_count;
+    /* bearings (2D): ascending table in radians (SensorInfo2D::bearings_)         */
+    const float* bearings;
+    uint32_t bearing_count;
+    int bearing_axis;           /* RTSIM_AXIS_*                                    */
+    /* PSF taps: complex64 interleaved, row-major [range tap][bearing tap]         */
+    const float* psf;
+    uint32_t psf_range_taps, psf_bearing_taps; /* 1D: psf_bearing_taps = 1         */
+    float psf_range_span;       /* metres covered by the range taps (c * tau)      */
+    float psf_bearing_span;     /* bearing-axis units covered by the bearing taps  */
+    float margin;               /* Binner margin (m); < 0 selects c * tau          */
+    /* receiver directivity table [dir_height][dir_width] (Directivity texture);   */
+    /* NULL selects a uniform directivity                                          */
+    const float* directivity;
+    uint32_t dir_width, dir_height;
+    float dir_xscale, dir_yscale; /* DirectivityView scalings (1/pi by default)   */
+    float frequency;            /* Hz, for the round-trip phase                    */
+    float sound_speed;          /* m/s                                             */
+    int spreading;              /* RTSIM_SPREAD_*                                  */
+    float rotation[9];          /* receiver pose R, row-major; local = R^T dir     */
+} rtsim_sensor_desc;
+
+/* Ray hits handed to the streaming kernel (SoA, device pointers). */
+typedef struct {
+    const float* range;         /* one-way travel r (m); r < 1e-4 -> null sample   */
+    const float* dir_x;         /* direction toward the hit, world frame           */
+    const float* dir_y;
+    const float* dir_z;
+    const float* amplitude;     /* complex64 interleaved                           */
+} rtsim_hits;
+
+/* ---- library ---------------------------------------------------------------- */
+const char* rtsim_last_error(void);
+const char* rtsim_version(void);
+/* Number of visible HIP devices (0 when no GPU). */
+int rtsim_device_count(int* count);
+int rtsim_stream_sync(void* stream);
+
+/* ---- sensor lifetime: SensorInstance2D_2<Complex<float>>::Create
+ *      (SensorInstance2D.h:122-139), SensorInstance1D_Complex::Create
+ *      (SensorInstance1D.h:93-98) --------------------------------------------- */
+int rtsim_sensor_create(const rtsim_sensor_desc* desc, rtsim_sensor* out);
+int rtsim_sensor_destroy(rtsim_sensor h);
+
+/* SensorInstance::set_ranges (SensorInstance.h:56-58, SensorInstance.cu:23-34):
+ * new output ranges and Binner margin (< 0 keeps c * tau).                         */
+int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
+                            uint32_t range_count, float margin);
+
+/* SensorInstance{1D,2D}::set_sample_count / sample_count
+ * (SensorInstance1D.h:47-48, SensorInstance2D.h:59-60).                            */
+int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count);
+int rtsim_sensor_sample_count(rtsim_sensor h, size_t* count);
+
+/* receiver_view() samples pointer (SensorInstance.cu:41-50, Receiver.h:18-35):
+ * the device array of 16-B records a producer may fill directly.                  */
+int rtsim_sensor_samples(rtsim_sensor h, void** dev_ptr);
+
+/* ReceiverView<T>::set_sample / set_null_sample (Receiver.h:51-65) applied to
+ * n hits with the phase/spreading of RayCaster.cu:62-74: writes records
+ * [offset, offset + n).                                                           */
+int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t n,
+                                 size_t offset, void* stream);
+
+/* SensorInstance::compute_output (SensorInstance.h:67; 2D: fill_ping
+ * SensorInstance2D.h:84-93; 1D: reduce_samples SensorInstance1D.h:63-69):
+ * sort -> Binner::compute -> PSF convolution, in the chosen form.                 */
+int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
+
+/* get_ping()/output() (SensorInstance2D.h:144-150, SensorInstance1D.h:103)         */
+int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height,
+                        uint32_t* width);
+
+/* Sorted samples after compute (receivedSamples_ after sort(), Receiver.cu:28).   */
+int rtsim_sensor_sorted_samples(rtsim_sensor h, const void** dev_ptr);
+
+/* Binner internals (Binner.h:56-77) copied to host for bit-exact checks.  Any
+ * pointer may be NULL.  keys: sorted keys (n); segments: (start, end) per bin
+ * (2N); windows: (start, size) per bin (2N); order: sort permutation (n);
+ * overlap: binOverlap.  Synchronises the stream.                                 */
+int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments,
+                              uint32_t* windows, uint32_t* order, int* overlap,
+                              void* stream);
+
+/* Host <-> device helpers (synchronous on the given stream).                      */
+int rtsim_sensor_upload_samples(rtsim_sensor h, const void* host, size_t n, void* stream);
+int rtsim_sensor_download_samples(rtsim_sensor h, void* host, size_t n, int sorted,
+                                  void* stream);
+int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream);
+
+/* Per-stage timing with HIP events recorded on the compute stream.  When on,
+ * each compute records events around its stages; stage_times returns the
+ * mean milliseconds per stage over the computes since the last reset.
+ * Stages: 0 stream, 1 sort, 2 bins (keys+segments+windows), 3 grid deposit
+ * (MFMA form only), 4 convolution kernel (plain gather or MFMA correlation).    */
+int rtsim_sensor_set_profiling(rtsim_sensor h, int on);
+int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes);
+
+/* Streams and event timers for callers without their own HIP binding
+ * (hipStreamCreate at the default priority, hipEvent pairs).                     */
+int rtsim_stream_create(void** stream);
+int rtsim_stream_destroy(void* stream);
+int rtsim_timer_create(void** timer);
+int rtsim_timer_destroy(void* timer);
+int rtsim_timer_start(void* timer, void* stream);
+int rtsim_timer_stop(void* timer, void* stream);
+/* Waits for the stop event and returns start -> stop in milliseconds.            */
+int rtsim_timer_elapsed(void* timer, float* ms);
+
+/* Device memory helpers so callers without a GPU runtime binding can stage
+ * inputs (hipMalloc / hipMemcpy / hipFree).                                       */
+int rtsim_device_alloc(void** ptr, size_t bytes);
+int rtsim_device_free(void* ptr);
+int rtsim_memcpy_h2d(void* dst, const void* src, size_t bytes);
+int rtsim_memcpy_d2h(void* dst, const void* src, size_t bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTSIM_H */

@@ -1,0 +1,414 @@
+"""rtac_simulation_amd: MI355X-native sample-to-ping path.
+
+Python binding (ctypes) of the C ABI in ``include/rtsim.h`` /
+``include/rtsim_scene.h``, used by the tests and ``bench.py``.  The compute
+path is the in-tree HIP library ``librtsim.so`` built for gfx950; importing
+this package never falls back to anything else: if the library is missing
+every call raises.
+
+Class names follow the reference's sensor-instance API
+(``SensorInstance1D_Complex`` / ``SensorInstance2D_Complex``,
+SensorInstance1D.h:72-110, SensorInstance2D.h:153-205).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+PKG_DIR = Path(__file__).resolve().parent
+LIB_PATH = PKG_DIR / "librtsim.so"
+
+RTSIM_OK = 0
+ERR_INVALID, ERR_HIP, ERR_UNSUPPORTED, ERR_CONFIG, ERR_IO = 1, 2, 3, 4, 5
+SENSOR_1D, SENSOR_2D = 1, 2
+FORM_PLAIN_FP32, FORM_MFMA_BF16, FORM_MFMA_FP16 = 0, 1, 2
+FORM_NAMES = {FORM_PLAIN_FP32: "plain_fp32", FORM_MFMA_BF16: "mfma_bf16",
+              FORM_MFMA_FP16: "mfma_fp16"}
+AXIS_ANGLE, AXIS_SINE = 0, 1
+SPREAD_NONE, SPREAD_INV_R2 = 0, 1
+SCENE_CONE, SCENE_SEABED = 1, 2
+KEY_OUT_OF_RANGE = 0xFFFFFFFF
+
+
+class RtsimError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"rtsim error {code}: {msg}")
+        self.code = code
+
+
+class SensorDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("range_min", C.c_float), ("range_max", C.c_float), ("range_count", C.c_uint32),
+        ("bearings", C.POINTER(C.c_float)), ("bearing_count", C.c_uint32),
+        ("bearing_axis", C.c_int),
+        ("psf", C.POINTER(C.c_float)),
+        ("psf_range_taps", C.c_uint32), ("psf_bearing_taps", C.c_uint32),
+        ("psf_range_span", C.c_float), ("psf_bearing_span", C.c_float),
+        ("margin", C.c_float),
+        ("directivity", C.POINTER(C.c_float)),
+        ("dir_width", C.c_uint32), ("dir_height", C.c_uint32),
+        ("dir_xscale", C.c_float), ("dir_yscale", C.c_float),
+        ("frequency", C.c_float), ("sound_speed", C.c_float),
+        ("spreading", C.c_int),
+        ("rotation", C.c_float * 9),
+    ]
+
+
+class Hits(C.Structure):
+    _fields_ = [("range", C.c_void_p), ("dir_x", C.c_void_p), ("dir_y", C.c_void_p),
+                ("dir_z", C.c_void_p), ("amplitude", C.c_void_p)]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("range_min", C.c_float), ("range_max", C.c_float),
+        ("off_axis_max", C.c_float),
+        ("bearing_max", C.c_float), ("elevation_max", C.c_float),
+        ("plane_count", C.c_uint32), ("planes", (C.c_float * 4) * 4),
+        ("height_noise", C.c_float),
+        ("sphere_count", C.c_uint32),
+        ("sphere_radius_min", C.c_float), ("sphere_radius_max", C.c_float),
+        ("sphere_range_min", C.c_float), ("sphere_range_max", C.c_float),
+        ("sphere_fraction", C.c_float), ("dynamic_range_db", C.c_float),
+        ("rayleigh_sigma", C.c_float), ("incidence_power", C.c_int),
+    ]
+
+
+# (name, restype, argtypes) of every exported entry point
+_SIGNATURES = [
+    ("rtsim_last_error", C.c_char_p, []),
+    ("rtsim_version", C.c_char_p, []),
+    ("rtsim_device_count", C.c_int, [C.POINTER(C.c_int)]),
+    ("rtsim_stream_sync", C.c_int, [C.c_void_p]),
+    ("rtsim_sensor_create", C.c_int, [C.POINTER(SensorDesc), C.POINTER(C.c_void_p)]),
+    ("rtsim_sensor_destroy", C.c_int, [C.c_void_p]),
+    ("rtsim_sensor_set_ranges", C.c_int,
+     [C.c_void_p, C.c_float, C.c_float, C.c_uint32, C.c_float]),
+    ("rtsim_sensor_set_sample_count", C.c_int, [C.c_void_p, C.c_size_t]),
+    ("rtsim_sensor_sample_count", C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    ("rtsim_sensor_samples", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("rtsim_receiver_write_samples", C.c_int,
+     [C.c_void_p, C.POINTER(Hits), C.c_size_t, C.c_size_t, C.c_void_p]),
+    ("rtsim_sensor_compute", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtsim_sensor_output", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rtsim_sensor_sorted_samples", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
+    ("rtsim_sensor_binner_debug", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.POINTER(C.c_int),
+      C.c_void_p]),
+    ("rtsim_sensor_upload_samples", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+    ("rtsim_sensor_download_samples", C.c_int,
+     [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int, C.c_void_p]),
+    ("rtsim_sensor_download_output", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtsim_sensor_set_profiling", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_sensor_stage_times", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_float), C.c_int, C.POINTER(C.c_int)]),
+    ("rtsim_stream_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("rtsim_stream_destroy", C.c_int, [C.c_void_p]),
+    ("rtsim_timer_create", C.c_int, [C.POINTER(C.c_void_p)]),
+    ("rtsim_timer_destroy", C.c_int, [C.c_void_p]),
+    ("rtsim_timer_start", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("rtsim_timer_stop", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("rtsim_timer_elapsed", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    ("rtsim_device_alloc", C.c_int, [C.POINTER(C.c_void_p), C.c_size_t]),
+    ("rtsim_device_free", C.c_int, [C.c_void_p]),
+    ("rtsim_memcpy_h2d", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("rtsim_memcpy_d2h", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("rtsim_generate_hits", C.c_int,
+     [C.POINTER(SceneDesc), C.c_uint64, C.c_uint64, C.c_size_t, C.c_void_p, C.c_void_p,
+      C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]),
+    ("rtsim_directivity_disk", C.c_int,
+     [C.c_float, C.c_float, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t,
+      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rtsim_directivity_rectangle", C.c_int,
+     [C.c_float, C.c_float, C.c_float, C.c_int, C.c_uint32, C.c_void_p, C.c_size_t,
+      C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rtsim_directivity_uniform", C.c_int,
+     [C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+]
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    """Load ``librtsim.so`` (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(str(LIB_PATH))
+        for name, res, args in _SIGNATURES:
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def exported_names() -> list[str]:
+    return [name for name, _, _ in _SIGNATURES]
+
+
+def check(rc: int) -> None:
+    if rc != RTSIM_OK:
+        raise RtsimError(rc, lib().rtsim_last_error().decode())
+
+
+def device_count() -> int:
+    c = C.c_int(0)
+    check(lib().rtsim_device_count(C.byref(c)))
+    return c.value
+
+
+def _ptr(a: np.ndarray) -> C.c_void_p:
+    assert a.flags.c_contiguous
+    return C.c_void_p(a.ctypes.data)
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class DeviceBuffer:
+    """A hipMalloc'd buffer owned through the C ABI."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib().rtsim_device_alloc(C.byref(p), max(1, self.nbytes)))
+        self.ptr = p
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        if a.nbytes:
+            check(lib().rtsim_memcpy_h2d(b.ptr, _ptr(a), a.nbytes))
+        return b
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        if out.nbytes:
+            check(lib().rtsim_memcpy_d2h(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().rtsim_device_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().rtsim_stream_create(C.byref(p)))
+        self.handle = p
+
+    def sync(self):
+        check(lib().rtsim_stream_sync(self.handle))
+
+    def __del__(self):
+        try:
+            lib().rtsim_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Timer:
+    """hipEvent pair recorded on a stream."""
+
+    def __init__(self):
+        p = C.c_void_p()
+        check(lib().rtsim_timer_create(C.byref(p)))
+        self.handle = p
+
+    def start(self, stream=None):
+        check(lib().rtsim_timer_start(self.handle, stream.handle if stream else None))
+
+    def stop(self, stream=None):
+        check(lib().rtsim_timer_stop(self.handle, stream.handle if stream else None))
+
+    def elapsed_ms(self) -> float:
+        ms = C.c_float()
+        check(lib().rtsim_timer_elapsed(self.handle, C.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        try:
+            lib().rtsim_timer_destroy(self.handle)
+        except Exception:
+            pass
+
+
+# ----------------------------------------------------------------------------
+# synthetic hits
+# ----------------------------------------------------------------------------
+class HostHits:
+    """SoA ray hits on the host."""
+
+    def __init__(self, n: int):
+        self.n = int(n)
+        self.range = np.zeros(n, np.float32)
+        self.dir_x = np.zeros(n, np.float32)
+        self.dir_y = np.zeros(n, np.float32)
+        self.dir_z = np.zeros(n, np.float32)
+        self.amplitude = np.zeros(2 * n, np.float32)
+
+    def to_device(self) -> "DeviceHits":
+        return DeviceHits(self)
+
+
+class DeviceHits:
+    def __init__(self, h: HostHits):
+        self.n = h.n
+        self.bufs = [DeviceBuffer.from_array(a) for a in
+                     (h.range, h.dir_x, h.dir_y, h.dir_z, h.amplitude)]
+        self.struct = Hits(*[b.ptr for b in self.bufs])
+
+
+def generate_hits(scene: SceneDesc, seed: int, n: int, first: int = 0,
+                  nthreads: int = 8) -> HostHits:
+    h = HostHits(n)
+    check(lib().rtsim_generate_hits(C.byref(scene), seed, first, n, _ptr(h.range),
+                                    _ptr(h.dir_x), _ptr(h.dir_y), _ptr(h.dir_z),
+                                    _ptr(h.amplitude), nthreads))
+    return h
+
+
+def directivity_disk(diameter: float, wavelength: float, single_sided: bool = True,
+                     oversampling: int = 8) -> np.ndarray:
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().rtsim_directivity_disk(diameter, wavelength, int(single_sided), oversampling,
+                                       None, 0, C.byref(w), C.byref(h)))
+    out = np.zeros((h.value, w.value), np.float32)
+    check(lib().rtsim_directivity_disk(diameter, wavelength, int(single_sided), oversampling,
+                                       _ptr(out), out.size, C.byref(w), C.byref(h)))
+    return out
+
+
+def directivity_rectangle(width: float, height: float, wavelength: float,
+                          single_sided: bool = True, oversampling: int = 8) -> np.ndarray:
+    w, h = C.c_uint32(), C.c_uint32()
+    check(lib().rtsim_directivity_rectangle(width, height, wavelength, int(single_sided),
+                                            oversampling, None, 0, C.byref(w), C.byref(h)))
+    out = np.zeros((h.value, w.value), np.float32)
+    check(lib().rtsim_directivity_rectangle(width, height, wavelength, int(single_sided),
+                                            oversampling, _ptr(out), out.size, C.byref(w),
+                                            C.byref(h)))
+    return out
+
+
+# ----------------------------------------------------------------------------
+# sensor instances
+# ----------------------------------------------------------------------------
+class SensorInstance:
+    """Owner of one rtsim_sensor handle (SensorInstance, SensorInstance.h:21-72)."""
+
+    def __init__(self, cfg: "SensorConfig"):
+        self.cfg = cfg
+        d = cfg.desc()
+        h = C.c_void_p()
+        check(lib().rtsim_sensor_create(C.byref(d), C.byref(h)))
+        self.handle = h
+        self.kind = cfg.kind
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().rtsim_sensor_destroy(self.handle)
+        except Exception:
+            pass
+
+    # -- producer side -----------------------------------------------------------
+    def set_sample_count(self, n: int):
+        check(lib().rtsim_sensor_set_sample_count(self.handle, n))
+
+    def sample_count(self) -> int:
+        c = C.c_size_t()
+        check(lib().rtsim_sensor_sample_count(self.handle, C.byref(c)))
+        return c.value
+
+    def set_ranges(self, lo: float, hi: float, n: int, margin: float = -1.0):
+        check(lib().rtsim_sensor_set_ranges(self.handle, lo, hi, n, margin))
+
+    def write_samples(self, hits: DeviceHits, offset: int = 0, stream: Stream | None = None):
+        check(lib().rtsim_receiver_write_samples(self.handle, C.byref(hits.struct), hits.n,
+                                                 offset, stream.handle if stream else None))
+
+    def upload_samples(self, rec: np.ndarray):
+        rec = np.ascontiguousarray(rec, dtype=np.float32).reshape(-1, 4)
+        check(lib().rtsim_sensor_upload_samples(self.handle, _ptr(rec), rec.shape[0], None))
+
+    # -- consumer side -------------------------------------------------------------
+    def compute_output(self, form: int = FORM_PLAIN_FP32, stream: Stream | None = None):
+        check(lib().rtsim_sensor_compute(self.handle, form, stream.handle if stream else None))
+
+    def shape(self):
+        h, w = C.c_uint32(), C.c_uint32()
+        check(lib().rtsim_sensor_output(self.handle, None, C.byref(h), C.byref(w)))
+        return h.value, w.value
+
+    def output(self) -> np.ndarray:
+        """The ping as complex64 [range][bearing] (1D: [range])."""
+        H, W = self.shape()
+        out = np.empty(2 * H * W, np.float32)
+        check(lib().rtsim_sensor_download_output(self.handle, _ptr(out), None))
+        c = out.view(np.complex64)
+        return c.reshape(H, W) if self.kind == SENSOR_2D else c
+
+    def samples(self, sorted: bool = False) -> np.ndarray:
+        n = self.sample_count()
+        out = np.empty((n, 4), np.float32)
+        check(lib().rtsim_sensor_download_samples(self.handle, _ptr(out), n, int(sorted), None))
+        return out
+
+    def binner_debug(self) -> dict:
+        n = self.sample_count()
+        H, _ = self.shape()
+        keys = np.empty(n, np.uint32)
+        seg = np.empty(2 * H, np.uint32)
+        win = np.empty(2 * H, np.uint32)
+        order = np.empty(n, np.uint32)
+        ov = C.c_int()
+        check(lib().rtsim_sensor_binner_debug(self.handle, _ptr(keys), _ptr(seg), _ptr(win),
+                                              _ptr(order), C.byref(ov), None))
+        return {"keys": keys, "segments": seg.reshape(H, 2), "windows": win.reshape(H, 2),
+                "order": order, "overlap": ov.value}
+
+    def set_profiling(self, on: bool):
+        check(lib().rtsim_sensor_set_profiling(self.handle, int(on)))
+
+    STAGES = ("stream", "sort", "bins", "deposit", "convolution")
+
+    def stage_times(self) -> tuple[dict, int]:
+        """Mean ms per stage (HIP events on the compute stream) since profiling on."""
+        ms = (C.c_float * 5)()
+        n = C.c_int()
+        check(lib().rtsim_sensor_stage_times(self.handle, ms, 5, C.byref(n)))
+        return dict(zip(self.STAGES, list(ms))), n.value
+
+
+class SensorInstance1D_Complex(SensorInstance):
+    pass
+
+
+class SensorInstance2D_Complex(SensorInstance):
+    pass
+
+
+def make_sensor(cfg: "SensorConfig") -> SensorInstance:
+    return (SensorInstance2D_Complex if cfg.kind == SENSOR_2D else SensorInstance1D_Complex)(cfg)
+
+
+from .configs import SensorConfig  # noqa: E402  (re-export)

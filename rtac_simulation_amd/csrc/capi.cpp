@@ -1,0 +1,713 @@
+// Host runtime of the C ABI declared in include/rtsim.h.
+//
+// One rtsim_sensor owns every device buffer of one sensor instance, the way
+// SensorInstance{1D,2D} own receivedSamples_, bins_, the Binner buffers, the
+// PSF texture and the ping (SensorInstance.h:21-72, SensorInstance2D.h:11-93).
+// All work is queued on the caller's stream; nothing here synchronises the
+// device except the explicit download / debug helpers.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rtsim.h"
+#include "rtsim_internal.h"
+#include "sensor.h"
+
+namespace rtsim {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char* fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    std::vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_last_error = buf;
+    return code;
+}
+
+const char* fail_message() { return g_last_error.c_str(); }
+
+int hip_fail(hipError_t e, const char* what)
+{
+    return fail(RTSIM_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+#define HIP_TRY(expr)                                   \
+    do {                                                \
+        hipError_t e_ = (expr);                         \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+// Binner::Key host constructor (Binner.h:34-45), FP32 without contraction.
+#pragma clang fp contract(off)
+KeyParams make_key(uint32_t N, float lo, float hi, float margin)
+{
+    KeyParams k;
+    float resolution = (hi - lo) / (float)N;
+    float m = lo - 0.5f * resolution;
+    float M = hi + 0.5f * resolution;
+    k.n = N;
+    k.a = (float)(N - 1u) / (M - m);
+    k.b = -m * k.a + 0.5f;
+    k.lower = m - margin;
+    k.upper = M + margin;
+    return k;
+}
+
+// binOverlap = margin / resolution truncated (Binner.cu:112-113), clamped to
+// [0, N] (any overlap >= N - 1 covers every bin; <= 0 empties every window).
+int make_overlap(float margin, float lo, float hi, uint32_t N)
+{
+    float resolution = (hi - lo) / (float)(N - 1u);
+    float q = margin / resolution;
+    int ov;
+    if (!(q == q)) ov = 0;
+    else if (q >= 2147483647.0f) ov = INT32_MAX;
+    else if (q <= -2147483648.0f) ov = INT32_MIN;
+    else ov = (int)q;
+    if (ov < 0) ov = 0;
+    if (ov > (int)N) ov = (int)N;
+    return ov;
+}
+#pragma clang fp contract(on)
+
+static float2 split_df(double x)
+{
+    float h = (float)x;
+    return make_float2(h, (float)(x - (double)h));
+}
+
+template <typename T>
+static int dev_alloc(T** p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    HIP_TRY(hipMalloc((void**)p, sizeof(T) * count));
+    return RTSIM_OK;
+}
+
+template <typename T>
+static void dev_free(T*& p)
+{
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+Sensor::~Sensor()
+{
+    free_samples();
+    free_grid();
+    dev_free(d_psf);
+    dev_free(d_bear);
+    dev_free(d_rng);
+    dev_free(d_dir);
+    dev_free(d_seg);
+    dev_free(d_win);
+    dev_free(d_out);
+    dev_free(d_hist);
+    dev_free(d_totals);
+    for (auto& p : events) {
+        (void)hipEventDestroy(p.start);
+        (void)hipEventDestroy(p.stop);
+    }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+}
+
+void Sensor::free_samples()
+{
+    dev_free(d_samples);
+    dev_free(d_sorted);
+    dev_free(d_coord);
+    dev_free(d_kA);
+    dev_free(d_kB);
+    dev_free(d_vA);
+    dev_free(d_vB);
+    dev_free(d_skeys);
+    capacity = 0;
+}
+
+int Sensor::reserve(size_t count)
+{
+    if (count <= capacity) return RTSIM_OK;
+    free_samples();
+    int rc;
+    if ((rc = dev_alloc(&d_samples, count))) return rc;
+    if ((rc = dev_alloc(&d_sorted, count))) return rc;
+    if ((rc = dev_alloc(&d_coord, count))) return rc;
+    if ((rc = dev_alloc(&d_kA, count))) return rc;
+    if ((rc = dev_alloc(&d_kB, count))) return rc;
+    if ((rc = dev_alloc(&d_vA, count))) return rc;
+    if ((rc = dev_alloc(&d_vB, count))) return rc;
+    if ((rc = dev_alloc(&d_skeys, count))) return rc;
+    capacity = count;
+    return RTSIM_OK;
+}
+
+// Ranges-dependent state: Binner::reconfigure (Binner.cu:15-22) + ping resize.
+int Sensor::configure_ranges()
+{
+    margin = margin_request >= 0.f ? margin_request : psf_range_span;
+    key = make_key(N, range_min, range_max, margin);
+    overlap = make_overlap(margin, range_min, range_max, N);
+    std::vector<float2> rc(N);
+    for (uint32_t r = 0; r < N; r++) {
+        double rr = N > 1 ? (double)range_min + (double)r * ((double)range_max - (double)range_min)
+                                                    / (double)(N - 1)
+                          : (double)range_min;
+        rc[r] = split_df(rr);
+    }
+    dev_free(d_rng);
+    dev_free(d_seg);
+    dev_free(d_win);
+    dev_free(d_out);
+    int rc_;
+    if ((rc_ = dev_alloc(&d_rng, N))) return rc_;
+    if ((rc_ = dev_alloc(&d_seg, N))) return rc_;
+    if ((rc_ = dev_alloc(&d_win, N))) return rc_;
+    if ((rc_ = dev_alloc(&d_out, (size_t)N * W))) return rc_;
+    HIP_TRY(hipMemcpy(d_rng, rc.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemset(d_out, 0, sizeof(float2) * (size_t)N * W));
+    free_grid();
+    return RTSIM_OK;
+}
+
+hipEvent_t Sensor::take_event()
+{
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+void Sensor::stage_begin(int stage, hipStream_t s)
+{
+    if (!profiling) return;
+    StageEvent p;
+    p.stage = stage;
+    p.start = take_event();
+    p.stop = take_event();
+    if (!p.start || !p.stop) return;
+    (void)hipEventRecord(p.start, s);
+    events.push_back(p);
+    open_event = (int)events.size() - 1;
+}
+
+void Sensor::stage_end(hipStream_t s)
+{
+    if (!profiling || open_event < 0) return;
+    (void)hipEventRecord(events[open_event].stop, s);
+    open_event = -1;
+}
+
+int Sensor::check_plain_limits() const
+{
+    if (kind == RTSIM_SENSOR_2D) {
+        size_t shm = sizeof(float) * 2 * W + sizeof(float2) * W + sizeof(float2) * Tr * Tb;
+        if (shm > 150 * 1024)
+            return fail(RTSIM_ERR_UNSUPPORTED,
+                        "plain form: %u bearings x %ux%u taps exceed the LDS budget", W, Tr, Tb);
+    }
+    return RTSIM_OK;
+}
+
+int Sensor::compute(int form, hipStream_t s)
+{
+    const uint32_t n = (uint32_t)count;
+    // ---- sort (Receiver.cu:28-34) -------------------------------------
+    stage_begin(1, s);
+    SortBuffers sb;
+    sb.keys[0] = d_kA;
+    sb.keys[1] = d_kB;
+    sb.vals[0] = d_vA;
+    sb.vals[1] = d_vB;
+    sb.hist = d_hist;
+    sb.totals = d_totals;
+    sb.max_stripes = kMaxStripes;
+    const uint32_t* order = nullptr;
+    const uint32_t* img = nullptr;
+    HIP_TRY(launch_sort(d_samples, n, sb, &order, &img, s));
+    order_ptr = order;
+    stage_end(s);
+
+    // ---- Binner::compute (Binner.h:96-102) ------------------------------
+    stage_begin(2, s);
+    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(d_seg), 0u, 2 * (size_t)N, s));
+    GatherArgs g;
+    g.rec = d_samples;
+    g.order = order;
+    g.img = img;
+    g.n = n;
+    g.key = key;
+    g.sorted = d_sorted;
+    g.coord = d_coord;
+    g.skeys = d_skeys;
+    g.seg = d_seg;
+    g.axis = axis;
+    HIP_TRY(launch_gather(g, s));
+    HIP_TRY(launch_windows(d_seg, N, overlap, d_win, s));
+    stage_end(s);
+
+    // ---- PSF convolution ------------------------------------------------
+    int rc = RTSIM_OK;
+    if (form == RTSIM_FORM_PLAIN_FP32) {
+        if ((rc = check_plain_limits())) return rc;
+        ConvArgs c = conv_args();
+        stage_begin(4, s);
+        if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(c, s));
+        else HIP_TRY(launch_conv1d_plain(c, s));
+        stage_end(s);
+    } else if (form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16) {
+        rc = compute_grid_form(form, s);   // stages 3 (deposit) and 4 (correlation)
+    } else {
+        rc = fail(RTSIM_ERR_INVALID, "unknown form %d", form);
+    }
+    if (rc == RTSIM_OK) computed = true;
+    return rc;
+}
+
+ConvArgs Sensor::conv_args() const
+{
+    ConvArgs c;
+    c.sorted = d_sorted;
+    c.coord = d_coord;
+    c.win = d_win;
+    c.range_c = d_rng;
+    c.bear_c = d_bear;
+    c.psf = d_psf;
+    c.out = d_out;
+    c.H = (int)N;
+    c.W = (int)W;
+    c.Tr = (int)Tr;
+    c.Tb = (int)Tb;
+    c.inv_pr = (float)((double)Tr / (double)psf_range_span);
+    c.half_r = 0.5f * (float)(Tr - 1);
+    c.inv_pb = (float)((double)Tb / (double)psf_bearing_span);
+    c.half_b = 0.5f * (float)(Tb - 1);
+    c.supp_b = (float)(0.5 * (double)(Tb + 1) * (double)psf_bearing_span / (double)Tb);
+    return c;
+}
+
+}  // namespace rtsim
+
+using namespace rtsim;
+
+// ============================================================================
+// C ABI
+// ============================================================================
+extern "C" {
+
+const char* rtsim_last_error(void) { return g_last_error.c_str(); }
+
+const char* rtsim_version(void) { return "rtsim 0.1.0 (gfx950)"; }
+
+int rtsim_device_count(int* count)
+{
+    if (!count) return fail(RTSIM_ERR_INVALID, "count is NULL");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return RTSIM_OK;
+}
+
+int rtsim_stream_sync(void* stream)
+{
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return RTSIM_OK;
+}
+
+static int validate_desc(const rtsim_sensor_desc* d)
+{
+    if (!d) return fail(RTSIM_ERR_INVALID, "desc is NULL");
+    if (d->kind != RTSIM_SENSOR_1D && d->kind != RTSIM_SENSOR_2D)
+        return fail(RTSIM_ERR_INVALID, "unknown sensor kind %d", d->kind);
+    if (d->range_count < 1) return fail(RTSIM_ERR_INVALID, "range_count must be >= 1");
+    if (!(d->range_max > d->range_min))
+        return fail(RTSIM_ERR_INVALID, "range_max must exceed range_min");
+    if (d->kind == RTSIM_SENSOR_2D) {
+        if (!d->bearings || d->bearing_count < 1)
+            return fail(RTSIM_ERR_INVALID, "2D sensor needs a bearing table");
+        for (uint32_t b = 1; b < d->bearing_count; b++)
+            if (!(d->bearings[b] > d->bearings[b - 1]))
+                return fail(RTSIM_ERR_INVALID, "bearings must be strictly ascending");
+        if (d->bearing_axis != RTSIM_AXIS_ANGLE && d->bearing_axis != RTSIM_AXIS_SINE)
+            return fail(RTSIM_ERR_INVALID, "unknown bearing axis %d", d->bearing_axis);
+    }
+    if (!d->psf || d->psf_range_taps < 1 || d->psf_bearing_taps < 1)
+        return fail(RTSIM_ERR_INVALID, "PSF taps missing");
+    if (d->kind == RTSIM_SENSOR_1D && d->psf_bearing_taps != 1)
+        return fail(RTSIM_ERR_INVALID, "1D sensor needs psf_bearing_taps == 1");
+    if (!(d->psf_range_span > 0.f)) return fail(RTSIM_ERR_INVALID, "psf_range_span must be > 0");
+    if (d->kind == RTSIM_SENSOR_2D && !(d->psf_bearing_span > 0.f))
+        return fail(RTSIM_ERR_INVALID, "psf_bearing_span must be > 0");
+    if (d->directivity && (d->dir_width < 1 || d->dir_height < 1))
+        return fail(RTSIM_ERR_INVALID, "directivity table size");
+    if (!(d->sound_speed > 0.f)) return fail(RTSIM_ERR_INVALID, "sound_speed must be > 0");
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_create(const rtsim_sensor_desc* d, rtsim_sensor* out)
+{
+    if (!out) return fail(RTSIM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_desc(d);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(RTSIM_ERR_HIP, "no HIP device available");
+    Sensor* s = new Sensor();
+    s->kind = d->kind;
+    s->range_min = d->range_min;
+    s->range_max = d->range_max;
+    s->N = d->range_count;
+    s->W = d->kind == RTSIM_SENSOR_2D ? d->bearing_count : 1u;
+    s->axis = d->kind == RTSIM_SENSOR_2D ? d->bearing_axis : RTSIM_AXIS_ANGLE;
+    if (d->kind == RTSIM_SENSOR_2D) s->bearings.assign(d->bearings, d->bearings + d->bearing_count);
+    else s->bearings.assign(1, 0.f);
+    s->Tr = d->psf_range_taps;
+    s->Tb = d->psf_bearing_taps;
+    s->psf.assign(d->psf, d->psf + 2 * (size_t)s->Tr * s->Tb);
+    s->psf_range_span = d->psf_range_span;
+    s->psf_bearing_span = d->kind == RTSIM_SENSOR_2D ? d->psf_bearing_span : 1.0f;
+    s->margin_request = d->margin;
+    s->frequency = d->frequency;
+    s->sound_speed = d->sound_speed;
+    s->spreading = d->spreading;
+    std::memcpy(s->R, d->rotation, sizeof(s->R));
+    if (d->directivity) {
+        s->dir.assign(d->directivity, d->directivity + (size_t)d->dir_width * d->dir_height);
+        s->dir_w = d->dir_width;
+        s->dir_h = d->dir_height;
+        s->dir_xs = d->dir_xscale;
+        s->dir_ys = d->dir_yscale;
+    }
+    // device-side constant tables
+    std::vector<float2> bc(s->W);
+    for (uint32_t b = 0; b < s->W; b++) {
+        double x = s->axis == RTSIM_AXIS_SINE ? std::sin((double)s->bearings[b])
+                                               : (double)s->bearings[b];
+        bc[b] = split_df(x);
+    }
+    if ((rc = dev_alloc(&s->d_bear, s->W)) || (rc = dev_alloc(&s->d_psf, (size_t)s->Tr * s->Tb)) ||
+        (rc = dev_alloc(&s->d_hist, 256 * (size_t)kMaxStripes)) ||
+        (rc = dev_alloc(&s->d_totals, 256))) {
+        delete s;
+        return rc;
+    }
+    if (hipMemcpy(s->d_bear, bc.data(), sizeof(float2) * s->W, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(s->d_psf, s->psf.data(), sizeof(float) * s->psf.size(), hipMemcpyHostToDevice) !=
+            hipSuccess) {
+        delete s;
+        return fail(RTSIM_ERR_HIP, "uploading sensor tables failed");
+    }
+    if (!s->dir.empty()) {
+        if ((rc = dev_alloc(&s->d_dir, s->dir.size()))) { delete s; return rc; }
+        if (hipMemcpy(s->d_dir, s->dir.data(), sizeof(float) * s->dir.size(),
+                      hipMemcpyHostToDevice) != hipSuccess) {
+            delete s;
+            return fail(RTSIM_ERR_HIP, "uploading the directivity table failed");
+        }
+    }
+    if ((rc = s->configure_ranges())) { delete s; return rc; }
+    *out = reinterpret_cast<rtsim_sensor>(s);
+    return RTSIM_OK;
+}
+
+static Sensor* get(rtsim_sensor h) { return reinterpret_cast<Sensor*>(h); }
+
+#define GET_SENSOR(h)                                                    \
+    Sensor* s = get(h);                                                  \
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+
+int rtsim_sensor_destroy(rtsim_sensor h)
+{
+    delete get(h);
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
+                            uint32_t range_count, float margin)
+{
+    GET_SENSOR(h);
+    if (range_count < 1) return fail(RTSIM_ERR_INVALID, "range_count must be >= 1");
+    if (!(range_max > range_min)) return fail(RTSIM_ERR_INVALID, "range_max must exceed range_min");
+    s->range_min = range_min;
+    s->range_max = range_max;
+    s->N = range_count;
+    s->margin_request = margin;
+    s->computed = false;
+    return s->configure_ranges();
+}
+
+int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count)
+{
+    GET_SENSOR(h);
+    if (count > 0x7fffffffu) return fail(RTSIM_ERR_INVALID, "at most 2^31 - 1 samples");
+    int rc = s->reserve(count);
+    if (rc) return rc;
+    s->count = count;
+    s->computed = false;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_sample_count(rtsim_sensor h, size_t* count)
+{
+    GET_SENSOR(h);
+    if (!count) return fail(RTSIM_ERR_INVALID, "count is NULL");
+    *count = s->count;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_samples(rtsim_sensor h, void** dev_ptr)
+{
+    GET_SENSOR(h);
+    if (!dev_ptr) return fail(RTSIM_ERR_INVALID, "dev_ptr is NULL");
+    *dev_ptr = s->d_samples;
+    return RTSIM_OK;
+}
+
+int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t n,
+                                 size_t offset, void* stream)
+{
+    GET_SENSOR(h);
+    if (!hits || !hits->range || !hits->dir_x || !hits->dir_y || !hits->dir_z || !hits->amplitude)
+        return fail(RTSIM_ERR_INVALID, "hits has a NULL array");
+    if (offset + n > s->count)
+        return fail(RTSIM_ERR_INVALID, "samples [%zu, %zu) exceed sample_count %zu", offset,
+                    offset + n, s->count);
+    hipStream_t st = (hipStream_t)stream;
+    StreamArgs a;
+    a.range = hits->range;
+    a.dx = hits->dir_x;
+    a.dy = hits->dir_y;
+    a.dz = hits->dir_z;
+    a.amp = reinterpret_cast<const float2*>(hits->amplitude);
+    a.out = s->d_samples + offset;
+    a.n = (uint32_t)n;
+    std::memcpy(a.R, s->R, sizeof(a.R));
+    a.k2 = 2.0 * (double)s->frequency / (double)s->sound_speed;
+    a.spreading = s->spreading;
+    a.is2d = s->kind == RTSIM_SENSOR_2D;
+    a.dir = s->d_dir;
+    a.dir_w = (int)s->dir_w;
+    a.dir_h = (int)s->dir_h;
+    a.dir_xs = s->dir_xs;
+    a.dir_ys = s->dir_ys;
+    s->stage_begin(0, st);
+    HIP_TRY(launch_stream(a, st));
+    s->stage_end(st);
+    s->computed = false;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream)
+{
+    GET_SENSOR(h);
+    return s->compute(form, (hipStream_t)stream);
+}
+
+int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height, uint32_t* width)
+{
+    GET_SENSOR(h);
+    if (dev_ptr) *dev_ptr = s->d_out;
+    if (height) *height = s->N;
+    if (width) *width = s->W;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_sorted_samples(rtsim_sensor h, const void** dev_ptr)
+{
+    GET_SENSOR(h);
+    if (!dev_ptr) return fail(RTSIM_ERR_INVALID, "dev_ptr is NULL");
+    *dev_ptr = s->d_sorted;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments,
+                              uint32_t* windows, uint32_t* order, int* overlap, void* stream)
+{
+    GET_SENSOR(h);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (!s->computed) return fail(RTSIM_ERR_INVALID, "compute has not run since the last change");
+    size_t n = s->count;
+    if (keys && n) HIP_TRY(hipMemcpy(keys, s->d_skeys, 4 * n, hipMemcpyDeviceToHost));
+    if (segments) HIP_TRY(hipMemcpy(segments, s->d_seg, 8 * (size_t)s->N, hipMemcpyDeviceToHost));
+    if (windows) HIP_TRY(hipMemcpy(windows, s->d_win, 8 * (size_t)s->N, hipMemcpyDeviceToHost));
+    if (order && n) HIP_TRY(hipMemcpy(order, s->order_ptr, 4 * n, hipMemcpyDeviceToHost));
+    if (overlap) *overlap = s->overlap;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_upload_samples(rtsim_sensor h, const void* host, size_t n, void* stream)
+{
+    GET_SENSOR(h);
+    if (n > s->count) return fail(RTSIM_ERR_INVALID, "n exceeds sample_count");
+    hipStream_t st = (hipStream_t)stream;
+    if (n) HIP_TRY(hipMemcpyAsync(s->d_samples, host, 16 * n, hipMemcpyHostToDevice, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    s->computed = false;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_download_samples(rtsim_sensor h, void* host, size_t n, int sorted, void* stream)
+{
+    GET_SENSOR(h);
+    if (n > s->count) return fail(RTSIM_ERR_INVALID, "n exceeds sample_count");
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    if (n)
+        HIP_TRY(hipMemcpy(host, sorted ? (const void*)s->d_sorted : (const void*)s->d_samples,
+                          16 * n, hipMemcpyDeviceToHost));
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream)
+{
+    GET_SENSOR(h);
+    hipStream_t st = (hipStream_t)stream;
+    HIP_TRY(hipStreamSynchronize(st));
+    HIP_TRY(hipMemcpy(host, s->d_out, sizeof(float2) * (size_t)s->N * s->W, hipMemcpyDeviceToHost));
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_set_profiling(rtsim_sensor h, int on)
+{
+    GET_SENSOR(h);
+    HIP_TRY(hipDeviceSynchronize());
+    for (auto& p : s->events) {
+        s->event_pool.push_back(p.start);
+        s->event_pool.push_back(p.stop);
+    }
+    s->events.clear();
+    s->open_event = -1;
+    s->profiling = on != 0;
+    return RTSIM_OK;
+}
+
+int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes)
+{
+    GET_SENSOR(h);
+    if (!ms || nstages < 1) return fail(RTSIM_ERR_INVALID, "ms buffer");
+    std::vector<double> sum(nstages, 0.0);
+    std::vector<int> cnt(nstages, 0);
+    for (auto& p : s->events) {
+        HIP_TRY(hipEventSynchronize(p.stop));
+        float t = 0.f;
+        HIP_TRY(hipEventElapsedTime(&t, p.start, p.stop));
+        if (p.stage < nstages) {
+            sum[p.stage] += t;
+            cnt[p.stage]++;
+        }
+    }
+    int maxc = 0;
+    for (int i = 0; i < nstages; i++) {
+        ms[i] = cnt[i] ? (float)(sum[i] / cnt[i]) : 0.f;
+        maxc = std::max(maxc, cnt[i]);
+    }
+    if (ncomputes) *ncomputes = maxc;
+    return RTSIM_OK;
+}
+
+int rtsim_stream_create(void** stream)
+{
+    if (!stream) return fail(RTSIM_ERR_INVALID, "stream is NULL");
+    hipStream_t st = nullptr;
+    HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    *stream = st;
+    return RTSIM_OK;
+}
+
+int rtsim_stream_destroy(void* stream)
+{
+    if (stream) HIP_TRY(hipStreamDestroy((hipStream_t)stream));
+    return RTSIM_OK;
+}
+
+struct Timer {
+    hipEvent_t start = nullptr, stop = nullptr;
+};
+
+int rtsim_timer_create(void** timer)
+{
+    if (!timer) return fail(RTSIM_ERR_INVALID, "timer is NULL");
+    Timer* t = new Timer();
+    if (hipEventCreate(&t->start) != hipSuccess || hipEventCreate(&t->stop) != hipSuccess) {
+        delete t;
+        return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    }
+    *timer = t;
+    return RTSIM_OK;
+}
+
+int rtsim_timer_destroy(void* timer)
+{
+    Timer* t = (Timer*)timer;
+    if (!t) return RTSIM_OK;
+    (void)hipEventDestroy(t->start);
+    (void)hipEventDestroy(t->stop);
+    delete t;
+    return RTSIM_OK;
+}
+
+int rtsim_timer_start(void* timer, void* stream)
+{
+    if (!timer) return fail(RTSIM_ERR_INVALID, "timer is NULL");
+    HIP_TRY(hipEventRecord(((Timer*)timer)->start, (hipStream_t)stream));
+    return RTSIM_OK;
+}
+
+int rtsim_timer_stop(void* timer, void* stream)
+{
+    if (!timer) return fail(RTSIM_ERR_INVALID, "timer is NULL");
+    HIP_TRY(hipEventRecord(((Timer*)timer)->stop, (hipStream_t)stream));
+    return RTSIM_OK;
+}
+
+int rtsim_timer_elapsed(void* timer, float* ms)
+{
+    if (!timer || !ms) return fail(RTSIM_ERR_INVALID, "timer/ms is NULL");
+    Timer* t = (Timer*)timer;
+    HIP_TRY(hipEventSynchronize(t->stop));
+    HIP_TRY(hipEventElapsedTime(ms, t->start, t->stop));
+    return RTSIM_OK;
+}
+
+int rtsim_device_alloc(void** ptr, size_t bytes)
+{
+    if (!ptr) return fail(RTSIM_ERR_INVALID, "ptr is NULL");
+    HIP_TRY(hipMalloc(ptr, bytes ? bytes : 1));
+    return RTSIM_OK;
+}
+
+int rtsim_device_free(void* ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return RTSIM_OK;
+}
+
+int rtsim_memcpy_h2d(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return RTSIM_OK;
+}
+
+int rtsim_memcpy_d2h(void* dst, const void* src, size_t bytes)
+{
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return RTSIM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,242 @@
+// Grid (MFMA) form host orchestration: validity checks, extended-grid
+// geometry, tap preparation (scaled, hi/lo split in BF16 and FP16) and the
+// deposit + correlation launches.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/rtsim.h"
+#include "grid_internal.h"
+#include "sensor.h"
+
+namespace rtsim {
+
+namespace {
+
+uint16_t host_bf16(float x)
+{
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+float host_bf16_f32(uint16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// IEEE binary16, round to nearest even (inputs here are finite, |x| < 2^15).
+uint16_t host_f16(float x)
+{
+    _Float16 h = (_Float16)x;
+    uint16_t u;
+    std::memcpy(&u, &h, 2);
+    return u;
+}
+
+float host_f16_f32(uint16_t u)
+{
+    _Float16 h;
+    std::memcpy(&h, &u, 2);
+    return (float)h;
+}
+
+struct GridState {
+    bool ready = false;
+    int status = RTSIM_OK;
+    std::string why;
+    int Hp = 0, Wp = 0, GH = 0, GW = 0, KB = 0, fr = 0, fb = 0, AS = 0, BL = 0;
+    float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
+    float bear0_hi = 0, bear0_lo = 0;
+    float tap_scale = 1.f;
+    float2* d_grid = nullptr;
+    uint32_t* d_row_start = nullptr;
+    uint16_t* d_taps[2] = {nullptr, nullptr};   // [0] bf16, [1] fp16
+    size_t lds = 0;
+
+    ~GridState()
+    {
+        if (d_grid) (void)hipFree(d_grid);
+        if (d_row_start) (void)hipFree(d_row_start);
+        for (auto p : d_taps)
+            if (p) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+void Sensor::free_grid()
+{
+    delete static_cast<GridState*>(grid_state);
+    grid_state = nullptr;
+}
+
+static int prepare_grid(Sensor* s, GridState* g)
+{
+    if (s->kind != RTSIM_SENSOR_2D)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: 1D sensors use the plain form");
+    if (s->W < 2 || s->N < 2)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form needs >= 2 bearings and >= 2 ranges");
+    // uniform bearing axis at the PSF bearing pitch
+    std::vector<double> x(s->W);
+    for (uint32_t b = 0; b < s->W; b++)
+        x[b] = s->axis == RTSIM_AXIS_SINE ? std::sin((double)s->bearings[b]) : (double)s->bearings[b];
+    const double pb = (x[s->W - 1] - x[0]) / (double)(s->W - 1);
+    double dev = 0;
+    for (uint32_t b = 0; b < s->W; b++) dev = std::fmax(dev, std::fabs(x[b] - (x[0] + b * pb)));
+    if (dev > 5e-5 * pb)
+        return fail(RTSIM_ERR_UNSUPPORTED,
+                    "MFMA form needs bearings uniform in the PSF axis (deviation %.3g pitch)",
+                    dev / pb);
+    const double psf_pb = (double)s->psf_bearing_span / s->Tb;
+    if (std::fabs(psf_pb - pb) > 5e-5 * pb)
+        return fail(RTSIM_ERR_UNSUPPORTED,
+                    "MFMA form needs the PSF bearing pitch (%.6g) equal to the bearing pitch (%.6g)",
+                    psf_pb, pb);
+    const double dr = ((double)s->range_max - (double)s->range_min) / (double)(s->N - 1);
+    const double psf_pr = (double)s->psf_range_span / s->Tr;
+    if (std::fabs(psf_pr - dr) > 5e-5 * dr)
+        return fail(RTSIM_ERR_UNSUPPORTED,
+                    "MFMA form needs the PSF range pitch (%.6g) equal to the range step (%.6g)",
+                    psf_pr, dr);
+    const int ov_req = ((int)s->Tr + 3) / 2;
+    if (s->overlap < ov_req)
+        return fail(RTSIM_ERR_UNSUPPORTED,
+                    "MFMA form needs a Binner overlap >= %d bins for %u range taps (have %d)",
+                    ov_req, s->Tr, s->overlap);
+
+    g->Hp = (int)((s->N + kCorrRows - 1) / kCorrRows) * kCorrRows;
+    g->Wp = (int)((s->W + kCorrCols - 1) / kCorrCols) * kCorrCols;
+    g->KB = ((int)s->Tb + 15 + 31) / 32;
+    g->GH = g->Hp + (int)s->Tr - 1;
+    g->GW = g->Wp - 16 + 32 * g->KB;
+    g->AS = 16 + 32 * g->KB + 8;
+    g->BL = 32 * g->KB + 16;
+    g->fr = ((int)s->Tr - 1) / 2;
+    g->fb = ((int)s->Tb - 1) / 2;
+    g->off_r = (float)(0.5 * (s->Tr - 1) - g->fr);
+    g->off_b = (float)(0.5 * (s->Tb - 1) - g->fb);
+    g->inv_dr = (float)(1.0 / dr);
+    g->inv_db = (float)(1.0 / psf_pb);
+    g->bear0_hi = (float)x[0];
+    g->bear0_lo = (float)(x[0] - (double)g->bear0_hi);
+
+    CorrArgs probe{};
+    probe.Tr = (int)s->Tr;
+    probe.AS = g->AS;
+    probe.BL = g->BL;
+    g->lds = correlate_lds_bytes(probe);
+    int dev_id = 0, max_lds = 0;
+    (void)hipGetDevice(&dev_id);
+    if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id) !=
+            hipSuccess || max_lds <= 0)
+        max_lds = 64 * 1024;
+    if (g->lds > (size_t)max_lds)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %zu B of LDS per tile exceed %d B",
+                    g->lds, max_lds);
+    const size_t dep_lds = sizeof(float) * 2 * kDepositRows * g->GW;
+    if (dep_lds > (size_t)max_lds)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
+
+    // taps: power-of-two scale so max |T| in [0.5, 1), then hi/lo planes
+    const size_t T = (size_t)s->Tr * s->Tb;
+    float mx = 0.f;
+    for (size_t k = 0; k < 2 * T; k++) mx = std::fmax(mx, std::fabs(s->psf[k]));
+    int e = 0;
+    if (mx > 0.f) std::frexp(mx, &e);
+    g->tap_scale = std::ldexp(1.0f, -e);
+    std::vector<uint16_t> planes[2];
+    for (int d = 0; d < 2; d++) planes[d].assign(4 * T, 0);
+    for (size_t k = 0; k < T; k++) {
+        for (int c = 0; c < 2; c++) {               // re, im
+            float v = s->psf[2 * k + c] * g->tap_scale;
+            uint16_t h = host_bf16(v);
+            uint16_t l = host_bf16(v - host_bf16_f32(h));
+            planes[0][(2 * c + 0) * T + k] = h;
+            planes[0][(2 * c + 1) * T + k] = l;
+            h = host_f16(v);
+            l = host_f16(v - host_f16_f32(h));
+            planes[1][(2 * c + 0) * T + k] = h;
+            planes[1][(2 * c + 1) * T + k] = l;
+        }
+    }
+    hipError_t err;
+    if ((err = hipMalloc((void**)&g->d_grid, sizeof(float2) * (size_t)g->GH * g->GW)) != hipSuccess)
+        return hip_fail(err, "hipMalloc(grid)");
+    if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
+        return hip_fail(err, "hipMalloc(row_start)");
+    for (int d = 0; d < 2; d++) {
+        if ((err = hipMalloc((void**)&g->d_taps[d], sizeof(uint16_t) * 4 * T)) != hipSuccess)
+            return hip_fail(err, "hipMalloc(taps)");
+        if ((err = hipMemcpy(g->d_taps[d], planes[d].data(), sizeof(uint16_t) * 4 * T,
+                             hipMemcpyHostToDevice)) != hipSuccess)
+            return hip_fail(err, "hipMemcpy(taps)");
+    }
+    g->ready = true;
+    return RTSIM_OK;
+}
+
+int Sensor::compute_grid_form(int form, hipStream_t s)
+{
+    GridState* g = static_cast<GridState*>(grid_state);
+    if (!g) {
+        g = new GridState();
+        grid_state = g;
+        g->status = prepare_grid(this, g);
+        if (g->status) g->why = fail_message();
+    }
+    if (!g->ready) return fail(g->status, "%s", g->why.c_str());
+
+    DepositArgs d;
+    d.sorted = d_sorted;
+    d.coord = d_coord;
+    d.skeys = d_skeys;
+    d.n = (uint32_t)count;
+    d.row_start = g->d_row_start;
+    d.grid = g->d_grid;
+    d.GH = g->GH;
+    d.GW = g->GW;
+    d.fr = g->fr;
+    d.fb = g->fb;
+    d.range0 = range_min;
+    d.inv_dr = g->inv_dr;
+    d.off_r = g->off_r;
+    d.bear0_hi = g->bear0_hi;
+    d.bear0_lo = g->bear0_lo;
+    d.inv_db = g->inv_db;
+    d.off_b = g->off_b;
+    stage_begin(3, s);
+    hipError_t err = launch_deposit(d, s);
+    stage_end(s);
+    if (err != hipSuccess) return hip_fail(err, "deposit");
+
+    CorrArgs c;
+    c.grid = g->d_grid;
+    c.taps = g->d_taps[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
+    c.win = d_win;
+    c.out = d_out;
+    c.H = (int)N;
+    c.W = (int)W;
+    c.Tr = (int)Tr;
+    c.Tb = (int)Tb;
+    c.GH = g->GH;
+    c.GW = g->GW;
+    c.KB = g->KB;
+    c.AS = g->AS;
+    c.BL = g->BL;
+    c.tap_scale = g->tap_scale;
+    stage_begin(4, s);
+    err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);
+    stage_end(s);
+    if (err != hipSuccess) return hip_fail(err, "correlate");
+    return RTSIM_OK;
+}
+
+}  // namespace rtsim

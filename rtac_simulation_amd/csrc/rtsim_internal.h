@@ -1,0 +1,90 @@
+// Internal declarations shared by the host runtime (capi.cpp) and the HIP
+// kernels (kernels_*.hip).  Not part of the public ABI (include/rtsim.h).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace rtsim {
+
+constexpr uint32_t kOutOfRange = 0xFFFFFFFFu;
+
+// Binner::Key constants (Binner.h:25-45), computed on the host in FP32.
+struct KeyParams {
+    uint32_t n;
+    float a, b, lower, upper;
+};
+
+// ---- streaming kernel: ReceiverView::set_sample + raygen phase -------------
+struct StreamArgs {
+    const float* range;
+    const float* dx;
+    const float* dy;
+    const float* dz;
+    const float2* amp;
+    float4* out;          // records {re, im, travel, bearing}
+    uint32_t n;
+    float R[9];           // row-major pose rotation; local = R^T dir
+    double k2;            // 2 f / c (cycles per metre of one-way travel)
+    int spreading;
+    int is2d;
+    const float* dir;     // [dir_h][dir_w] or nullptr
+    int dir_w, dir_h;
+    float dir_xs, dir_ys;
+};
+hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
+
+// ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
+struct SortBuffers {
+    uint32_t* keys[2];
+    uint32_t* vals[2];
+    uint32_t* hist;       // 256 * max_stripes
+    uint32_t* totals;     // 256
+    uint32_t max_stripes;
+};
+// Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
+// buffers that hold the permutation and the sorted travel images.
+hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
+                       const uint32_t** order_out, const uint32_t** img_out,
+                       hipStream_t s);
+
+// ---- gather sorted records + keys + segments (Binner.cu:24-80) --------------
+struct GatherArgs {
+    const float4* rec;
+    const uint32_t* order;
+    const uint32_t* img;
+    uint32_t n;
+    KeyParams key;
+    float4* sorted;
+    float2* coord;        // bearing-axis coordinate as (hi, lo) floats
+    uint32_t* skeys;
+    uint2* seg;           // must be zeroed beforehand
+    int axis;
+};
+hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
+
+hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
+
+// ---- windows (Binner.cu:82-136) ---------------------------------------------
+hipError_t launch_windows(const uint2* seg, uint32_t N, int ov, uint2* win, hipStream_t s);
+
+// ---- plain FP32 PSF convolution (SensorInstance{1D,2D}.cu) ------------------
+struct ConvArgs {
+    const float4* sorted;
+    const float2* coord;
+    const uint2* win;
+    const float2* range_c;   // per output row: (hi, lo) of the range in metres
+    const float2* bear_c;    // per output bearing: (hi, lo) of the axis coordinate
+    const float2* psf;       // [Tr][Tb] complex
+    float2* out;             // [H][W]
+    int H, W, Tr, Tb;
+    float inv_pr, half_r;    // range tap coordinate yB = dr * inv_pr + half_r
+    float inv_pb, half_b;    // bearing tap coordinate xB = dx * inv_pb + half_b
+    float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
+};
+hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s);
+hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
+
+}  // namespace rtsim

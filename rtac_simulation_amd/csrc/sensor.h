@@ -1,0 +1,92 @@
+// Host-side sensor instance state (one per rtsim_sensor handle).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "rtsim_internal.h"
+
+namespace rtsim {
+
+int fail(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+const char* fail_message();
+KeyParams make_key(uint32_t N, float lo, float hi, float margin);
+int make_overlap(float margin, float lo, float hi, uint32_t N);
+
+constexpr uint32_t kMaxStripes = 1024;
+
+struct StageEvent {
+    int stage;
+    hipEvent_t start, stop;
+};
+
+struct Sensor {
+    // ---- configuration (SensorInfo / SensorInstance members) -------------
+    int kind = 0;
+    float range_min = 0.f, range_max = 1.f;
+    uint32_t N = 1;              // range bins
+    uint32_t W = 1;              // bearings (1 for 1D)
+    int axis = 0;
+    std::vector<float> bearings;
+    uint32_t Tr = 1, Tb = 1;
+    std::vector<float> psf;      // complex interleaved [Tr][Tb]
+    float psf_range_span = 1.f, psf_bearing_span = 1.f;
+    float margin_request = -1.f;
+    float margin = 0.f;
+    std::vector<float> dir;
+    uint32_t dir_w = 0, dir_h = 0;
+    float dir_xs = 0.f, dir_ys = 0.f;
+    float frequency = 0.f, sound_speed = 1500.f;
+    int spreading = 0;
+    float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    KeyParams key{};
+    int overlap = 0;
+
+    // ---- samples -----------------------------------------------------------
+    size_t count = 0, capacity = 0;
+    bool computed = false;
+    float4* d_samples = nullptr;  // receivedSamples_ (unsorted)
+    float4* d_sorted = nullptr;   // receivedSamples_ after sort()
+    float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
+    uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_vA = nullptr, *d_vB = nullptr;
+    uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
+    const uint32_t* order_ptr = nullptr;
+    uint32_t* d_hist = nullptr;
+    uint32_t* d_totals = nullptr;
+
+    // ---- per-configuration device tables -----------------------------------
+    uint2* d_seg = nullptr;       // Binner::binIndexes_
+    uint2* d_win = nullptr;       // bins_ as (start, size)
+    float2* d_out = nullptr;      // ping [N][W]
+    float2* d_psf = nullptr;
+    float2* d_bear = nullptr;     // bearing coordinate (hi, lo)
+    float2* d_rng = nullptr;      // range coordinate (hi, lo)
+    float* d_dir = nullptr;
+
+    // ---- grid (MFMA) form state --------------------------------------------
+    void* grid_state = nullptr;
+
+    // ---- profiling -----------------------------------------------------------
+    bool profiling = false;
+    std::vector<StageEvent> events;
+    std::vector<hipEvent_t> event_pool;
+    int open_event = -1;
+
+    ~Sensor();
+    void free_samples();
+    void free_grid();
+    int reserve(size_t count);
+    int configure_ranges();
+    int check_plain_limits() const;
+    int compute(int form, hipStream_t s);
+    int compute_grid_form(int form, hipStream_t s);
+    ConvArgs conv_args() const;
+    hipEvent_t take_event();
+    void stage_begin(int stage, hipStream_t s);
+    void stage_end(hipStream_t s);
+};
+
+}  // namespace rtsim

@@ -1,0 +1,113 @@
+"""GPU parity of the MFMA grid form (BF16 / FP16 hi+lo split, FP32
+accumulation) against the FP64 oracle: relative L2 <= 1e-3 (tolerance of the
+MFMA form), through the C ABI."""
+import numpy as np
+import pytest
+
+import oracle as O
+from rtac_simulation_amd import configs
+
+pytestmark = pytest.mark.gpu
+
+MFMA_TOL = 1e-3
+FORMS = [1, 2]          # RTSIM_FORM_MFMA_BF16, RTSIM_FORM_MFMA_FP16
+
+
+def _sensor(R, cfg, hits):
+    s = R.make_sensor(cfg)
+    s.set_sample_count(hits.n)
+    s.write_samples(hits.to_device())
+    return s
+
+
+def _psf(kind):
+    if kind == "15x9":
+        return np.outer(configs.baseband_pulse(15), configs.sinc2_beam(9, 3.0))
+    if kind == "31x17":
+        return configs.psf_31x17()
+    if kind == "21x33":
+        return configs.perturbed(np.outer(configs.lfm_pulse(21, 0.03),
+                                          configs.sinc2_beam(33, 8.0)), -30.0, 9)
+    if kind == "8x6":
+        return configs.perturbed(np.outer(configs.baseband_pulse(8), configs.sinc2_beam(6, 2.0)),
+                                 -30.0, 10)
+    raise KeyError(kind)
+
+
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("W,H,psf", [(48, 96, "31x17"), (64, 128, "15x9"), (40, 100, "21x33"),
+                                     (37, 70, "8x6")])
+def test_mfma_small_vs_oracle(gpu, form, W, H, psf):
+    R = gpu
+    cfg = configs._fls(W, H, 0.1, 30.0, 1.2e6, _psf(psf), f"m{W}x{H}")
+    hits = R.generate_hits(configs.fls_scene(), 21, 40_000)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    got = s.output()
+    ref, b = O.reference_ping(cfg, s.samples())
+    err = O.rel_l2(got, ref)
+    assert err <= MFMA_TOL, err
+    # masked rows are exactly zero
+    empty = b["windows"][:, 1] == 0
+    assert not np.abs(got[empty]).any()
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_full_size_vs_plain_and_oracle(gpu, form):
+    R = gpu
+    cfg, scene, S, seed = configs.workload("fls_512x1024")
+    hits = R.generate_hits(scene, seed, S)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(0)
+    plain = s.output()
+    s.compute_output(form)
+    mf = s.output()
+    assert O.rel_l2(mf, plain) <= MFMA_TOL
+    b = O.binning(cfg, s.samples())
+    rows = np.arange(7, cfg.H, 64)
+    win = np.zeros_like(b["windows"])
+    win[rows] = b["windows"][rows]
+    ref = O.conv(cfg, b["sorted"], win)
+    assert O.rel_l2(mf[rows], ref[rows]) <= MFMA_TOL
+
+
+def test_mfma_rejects_what_it_cannot_restate(gpu):
+    R = gpu
+    # 1D sensor
+    cfg1, scene1, S1, seed1 = configs.workload("single_beam_38k")
+    s = _sensor(R, cfg1, R.generate_hits(scene1, seed1, 1000))
+    with pytest.raises(R.RtsimError) as e:
+        s.compute_output(1)
+    assert e.value.code == R.ERR_UNSUPPORTED
+    # bearing table not uniform in the PSF axis (reference angle axis on a sine grid)
+    cfg = configs._fls(32, 64, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "angle")
+    cfg.bearing_axis = 0
+    s = _sensor(R, cfg, R.generate_hits(configs.fls_scene(), 3, 1000))
+    with pytest.raises(R.RtsimError) as e:
+        s.compute_output(2)
+    assert e.value.code == R.ERR_UNSUPPORTED
+    # overlap below the PSF support: the windowed gather truncates the PSF
+    cfg = configs._fls(32, 64, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "ov")
+    cfg.margin = configs.overlap_margin(3, cfg.range_step)
+    s = _sensor(R, cfg, R.generate_hits(configs.fls_scene(), 3, 1000))
+    with pytest.raises(R.RtsimError) as e:
+        s.compute_output(1)
+    assert e.value.code == R.ERR_UNSUPPORTED
+    # the plain form still serves it
+    s.compute_output(0)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_dynamic_range(gpu, form):
+    """Samples spanning 60 dB: per-tile power-of-two scaling keeps FP16 in range."""
+    R = gpu
+    cfg = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "dr")
+    hits = R.generate_hits(configs.fls_scene(), 22, 40_000)
+    rng = np.random.default_rng(0)
+    g = 10.0 ** (-3.0 * rng.uniform(0, 1, hits.n)).astype(np.float32)
+    hits.amplitude[0::2] *= g
+    hits.amplitude[1::2] *= g
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    ref, _ = O.reference_ping(cfg, s.samples())
+    assert O.rel_l2(s.output(), ref) <= MFMA_TOL
