@@ -1,0 +1,41 @@
+/*
+ * rtsim_batch.h -- batched pings of the rtsim C ABI.
+ *
+ * The reference computes one ping per SensorInstance::compute_output call
+ * (SensorInstance.h:67) and synchronises the device after every kernel
+ * (Binner.cu:78,118,134, SensorInstance2D.cu:100,117).  A batch computes many
+ * sensor instances (mixed 1D / 2D, uneven sizes, per-sensor form) from one
+ * hipGraph launch; the graph is captured on the first compute and again only
+ * after a sensor's ranges or sample count change.
+ */
+#ifndef RTSIM_BATCH_H
+#define RTSIM_BATCH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include "rtsim.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rtsim_batch_s* rtsim_batch;
+
+/* forms[i] is the RTSIM_FORM_* of sensors[i] (NULL: plain FP32 for all).
+ * Form-specific device state is prepared here, so an unsupported form is
+ * reported at creation.  The sensors must outlive the batch.                  */
+int rtsim_batch_create(const rtsim_sensor* sensors, const int* forms, uint32_t count,
+                       rtsim_batch* out);
+/* Includes the streaming kernel (rtsim_receiver_write_samples) of sensor
+ * `index` in the batch: n must equal its sample count.  hits == NULL unbinds. */
+int rtsim_batch_bind_hits(rtsim_batch batch, uint32_t index, const rtsim_hits* hits, size_t n);
+/* Queues the whole batch on `stream` (one graph launch).                       */
+int rtsim_batch_compute(rtsim_batch batch, void* stream);
+int rtsim_batch_destroy(rtsim_batch batch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTSIM_BATCH_H */

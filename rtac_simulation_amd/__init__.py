@@ -129,6 +129,17 @@ _SIGNATURES = [
       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("rtsim_directivity_uniform", C.c_int,
      [C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rtsim_info_from_yaml", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("rtsim_info_desc", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(SensorDesc))]),
+    ("rtsim_info_destroy", C.c_int, [C.c_void_p]),
+    ("rtsim_sensor_create_from_yaml", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    ("rtsim_config_add_search_path", C.c_int, [C.c_char_p]),
+    ("rtsim_config_find", C.c_int, [C.c_char_p, C.c_char_p, C.c_size_t]),
+    ("rtsim_batch_create", C.c_int,
+     [C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.c_uint32, C.POINTER(C.c_void_p)]),
+    ("rtsim_batch_bind_hits", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Hits), C.c_size_t]),
+    ("rtsim_batch_compute", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("rtsim_batch_destroy", C.c_int, [C.c_void_p]),
 ]
 
 _lib = None
@@ -409,6 +420,76 @@ class SensorInstance2D_Complex(SensorInstance):
 
 def make_sensor(cfg: "SensorConfig") -> SensorInstance:
     return (SensorInstance2D_Complex if cfg.kind == SENSOR_2D else SensorInstance1D_Complex)(cfg)
+
+
+class Batch:
+    """Many sensor instances computed from one hipGraph launch (rtsim_batch.h)."""
+
+    def __init__(self, sensors: list, forms: list[int]):
+        self.sensors = list(sensors)
+        n = len(self.sensors)
+        hs = (C.c_void_p * n)(*[s.handle for s in self.sensors])
+        fs = (C.c_int * n)(*forms)
+        h = C.c_void_p()
+        check(lib().rtsim_batch_create(hs, fs, n, C.byref(h)))
+        self.handle = h
+        self._hits = {}
+
+    def bind_hits(self, index: int, hits: "DeviceHits | None"):
+        if hits is None:
+            check(lib().rtsim_batch_bind_hits(self.handle, index, None, 0))
+            self._hits.pop(index, None)
+        else:
+            check(lib().rtsim_batch_bind_hits(self.handle, index, C.byref(hits.struct), hits.n))
+            self._hits[index] = hits
+
+    def compute(self, stream: "Stream | None" = None):
+        check(lib().rtsim_batch_compute(self.handle, stream.handle if stream else None))
+
+    def __del__(self):
+        try:
+            lib().rtsim_batch_destroy(self.handle)
+        except Exception:
+            pass
+
+
+CONFIG_DIR = PKG_DIR / "configs"
+
+
+def config_from_yaml(path) -> "SensorConfig":
+    """SensorInfoFactory::Make(path) through the C ABI, as a SensorConfig
+    (host only: no device work)."""
+    from .configs import SensorConfig
+    info = C.c_void_p()
+    check(lib().rtsim_info_from_yaml(str(path).encode(), C.byref(info)))
+    try:
+        dp = C.POINTER(SensorDesc)()
+        check(lib().rtsim_info_desc(info, C.byref(dp)))
+        d = dp.contents
+        Tr, Tb = d.psf_range_taps, d.psf_bearing_taps
+        psf = np.ctypeslib.as_array(d.psf, (2 * Tr * Tb,)).copy().view(np.complex64)
+        cfg = SensorConfig(kind=d.kind, range_min=d.range_min, range_max=d.range_max,
+                           range_count=d.range_count, psf=psf.reshape(Tr, Tb),
+                           psf_range_span=d.psf_range_span, psf_bearing_span=d.psf_bearing_span,
+                           margin=d.margin, bearing_axis=d.bearing_axis,
+                           frequency=d.frequency, sound_speed=d.sound_speed,
+                           spreading=d.spreading, name=Path(str(path)).stem)
+        if d.kind == SENSOR_2D:
+            cfg.bearings = np.ctypeslib.as_array(d.bearings, (d.bearing_count,)).copy()
+        if d.dir_width and d.dir_height:
+            cfg.directivity = np.ctypeslib.as_array(
+                d.directivity, (d.dir_width * d.dir_height,)).copy().reshape(d.dir_height,
+                                                                              d.dir_width)
+            cfg.dir_xscale, cfg.dir_yscale = d.dir_xscale, d.dir_yscale
+        cfg.rotation = np.array(list(d.rotation), np.float32).reshape(3, 3)
+        return cfg
+    finally:
+        lib().rtsim_info_destroy(info)
+
+
+def sensor_from_yaml(path) -> SensorInstance:
+    """SensorFactory::Make(path): a sensor instance built from a YAML file."""
+    return make_sensor(config_from_yaml(path))
 
 
 from .configs import SensorConfig  # noqa: E402  (re-export)

@@ -12,6 +12,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -79,6 +80,20 @@ int make_overlap(float margin, float lo, float hi, uint32_t N)
     return ov;
 }
 #pragma clang fp contract(on)
+
+void init_kernel_attributes()
+{
+    static std::once_flag once;
+    std::call_once(once, []() {
+        int dev = 0, max_lds = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess || max_lds <= 0)
+            max_lds = 64 * 1024;
+        init_plain_attributes(max_lds);
+        init_grid_attributes(max_lds);
+    });
+}
 
 static float2 split_df(double x)
 {
@@ -177,7 +192,37 @@ int Sensor::configure_ranges()
     HIP_TRY(hipMemcpy(d_rng, rc.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d_out, 0, sizeof(float2) * (size_t)N * W));
     free_grid();
+    generation++;
     return RTSIM_OK;
+}
+
+int Sensor::prepare(int form)
+{
+    if (form == RTSIM_FORM_PLAIN_FP32) return check_plain_limits();
+    if (form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16) return ensure_grid();
+    return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
+}
+
+StreamArgs Sensor::stream_args(const rtsim_hits& hits, size_t n, size_t offset) const
+{
+    StreamArgs a;
+    a.range = hits.range;
+    a.dx = hits.dir_x;
+    a.dy = hits.dir_y;
+    a.dz = hits.dir_z;
+    a.amp = reinterpret_cast<const float2*>(hits.amplitude);
+    a.out = d_samples + offset;
+    a.n = (uint32_t)n;
+    std::memcpy(a.R, R, sizeof(a.R));
+    a.k2 = 2.0 * (double)frequency / (double)sound_speed;
+    a.spreading = spreading;
+    a.is2d = kind == RTSIM_SENSOR_2D;
+    a.dir = d_dir;
+    a.dir_w = (int)dir_w;
+    a.dir_h = (int)dir_h;
+    a.dir_xs = dir_xs;
+    a.dir_ys = dir_ys;
+    return a;
 }
 
 hipEvent_t Sensor::take_event()
@@ -215,10 +260,15 @@ void Sensor::stage_end(hipStream_t s)
 int Sensor::check_plain_limits() const
 {
     if (kind == RTSIM_SENSOR_2D) {
-        size_t shm = sizeof(float) * 2 * W + sizeof(float2) * W + sizeof(float2) * Tr * Tb;
-        if (shm > 150 * 1024)
+        int dev = 0, max_lds = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) !=
+                hipSuccess || max_lds <= 0)
+            max_lds = 64 * 1024;
+        ConvArgs c = conv_args();
+        if (plain2d_lds_bytes(c) > (size_t)max_lds)
             return fail(RTSIM_ERR_UNSUPPORTED,
-                        "plain form: %u bearings x %ux%u taps exceed the LDS budget", W, Tr, Tb);
+                        "plain form: a row of %u bearings exceeds the %d B LDS budget", W, max_lds);
     }
     return RTSIM_OK;
 }
@@ -297,6 +347,8 @@ ConvArgs Sensor::conv_args() const
     c.inv_pb = (float)((double)Tb / (double)psf_bearing_span);
     c.half_b = 0.5f * (float)(Tb - 1);
     c.supp_b = (float)(0.5 * (double)(Tb + 1) * (double)psf_bearing_span / (double)Tb);
+    // tables up to 32 KiB go to LDS; larger ones are read through L1/L2
+    c.psf_in_lds = sizeof(float2) * (size_t)Tr * Tb <= 32 * 1024;
     return c;
 }
 
@@ -367,6 +419,7 @@ int rtsim_sensor_create(const rtsim_sensor_desc* d, rtsim_sensor* out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return fail(RTSIM_ERR_HIP, "no HIP device available");
+    init_kernel_attributes();
     Sensor* s = new Sensor();
     s->kind = d->kind;
     s->range_min = d->range_min;
@@ -457,6 +510,7 @@ int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count)
     if (count > 0x7fffffffu) return fail(RTSIM_ERR_INVALID, "at most 2^31 - 1 samples");
     int rc = s->reserve(count);
     if (rc) return rc;
+    if (s->count != count) s->generation++;
     s->count = count;
     s->computed = false;
     return RTSIM_OK;
@@ -488,23 +542,7 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
         return fail(RTSIM_ERR_INVALID, "samples [%zu, %zu) exceed sample_count %zu", offset,
                     offset + n, s->count);
     hipStream_t st = (hipStream_t)stream;
-    StreamArgs a;
-    a.range = hits->range;
-    a.dx = hits->dir_x;
-    a.dy = hits->dir_y;
-    a.dz = hits->dir_z;
-    a.amp = reinterpret_cast<const float2*>(hits->amplitude);
-    a.out = s->d_samples + offset;
-    a.n = (uint32_t)n;
-    std::memcpy(a.R, s->R, sizeof(a.R));
-    a.k2 = 2.0 * (double)s->frequency / (double)s->sound_speed;
-    a.spreading = s->spreading;
-    a.is2d = s->kind == RTSIM_SENSOR_2D;
-    a.dir = s->d_dir;
-    a.dir_w = (int)s->dir_w;
-    a.dir_h = (int)s->dir_h;
-    a.dir_xs = s->dir_xs;
-    a.dir_ys = s->dir_ys;
+    StreamArgs a = s->stream_args(*hits, n, offset);
     s->stage_begin(0, st);
     HIP_TRY(launch_stream(a, st));
     s->stage_end(st);

@@ -183,7 +183,7 @@ static int prepare_grid(Sensor* s, GridState* g)
     return RTSIM_OK;
 }
 
-int Sensor::compute_grid_form(int form, hipStream_t s)
+int Sensor::ensure_grid()
 {
     GridState* g = static_cast<GridState*>(grid_state);
     if (!g) {
@@ -193,6 +193,14 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
         if (g->status) g->why = fail_message();
     }
     if (!g->ready) return fail(g->status, "%s", g->why.c_str());
+    return RTSIM_OK;
+}
+
+int Sensor::compute_grid_form(int form, hipStream_t s)
+{
+    int rc = ensure_grid();
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(grid_state);
 
     DepositArgs d;
     d.sorted = d_sorted;

@@ -302,24 +302,19 @@ hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s)
 {
     dim3 grid((a.W + kCorrCols - 1) / kCorrCols, (a.H + kCorrRows - 1) / kCorrRows);
     const size_t shm = correlate_lds_bytes(a);
-    if (fp16) {
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void*)k_correlate<true>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr = true;
-        }
-        hipLaunchKernelGGL(k_correlate<true>, grid, dim3(256), shm, s, a);
-    } else {
-        static bool attr = false;
-        if (!attr) {
-            (void)hipFuncSetAttribute((const void*)k_correlate<false>,
-                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-            attr = true;
-        }
-        hipLaunchKernelGGL(k_correlate<false>, grid, dim3(256), shm, s, a);
-    }
+    if (fp16) hipLaunchKernelGGL(k_correlate<true>, grid, dim3(256), shm, s, a);
+    else hipLaunchKernelGGL(k_correlate<false>, grid, dim3(256), shm, s, a);
     return hipGetLastError();
+}
+
+void init_grid_attributes(int max_lds)
+{
+    (void)hipFuncSetAttribute((const void*)k_correlate<true>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    (void)hipFuncSetAttribute((const void*)k_correlate<false>,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    (void)hipFuncSetAttribute((const void*)k_deposit,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
 }
 
 }  // namespace rtsim

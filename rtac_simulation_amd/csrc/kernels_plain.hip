@@ -43,15 +43,22 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
     return acc;
 }
 
+size_t plain2d_lds_bytes(const ConvArgs& a)
+{
+    return sizeof(float) * 2 * a.W + sizeof(float2) * a.W +
+           (a.psf_in_lds ? sizeof(float2) * a.Tr * a.Tb : 0);
+}
+
 // One workgroup per output range row.  LDS holds the row accumulator, the
-// bearing coordinates and the PSF table.
+// bearing coordinates and (when it fits) the PSF table.
 __global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
 {
     extern __shared__ float smem[];
     const int W = a.W;
     float* acc = smem;                                  // 2 W
     float2* bear = reinterpret_cast<float2*>(smem + 2 * W); // W
-    float2* psf = bear + W;                             // Tr * Tb
+    // PSF table in LDS when it fits, else read through the caches
+    const float2* psf = a.psf_in_lds ? bear + W : a.psf;
     const int r = blockIdx.x;
     const int t = threadIdx.x;
     for (int b = t; b < W; b += blockDim.x) {
@@ -59,7 +66,8 @@ __global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
         acc[2 * b + 1] = 0.f;
         bear[b] = a.bear_c[b];
     }
-    for (int k = t; k < a.Tr * a.Tb; k += blockDim.x) psf[k] = a.psf[k];
+    if (a.psf_in_lds)
+        for (int k = t; k < a.Tr * a.Tb; k += blockDim.x) (bear + W)[k] = a.psf[k];
     __syncthreads();
 
     const uint2 win = a.win[r];
@@ -96,9 +104,15 @@ __global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
 {
     if (a.H == 0 || a.W == 0) return hipSuccess;
-    size_t shm = sizeof(float) * 2 * a.W + sizeof(float2) * a.W + sizeof(float2) * a.Tr * a.Tb;
+    size_t shm = plain2d_lds_bytes(a);
     hipLaunchKernelGGL(k_conv2d_plain, dim3(a.H), dim3(256), shm, s, a);
     return hipGetLastError();
+}
+
+void init_plain_attributes(int max_lds)
+{
+    (void)hipFuncSetAttribute((const void*)k_conv2d_plain,
+                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
 }
 
 // 1D: one wave per output range bin, FP32 partial sums reduced across the

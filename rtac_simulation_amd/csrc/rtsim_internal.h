@@ -83,7 +83,11 @@ struct ConvArgs {
     float inv_pr, half_r;    // range tap coordinate yB = dr * inv_pr + half_r
     float inv_pb, half_b;    // bearing tap coordinate xB = dx * inv_pb + half_b
     float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
+    int psf_in_lds;          // stage the PSF table in LDS (else read it from global)
 };
+size_t plain2d_lds_bytes(const ConvArgs& a);
+void init_plain_attributes(int max_lds);
+void init_grid_attributes(int max_lds);
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s);
 hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
 

@@ -6,9 +6,13 @@
 #include <string>
 #include <vector>
 
+#include "../../include/rtsim.h"
 #include "rtsim_internal.h"
 
 namespace rtsim {
+
+// one-time kernel attribute setup (dynamic LDS limits); never under capture
+void init_kernel_attributes();
 
 int fail(int code, const char* fmt, ...);
 int hip_fail(hipError_t e, const char* what);
@@ -75,14 +79,21 @@ struct Sensor {
     std::vector<hipEvent_t> event_pool;
     int open_event = -1;
 
+    // bumped whenever buffers or kernel arguments change (captured graphs
+    // built from this sensor must then be re-captured)
+    uint64_t generation = 0;
+
     ~Sensor();
     void free_samples();
     void free_grid();
     int reserve(size_t count);
     int configure_ranges();
     int check_plain_limits() const;
+    int prepare(int form);          // form-specific state, no device work queued
+    int ensure_grid();
     int compute(int form, hipStream_t s);
     int compute_grid_form(int form, hipStream_t s);
+    StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
     ConvArgs conv_args() const;
     hipEvent_t take_event();
     void stage_begin(int stage, hipStream_t s);

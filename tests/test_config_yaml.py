@@ -1,0 +1,140 @@
+"""YAML sensor configuration drop-in (SensorInfoFactory.cpp:7-235,
+factories/utilities.cpp:22-257) through the C ABI, host only (no GPU).
+
+The reference example receiver configs under tests/golden/reference_configs
+are data fixtures copied from the reference's examples/ folder."""
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle as O
+import rtac_simulation_amd as R
+from rtac_simulation_amd import configs
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "reference_configs"
+
+
+@pytest.mark.parametrize("name", ["single_beam_38k", "fls_512x1024_sep", "fls_512x1024",
+                                  "fls_1024x4096"])
+def test_workload_yaml_matches_python_config(name):
+    y = R.config_from_yaml(R.CONFIG_DIR / f"{name}.yaml")
+    p, _, _, _ = configs.workload(name, samples=10)
+    assert (y.kind, y.range_count) == (p.kind, p.range_count)
+    assert np.float32(y.range_min) == np.float32(p.range_min)
+    assert np.float32(y.range_max) == np.float32(p.range_max)
+    assert y.psf.shape == p.psf.shape
+    assert np.allclose(y.psf, p.psf.astype(np.complex64), rtol=1e-6, atol=1e-7)
+    assert abs(y.psf_range_span / p.psf_range_span - 1) < 1e-6
+    if p.kind == 2:
+        assert abs(y.psf_bearing_span / p.psf_bearing_span - 1) < 1e-6
+        assert y.bearing_axis == p.bearing_axis
+        assert np.allclose(y.bearings, p.bearings, rtol=0, atol=2e-7)
+    assert O.overlap(y.margin, y.range_min, y.range_max, y.range_count) == \
+        O.overlap(p.margin, p.range_min, p.range_max, p.range_count)
+    # the reference disk builder leaves NaN texels on the 90 deg edge (denominator 0)
+    assert np.allclose(y.directivity, p.directivity, atol=1e-6, equal_nan=True)
+    assert np.float32(y.frequency) == np.float32(p.frequency)
+
+
+def test_reference_oculus_config():
+    cfg = R.config_from_yaml(GOLDEN / "oculus_M1200d_1_receiver.yaml")
+    assert cfg.kind == 2
+    # bearings: 256 values in degrees from the csv (utilities.cpp:114-153)
+    raw = open(GOLDEN / "oculus_M1200d_1_bearings.csv").read().strip().split(",")
+    assert raw[0] == "deg"
+    want = np.float32(np.pi / 180.0) * np.array([np.float32(v) for v in raw[1:]], np.float32)
+    assert np.array_equal(cfg.bearings, want.astype(np.float32))
+    assert (cfg.range_count, np.float32(cfg.range_min), np.float32(cfg.range_max)) == \
+        (500, np.float32(0.1), np.float32(6.5))
+    # rectangle antenna table size (Directivity.cpp:63-64) for 1.56 mm x 8 mm at 1.25 mm
+    Nw = 2 * int(np.float32(0.5 * 8 * 2 * np.pi * 0.00156 / 0.00125) + 1)
+    Nh = 2 * int(np.float32(0.5 * 8 * 2 * np.pi * 0.008 / 0.00125) + 1)
+    assert cfg.directivity.shape == (Nh, Nw)
+    # adaptive sine waveform: tau = resolution / c, size = (2 f os) tau (Waveform.h:127-130)
+    res = (np.float32(6.5) - np.float32(0.1)) / np.float32(499)
+    tau = np.float32(res / np.float32(1500))
+    Tr = int(np.float32(2 * 1.2e6 * 8) * tau)
+    assert cfg.psf.shape[0] == Tr
+    assert abs(cfg.psf_range_span - float(tau * np.float32(1500))) < 1e-7
+    # waveform taps exp(i pi n / 8) (Waveform.h:131-134), beam taps real: column 0
+    # of the table is waveform x beam[0]
+    col = cfg.psf[:, cfg.psf.shape[1] // 2]
+    wave = np.exp(1j * np.pi * np.arange(Tr) / 8)
+    assert np.allclose(col / col[0], wave, atol=1e-5)
+    assert abs(cfg.psf_bearing_span - np.deg2rad(130.0)) < 1e-6
+
+
+def test_reference_single_beam_config():
+    cfg = R.config_from_yaml(GOLDEN / "EA400_38KHz_receiver.yaml")
+    assert cfg.kind == 1 and cfg.range_count == 1049
+    # fixed duration 128 us: size = (unsigned)(2 * 38e3 * 8 * 128e-6)
+    assert cfg.psf.shape == (int(np.float32(2 * 38e3 * 8) * np.float32(128e-6)), 1)
+    # binOverlap = c tau / resolution = 0.192 / 0.0477 -> 4
+    assert O.overlap(cfg.margin if cfg.margin >= 0 else cfg.psf_range_span, cfg.range_min,
+                     cfg.range_max, cfg.range_count) == 4
+    lam, D = 0.0395, 0.15
+    assert cfg.directivity.shape[0] == max(64, 8 * int(8 * np.pi / np.float32(2 * lam / D)))
+
+
+@pytest.mark.parametrize("fname", ["EA400_200KHz_receiver.yaml", "oculus_M750d_1_receiver.yaml"])
+def test_other_reference_configs_parse(fname):
+    cfg = R.config_from_yaml(GOLDEN / fname)
+    assert cfg.psf.size > 0 and cfg.range_count > 1
+
+
+def _err(path):
+    info = C.c_void_p()
+    rc = R.lib().rtsim_info_from_yaml(str(path).encode(), C.byref(info))
+    if rc == 0:
+        R.lib().rtsim_info_destroy(info)
+    return rc, R.lib().rtsim_last_error().decode()
+
+
+@pytest.mark.parametrize("text,code,msg", [
+    ("name: x\n", R.ERR_CONFIG, "no type"),
+    ("type: sidescan\n", R.ERR_CONFIG, "unsupported sensor type"),
+    ("type: front-scan\n", R.ERR_CONFIG, "sampling"),
+    ("type: single-beam\noutput-node: real\n", R.ERR_CONFIG, "real sensor output"),
+    ("type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: 5\n    unit: km\n",
+     R.ERR_CONFIG, "unit"),
+    ("type: single-beam\nsampling:\n  ranges:\n    size: 10\n", R.ERR_CONFIG, "range-max"),
+    ("type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: 5\n"
+     "waveform:\n  type: square\n", R.ERR_CONFIG, "waveform type"),
+    ("type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: 5\n"
+     "waveform:\n  type: hann-baseband\n  taps: 5\ndirectivity:\n  type: cardioid\n",
+     R.ERR_CONFIG, "directivity"),
+    ("type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: 5\n"
+     "waveform:\n  type: hann-baseband\n  taps: 5\ndirectivity:\n  type: disk\n  diameter: 1\n",
+     R.ERR_CONFIG, "wavelength"),
+    ("type: front-scan\nsampling:\n  bearings:\n    type: csv\n    path: nowhere.csv\n"
+     "  ranges:\n    size: 10\n    range-max: 5\n", R.ERR_CONFIG, "not found"),
+    ("a: *missing\n", R.ERR_CONFIG, "alias"),
+])
+def test_config_errors(tmp_path, text, code, msg):
+    p = tmp_path / "bad.yaml"
+    p.write_text(text)
+    rc, err = _err(p)
+    assert rc == code and msg in err, err
+
+
+def test_missing_file_is_io_error():
+    rc, err = _err("/nonexistent/sensor.yaml")
+    assert rc == R.ERR_IO
+
+
+def test_yaml_reader_features(tmp_path):
+    """anchors / aliases, flow sequences, comments, quoted scalars, 'file' type."""
+    (tmp_path / "sub").mkdir()
+    (tmp_path / "sub" / "inner.yaml").write_text(
+        "# comment\n---\ntype: single-beam   # trailing comment\n"
+        "g: &g\n  range-min: 2.0\n  tags: [a, 'b c', 3]\n"
+        "sampling:\n  ranges:\n    globals: *g\n    size: 64\n    range-max: \"12.5\"\n"
+        "waveform:\n  type: hann-baseband\n  taps: 7\n"
+        "directivity:\n  type: uniform\n")
+    (tmp_path / "outer.yaml").write_text("type: file\npath: inner.yaml\n")
+    assert R.lib().rtsim_config_add_search_path(str(tmp_path / "sub").encode()) == 0
+    cfg = R.config_from_yaml(tmp_path / "outer.yaml")
+    assert (cfg.range_count, cfg.range_min, cfg.range_max) == (64, 2.0, 12.5)
+    assert cfg.psf.shape == (7, 1)
