@@ -74,9 +74,28 @@ def build_oracle(verbose: bool = True) -> Path:
     return odir / "liboracle.so"
 
 
+EXAMPLE_SRC = ROOT / "tests" / "cpp" / "dropin_example.cpp"
+EXAMPLE_BIN = BUILD / "dropin_example"
+
+
+def build_example(verbose: bool = True) -> Path:
+    """Host-only C++ program using the drop-in header, linked to librtsim.so."""
+    BUILD.mkdir(exist_ok=True)
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(EXAMPLE_SRC),
+           "-o", str(EXAMPLE_BIN), f"-L{PKG}", "-lrtsim", f"-Wl,-rpath,{PKG}",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,--disable-new-dtags"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"example build failed:\n{res.stdout}\n{res.stderr}")
+    if verbose:
+        print(f"built {EXAMPLE_BIN}", flush=True)
+    return EXAMPLE_BIN
+
+
 def build_all(verbose: bool = True) -> None:
     build_oracle(verbose)
     build_library(verbose=verbose)
+    build_example(verbose)
 
 
 def clean() -> None:
