@@ -276,6 +276,7 @@ int Sensor::check_plain_limits() const
 int Sensor::compute(int form, hipStream_t s)
 {
     const uint32_t n = (uint32_t)count;
+    (void)hipGetLastError();   // start from a clean error state: launches are checked below
     // ---- sort (Receiver.cu:28-34) -------------------------------------
     stage_begin(1, s);
     SortBuffers sb;
@@ -543,6 +544,7 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
                     offset + n, s->count);
     hipStream_t st = (hipStream_t)stream;
     StreamArgs a = s->stream_args(*hits, n, offset);
+    (void)hipGetLastError();
     s->stage_begin(0, st);
     HIP_TRY(launch_stream(a, st));
     s->stage_end(st);

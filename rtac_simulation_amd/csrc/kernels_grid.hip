@@ -192,7 +192,7 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
     }
     int e;
     frexpf(mx, &e);                          // mx in [2^(e-1), 2^e)
-    const float scale = ldexpf(1.0f, 14 - e); // mx * scale in [2^13, 2^14)
+    const float scale = ldexpf(1.0f, min(14 - e, 126)); // mx * scale in [2^13, 2^14)
 
     // ---- stage A (scaled, hi/lo split) ---------------------------------------
     for (int k = t; k < AR * AC; k += 256) {
@@ -309,12 +309,9 @@ hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s)
 
 void init_grid_attributes(int max_lds)
 {
-    (void)hipFuncSetAttribute((const void*)k_correlate<true>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-    (void)hipFuncSetAttribute((const void*)k_correlate<false>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
-    (void)hipFuncSetAttribute((const void*)k_deposit,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_deposit, max_lds);
 }
 
 }  // namespace rtsim

@@ -111,8 +111,7 @@ hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
 
 void init_plain_attributes(int max_lds)
 {
-    (void)hipFuncSetAttribute((const void*)k_conv2d_plain,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain, max_lds);
 }
 
 // 1D: one wave per output range bin, FP32 partial sums reduced across the

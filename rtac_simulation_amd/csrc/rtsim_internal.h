@@ -88,6 +88,18 @@ struct ConvArgs {
 size_t plain2d_lds_bytes(const ConvArgs& a);
 void init_plain_attributes(int max_lds);
 void init_grid_attributes(int max_lds);
+
+// Raises a kernel's dynamic LDS limit to what the device allows next to its
+// static LDS; clears the error state so a refusal cannot leak into a later
+// hipGetLastError().
+inline void set_dynamic_lds_limit(const void* fn, int max_lds)
+{
+    hipFuncAttributes attr{};
+    int limit = max_lds;
+    if (hipFuncGetAttributes(&attr, fn) == hipSuccess) limit -= (int)attr.sharedSizeBytes;
+    if (limit > 0) (void)hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, limit);
+    (void)hipGetLastError();
+}
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s);
 hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
 
