@@ -137,6 +137,20 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
  * sort -> Binner::compute -> PSF convolution, in the chosen form.                 */
 int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
 
+/* Tile-shape autotuner of the MFMA forms (no reference counterpart: the
+ * reference has one fixed launch shape, SensorInstance2D.cu:93).  Runs one
+ * compute, times every correlation tile shape that fits over `iterations`
+ * launches with HIP events on `stream`, keeps the fastest for this sensor and
+ * caches it for every sensor of the same (ranges, bearings, taps) class.
+ * Synchronises the stream; shape / ms may be NULL.                             */
+int rtsim_sensor_autotune(rtsim_sensor h, int form, int iterations, void* stream, int* shape,
+                          float* ms);
+/* Sets / reads the MFMA correlation tile shape (0: 64x32, 1: 32x32, 2: 32x64,
+ * 3: 64x16 range rows x bearings).  A shape whose LDS does not fit is
+ * RTSIM_ERR_UNSUPPORTED.  No reference counterpart (see autotune).            */
+int rtsim_sensor_set_tile_shape(rtsim_sensor h, int shape);
+int rtsim_sensor_tile_shape(rtsim_sensor h, int* shape, int* rows, int* cols);
+
 /* get_ping()/output() (SensorInstance2D.h:144-150, SensorInstance1D.h:103)         */
 int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height,
                         uint32_t* width);

@@ -94,6 +94,11 @@ _SIGNATURES = [
     ("rtsim_receiver_write_samples", C.c_int,
      [C.c_void_p, C.POINTER(Hits), C.c_size_t, C.c_size_t, C.c_void_p]),
     ("rtsim_sensor_compute", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtsim_sensor_autotune", C.c_int,
+     [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
+    ("rtsim_sensor_set_tile_shape", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_sensor_tile_shape", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("rtsim_sensor_output", C.c_int,
      [C.c_void_p, C.POINTER(C.c_void_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("rtsim_sensor_sorted_samples", C.c_int, [C.c_void_p, C.POINTER(C.c_void_p)]),
@@ -364,6 +369,22 @@ class SensorInstance:
     # -- consumer side -------------------------------------------------------------
     def compute_output(self, form: int = FORM_PLAIN_FP32, stream: Stream | None = None):
         check(lib().rtsim_sensor_compute(self.handle, form, stream.handle if stream else None))
+
+    def autotune(self, form: int, iterations: int = 10, stream: Stream | None = None):
+        """Pick the fastest MFMA correlation tile shape; returns (shape, ms)."""
+        sh, ms = C.c_int(), C.c_float()
+        check(lib().rtsim_sensor_autotune(self.handle, form, iterations,
+                                          stream.handle if stream else None, C.byref(sh),
+                                          C.byref(ms)))
+        return sh.value, ms.value
+
+    def set_tile_shape(self, shape: int):
+        check(lib().rtsim_sensor_set_tile_shape(self.handle, shape))
+
+    def tile_shape(self):
+        sh, r, c = C.c_int(), C.c_int(), C.c_int()
+        check(lib().rtsim_sensor_tile_shape(self.handle, C.byref(sh), C.byref(r), C.byref(c)))
+        return sh.value, r.value, c.value
 
     def shape(self):
         h, w = C.c_uint32(), C.c_uint32()

@@ -291,6 +291,7 @@ int Sensor::compute(int form, hipStream_t s)
     const uint32_t* img = nullptr;
     HIP_TRY(launch_sort(d_samples, n, sb, &order, &img, s));
     order_ptr = order;
+    img_ptr = img;
     stage_end(s);
 
     // ---- Binner::compute (Binner.h:96-102) ------------------------------

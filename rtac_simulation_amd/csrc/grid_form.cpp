@@ -5,7 +5,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/rtsim.h"
@@ -48,15 +51,51 @@ float host_f16_f32(uint16_t u)
     return (float)h;
 }
 
+// LDS row stride (elements) of the A planes for tile shape `sh`
+int shape_stride(int sh, int KB)
+{
+    int rows, cols;
+    corr_tile(sh, &rows, &cols);
+    return cols - 16 + 32 * KB + 8;
+}
+
+// Tuned tile shapes, cached per shape class for the whole process (a
+// sensor of the same grid and PSF size starts from the tuned shape).
+struct TuneKey {
+    int H, W, Tr, Tb;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(H, W, Tr, Tb) < std::tie(o.H, o.W, o.Tr, o.Tb);
+    }
+};
+std::mutex g_tune_mutex;
+std::map<TuneKey, int> g_tuned;
+
+int tuned_shape(const TuneKey& k)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    auto it = g_tuned.find(k);
+    return it == g_tuned.end() ? -1 : it->second;
+}
+
+void store_tuned(const TuneKey& k, int shape)
+{
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
+    g_tuned[k] = shape;
+}
+
 struct GridState {
     bool ready = false;
     int status = RTSIM_OK;
     std::string why;
-    int Hp = 0, Wp = 0, GH = 0, GW = 0, KB = 0, fr = 0, fb = 0, AS = 0, BL = 0;
+    int Hp = 0, Wp = 0, GH = 0, GW = 0, KB = 0, fr = 0, fb = 0, BL = 0;
+    int shape = 1;                       // correlation tile shape in use
+    bool fits[kCorrShapes] = {false, false, false, false};
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
     float bear0_hi = 0, bear0_lo = 0;
     float tap_scale = 1.f;
     float2* d_grid = nullptr;
+    float* d_colmax = nullptr;
     uint32_t* d_row_start = nullptr;
     uint16_t* d_taps[2] = {nullptr, nullptr};   // [0] bf16, [1] fp16
     size_t lds = 0;
@@ -64,6 +103,7 @@ struct GridState {
     ~GridState()
     {
         if (d_grid) (void)hipFree(d_grid);
+        if (d_colmax) (void)hipFree(d_colmax);
         if (d_row_start) (void)hipFree(d_row_start);
         for (auto p : d_taps)
             if (p) (void)hipFree(p);
@@ -116,8 +156,7 @@ static int prepare_grid(Sensor* s, GridState* g)
     g->Wp = (int)((s->W + kCorrCols - 1) / kCorrCols) * kCorrCols;
     g->KB = ((int)s->Tb + 15 + 31) / 32;
     g->GH = g->Hp + (int)s->Tr - 1;
-    g->GW = g->Wp - 16 + 32 * g->KB;
-    g->AS = 16 + 32 * g->KB + 8;
+    g->GW = g->Wp - 16 + 32 * g->KB;   // Wp rounded to the widest tile: every shape fits
     g->BL = 32 * g->KB + 16;
     g->fr = ((int)s->Tr - 1) / 2;
     g->fb = ((int)s->Tb - 1) / 2;
@@ -128,19 +167,30 @@ static int prepare_grid(Sensor* s, GridState* g)
     g->bear0_hi = (float)x[0];
     g->bear0_lo = (float)(x[0] - (double)g->bear0_hi);
 
-    CorrArgs probe{};
-    probe.Tr = (int)s->Tr;
-    probe.AS = g->AS;
-    probe.BL = g->BL;
-    g->lds = correlate_lds_bytes(probe);
     int dev_id = 0, max_lds = 0;
     (void)hipGetDevice(&dev_id);
     if (hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev_id) !=
             hipSuccess || max_lds <= 0)
         max_lds = 64 * 1024;
-    if (g->lds > (size_t)max_lds)
-        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %zu B of LDS per tile exceed %d B",
-                    g->lds, max_lds);
+    max_lds -= 64;   // the kernel's static LDS
+    // tile shapes whose LDS fits; default: the 32x32 tile when it fits
+    g->shape = -1;
+    for (int sh = 0; sh < kCorrShapes; sh++) {
+        CorrArgs probe{};
+        probe.Tr = (int)s->Tr;
+        probe.Tb = (int)s->Tb;
+        probe.shape = sh;
+        probe.AS = shape_stride(sh, g->KB);
+        probe.BL = g->BL;
+        g->fits[sh] = correlate_lds_bytes(probe) <= (size_t)max_lds;
+        if (g->fits[sh] && (g->shape < 0 || sh == 1)) g->shape = sh;
+    }
+    if (g->shape < 0)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: no tile shape fits %d B of LDS for %ux%u taps",
+                    max_lds, s->Tr, s->Tb);
+    const TuneKey key{(int)s->N, (int)s->W, (int)s->Tr, (int)s->Tb};
+    int cached = tuned_shape(key);
+    if (cached >= 0 && g->fits[cached]) g->shape = cached;
     const size_t dep_lds = sizeof(float) * 2 * kDepositRows * g->GW;
     if (dep_lds > (size_t)max_lds)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
@@ -172,6 +222,9 @@ static int prepare_grid(Sensor* s, GridState* g)
         return hip_fail(err, "hipMalloc(grid)");
     if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
         return hip_fail(err, "hipMalloc(row_start)");
+    if ((err = hipMalloc((void**)&g->d_colmax, sizeof(float) * (size_t)g->GH * (g->GW / 16))) !=
+        hipSuccess)
+        return hip_fail(err, "hipMalloc(colmax)");
     for (int d = 0; d < 2; d++) {
         if ((err = hipMalloc((void**)&g->d_taps[d], sizeof(uint16_t) * 4 * T)) != hipSuccess)
             return hip_fail(err, "hipMalloc(taps)");
@@ -203,12 +256,14 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     GridState* g = static_cast<GridState*>(grid_state);
 
     DepositArgs d;
+    d.img = img_ptr;
     d.sorted = d_sorted;
     d.coord = d_coord;
     d.skeys = d_skeys;
     d.n = (uint32_t)count;
     d.row_start = g->d_row_start;
     d.grid = g->d_grid;
+    d.colmax = g->d_colmax;
     d.GH = g->GH;
     d.GW = g->GW;
     d.fr = g->fr;
@@ -227,6 +282,8 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
 
     CorrArgs c;
     c.grid = g->d_grid;
+    c.colmax = g->d_colmax;
+    c.CG = g->GW / 16;
     c.taps = g->d_taps[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
     c.win = d_win;
     c.out = d_out;
@@ -237,14 +294,104 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     c.GH = g->GH;
     c.GW = g->GW;
     c.KB = g->KB;
-    c.AS = g->AS;
+    c.shape = g->shape;
+    c.AS = shape_stride(g->shape, g->KB);
     c.BL = g->BL;
     c.tap_scale = g->tap_scale;
     stage_begin(4, s);
     err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);
     stage_end(s);
     if (err != hipSuccess) return hip_fail(err, "correlate");
+    last_corr = c;
     return RTSIM_OK;
 }
 
+// Times every tile shape that fits (correlation kernel only, HIP events on
+// the stream, after one full compute so G is current) and keeps the fastest;
+// the result is cached for the shape class (H, W, Tr, Tb).
+int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms)
+{
+    if (form != RTSIM_FORM_MFMA_BF16 && form != RTSIM_FORM_MFMA_FP16)
+        return fail(RTSIM_ERR_INVALID, "autotune applies to the MFMA forms");
+    if (iterations < 1) iterations = 1;
+    int rc = compute(form, s);
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(grid_state);
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    int best = -1;
+    float bms = 0.f;
+    for (int sh = 0; sh < kCorrShapes; sh++) {
+        if (!g->fits[sh]) continue;
+        CorrArgs c = last_corr;
+        c.shape = sh;
+        c.AS = shape_stride(sh, g->KB);
+        hipError_t err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);   // warm
+        (void)hipEventRecord(e0, s);
+        for (int it = 0; it < iterations && err == hipSuccess; it++)
+            err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);
+        (void)hipEventRecord(e1, s);
+        float ms = 0.f;
+        if (err == hipSuccess) err = hipEventSynchronize(e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, e0, e1);
+        if (err != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            return hip_fail(err, "autotune");
+        }
+        ms /= iterations;
+        if (best < 0 || ms < bms) { best = sh; bms = ms; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g->shape = best;
+    store_tuned(TuneKey{(int)N, (int)W, (int)Tr, (int)Tb}, best);
+    generation++;   // captured graphs hold the old shape
+    if (best_shape) *best_shape = best;
+    if (best_ms) *best_ms = bms;
+    // leave the ping consistent with the chosen shape
+    return compute(form, s);
+}
+
 }  // namespace rtsim
+
+using namespace rtsim;
+
+extern "C" int rtsim_sensor_autotune(rtsim_sensor h, int form, int iterations, void* stream,
+                                     int* shape, float* ms)
+{
+    Sensor* s = reinterpret_cast<Sensor*>(h);
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+    return s->autotune(form, iterations, (hipStream_t)stream, shape, ms);
+}
+
+extern "C" int rtsim_sensor_set_tile_shape(rtsim_sensor h, int shape)
+{
+    Sensor* s = reinterpret_cast<Sensor*>(h);
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+    if (shape < 0 || shape >= kCorrShapes) return fail(RTSIM_ERR_INVALID, "unknown tile shape %d", shape);
+    int rc = s->ensure_grid();
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(s->grid_state);
+    if (!g->fits[shape])
+        return fail(RTSIM_ERR_UNSUPPORTED, "tile shape %d does not fit this sensor's LDS budget", shape);
+    g->shape = shape;
+    s->generation++;
+    return RTSIM_OK;
+}
+
+extern "C" int rtsim_sensor_tile_shape(rtsim_sensor h, int* shape, int* rows, int* cols)
+{
+    Sensor* s = reinterpret_cast<Sensor*>(h);
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+    int rc = s->ensure_grid();
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(s->grid_state);
+    if (shape) *shape = g->shape;
+    int r = 0, c = 0;
+    corr_tile(g->shape, &r, &c);
+    if (rows) *rows = r;
+    if (cols) *cols = c;
+    return RTSIM_OK;
+}

@@ -7,20 +7,27 @@
 
 namespace rtsim {
 
-constexpr int kDepositRows = 4;   // G rows per deposit workgroup
-constexpr int kCorrRows = 64;     // output rows per correlation workgroup (4 waves x 16)
-constexpr int kCorrCols = 32;     // output bearings per workgroup (2 MFMA N tiles)
+constexpr int kDepositRows = 1;   // G rows per deposit workgroup (many short blocks
+                                  // keep enough waves in flight to hide load latency)
+constexpr int kCorrRows = 64;     // largest tile height of the correlation shapes
+constexpr int kCorrCols = 64;     // largest tile width of the correlation shapes
+constexpr int kCorrShapes = 4;    // {64x32, 32x32, 32x64, 64x16} (rows x bearings)
 constexpr int kTapChunk = 16;     // range taps staged in LDS at a time
+
+// rows x bearings of correlation tile shape `shape`
+void corr_tile(int shape, int* rows, int* cols);
 
 // Extended grid G: rho = grid row + fr, kappa = grid column + fb, so that
 // out(r, b) = sum T[i][j] G[r + i][b + j].
 struct DepositArgs {
+    const uint32_t* img;       // sorted travel images (4 B per sample)
     const float4* sorted;
     const float2* coord;       // bearing-axis coordinate (hi, lo) per sorted sample
     const uint32_t* skeys;     // OutOfRange samples are skipped
     uint32_t n;
     uint32_t* row_start;       // GH + 3 entries
     float2* grid;              // [GH][GW]
+    float* colmax;             // [GH][GW / 16]: max |re|, |im| per 16 columns
     int GH, GW;
     int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)
     float range0, inv_dr, off_r;        // y = (t - range0) / dr + off_r
@@ -29,12 +36,15 @@ struct DepositArgs {
 
 struct CorrArgs {
     const float2* grid;        // [GH][GW]
+    const float* colmax;       // [GH][CG]
+    int CG;                    // GW / 16
     const uint16_t* taps;      // [4 planes][Tr][Tb]: re_hi, re_lo, im_hi, im_lo
     const uint2* win;          // empty window -> masked row
     float2* out;               // [H][W]
     int H, W, Tr, Tb;
     int GH, GW;
     int KB;                    // 32-wide K blocks per 16-wide N tile
+    int shape;                 // tile shape index (corr_tile)
     int AS;                    // LDS row stride of the A planes (elements)
     int BL;                    // LDS length of one padded tap row (elements)
     float tap_scale;           // power of two applied to the taps
@@ -43,5 +53,6 @@ struct CorrArgs {
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s);
 hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s);
 size_t correlate_lds_bytes(const CorrArgs& a);
+void init_corr_attributes(int max_lds);
 
 }  // namespace rtsim

@@ -129,6 +129,8 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // ---------------------------------------------------------------------------
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;
+constexpr int kSortItems = 8;                          // items per thread per tile
+constexpr int kSortTile = kSortThreads * kSortItems;   // 2048 items staged in LDS
 
 __global__ __launch_bounds__(256) void k_radix_init(const float4* __restrict__ rec, uint32_t n,
                                                     uint32_t* __restrict__ keys,
@@ -149,9 +151,18 @@ __global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__
     const uint32_t t = threadIdx.x, w = t >> 6;
     for (int k = 0; k < kSortWaves; k++) h[k][t] = 0;
     __syncthreads();
-    const uint32_t beg = blockIdx.x * stripe;
+    const uint32_t beg = blockIdx.x * stripe;     // multiple of 256
     const uint32_t end = min(beg + stripe, n);
-    for (uint32_t i = beg + t; i < end; i += kSortThreads)
+    const uint32_t end4 = beg + ((end - beg) & ~3u);
+    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
+    for (uint32_t i = beg + 4 * t; i < end4; i += 4 * kSortThreads) {
+        const uint4 v = k4[i >> 2];
+        atomicAdd(&h[w][(v.x >> shift) & 255u], 1u);
+        atomicAdd(&h[w][(v.y >> shift) & 255u], 1u);
+        atomicAdd(&h[w][(v.z >> shift) & 255u], 1u);
+        atomicAdd(&h[w][(v.w >> shift) & 255u], 1u);
+    }
+    for (uint32_t i = end4 + t; i < end; i += kSortThreads)
         atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
     __syncthreads();
     uint32_t c = 0;
@@ -222,37 +233,83 @@ __global__ __launch_bounds__(256) void k_radix_scatter(
     run[t] = base + incl - tot + hist[t * P + blockIdx.x];
     __syncthreads();
 
+    __shared__ uint32_t tcnt[256];     // digit counts of the current tile
+    __shared__ uint32_t toff[256];     // exclusive prefix of tcnt
+    __shared__ uint32_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+
     const uint32_t beg = blockIdx.x * stripe;
     const uint32_t end = min(beg + stripe, n);
-    for (uint32_t rb = beg; rb < end; rb += kSortThreads) {
-        const uint32_t i = rb + t;
-        const bool valid = i < end;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const uint32_t val = valid ? vin[i] : 0u;
-        const uint32_t d = (key >> shift) & 255u;
-        uint64_t peers = __ballot(valid);
+    for (uint32_t tb = beg; tb < end; tb += kSortTile) {
+        const uint32_t tn = min((uint32_t)kSortTile, end - tb);
+        // all items of the tile in registers: item j of thread t is tb + j*256 + t
+        uint32_t key[kSortItems], val[kSortItems], lrank[kSortItems];
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
-            const uint32_t bit = (d >> b) & 1u;
-            const uint64_t m = __ballot(bit);
-            peers &= bit ? m : ~m;
+        for (int j = 0; j < kSortItems; j++) {
+            const uint32_t q = j * kSortThreads + t;
+            key[j] = q < tn ? kin[tb + q] : 0u;
+            val[j] = q < tn ? vin[tb + q] : 0u;
         }
-        const uint32_t rank = popc_below(peers);
-        if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+        tcnt[t] = 0;
         __syncthreads();
-        if (valid) {
-            uint32_t pos = run[d] + rank;
-            for (uint32_t k = 0; k < w; k++) pos += wc[k][d];
-            kout[pos] = key;
-            vout[pos] = val;
+        // stable ranks within the tile, one round of 256 items at a time
+#pragma unroll
+        for (int j = 0; j < kSortItems; j++) {
+            const bool valid = j * kSortThreads + t < tn;
+            const uint32_t d = (key[j] >> shift) & 255u;
+            uint64_t peers = __ballot(valid);
+#pragma unroll
+            for (int b = 0; b < 8; b++) {
+                const uint32_t bit = (d >> b) & 1u;
+                const uint64_t m = __ballot(bit);
+                peers &= bit ? m : ~m;
+            }
+            const uint32_t rank = popc_below(peers);
+            if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            uint32_t r = tcnt[d] + rank;
+            for (uint32_t k = 0; k < w; k++) r += wc[k][d];
+            lrank[j] = r;
+            __syncthreads();
+            uint32_t add = 0;
+            for (int k = 0; k < kSortWaves; k++) {
+                add += wc[k][t];
+                wc[k][t] = 0;
+            }
+            tcnt[t] += add;
+            __syncthreads();
+        }
+        // local digit offsets, then stage the tile in LDS in digit order
+        {
+            const uint32_t c = tcnt[t];
+            const uint32_t inc = wave_incl_scan(c, lane);
+            if (lane == 63) wsum[w] = inc;
+            __syncthreads();
+            uint32_t base = 0;
+            for (uint32_t k = 0; k < w; k++) base += wsum[k];
+            toff[t] = base + inc - c;
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kSortItems; j++) {
+            if (j * kSortThreads + t < tn) {
+                const uint32_t d = (key[j] >> shift) & 255u;
+                const uint32_t p = toff[d] + lrank[j];
+                skey[p] = key[j];
+                sval[p] = val[j];
+            }
         }
         __syncthreads();
-        uint32_t add = 0;
-        for (int k = 0; k < kSortWaves; k++) {
-            add += wc[k][t];
-            wc[k][t] = 0;
+        // consecutive lanes write consecutive addresses inside each digit run
+        for (uint32_t q = t; q < tn; q += kSortThreads) {
+            const uint32_t k = skey[q];
+            const uint32_t d = (k >> shift) & 255u;
+            const uint32_t pos = run[d] + (q - toff[d]);
+            kout[pos] = k;
+            vout[pos] = sval[q];
         }
-        run[t] += add;
+        __syncthreads();
+        run[t] += tcnt[t];
         __syncthreads();
     }
 }
@@ -270,7 +327,7 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
     if (P > b.max_stripes) P = b.max_stripes;
     if (P < 1) P = 1;
     uint32_t stripe = (n + P - 1) / P;
-    stripe = (stripe + 255) / 256 * 256;
+    stripe = (stripe + kSortTile - 1) / kSortTile * kSortTile;   // whole tiles
     P = (n + stripe - 1) / stripe;
     int cur = 0;
     for (int pass = 0; pass < 4; pass++) {

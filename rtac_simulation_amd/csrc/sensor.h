@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/rtsim.h"
+#include "grid_internal.h"
 #include "rtsim_internal.h"
 
 namespace rtsim {
@@ -57,7 +58,8 @@ struct Sensor {
     float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
     uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_vA = nullptr, *d_vB = nullptr;
     uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
-    const uint32_t* order_ptr = nullptr;
+    const uint32_t* order_ptr = nullptr;  // sort permutation (after compute)
+    const uint32_t* img_ptr = nullptr;    // sorted travel images (after compute)
     uint32_t* d_hist = nullptr;
     uint32_t* d_totals = nullptr;
 
@@ -72,6 +74,7 @@ struct Sensor {
 
     // ---- grid (MFMA) form state --------------------------------------------
     void* grid_state = nullptr;
+    CorrArgs last_corr{};
 
     // ---- profiling -----------------------------------------------------------
     bool profiling = false;
@@ -93,6 +96,7 @@ struct Sensor {
     int ensure_grid();
     int compute(int form, hipStream_t s);
     int compute_grid_form(int form, hipStream_t s);
+    int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
     ConvArgs conv_args() const;
     hipEvent_t take_event();

@@ -111,3 +111,43 @@ def test_mfma_dynamic_range(gpu, form):
     s.compute_output(form)
     ref, _ = O.reference_ping(cfg, s.samples())
     assert O.rel_l2(s.output(), ref) <= MFMA_TOL
+
+
+@pytest.mark.parametrize("shape", [0, 1, 2, 3])
+@pytest.mark.parametrize("W,H,psf", [(48, 96, "31x17"), (37, 70, "8x6"), (40, 100, "21x33")])
+def test_mfma_every_tile_shape_vs_oracle(gpu, shape, W, H, psf):
+    """Every correlation tile shape the autotuner may pick restates the gather."""
+    R = gpu
+    cfg = configs._fls(W, H, 0.1, 30.0, 1.2e6, _psf(psf), f"t{W}x{H}")
+    hits = R.generate_hits(configs.fls_scene(), 23, 30_000)
+    s = _sensor(R, cfg, hits)
+    s.set_tile_shape(shape)
+    assert s.tile_shape()[0] == shape
+    ref = None
+    for form in FORMS:
+        s.compute_output(form)
+        got = s.output()
+        if ref is None:
+            ref, b = O.reference_ping(cfg, s.samples())
+        assert O.rel_l2(got, ref) <= MFMA_TOL, (form, shape)
+        empty = b["windows"][:, 1] == 0
+        assert not np.abs(got[empty]).any()
+
+
+def test_mfma_autotune_keeps_results(gpu):
+    R = gpu
+    cfg = configs._fls(96, 160, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "tune")
+    hits = R.generate_hits(configs.fls_scene(), 24, 60_000)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(2)
+    before = s.output()
+    shape, ms = s.autotune(2, iterations=3)
+    assert 0 <= shape < 4 and ms > 0
+    assert s.tile_shape()[0] == shape
+    s.compute_output(2)
+    after = s.output()
+    assert O.rel_l2(after, before) <= MFMA_TOL
+    # a second sensor of the same class starts on the cached shape
+    s2 = _sensor(R, cfg, hits)
+    s2.compute_output(2)
+    assert s2.tile_shape()[0] == shape
