@@ -91,21 +91,20 @@ struct GridState {
     int Hp = 0, Wp = 0, GH = 0, GW = 0, KB = 0, fr = 0, fb = 0, BL = 0;
     int shape = 1;                       // correlation tile shape in use
     bool fits[kCorrShapes] = {false, false, false, false};
+    int tch[kCorrShapes] = {0, 0, 0, 0};  // tap chunk per shape
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
     float bear0_hi = 0, bear0_lo = 0;
     float tap_scale = 1.f;
     float2* d_grid = nullptr;
-    float* d_colmax = nullptr;
     uint32_t* d_row_start = nullptr;
-    uint16_t* d_taps[2] = {nullptr, nullptr};   // [0] bf16, [1] fp16
+    uint16_t* d_toep[2] = {nullptr, nullptr};   // Toeplitz tap rows, [0] bf16, [1] fp16
     size_t lds = 0;
 
     ~GridState()
     {
         if (d_grid) (void)hipFree(d_grid);
-        if (d_colmax) (void)hipFree(d_colmax);
         if (d_row_start) (void)hipFree(d_row_start);
-        for (auto p : d_taps)
+        for (auto p : d_toep)
             if (p) (void)hipFree(p);
     }
 };
@@ -173,7 +172,8 @@ static int prepare_grid(Sensor* s, GridState* g)
             hipSuccess || max_lds <= 0)
         max_lds = 64 * 1024;
     max_lds -= 64;   // the kernel's static LDS
-    // tile shapes whose LDS fits; default: the 32x32 tile when it fits
+    // tile shapes whose LDS fits (with the tap chunk each can hold);
+    // default: the 32x32 tile when it fits
     g->shape = -1;
     for (int sh = 0; sh < kCorrShapes; sh++) {
         CorrArgs probe{};
@@ -182,7 +182,8 @@ static int prepare_grid(Sensor* s, GridState* g)
         probe.shape = sh;
         probe.AS = shape_stride(sh, g->KB);
         probe.BL = g->BL;
-        g->fits[sh] = correlate_lds_bytes(probe) <= (size_t)max_lds;
+        g->tch[sh] = correlate_chunk(probe, (size_t)max_lds);
+        g->fits[sh] = g->tch[sh] > 0;
         if (g->fits[sh] && (g->shape < 0 || sh == 1)) g->shape = sh;
     }
     if (g->shape < 0)
@@ -191,30 +192,40 @@ static int prepare_grid(Sensor* s, GridState* g)
     const TuneKey key{(int)s->N, (int)s->W, (int)s->Tr, (int)s->Tb};
     int cached = tuned_shape(key);
     if (cached >= 0 && g->fits[cached]) g->shape = cached;
-    const size_t dep_lds = sizeof(float) * 2 * kDepositRows * g->GW;
+    const size_t dep_lds = sizeof(long long) * 2 * g->GW;
     if (dep_lds > (size_t)max_lds)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
 
-    // taps: power-of-two scale so max |T| in [0.5, 1), then hi/lo planes
+    // taps: power-of-two scale so max |T| in [0.5, 1), split hi/lo, laid out
+    // as Toeplitz rows [shift c][plane p][tap i][x] = T_p[i][x + c - 15]
+    // (zero outside [0, Tb)), planes re_hi, re_lo, im_hi, im_lo
     const size_t T = (size_t)s->Tr * s->Tb;
     float mx = 0.f;
     for (size_t k = 0; k < 2 * T; k++) mx = std::fmax(mx, std::fabs(s->psf[k]));
     int e = 0;
     if (mx > 0.f) std::frexp(mx, &e);
     g->tap_scale = std::ldexp(1.0f, -e);
-    std::vector<uint16_t> planes[2];
-    for (int d = 0; d < 2; d++) planes[d].assign(4 * T, 0);
-    for (size_t k = 0; k < T; k++) {
-        for (int c = 0; c < 2; c++) {               // re, im
-            float v = s->psf[2 * k + c] * g->tap_scale;
-            uint16_t h = host_bf16(v);
-            uint16_t l = host_bf16(v - host_bf16_f32(h));
-            planes[0][(2 * c + 0) * T + k] = h;
-            planes[0][(2 * c + 1) * T + k] = l;
-            h = host_f16(v);
-            l = host_f16(v - host_f16_f32(h));
-            planes[1][(2 * c + 0) * T + k] = h;
-            planes[1][(2 * c + 1) * T + k] = l;
+    const size_t BL = (size_t)g->BL, Tr = s->Tr;
+    const size_t tsize = 2 * 4 * Tr * BL;
+    std::vector<uint16_t> toep[2];
+    for (int d = 0; d < 2; d++) toep[d].assign(tsize, 0);
+    for (size_t i = 0; i < Tr; i++) {
+        for (size_t j = 0; j < s->Tb; j++) {
+            for (int c = 0; c < 2; c++) {               // re, im
+                const float v = s->psf[2 * (i * s->Tb + j) + c] * g->tap_scale;
+                uint16_t planes[2][2];
+                planes[0][0] = host_bf16(v);
+                planes[0][1] = host_bf16(v - host_bf16_f32(planes[0][0]));
+                planes[1][0] = host_f16(v);
+                planes[1][1] = host_f16(v - host_f16_f32(planes[1][0]));
+                for (int d = 0; d < 2; d++)
+                    for (int hl = 0; hl < 2; hl++)
+                        for (size_t sh = 0; sh < 2; sh++) {
+                            const size_t x = j + 15 - sh;   // x + sh - 15 = j
+                            const size_t p = 2 * c + hl;
+                            toep[d][((sh * 4 + p) * Tr + i) * BL + x] = planes[d][hl];
+                        }
+            }
         }
     }
     hipError_t err;
@@ -222,15 +233,12 @@ static int prepare_grid(Sensor* s, GridState* g)
         return hip_fail(err, "hipMalloc(grid)");
     if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
         return hip_fail(err, "hipMalloc(row_start)");
-    if ((err = hipMalloc((void**)&g->d_colmax, sizeof(float) * (size_t)g->GH * (g->GW / 16))) !=
-        hipSuccess)
-        return hip_fail(err, "hipMalloc(colmax)");
     for (int d = 0; d < 2; d++) {
-        if ((err = hipMalloc((void**)&g->d_taps[d], sizeof(uint16_t) * 4 * T)) != hipSuccess)
-            return hip_fail(err, "hipMalloc(taps)");
-        if ((err = hipMemcpy(g->d_taps[d], planes[d].data(), sizeof(uint16_t) * 4 * T,
+        if ((err = hipMalloc((void**)&g->d_toep[d], sizeof(uint16_t) * tsize)) != hipSuccess)
+            return hip_fail(err, "hipMalloc(toeplitz)");
+        if ((err = hipMemcpy(g->d_toep[d], toep[d].data(), sizeof(uint16_t) * tsize,
                              hipMemcpyHostToDevice)) != hipSuccess)
-            return hip_fail(err, "hipMemcpy(taps)");
+            return hip_fail(err, "hipMemcpy(toeplitz)");
     }
     g->ready = true;
     return RTSIM_OK;
@@ -263,7 +271,6 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     d.n = (uint32_t)count;
     d.row_start = g->d_row_start;
     d.grid = g->d_grid;
-    d.colmax = g->d_colmax;
     d.GH = g->GH;
     d.GW = g->GW;
     d.fr = g->fr;
@@ -282,9 +289,7 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
 
     CorrArgs c;
     c.grid = g->d_grid;
-    c.colmax = g->d_colmax;
-    c.CG = g->GW / 16;
-    c.taps = g->d_taps[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
+    c.toep = g->d_toep[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
     c.win = d_win;
     c.out = d_out;
     c.H = (int)N;
@@ -297,6 +302,7 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     c.shape = g->shape;
     c.AS = shape_stride(g->shape, g->KB);
     c.BL = g->BL;
+    c.TCH = g->tch[g->shape];
     c.tap_scale = g->tap_scale;
     stage_begin(4, s);
     err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);
@@ -327,6 +333,7 @@ int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, f
         CorrArgs c = last_corr;
         c.shape = sh;
         c.AS = shape_stride(sh, g->KB);
+        c.TCH = g->tch[sh];
         hipError_t err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);   // warm
         (void)hipEventRecord(e0, s);
         for (int it = 0; it < iterations && err == hipSuccess; it++)

@@ -3,16 +3,15 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstddef>
 #include <cstdint>
 
 namespace rtsim {
 
-constexpr int kDepositRows = 1;   // G rows per deposit workgroup (many short blocks
-                                  // keep enough waves in flight to hide load latency)
 constexpr int kCorrRows = 64;     // largest tile height of the correlation shapes
 constexpr int kCorrCols = 64;     // largest tile width of the correlation shapes
 constexpr int kCorrShapes = 4;    // {64x32, 32x32, 32x64, 64x16} (rows x bearings)
-constexpr int kTapChunk = 16;     // range taps staged in LDS at a time
+constexpr size_t kCorrLdsTarget = 80 * 1024;   // two workgroups per CU
 
 // rows x bearings of correlation tile shape `shape`
 void corr_tile(int shape, int* rows, int* cols);
@@ -27,7 +26,6 @@ struct DepositArgs {
     uint32_t n;
     uint32_t* row_start;       // GH + 3 entries
     float2* grid;              // [GH][GW]
-    float* colmax;             // [GH][GW / 16]: max |re|, |im| per 16 columns
     int GH, GW;
     int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)
     float range0, inv_dr, off_r;        // y = (t - range0) / dr + off_r
@@ -36,9 +34,7 @@ struct DepositArgs {
 
 struct CorrArgs {
     const float2* grid;        // [GH][GW]
-    const float* colmax;       // [GH][CG]
-    int CG;                    // GW / 16
-    const uint16_t* taps;      // [4 planes][Tr][Tb]: re_hi, re_lo, im_hi, im_lo
+    const uint16_t* toep;      // Toeplitz tap rows [2 shifts][4 planes][Tr][BL]
     const uint2* win;          // empty window -> masked row
     float2* out;               // [H][W]
     int H, W, Tr, Tb;
@@ -46,13 +42,18 @@ struct CorrArgs {
     int KB;                    // 32-wide K blocks per 16-wide N tile
     int shape;                 // tile shape index (corr_tile)
     int AS;                    // LDS row stride of the A planes (elements)
-    int BL;                    // LDS length of one padded tap row (elements)
+    int BL;                    // length of one padded tap row (elements)
+    int TCH;                   // range taps whose Toeplitz rows are resident in LDS at once
     float tap_scale;           // power of two applied to the taps
 };
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s);
 hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s);
 size_t correlate_lds_bytes(const CorrArgs& a);
+// Tap chunk for a.shape: all Tr taps when that fits kCorrLdsTarget, else
+// the largest chunk that does, else the largest that fits max_lds; 0 when
+// even one tap row does not fit.
+int correlate_chunk(const CorrArgs& a, size_t max_lds);
 void init_corr_attributes(int max_lds);
 
 }  // namespace rtsim

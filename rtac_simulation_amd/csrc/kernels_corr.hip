@@ -9,24 +9,27 @@
 //   Complex products use the real embedding (re = ArBr - AiBi,
 //   im = ArBi + AiBr); every real product is split hi+lo,
 //   X Y ~ XhYh + XlYh + XhYl, with FP32 accumulation.
-// Scaling: G is scaled per workgroup tile by a power of two (max |G| in
-// [2^13, 2^14), from the per-row / per-16-column maxima the deposit writes)
-// and the taps by a host power of two, so FP16 operands stay in range; the
-// epilogue divides both out exactly.  Rows whose reference window is empty
+// Scaling: each workgroup scales its G tile by a power of two (max |G| in
+// [2^13, 2^14)), found from the staged values themselves, and the taps
+// carry a host power of two, so FP16 operands stay in range; the epilogue
+// divides both out exactly.  Rows whose reference window is empty
 // (Binner.cu:96-97) are written as zero.
 //
 // Tile shapes (template WR x WC waves, NT 16-wide N tiles per wave): a
-// workgroup computes 16 WR output rows x 16 NT WC bearings with 4 waves.
-// Smaller tiles put more workgroups on a CU so one workgroup's staging
-// overlaps another's MFMA work; the shape is chosen per sensor by timing
-// (Sensor::tune_grid).  LDS (16-bit elements):
+// workgroup computes 16 WR output rows x 16 NT WC bearings with 4 waves;
+// the shape is chosen per sensor by timing (Sensor::autotune).
+// LDS (16-bit elements):
 //   A: 4 planes (re_hi, re_lo, im_hi, im_lo) x AR rows x AS stride
-//   B: 2 copies (shift 0 / 1) x 4 planes x kTapChunk taps x BL
-//   TC: the compact split taps [4][Tr][Tb]
-// Per K step a wave reads its A fragments (ds_read_b128) and the shared
-// Toeplitz B fragment (4-byte reads of a pre-shifted copy) with pointer
-// increments only; fragments are double-buffered so the next step's reads
-// overlap the current step's MFMAs.
+//   B: the Toeplitz rows of TCH range taps, copied from the prebuilt table
+//      [2 shifts][4 planes][Tr][BL] (shift c holds Tpad[x + c],
+//      Tpad[x] = T[x - 15]); the two shifts keep every lane's 8-element B
+//      fragment 4-byte aligned.
+// Staging: every thread issues all its grid loads (float4 = two complex
+// cells) and the B copy at once, so a workgroup pays one memory round trip
+// before its MFMA loop.  Per K step a wave reads its A fragments
+// (ds_read_b128) and the shared B fragment with pointer increments only;
+// fragments are double-buffered so the next step's reads overlap the
+// current step's MFMAs.
 #include "device_common.h"
 #include "grid_internal.h"
 
@@ -36,6 +39,8 @@ typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kRegCells = 16;   // float4 grid loads per thread held in registers
 
 __device__ __forceinline__ uint16_t bf16_rne(float x)
 {
@@ -112,6 +117,29 @@ __device__ __forceinline__ void load_frags(Frags<NT>& f, const uint16_t* aptr, i
             f.a[n][p] = *reinterpret_cast<const u32x4*>(aptr + p * plane_stride + 16 * n);
 }
 
+// q = floor(k / d) for 0 <= k < 2^20, d >= 1: (k + 0.5) / d lies at least
+// 0.5 / d from an integer, and the float product's error is below
+// (k + 0.5) / d * 2^-23 < 0.5 / d.
+__device__ __forceinline__ int small_div(int k, float inv_d)
+{
+    return (int)(((float)k + 0.5f) * inv_d);
+}
+
+// Copies the Toeplitz rows of range taps [i0, i0 + tc) into LDS B: for each
+// (shift, plane) run a contiguous tc * BL elements, moved as uint4.
+__device__ __forceinline__ void copy_toeplitz(const CorrArgs& a, uint16_t* B, int i0, int tc, int t)
+{
+    const int n16 = tc * a.BL / 8;                 // uint4 per (shift, plane) run
+    const float inv_n16 = 1.0f / (float)n16;
+    const uint4* src = reinterpret_cast<const uint4*>(a.toep);
+    uint4* dst = reinterpret_cast<uint4*>(B);
+    const int run_src = a.Tr * a.BL / 8, run_dst = a.TCH * a.BL / 8, off = i0 * a.BL / 8;
+    for (int k = t; k < 8 * n16; k += 256) {
+        const int cp = small_div(k, inv_n16), o = k - cp * n16;
+        dst[cp * run_dst + o] = src[cp * run_src + off + o];
+    }
+}
+
 template <bool FP16, int WR, int NT>
 __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
 {
@@ -120,24 +148,57 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
     extern __shared__ uint16_t lds[];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave % WR, wc = wave / WR;
+    const int m = lane & 15, kg = lane >> 4;
     const int r0 = blockIdx.y * ROWS;
     const int c0 = blockIdx.x * COLS;
     const int AR = ROWS + a.Tr - 1;
     const int AC = COLS - 16 + 32 * a.KB;   // columns [c0, c0 + COLS - 16 + 32 KB)
+    const int PR = AC / 2;                  // float4 loads per A row
+    const int total = AR * PR;
+    const float inv_pr = 1.0f / (float)PR;
     const int AS = a.AS;
-    const int BL = a.BL;
+    const int bps = a.TCH * a.BL;           // B plane stride (elements)
     uint16_t* A = lds;
     uint16_t* B = lds + 4 * AR * AS;
-    uint16_t* TC = B + 2 * 4 * kTapChunk * BL;
     __shared__ float red[4];
 
-    // ---- tile scale from the deposit's per-row / per-16-column maxima --------
+    // window sizes of this lane's output rows, for the epilogue mask
+    uint32_t wsz[4];
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+        const int rr = r0 + 16 * wr + 4 * kg + e;
+        wsz[e] = a.win[min(rr, a.H - 1)].y;
+    }
+
+    // ---- grid tile loads: all in flight together ------------------------------
+    const float4* gtile = reinterpret_cast<const float4*>(a.grid + (size_t)r0 * a.GW + c0);
+    const int grow4 = a.GW / 2;             // float4 per grid row
+    const bool in_regs = total <= kRegCells * 256;
+    float4 v[kRegCells];
     float mx = 0.f;
-    {
-        const int g0 = c0 / 16, ng = AC / 16;
-        for (int k = t; k < AR * ng; k += 256) {
-            const int rr = k / ng, g = k - rr * ng;
-            mx = fmaxf(mx, a.colmax[(size_t)(r0 + rr) * a.CG + g0 + g]);
+    if (in_regs) {
+        const int nreg = (total + 255) / 256;
+#pragma unroll
+        for (int j = 0; j < kRegCells; j++) {
+            if (j < nreg) {
+                const int k = min(t + 256 * j, total - 1);
+                const int rr = small_div(k, inv_pr);
+                v[j] = gtile[rr * grow4 + (k - rr * PR)];
+            }
+        }
+        copy_toeplitz(a, B, 0, min(a.TCH, a.Tr), t);
+#pragma unroll
+        for (int j = 0; j < kRegCells; j++) {
+            if (j < nreg && t + 256 * j < total)
+                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[j].x), fabsf(v[j].y)),
+                                     fmaxf(fabsf(v[j].z), fabsf(v[j].w))));
+        }
+    } else {
+        copy_toeplitz(a, B, 0, min(a.TCH, a.Tr), t);
+        for (int k = t; k < total; k += 256) {
+            const int rr = small_div(k, inv_pr);
+            const float4 g = gtile[rr * grow4 + (k - rr * PR)];
+            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w))));
         }
     }
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
@@ -155,34 +216,30 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
     frexpf(mx, &e);                                       // mx in [2^(e-1), 2^e)
     const float scale = ldexpf(1.0f, min(14 - e, 126));   // mx * scale in [2^13, 2^14)
 
-    // ---- compact taps -> LDS (wide loads) --------------------------------------
+    // ---- A: scaled, hi/lo split, two complex cells per 32-bit LDS word pair ----
     {
-        const int nw = 2 * a.Tr * a.Tb;   // 32-bit words of [4][Tr][Tb] 16-bit taps
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(a.taps);
-        uint32_t* dst = reinterpret_cast<uint32_t*>(TC);
-        for (int k = t; k < nw; k += 256) dst[k] = src[k];
-    }
-
-    // ---- stage A: scaled, hi/lo split, two complex cells per lane per row ------
-    {
-        const int PR = AC / 2;
-#pragma unroll 4
-        for (int rr = wave; rr < AR; rr += 4) {
-            const float4* src = reinterpret_cast<const float4*>(
-                a.grid + (size_t)(r0 + rr) * a.GW + c0);
-            for (int cp = lane; cp < PR; cp += 64) {
-                const float4 g = src[cp];
-                uint16_t h0, l0, h1, l1, h2, l2, h3, l3;
-                split_hi_lo<FP16>(g.x * scale, h0, l0);   // re, column 2cp
-                split_hi_lo<FP16>(g.z * scale, h1, l1);   // re, column 2cp + 1
-                split_hi_lo<FP16>(g.y * scale, h2, l2);   // im, column 2cp
-                split_hi_lo<FP16>(g.w * scale, h3, l3);   // im, column 2cp + 1
-                uint32_t* base = reinterpret_cast<uint32_t*>(A + rr * AS) + cp;
-                const int ps = AR * AS / 2;   // plane stride in words
-                base[0] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-                base[ps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-                base[2 * ps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
-                base[3 * ps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
+        const int ps = AR * AS / 2;   // plane stride in words
+        auto put = [&](int k, const float4& g) {
+            const int rr = small_div(k, inv_pr), cp = k - rr * PR;
+            uint16_t h0, l0, h1, l1, h2, l2, h3, l3;
+            split_hi_lo<FP16>(g.x * scale, h0, l0);   // re, column 2cp
+            split_hi_lo<FP16>(g.z * scale, h1, l1);   // re, column 2cp + 1
+            split_hi_lo<FP16>(g.y * scale, h2, l2);   // im, column 2cp
+            split_hi_lo<FP16>(g.w * scale, h3, l3);   // im, column 2cp + 1
+            uint32_t* base = reinterpret_cast<uint32_t*>(A + rr * AS) + cp;
+            base[0] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+            base[ps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            base[2 * ps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
+            base[3 * ps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
+        };
+        if (in_regs) {
+#pragma unroll
+            for (int j = 0; j < kRegCells; j++)
+                if (t + 256 * j < total) put(t + 256 * j, v[j]);
+        } else {
+            for (int k = t; k < total; k += 256) {
+                const int rr = small_div(k, inv_pr);
+                put(k, gtile[rr * grow4 + (k - rr * PR)]);
             }
         }
     }
@@ -193,30 +250,18 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
         re[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
         im[n] = re[n];
     }
-    const int m = lane & 15, kg = lane >> 4;
     // per-lane fragment origins (advanced by pointer increments)
     const int cpy = (15 - m) & 1;                       // parity of 32q + 8kg + 15 - m
     const uint16_t* a_lane = A + (16 * wr + m) * AS + 16 * NT * wc + 8 * kg;
-    const int bps = kTapChunk * BL;                     // B plane stride
     const uint16_t* b_lane = B + cpy * 4 * bps + (8 * kg + 15 - m - cpy);
 
-    for (int i0 = 0; i0 < a.Tr; i0 += kTapChunk) {
-        const int tc = min(kTapChunk, a.Tr - i0);
-        __syncthreads();   // A staged / previous chunk consumed
-        // ---- Toeplitz tap rows of this chunk (LDS -> LDS) ----------------------
-        // B[c][p][ii][x] = Tpad[p][i0 + ii][x + c], Tpad[.][x] = T[.][x - 15]
-        for (int rowi = wave; rowi < 2 * 4 * tc; rowi += 4) {
-            const int ii = rowi % tc;
-            const int cp = rowi / tc;
-            const int p = cp & 3, c = cp >> 2;
-            const uint16_t* trow = TC + (p * a.Tr + (i0 + ii)) * a.Tb;
-            uint16_t* brow = B + ((c * 4 + p) * kTapChunk + ii) * BL;
-            for (int x = lane; x < BL; x += 64) {
-                const int j = x + c - 15;
-                brow[x] = (j >= 0 && j < a.Tb) ? trow[j] : (uint16_t)0;
-            }
+    for (int i0 = 0; i0 < a.Tr; i0 += a.TCH) {
+        const int tc = min(a.TCH, a.Tr - i0);
+        if (i0 > 0) {
+            __syncthreads();   // previous chunk consumed
+            copy_toeplitz(a, B, i0, tc, t);
         }
-        __syncthreads();
+        __syncthreads();       // A (first chunk) and B staged
 
         // K steps (tap ii, K block q) of this chunk, ping-pong fragments
         const int nsteps = tc * a.KB;
@@ -227,7 +272,7 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
             if (++q == a.KB) {
                 q = 0;
                 ap += AS - 32 * (a.KB - 1);
-                bp += BL - 32 * (a.KB - 1);
+                bp += a.BL - 32 * (a.KB - 1);
             } else {
                 ap += 32;
                 bp += 32;
@@ -261,9 +306,9 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
         for (int e2 = 0; e2 < 4; e2++) {
             const int rr = r0 + 16 * wr + 4 * kg + e2;
             if (rr < a.H && cc < a.W) {
-                float2 v = make_float2(re[n][e2] * inv, im[n][e2] * inv);
-                if (a.win[rr].y == 0) v = make_float2(0.f, 0.f);
-                a.out[(size_t)rr * a.W + cc] = v;
+                float2 v2 = make_float2(re[n][e2] * inv, im[n][e2] * inv);
+                if (wsz[e2] == 0) v2 = make_float2(0.f, 0.f);
+                a.out[(size_t)rr * a.W + cc] = v2;
             }
         }
     }
@@ -285,8 +330,23 @@ size_t correlate_lds_bytes(const CorrArgs& a)
     int rows, cols;
     corr_tile(a.shape, &rows, &cols);
     const int AR = rows + a.Tr - 1;
-    return sizeof(uint16_t) *
-           (size_t)(4 * AR * a.AS + 2 * 4 * kTapChunk * a.BL + 4 * a.Tr * a.Tb + 2);
+    return sizeof(uint16_t) * ((size_t)4 * AR * a.AS + (size_t)8 * a.TCH * a.BL);
+}
+
+int correlate_chunk(const CorrArgs& a, size_t max_lds)
+{
+    CorrArgs c = a;
+    auto largest = [&](size_t budget) {
+        int lo = 0;
+        for (int tc = a.Tr; tc >= 1; tc--) {
+            c.TCH = tc;
+            if (correlate_lds_bytes(c) <= budget) { lo = tc; break; }
+        }
+        return lo;
+    };
+    int tc = largest(kCorrLdsTarget);
+    if (tc == 0) tc = largest(max_lds);
+    return tc;
 }
 
 template <bool FP16>

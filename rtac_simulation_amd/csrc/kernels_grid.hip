@@ -86,73 +86,115 @@ __global__ __launch_bounds__(256) void k_fill_u32_grid(uint32_t* p, uint32_t v, 
     if (i < n) p[i] = v;
 }
 
+// One workgroup per G row rho.  Its samples are the sorted run whose G row
+// is rho - 1 or rho (each sample's range weights go to rows gy and gy + 1);
+// each adds w_y w_x v to two columns of the row.  The row is accumulated in
+// LDS as 64-bit fixed point: integer adds are exact and independent of their
+// order, so the grid is bit-reproducible (and integer LDS atomics run at the
+// rate of plain LDS traffic, unlike FP32 ones).  A first pass finds M, the
+// largest |re|, |im| of the run's in-range samples; products are scaled by
+// 2^k with (run length) M 2^k < 2^62, so no partial sum can overflow.  Each
+// product is rounded once, in FP32, then summed exactly.  A run holding a
+// non-finite value yields a NaN row.
+constexpr int kDepositU = 4;   // samples per thread whose loads are in flight together
+
 __global__ __launch_bounds__(256) void k_deposit(DepositArgs a)
 {
-    extern __shared__ float sacc[];                   // [kDepositRows][GW][2]
-    const int t = threadIdx.x;
-    const int rho0 = blockIdx.x * kDepositRows;       // first G row of this block
+    extern __shared__ long long sacc[];               // [2][GW]: re plane, im plane
+    __shared__ float red[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int rho = blockIdx.x;
     const int GW = a.GW;
-    for (int k = t; k < kDepositRows * GW * 2; k += blockDim.x) sacc[k] = 0.f;
-    __syncthreads();
-    // samples whose G row is in [rho0 - 1, rho0 + R): they touch this block
-    const uint32_t s0 = a.row_start[rho0 - 1 + 1];
-    const uint32_t s1 = a.row_start[min(rho0 + kDepositRows, a.GH + 1) + 1];
-    constexpr int U = 4;   // samples per thread whose loads are in flight together
-    for (uint32_t sb = s0; sb < s1; sb += U * blockDim.x) {
+    unsigned long long* sre = reinterpret_cast<unsigned long long*>(sacc);
+    unsigned long long* sim = sre + GW;
+    for (int k = t; k < 2 * GW; k += 256) sacc[k] = 0;
+    const uint32_t s0 = a.row_start[rho];            // first sample with G row >= rho - 1
+    const uint32_t s1 = a.row_start[rho + 2];        // first sample with G row >= rho + 1
+    constexpr int U = kDepositU;
+
+    // ---- pass 1: largest component magnitude of the run -------------------
+    float mx = 0.f;
+    for (uint32_t sb = s0; sb < s1; sb += U * 256) {
+        uint32_t key[U];
+        float2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const uint32_t s = min(sb + u * 256 + t, s1 - 1);
+            key[u] = a.skeys[s];
+            const float4 r = a.sorted[s];
+            v[u] = make_float2(r.x, r.y);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const float m = fmaxf(fabsf(v[u].x), fabsf(v[u].y));
+            // NaN propagates as +inf so the row is flagged below
+            const float mm = m == m ? m : INFINITY;
+            if (key[u] != kOutOfRange) mx = fmaxf(mx, mm);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+    if (lane == 0) red[wave] = mx;
+    __syncthreads();                                  // also orders the LDS clear
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    float2* row = a.grid + (size_t)rho * GW;
+    if (!(mx <= 3.402823466e38f)) {
+        for (int c = t; c < GW; c += 256) row[c] = make_float2(NAN, NAN);
+        return;
+    }
+    int em = 0;
+    frexpf(mx, &em);                                  // mx < 2^em (em = 0 for mx = 0)
+    const int ec = 32 - __clz(max(s1 - s0, 1u));     // run length < 2^ec
+    const int k = 62 - em - ec;
+
+    // ---- pass 2: fixed-point deposit -----------------------------------------
+    for (uint32_t sb = s0; sb < s1; sb += U * 256) {
         uint32_t key[U];
         float4 rec[U];
         float2 xc[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            const uint32_t s = sb + u * blockDim.x + t;
-            const bool ok = s < s1;
-            key[u] = ok ? a.skeys[s] : kOutOfRange;
-            rec[u] = ok ? a.sorted[s] : make_float4(0.f, 0.f, 0.f, 0.f);
-            xc[u] = ok ? a.coord[s] : make_float2(0.f, 0.f);
+            const uint32_t s = min(sb + u * 256 + t, s1 - 1);
+            key[u] = a.skeys[s];
+            rec[u] = a.sorted[s];
+            xc[u] = a.coord[s];
         }
+        // weights for every slot first (no branch between the loads and their
+        // use), then the atomics; a slot past the run or out of range gets
+        // column -2 and adds nothing
+        int col[U];
+        float wx[U], vr[U], vi[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            if (key[u] == kOutOfRange) continue;
             const float y = grid_y(a, rec[u].z);
             const float x = ((xc[u].x - a.bear0_hi) + (xc[u].y - a.bear0_lo)) * a.inv_db + a.off_b;
             const float fy = floorf(y), fx = floorf(x);
-            const float wy1 = y - fy, wx1 = x - fx;
-            const int gy = (int)fy + a.fr, gx = (int)fx + a.fb;
+            const float wy1 = y - fy;
+            const int gy = (int)fy + a.fr;
+            // this row receives weight 1 - wy1 from gy == rho, wy1 from gy == rho - 1
+            const float wy = gy == rho ? 1.0f - wy1 : (gy == rho - 1 ? wy1 : 0.0f);
+            const bool ok = sb + u * 256 + t < s1 && key[u] != kOutOfRange;
+            wx[u] = x - fx;
+            col[u] = ok ? (int)fx + a.fb : -2;
+            vr[u] = wy * rec[u].x;
+            vi[u] = wy * rec[u].y;
+        }
 #pragma unroll
-            for (int dy = 0; dy < 2; dy++) {
-                const int lr = gy + dy - rho0;
-                if (lr < 0 || lr >= kDepositRows) continue;
-                const float wy = dy ? wy1 : 1.0f - wy1;
-#pragma unroll
-                for (int dx = 0; dx < 2; dx++) {
-                    const int c = gx + dx;
-                    if (c < 0 || c >= GW) continue;
-                    const float w = wy * (dx ? wx1 : 1.0f - wx1);
-                    atomicAdd(&sacc[2 * (lr * GW + c)], w * rec[u].x);
-                    atomicAdd(&sacc[2 * (lr * GW + c) + 1], w * rec[u].y);
-                }
+        for (int u = 0; u < U; u++) {
+            const int gx = col[u];
+            if (gx >= 0 && gx < GW) {
+                const float w = 1.0f - wx[u];
+                atomicAdd(&sre[gx], (unsigned long long)(long long)ldexpf(w * vr[u], k));
+                atomicAdd(&sim[gx], (unsigned long long)(long long)ldexpf(w * vi[u], k));
+            }
+            if (gx + 1 >= 0 && gx + 1 < GW) {
+                atomicAdd(&sre[gx + 1], (unsigned long long)(long long)ldexpf(wx[u] * vr[u], k));
+                atomicAdd(&sim[gx + 1], (unsigned long long)(long long)ldexpf(wx[u] * vi[u], k));
             }
         }
     }
     __syncthreads();
-    for (int k = t; k < kDepositRows * GW; k += blockDim.x) {
-        const int lr = k / GW, c = k - lr * GW;
-        const int rho = rho0 + lr;
-        if (rho < a.GH)
-            a.grid[(size_t)rho * GW + c] = make_float2(sacc[2 * k], sacc[2 * k + 1]);
-    }
-    // max |re|, |im| per 16 columns: the correlation's tile scales
-    const int CG = GW / 16;
-    for (int k = t; k < kDepositRows * CG; k += blockDim.x) {
-        const int lr = k / CG, g = k - lr * CG;
-        const int rho = rho0 + lr;
-        if (rho >= a.GH) continue;
-        const float* v = sacc + 2 * (lr * GW + 16 * g);
-        float mx = 0.f;
-#pragma unroll
-        for (int c = 0; c < 32; c++) mx = fmaxf(mx, fabsf(v[c]));
-        a.colmax[(size_t)rho * CG + g] = mx;
-    }
+    for (int c = t; c < GW; c += 256)
+        row[c] = make_float2(ldexpf((float)sacc[c], -k), ldexpf((float)sacc[GW + c], -k));
 }
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
@@ -161,9 +203,8 @@ hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
                        a.row_start, a.n, (uint32_t)(a.GH + 3));
     if (a.n) hipLaunchKernelGGL(k_row_marks, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
     hipLaunchKernelGGL(k_row_suffix_min, dim3(1), dim3(1024), 0, s, a.row_start, a.GH + 3);
-    const int blocks = (a.GH + kDepositRows - 1) / kDepositRows;
-    const size_t shm = sizeof(float) * 2 * kDepositRows * a.GW;
-    hipLaunchKernelGGL(k_deposit, dim3(blocks), dim3(256), shm, s, a);
+    const size_t shm = sizeof(long long) * 2 * a.GW;
+    hipLaunchKernelGGL(k_deposit, dim3(a.GH), dim3(256), shm, s, a);
     return hipGetLastError();
 }
 

@@ -151,3 +151,19 @@ def test_mfma_autotune_keeps_results(gpu):
     s2 = _sensor(R, cfg, hits)
     s2.compute_output(2)
     assert s2.tile_shape()[0] == shape
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_bitwise_reproducible(gpu, form):
+    """The deposit sums in exact fixed point and the correlation has a fixed
+    reduction order, so two computes of the same samples agree bit for bit."""
+    R = gpu
+    cfg = configs._fls(96, 192, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "rep")
+    hits = R.generate_hits(configs.fls_scene(), 25, 80_000)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    a = s.output().view(np.uint32).copy()
+    s.write_samples(hits.to_device())
+    s.compute_output(form)
+    b = s.output().view(np.uint32)
+    assert np.array_equal(a, b)
