@@ -128,7 +128,6 @@ Sensor::~Sensor()
     dev_free(d_seg);
     dev_free(d_win);
     dev_free(d_out);
-    dev_free(d_hist);
     dev_free(d_totals);
     for (auto& p : events) {
         (void)hipEventDestroy(p.start);
@@ -147,12 +146,15 @@ void Sensor::free_samples()
     dev_free(d_vA);
     dev_free(d_vB);
     dev_free(d_skeys);
+    dev_free(d_counts);
     capacity = 0;
 }
 
 int Sensor::reserve(size_t count)
 {
     if (count <= capacity) return RTSIM_OK;
+    if (count > 0xFFFFFFFFu)   // sample indices are 32-bit
+        return fail(RTSIM_ERR_INVALID, "sample count %zu exceeds 2^32 - 1", count);
     free_samples();
     int rc;
     if ((rc = dev_alloc(&d_samples, count))) return rc;
@@ -163,6 +165,7 @@ int Sensor::reserve(size_t count)
     if ((rc = dev_alloc(&d_vA, count))) return rc;
     if ((rc = dev_alloc(&d_vB, count))) return rc;
     if ((rc = dev_alloc(&d_skeys, count))) return rc;
+    if ((rc = dev_alloc(&d_counts, sort_count_words(count)))) return rc;
     capacity = count;
     return RTSIM_OK;
 }
@@ -284,9 +287,11 @@ int Sensor::compute(int form, hipStream_t s)
     sb.keys[1] = d_kB;
     sb.vals[0] = d_vA;
     sb.vals[1] = d_vB;
-    sb.hist = d_hist;
+    sb.counts = d_counts;
     sb.totals = d_totals;
-    sb.max_stripes = kMaxStripes;
+    sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
+    sb.zero = reinterpret_cast<uint32_t*>(d_seg);   // Binner segments start zeroed
+    sb.zero_count = 2 * (size_t)N;
     const uint32_t* order = nullptr;
     const uint32_t* img = nullptr;
     HIP_TRY(launch_sort(d_samples, n, sb, &order, &img, s));
@@ -296,7 +301,6 @@ int Sensor::compute(int form, hipStream_t s)
 
     // ---- Binner::compute (Binner.h:96-102) ------------------------------
     stage_begin(2, s);
-    HIP_TRY(launch_fill_u32(reinterpret_cast<uint32_t*>(d_seg), 0u, 2 * (size_t)N, s));
     GatherArgs g;
     g.rec = d_samples;
     g.order = order;
@@ -456,7 +460,6 @@ int rtsim_sensor_create(const rtsim_sensor_desc* d, rtsim_sensor* out)
         bc[b] = split_df(x);
     }
     if ((rc = dev_alloc(&s->d_bear, s->W)) || (rc = dev_alloc(&s->d_psf, (size_t)s->Tr * s->Tb)) ||
-        (rc = dev_alloc(&s->d_hist, 256 * (size_t)kMaxStripes)) ||
         (rc = dev_alloc(&s->d_totals, 256))) {
         delete s;
         return rc;

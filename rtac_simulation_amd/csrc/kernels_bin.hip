@@ -114,238 +114,6 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// LSD radix sort, 4 passes of 8 bits over (travel image, index).
-//
-// The array is cut into P contiguous stripes, one workgroup each.  Per pass:
-//   k_radix_hist    : per-stripe digit histogram -> hist[digit][stripe]
-//   k_radix_scan    : one workgroup per digit scans its row over stripes
-//                     (exclusive) and writes the digit total
-//   k_radix_scatter : each stripe walks its items in order, ranks them
-//                     stably (wave ballots over the digit bits give the
-//                     peers with the same digit, LDS carries counts across
-//                     waves and rounds) and scatters them.
-// Stable across stripes and within a stripe, so the final order is the
-// stable order of the travel images.
-// ---------------------------------------------------------------------------
-constexpr int kSortThreads = 256;
-constexpr int kSortWaves = kSortThreads / 64;
-constexpr int kSortItems = 8;                          // items per thread per tile
-constexpr int kSortTile = kSortThreads * kSortItems;   // 2048 items staged in LDS
-
-__global__ __launch_bounds__(256) void k_radix_init(const float4* __restrict__ rec, uint32_t n,
-                                                    uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ vals)
-{
-    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) {
-        keys[i] = travel_image(rec[i].z);
-        vals[i] = i;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_radix_hist(const uint32_t* __restrict__ keys,
-                                                    uint32_t n, uint32_t stripe, int shift,
-                                                    uint32_t* __restrict__ hist, uint32_t P)
-{
-    __shared__ uint32_t h[kSortWaves][256];
-    const uint32_t t = threadIdx.x, w = t >> 6;
-    for (int k = 0; k < kSortWaves; k++) h[k][t] = 0;
-    __syncthreads();
-    const uint32_t beg = blockIdx.x * stripe;     // multiple of 256
-    const uint32_t end = min(beg + stripe, n);
-    const uint32_t end4 = beg + ((end - beg) & ~3u);
-    const uint4* k4 = reinterpret_cast<const uint4*>(keys);
-    for (uint32_t i = beg + 4 * t; i < end4; i += 4 * kSortThreads) {
-        const uint4 v = k4[i >> 2];
-        atomicAdd(&h[w][(v.x >> shift) & 255u], 1u);
-        atomicAdd(&h[w][(v.y >> shift) & 255u], 1u);
-        atomicAdd(&h[w][(v.z >> shift) & 255u], 1u);
-        atomicAdd(&h[w][(v.w >> shift) & 255u], 1u);
-    }
-    for (uint32_t i = end4 + t; i < end; i += kSortThreads)
-        atomicAdd(&h[w][(keys[i] >> shift) & 255u], 1u);
-    __syncthreads();
-    uint32_t c = 0;
-    for (int k = 0; k < kSortWaves; k++) c += h[k][t];
-    hist[t * P + blockIdx.x] = c;
-}
-
-// Inclusive wave scan (wave64) of a 32-bit value.
-__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x, uint32_t lane)
-{
-    for (int o = 1; o < 64; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, 64);
-        if (lane >= (uint32_t)o) x += y;
-    }
-    return x;
-}
-
-// One workgroup per digit: exclusive scan of hist[digit][0..P) in place,
-// digit total to totals[digit].  P <= 4 * 256.
-__global__ __launch_bounds__(256) void k_radix_scan(uint32_t* __restrict__ hist, uint32_t P,
-                                                    uint32_t* __restrict__ totals)
-{
-    __shared__ uint32_t wsum[kSortWaves];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    uint32_t* row = hist + blockIdx.x * P;
-    uint32_t v[4];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint32_t idx = t * 4 + k;
-        v[k] = idx < P ? row[idx] : 0u;
-        s += v[k];
-    }
-    uint32_t incl = wave_incl_scan(s, lane);
-    if (lane == 63) wsum[w] = incl;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t k = 0; k < w; k++) base += wsum[k];
-    uint32_t run = base + incl - s;
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        uint32_t idx = t * 4 + k;
-        if (idx < P) row[idx] = run;
-        run += v[k];
-    }
-    if (t == kSortThreads - 1) totals[blockIdx.x] = run;
-}
-
-__global__ __launch_bounds__(256) void k_radix_scatter(
-    const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin,
-    uint32_t* __restrict__ kout, uint32_t* __restrict__ vout, uint32_t n, uint32_t stripe,
-    int shift, const uint32_t* __restrict__ hist, uint32_t P,
-    const uint32_t* __restrict__ totals)
-{
-    __shared__ uint32_t run[256];
-    __shared__ uint32_t wc[kSortWaves][256];
-    __shared__ uint32_t wsum[kSortWaves];
-    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-
-    // digit base = exclusive scan of the digit totals (block-wide)
-    uint32_t tot = totals[t];
-    uint32_t incl = wave_incl_scan(tot, lane);
-    if (lane == 63) wsum[w] = incl;
-    for (int k = 0; k < kSortWaves; k++) wc[k][t] = 0;
-    __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t k = 0; k < w; k++) base += wsum[k];
-    run[t] = base + incl - tot + hist[t * P + blockIdx.x];
-    __syncthreads();
-
-    __shared__ uint32_t tcnt[256];     // digit counts of the current tile
-    __shared__ uint32_t toff[256];     // exclusive prefix of tcnt
-    __shared__ uint32_t skey[kSortTile];
-    __shared__ uint32_t sval[kSortTile];
-
-    const uint32_t beg = blockIdx.x * stripe;
-    const uint32_t end = min(beg + stripe, n);
-    for (uint32_t tb = beg; tb < end; tb += kSortTile) {
-        const uint32_t tn = min((uint32_t)kSortTile, end - tb);
-        // all items of the tile in registers: item j of thread t is tb + j*256 + t
-        uint32_t key[kSortItems], val[kSortItems], lrank[kSortItems];
-#pragma unroll
-        for (int j = 0; j < kSortItems; j++) {
-            const uint32_t q = j * kSortThreads + t;
-            key[j] = q < tn ? kin[tb + q] : 0u;
-            val[j] = q < tn ? vin[tb + q] : 0u;
-        }
-        tcnt[t] = 0;
-        __syncthreads();
-        // stable ranks within the tile, one round of 256 items at a time
-#pragma unroll
-        for (int j = 0; j < kSortItems; j++) {
-            const bool valid = j * kSortThreads + t < tn;
-            const uint32_t d = (key[j] >> shift) & 255u;
-            uint64_t peers = __ballot(valid);
-#pragma unroll
-            for (int b = 0; b < 8; b++) {
-                const uint32_t bit = (d >> b) & 1u;
-                const uint64_t m = __ballot(bit);
-                peers &= bit ? m : ~m;
-            }
-            const uint32_t rank = popc_below(peers);
-            if (valid && rank == 0) wc[w][d] = (uint32_t)__popcll(peers);
-            __syncthreads();
-            uint32_t r = tcnt[d] + rank;
-            for (uint32_t k = 0; k < w; k++) r += wc[k][d];
-            lrank[j] = r;
-            __syncthreads();
-            uint32_t add = 0;
-            for (int k = 0; k < kSortWaves; k++) {
-                add += wc[k][t];
-                wc[k][t] = 0;
-            }
-            tcnt[t] += add;
-            __syncthreads();
-        }
-        // local digit offsets, then stage the tile in LDS in digit order
-        {
-            const uint32_t c = tcnt[t];
-            const uint32_t inc = wave_incl_scan(c, lane);
-            if (lane == 63) wsum[w] = inc;
-            __syncthreads();
-            uint32_t base = 0;
-            for (uint32_t k = 0; k < w; k++) base += wsum[k];
-            toff[t] = base + inc - c;
-            __syncthreads();
-        }
-#pragma unroll
-        for (int j = 0; j < kSortItems; j++) {
-            if (j * kSortThreads + t < tn) {
-                const uint32_t d = (key[j] >> shift) & 255u;
-                const uint32_t p = toff[d] + lrank[j];
-                skey[p] = key[j];
-                sval[p] = val[j];
-            }
-        }
-        __syncthreads();
-        // consecutive lanes write consecutive addresses inside each digit run
-        for (uint32_t q = t; q < tn; q += kSortThreads) {
-            const uint32_t k = skey[q];
-            const uint32_t d = (k >> shift) & 255u;
-            const uint32_t pos = run[d] + (q - toff[d]);
-            kout[pos] = k;
-            vout[pos] = sval[q];
-        }
-        __syncthreads();
-        run[t] += tcnt[t];
-        __syncthreads();
-    }
-}
-
-hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
-                       const uint32_t** order_out, const uint32_t** img_out, hipStream_t s)
-{
-    *order_out = b.vals[0];
-    *img_out = b.keys[0];
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_radix_init, dim3((n + 255) / 256), dim3(256), 0, s, records, n,
-                       b.keys[0], b.vals[0]);
-    // stripes of at least 2048 items, at most max_stripes (<= 1024) of them
-    uint32_t P = (n + 2047) / 2048;
-    if (P > b.max_stripes) P = b.max_stripes;
-    if (P < 1) P = 1;
-    uint32_t stripe = (n + P - 1) / P;
-    stripe = (stripe + kSortTile - 1) / kSortTile * kSortTile;   // whole tiles
-    P = (n + stripe - 1) / stripe;
-    int cur = 0;
-    for (int pass = 0; pass < 4; pass++) {
-        const int shift = 8 * pass;
-        hipLaunchKernelGGL(k_radix_hist, dim3(P), dim3(256), 0, s, b.keys[cur], n, stripe,
-                           shift, b.hist, P);
-        hipLaunchKernelGGL(k_radix_scan, dim3(256), dim3(256), 0, s, b.hist, P, b.totals);
-        hipLaunchKernelGGL(k_radix_scatter, dim3(P), dim3(256), 0, s, b.keys[cur],
-                           b.vals[cur], b.keys[cur ^ 1], b.vals[cur ^ 1], n, stripe, shift,
-                           b.hist, P, b.totals);
-        cur ^= 1;
-    }
-    *order_out = b.vals[cur];
-    *img_out = b.keys[cur];
-    return hipGetLastError();
-}
-
-// ---------------------------------------------------------------------------
 // gather sorted records, keys (Binner.cu:24-49) and segments (Binner.cu:51-69)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_gather(GatherArgs a)

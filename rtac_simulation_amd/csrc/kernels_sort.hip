@@ -1,0 +1,279 @@
+// Stable sort of the receiver samples by travel (Receiver.cu:9-42): an LSD
+// radix sort of (travel image, sample index) in four 8-bit passes over tiles
+// of 2048 items (one workgroup of 256 lanes x 8 items per tile).  Per pass:
+//
+//   k_sort_hist    : per-tile digit counts -> counts[digit][tile]
+//                    (pass 0 reads the records themselves)
+//   k_sort_scan    : one workgroup per digit scans its row over the tiles
+//                    (exclusive, in place) and writes the digit total
+//   k_sort_scatter : each tile ranks its items stably per wave (ballots over
+//                    the digit bits give each item's peers in its round; a
+//                    per-wave LDS row carries the running digit counts, and
+//                    since the LDS operations of one wave complete in order
+//                    no block barrier is needed between rounds), stages the
+//                    tile in LDS in digit order and writes every digit run to
+//                    consecutive global positions.
+// Kernel boundaries order the passes; no workgroup ever waits for another.
+// Items of a tile are ordered (wave, round, lane) = ascending index, ranks
+// count earlier rounds, lower lanes and earlier waves, tiles are scanned in
+// order: every pass is stable, so the result is the stable order of the
+// travel images.
+#include "device_common.h"
+#include "rtsim_internal.h"
+
+namespace rtsim {
+
+constexpr int kSortThreads = 256;
+constexpr int kSortWaves = kSortThreads / 64;                  // 4
+constexpr int kSortLaneItems = (int)kSortTile / kSortThreads;  // 16
+
+// Inclusive wave scan (wave64) of a 32-bit value.
+__device__ __forceinline__ uint32_t sort_wave_scan(uint32_t x, uint32_t lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Exclusive block scan (256 threads) of one value per thread; `ws` holds 4 words.
+__device__ __forceinline__ uint32_t sort_block_scan(uint32_t x, uint32_t* ws, uint32_t* total)
+{
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t incl = sort_wave_scan(x, lane);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t off = 0;
+    for (uint32_t k = 0; k < w; k++) off += ws[k];
+    if (total) *total = ws[0] + ws[1] + ws[2] + ws[3];
+    return off + incl - x;
+}
+
+template <bool FIRST>
+__device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
+                                          const uint32_t* __restrict__ kin,
+                                          const uint32_t* __restrict__ vin, uint32_t n,
+                                          uint32_t wbase, uint32_t lane, uint32_t* key,
+                                          uint32_t* val)
+{
+#pragma unroll
+    for (int j = 0; j < kSortLaneItems; j++) {
+        const uint32_t i = min(wbase + j * 64 + lane, n - 1);
+        if (FIRST) {
+            key[j] = travel_image(rec[i].z);
+            if (val) val[j] = i;
+        } else {
+            key[j] = kin[i];
+            if (val) val[j] = vin[i];
+        }
+    }
+}
+
+// ---- per-tile digit counts --------------------------------------------------
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sort_hist(const float4* __restrict__ rec,
+                                                   const uint32_t* __restrict__ kin, uint32_t n,
+                                                   int shift, uint32_t* __restrict__ counts,
+                                                   uint32_t tiles, uint32_t* __restrict__ zero,
+                                                   size_t zero_count)
+{
+    __shared__ uint32_t h[256];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    h[t] = 0u;
+    if (FIRST)
+        for (size_t i = (size_t)blockIdx.x * 256 + t; i < zero_count; i += (size_t)gridDim.x * 256)
+            zero[i] = 0u;
+    if (n == 0) return;
+    const uint32_t wbase = blockIdx.x * kSortTile + w * (64 * kSortLaneItems);
+    uint32_t key[kSortLaneItems];
+    load_tile<FIRST>(rec, kin, nullptr, n, wbase, lane, key, nullptr);
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kSortLaneItems; j++)
+        if (wbase + j * 64 + lane < n) atomicAdd(&h[(key[j] >> shift) & 255u], 1u);
+    __syncthreads();
+    counts[(size_t)t * tiles + blockIdx.x] = h[t];
+}
+
+// ---- one workgroup per digit: exclusive scan over tiles, digit total ---------
+constexpr int kScanPer = 32;   // tiles per thread: up to 8192 tiles (16.7M items)
+
+__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ counts, uint32_t tiles,
+                                                   uint32_t* __restrict__ totals)
+{
+    __shared__ uint32_t ws[4];
+    const uint32_t t = threadIdx.x;
+    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
+    const uint32_t per = (tiles + 255) / 256;
+    uint32_t v[kScanPer];
+    uint32_t s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) {
+        const uint32_t idx = t * per + k;
+        v[k] = (k < (int)per && idx < tiles) ? row[idx] : 0u;
+        s += v[k];
+    }
+    uint32_t total;
+    uint32_t run = sort_block_scan(s, ws, &total);
+#pragma unroll
+    for (int k = 0; k < kScanPer; k++) {
+        const uint32_t idx = t * per + k;
+        if (k < (int)per && idx < tiles) row[idx] = run;
+        run += v[k];
+    }
+    if (t == 0) totals[blockIdx.x] = total;
+}
+
+// ---- rank + stage + scatter ----------------------------------------------------
+struct ScatterArgs {
+    const float4* rec;        // pass 0 reads the records themselves
+    const uint32_t* kin;
+    const uint32_t* vin;
+    uint32_t* kout;
+    uint32_t* vout;
+    uint32_t n;
+    int shift;
+    const uint32_t* counts;   // scanned [digit][tile]
+    const uint32_t* totals;
+    uint32_t tiles;
+};
+
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
+{
+    __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
+    __shared__ uint32_t toff[256];               // digit run offsets in the staging tile
+    __shared__ uint32_t gpos[256];               // global position of each digit run
+    __shared__ uint32_t ws[4], ws2[4];
+    __shared__ uint32_t skey[kSortTile];
+    __shared__ uint32_t sval[kSortTile];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tb = tile * kSortTile;
+    const int shift = a.shift;
+
+    // global run start of digit t: digit base (scan of the totals) + earlier tiles
+    const uint32_t tot = a.totals[t];
+    const uint32_t pre = a.counts[(size_t)t * a.tiles + tile];
+#pragma unroll
+    for (int k = 0; k < kSortWaves; k++) wcnt[k][t] = 0u;
+
+    const uint32_t wbase = tb + w * (64 * kSortLaneItems);
+    uint32_t key[kSortLaneItems], val[kSortLaneItems], lrank[kSortLaneItems];
+    load_tile<FIRST>(a.rec, a.kin, a.vin, a.n, wbase, lane, key, val);
+    __syncthreads();
+
+    // stable ranks inside each wave
+#pragma unroll
+    for (int j = 0; j < kSortLaneItems; j++) {
+        const bool ok = wbase + j * 64 + lane < a.n;
+        const uint32_t d = (key[j] >> shift) & 255u;
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint32_t bit = (d >> b) & 1u;
+            const uint64_t m = __ballot(bit);
+            peers &= bit ? m : ~m;
+        }
+        const uint32_t r = popc_below(peers);
+        const uint32_t old = wcnt[w][d];
+        lrank[j] = old + r;
+        if (ok && r == 0) wcnt[w][d] = old + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+
+    // thread t = digit t: per-wave offsets, tile-local run offset, global run
+    uint32_t c[kSortWaves], cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kSortWaves; k++) {
+        c[k] = wcnt[k][t];
+        cnt += c[k];
+    }
+    const uint32_t o = sort_block_scan(cnt, ws, nullptr);
+    const uint32_t base = sort_block_scan(tot, ws2, nullptr);
+    toff[t] = o;
+    gpos[t] = base + pre;
+    {
+        uint32_t run = o;
+#pragma unroll
+        for (int k = 0; k < kSortWaves; k++) {
+            wcnt[k][t] = run;
+            run += c[k];
+        }
+    }
+    __syncthreads();
+
+    // stage the tile in digit order
+#pragma unroll
+    for (int j = 0; j < kSortLaneItems; j++) {
+        if (wbase + j * 64 + lane < a.n) {
+            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint32_t p = wcnt[w][d] + lrank[j];
+            skey[p] = key[j];
+            sval[p] = val[j];
+        }
+    }
+    __syncthreads();
+
+    // each digit run of the staged tile goes to consecutive global positions
+    const uint32_t tn = min(kSortTile, a.n - tb);
+    for (uint32_t q = t; q < tn; q += kSortThreads) {
+        const uint32_t k = skey[q];
+        const uint32_t d = (k >> shift) & 255u;
+        const uint32_t pos = gpos[d] + (q - toff[d]);
+        if (pos < a.n) {   // holds unless the records changed during the sort
+            a.kout[pos] = k;
+            a.vout[pos] = sval[q];
+        }
+    }
+}
+
+hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
+                       const uint32_t** order_out, const uint32_t** img_out, hipStream_t s)
+{
+    *order_out = b.vals[0];
+    *img_out = b.keys[0];
+    if (n == 0) {
+        if (b.zero_count)
+            hipLaunchKernelGGL(k_sort_hist<true>, dim3(1), dim3(256), 0, s, records, nullptr, 0u,
+                               0, b.counts, 1u, b.zero, b.zero_count);
+        return hipGetLastError();
+    }
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
+    ScatterArgs a;
+    a.rec = records;
+    a.n = n;
+    a.counts = b.counts;
+    a.totals = b.totals;
+    a.tiles = tiles;
+    int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
+    for (int pass = 0; pass < 4; pass++) {
+        const int shift = 8 * pass;
+        a.shift = shift;
+        a.kin = b.keys[cur];
+        a.vin = b.vals[cur];
+        a.kout = b.keys[cur ^ 1];
+        a.vout = b.vals[cur ^ 1];
+        if (pass == 0)
+            hipLaunchKernelGGL(k_sort_hist<true>, dim3(tiles), dim3(256), 0, s, records, nullptr, n,
+                               shift, b.counts, tiles, b.zero, b.zero_count);
+        else
+            hipLaunchKernelGGL(k_sort_hist<false>, dim3(tiles), dim3(256), 0, s, records,
+                               (const uint32_t*)b.keys[cur], n, shift, b.counts, tiles,
+                               (uint32_t*)nullptr, (size_t)0);
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, b.counts, tiles, b.totals);
+        if (pass == 0)
+            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(tiles), dim3(256), 0, s, a);
+        else
+            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(tiles), dim3(256), 0, s, a);
+        cur ^= 1;
+    }
+    *order_out = b.vals[cur];
+    *img_out = b.keys[cur];
+    return hipGetLastError();
+}
+
+}  // namespace rtsim

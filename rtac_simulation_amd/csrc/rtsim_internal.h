@@ -37,12 +37,21 @@ struct StreamArgs {
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
 
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
+constexpr uint32_t kSortTile = 2048;     // items per sort workgroup (256 lanes x 8)
+// per-tile digit counts of one pass for `capacity` items
+inline size_t sort_count_words(size_t capacity)
+{
+    return 256 * ((capacity + kSortTile - 1) / kSortTile);
+}
 struct SortBuffers {
     uint32_t* keys[2];
     uint32_t* vals[2];
-    uint32_t* hist;       // 256 * max_stripes
-    uint32_t* totals;     // 256
-    uint32_t max_stripes;
+    uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
+    uint32_t* totals;     // [256] digit totals of the current pass
+    uint32_t tiles_cap;
+    // cleared by the first sort kernel (before any later kernel reads them)
+    uint32_t* zero;       // zero[0..zero_count) = 0 (Binner segments)
+    size_t zero_count;
 };
 // Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
 // buffers that hold the permutation and the sorted travel images.

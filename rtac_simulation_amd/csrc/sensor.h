@@ -21,8 +21,6 @@ const char* fail_message();
 KeyParams make_key(uint32_t N, float lo, float hi, float margin);
 int make_overlap(float margin, float lo, float hi, uint32_t N);
 
-constexpr uint32_t kMaxStripes = 1024;
-
 struct StageEvent {
     int stage;
     hipEvent_t start, stop;
@@ -60,8 +58,8 @@ struct Sensor {
     uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
     const uint32_t* order_ptr = nullptr;  // sort permutation (after compute)
     const uint32_t* img_ptr = nullptr;    // sorted travel images (after compute)
-    uint32_t* d_hist = nullptr;
-    uint32_t* d_totals = nullptr;
+    uint32_t* d_totals = nullptr; // sort: digit totals of a pass
+    uint32_t* d_counts = nullptr; // sort: per-tile digit counts (capacity-sized)
 
     // ---- per-configuration device tables -----------------------------------
     uint2* d_seg = nullptr;       // Binner::binIndexes_
