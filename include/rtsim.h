@@ -175,8 +175,10 @@ int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream);
 /* Per-stage timing with HIP events recorded on the compute stream.  When on,
  * each compute records events around its stages; stage_times returns the
  * mean milliseconds per stage over the computes since the last reset.
- * Stages: 0 stream, 1 sort, 2 bins (keys+segments+windows), 3 grid deposit
- * (MFMA form only), 4 convolution kernel (plain gather or MFMA correlation).    */
+ * Stages: 0 stream, 1 sort (its last pass also gathers the records, keys and
+ * bearing coordinates), 2 bins (segments + windows; MFMA forms: + grid row
+ * table), 3 grid deposit (MFMA forms only), 4 convolution kernel (plain
+ * gather or MFMA correlation).                                                */
 int rtsim_sensor_set_profiling(rtsim_sensor h, int on);
 int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes);
 

@@ -280,7 +280,18 @@ int Sensor::compute(int form, hipStream_t s)
 {
     const uint32_t n = (uint32_t)count;
     (void)hipGetLastError();   // start from a clean error state: launches are checked below
-    // ---- sort (Receiver.cu:28-34) -------------------------------------
+    const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
+    int rc = RTSIM_OK;
+    if (form == RTSIM_FORM_PLAIN_FP32) {
+        if ((rc = check_plain_limits())) return rc;
+    } else if (!grid) {
+        return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
+    }
+    RowMap rows{};
+    if (grid && (rc = grid_row_map(&rows))) return rc;   // also checks the grid form applies
+
+    // ---- sort (Receiver.cu:28-34); the last pass gathers the records, their
+    //      keys (Binner.cu:24-49) and bearing coordinates ----------------------
     stage_begin(1, s);
     SortBuffers sb;
     sb.keys[0] = d_kA;
@@ -292,6 +303,14 @@ int Sensor::compute(int form, hipStream_t s)
     sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
     sb.zero = reinterpret_cast<uint32_t*>(d_seg);   // Binner segments start zeroed
     sb.zero_count = 2 * (size_t)N;
+    sb.fill = rows.row_start;                         // grid row table starts at n
+    sb.fill_value = n;
+    sb.fill_count = rows.row_start ? (size_t)rows.GH + 3 : 0;
+    sb.out.sorted = d_sorted;
+    sb.out.skeys = d_skeys;
+    sb.out.coord = d_coord;
+    sb.out.key = key;
+    sb.out.axis = axis;
     const uint32_t* order = nullptr;
     const uint32_t* img = nullptr;
     HIP_TRY(launch_sort(d_samples, n, sb, &order, &img, s));
@@ -299,36 +318,29 @@ int Sensor::compute(int form, hipStream_t s)
     img_ptr = img;
     stage_end(s);
 
-    // ---- Binner::compute (Binner.h:96-102) ------------------------------
+    // ---- Binner::compute (Binner.h:96-102): segments, windows ---------------
     stage_begin(2, s);
-    GatherArgs g;
-    g.rec = d_samples;
-    g.order = order;
-    g.img = img;
-    g.n = n;
-    g.key = key;
-    g.sorted = d_sorted;
-    g.coord = d_coord;
-    g.skeys = d_skeys;
-    g.seg = d_seg;
-    g.axis = axis;
-    HIP_TRY(launch_gather(g, s));
-    HIP_TRY(launch_windows(d_seg, N, overlap, d_win, s));
+    BinArgs ba;
+    ba.skeys = d_skeys;
+    ba.img = img;
+    ba.n = n;
+    ba.N = N;
+    ba.ov = overlap;
+    ba.seg = d_seg;
+    ba.win = d_win;
+    ba.rows = rows;
+    HIP_TRY(launch_bins(ba, s));
     stage_end(s);
 
     // ---- PSF convolution ------------------------------------------------
-    int rc = RTSIM_OK;
     if (form == RTSIM_FORM_PLAIN_FP32) {
-        if ((rc = check_plain_limits())) return rc;
         ConvArgs c = conv_args();
         stage_begin(4, s);
         if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(c, s));
         else HIP_TRY(launch_conv1d_plain(c, s));
         stage_end(s);
-    } else if (form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16) {
-        rc = compute_grid_form(form, s);   // stages 3 (deposit) and 4 (correlation)
     } else {
-        rc = fail(RTSIM_ERR_INVALID, "unknown form %d", form);
+        rc = compute_grid_form(form, s);   // stages 3 (deposit) and 4 (correlation)
     }
     if (rc == RTSIM_OK) computed = true;
     return rc;

@@ -43,6 +43,22 @@ __device__ __forceinline__ uint32_t bin_key(const KeyParams& k, float t)
     return (uint32_t)res;
 }
 
+// Grid (MFMA) form: range coordinate of a travel in grid rows, one explicit
+// fma so that the row table and the deposit round identically.
+__device__ __forceinline__ float grid_y_of(float t, float range0, float inv_dr, float off_r)
+{
+    return __builtin_fmaf(t - range0, inv_dr, off_r);
+}
+
+// G row (grid row + fr) of a travel, clamped to [-2, GH + 2]; NaN maps to the
+// top (the sort puts NaN travel last).
+__device__ __forceinline__ int g_row_of(const RowMap& m, float t)
+{
+    const float r = floorf(grid_y_of(t, m.range0, m.inv_dr, m.off_r)) + (float)m.fr;
+    if (!(r >= -2.0f)) return r != r ? m.GH + 2 : -2;
+    return r > (float)(m.GH + 2) ? m.GH + 2 : (int)r;
+}
+
 // Number of set bits of `mask` in lanes strictly below the calling lane.
 __device__ __forceinline__ uint32_t popc_below(uint64_t mask)
 {

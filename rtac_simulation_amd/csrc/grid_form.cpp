@@ -257,6 +257,20 @@ int Sensor::ensure_grid()
     return RTSIM_OK;
 }
 
+int Sensor::grid_row_map(RowMap* m)
+{
+    int rc = ensure_grid();
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(grid_state);
+    m->range0 = range_min;
+    m->inv_dr = g->inv_dr;
+    m->off_r = g->off_r;
+    m->fr = g->fr;
+    m->GH = g->GH;
+    m->row_start = g->d_row_start;
+    return RTSIM_OK;
+}
+
 int Sensor::compute_grid_form(int form, hipStream_t s)
 {
     int rc = ensure_grid();
@@ -264,7 +278,6 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     GridState* g = static_cast<GridState*>(grid_state);
 
     DepositArgs d;
-    d.img = img_ptr;
     d.sorted = d_sorted;
     d.coord = d_coord;
     d.skeys = d_skeys;

@@ -19,12 +19,11 @@ void corr_tile(int shape, int* rows, int* cols);
 // Extended grid G: rho = grid row + fr, kappa = grid column + fb, so that
 // out(r, b) = sum T[i][j] G[r + i][b + j].
 struct DepositArgs {
-    const uint32_t* img;       // sorted travel images (4 B per sample)
     const float4* sorted;
     const float2* coord;       // bearing-axis coordinate (hi, lo) per sorted sample
     const uint32_t* skeys;     // OutOfRange samples are skipped
     uint32_t n;
-    uint32_t* row_start;       // GH + 3 entries
+    uint32_t* row_start;       // GH + 3 entries (built after the sort, see RowMap)
     float2* grid;              // [GH][GW]
     int GH, GW;
     int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)

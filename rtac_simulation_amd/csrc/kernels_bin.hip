@@ -114,60 +114,82 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// gather sorted records, keys (Binner.cu:24-49) and segments (Binner.cu:51-69)
+// segments (Binner.cu:51-80) and the grid row table, over the sorted keys
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_gather(GatherArgs a)
+__global__ __launch_bounds__(256) void k_bounds(BinArgs a)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
-    const float4 r = a.rec[a.order[i]];
-    a.sorted[i] = r;
-    const uint32_t k = bin_key(a.key, r.z);
-    a.skeys[i] = k;
-    const double x = a.axis == 1 ? sin((double)r.w) : (double)r.w;
-    const float xh = (float)x;
-    a.coord[i] = make_float2(xh, (float)(x - (double)xh));
+    const uint32_t k = a.skeys[i];
     if (k != kOutOfRange) {
         if (i == 0)
             a.seg[0].x = 0;  // Binner.cu:56-58 (equal to the zero fill)
-        else if (bin_key(a.key, image_travel(a.img[i - 1])) != k)
+        else if (a.skeys[i - 1] != k)
             a.seg[k].x = i;
         if (i == a.n - 1)
             a.seg[k].y = a.n;
-        else if (bin_key(a.key, image_travel(a.img[i + 1])) != k)
+        else if (a.skeys[i + 1] != k)
             a.seg[k].y = i + 1;
+    }
+    if (a.rows.row_start) {
+        // first sample of each G row present (the table holds n elsewhere)
+        const int r = g_row_of(a.rows, image_travel(a.img[i]));
+        const int rp = i == 0 ? -3 : g_row_of(a.rows, image_travel(a.img[i - 1]));
+        if (r != rp && r >= -1 && r <= a.rows.GH + 1) a.rows.row_start[r + 1] = i;
     }
 }
 
-hipError_t launch_gather(const GatherArgs& a, hipStream_t s)
+// ---------------------------------------------------------------------------
+// windows: do_make_bins (Binner.cu:82-106); with a row table, the last
+// workgroup gives every row the minimum over itself and the rows above it
+// (rows without samples then start where the next row that has some starts)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_windows(BinArgs a, uint32_t win_blocks)
 {
-    if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_gather, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
-    return hipGetLastError();
+    if (blockIdx.x < win_blocks) {
+        const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+        if (tid >= (int)a.N) return;
+        const uint2* seg = a.seg;
+        int istart = max(0, tid - a.ov);
+        int iend = min((int)a.N - 1, tid + a.ov);
+        while (istart < iend && seg[istart].y == seg[istart].x) istart++;
+        while (iend > istart && seg[iend].y == seg[iend].x) iend--;
+        if (istart >= iend)
+            a.win[tid] = make_uint2(0u, 0u);
+        else
+            a.win[tid] = make_uint2(seg[istart].x, seg[iend].y - seg[istart].x);
+        return;
+    }
+    // reverse inclusive min-scan of the GH + 3 row-table entries
+    __shared__ uint32_t part[256];
+    uint32_t* rs = a.rows.row_start;
+    const int count = a.rows.GH + 3;
+    const int t = threadIdx.x;
+    const int per = (count + 255) / 256;
+    const int beg = t * per, end = min(beg + per, count);
+    uint32_t m = 0xffffffffu;
+    for (int k = end - 1; k >= beg; k--) m = min(m, rs[k]);
+    part[t] = m;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {   // suffix minimum of the chunk minima
+        const uint32_t v = t + o < 256 ? part[t + o] : 0xffffffffu;
+        __syncthreads();
+        part[t] = min(part[t], v);
+        __syncthreads();
+    }
+    uint32_t run = t + 1 < 256 ? part[t + 1] : 0xffffffffu;
+    for (int k = end - 1; k >= beg; k--) {
+        run = min(run, rs[k]);
+        rs[k] = run;
+    }
 }
 
-// ---------------------------------------------------------------------------
-// windows: do_make_bins (Binner.cu:82-106)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_windows(const uint2* __restrict__ seg, uint32_t N,
-                                                 int ov, uint2* __restrict__ win)
+hipError_t launch_bins(const BinArgs& a, hipStream_t s)
 {
-    const int tid = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tid >= (int)N) return;
-    int istart = max(0, tid - ov);
-    int iend = min((int)N - 1, tid + ov);
-    while (istart < iend && seg[istart].y == seg[istart].x) istart++;
-    while (iend > istart && seg[iend].y == seg[iend].x) iend--;
-    if (istart >= iend)
-        win[tid] = make_uint2(0u, 0u);
-    else
-        win[tid] = make_uint2(seg[istart].x, seg[iend].y - seg[istart].x);
-}
-
-hipError_t launch_windows(const uint2* seg, uint32_t N, int ov, uint2* win, hipStream_t s)
-{
-    if (N == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_windows, dim3((N + 255) / 256), dim3(256), 0, s, seg, N, ov, win);
+    if (a.n) hipLaunchKernelGGL(k_bounds, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
+    const uint32_t wb = (a.N + 255) / 256;
+    const uint32_t blocks = wb + (a.rows.row_start ? 1u : 0u);
+    if (blocks) hipLaunchKernelGGL(k_windows, dim3(blocks), dim3(256), 0, s, a, wb);
     return hipGetLastError();
 }
 

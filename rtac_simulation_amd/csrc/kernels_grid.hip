@@ -19,71 +19,10 @@ namespace rtsim {
 // ---------------------------------------------------------------------------
 // deposit
 // ---------------------------------------------------------------------------
-// grid range coordinate of a travel; one explicit fma so that the row table
-// and the deposit round identically
+// grid range coordinate of a travel (the row table uses the same rounding)
 __device__ __forceinline__ float grid_y(const DepositArgs& a, float t)
 {
-    return __builtin_fmaf(t - a.range0, a.inv_dr, a.off_r);
-}
-
-__device__ __forceinline__ float grid_row_of(const DepositArgs& a, float t)
-{
-    return floorf(grid_y(a, t));
-}
-
-// G row (grid row + fr) of a sorted sample, clamped to [-2, GH + 2]; the
-// travel is sorted so it is non-decreasing in the sample index (NaN travel
-// sorts last and maps to the top).
-__device__ __forceinline__ int g_row(const DepositArgs& a, float t)
-{
-    const float r = grid_row_of(a, t) + (float)a.fr;
-    if (!(r >= -2.0f)) return r != r ? a.GH + 2 : -2;
-    return r > (float)(a.GH + 2) ? a.GH + 2 : (int)r;
-}
-
-// row_start[rho + 1] = first sorted sample whose G row is >= rho, for rho in
-// [-1, GH + 1].  Built in two steps: k_row_marks writes, for every row that
-// has samples, the index of its first sample (the table is pre-filled with
-// n); k_row_suffix_min then gives every row the minimum over itself and the
-// rows above it.
-__global__ __launch_bounds__(256) void k_row_marks(DepositArgs a)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const int r = g_row(a, image_travel(a.img[i]));
-    const int rp = i == 0 ? -3 : g_row(a, image_travel(a.img[i - 1]));
-    if (r != rp && r >= -1 && r <= a.GH + 1) a.row_start[r + 1] = i;
-}
-
-// one workgroup of 1024 threads, reverse inclusive min-scan of GH + 3 entries
-__global__ __launch_bounds__(1024) void k_row_suffix_min(uint32_t* __restrict__ rs, int count)
-{
-    __shared__ uint32_t part[1024];
-    const int t = threadIdx.x;
-    const int per = (count + 1023) / 1024;
-    const int beg = t * per, end = min(beg + per, count);
-    uint32_t m = 0xffffffffu;
-    for (int k = end - 1; k >= beg; k--) m = min(m, rs[k]);
-    part[t] = m;
-    __syncthreads();
-    // suffix minimum of the chunk minima (Hillis-Steele, toward lower t)
-    for (int o = 1; o < 1024; o <<= 1) {
-        uint32_t v = t + o < 1024 ? part[t + o] : 0xffffffffu;
-        __syncthreads();
-        part[t] = min(part[t], v);
-        __syncthreads();
-    }
-    uint32_t run = t + 1 < 1024 ? part[t + 1] : 0xffffffffu;
-    for (int k = end - 1; k >= beg; k--) {
-        run = min(run, rs[k]);
-        rs[k] = run;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_fill_u32_grid(uint32_t* p, uint32_t v, uint32_t n)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
+    return grid_y_of(t, a.range0, a.inv_dr, a.off_r);
 }
 
 // One workgroup per G row rho.  Its samples are the sorted run whose G row
@@ -199,10 +138,6 @@ __global__ __launch_bounds__(256) void k_deposit(DepositArgs a)
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_fill_u32_grid, dim3((a.GH + 3 + 255) / 256), dim3(256), 0, s,
-                       a.row_start, a.n, (uint32_t)(a.GH + 3));
-    if (a.n) hipLaunchKernelGGL(k_row_marks, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
-    hipLaunchKernelGGL(k_row_suffix_min, dim3(1), dim3(1024), 0, s, a.row_start, a.GH + 3);
     const size_t shm = sizeof(long long) * 2 * a.GW;
     hipLaunchKernelGGL(k_deposit, dim3(a.GH), dim3(256), shm, s, a);
     return hipGetLastError();

@@ -12,7 +12,9 @@
 //                    since the LDS operations of one wave complete in order
 //                    no block barrier is needed between rounds), stages the
 //                    tile in LDS in digit order and writes every digit run to
-//                    consecutive global positions.
+//                    consecutive global positions.  The last pass also moves
+//                    the records to their sorted positions and writes their
+//                    Binner keys and bearing-axis coordinates.
 // Kernel boundaries order the passes; no workgroup ever waits for another.
 // Items of a tile are ordered (wave, round, lane) = ascending index, ranks
 // count earlier rounds, lower lanes and earlier waves, tiles are scanned in
@@ -76,14 +78,18 @@ __global__ __launch_bounds__(256) void k_sort_hist(const float4* __restrict__ re
                                                    const uint32_t* __restrict__ kin, uint32_t n,
                                                    int shift, uint32_t* __restrict__ counts,
                                                    uint32_t tiles, uint32_t* __restrict__ zero,
-                                                   size_t zero_count)
+                                                   size_t zero_count, uint32_t* __restrict__ fill,
+                                                   uint32_t fill_value, size_t fill_count)
 {
     __shared__ uint32_t h[256];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     h[t] = 0u;
-    if (FIRST)
+    if (FIRST) {
         for (size_t i = (size_t)blockIdx.x * 256 + t; i < zero_count; i += (size_t)gridDim.x * 256)
             zero[i] = 0u;
+        for (size_t i = (size_t)blockIdx.x * 256 + t; i < fill_count; i += (size_t)gridDim.x * 256)
+            fill[i] = fill_value;
+    }
     if (n == 0) return;
     const uint32_t wbase = blockIdx.x * kSortTile + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems];
@@ -137,9 +143,10 @@ struct ScatterArgs {
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;
+    SortOutputs out;          // last pass only
 };
 
-template <bool FIRST>
+template <bool FIRST, bool LAST>
 __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
 {
     __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
@@ -217,15 +224,43 @@ __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
     }
     __syncthreads();
 
-    // each digit run of the staged tile goes to consecutive global positions
+    // each digit run of the staged tile goes to consecutive global positions;
+    // the last pass also moves the records and writes their keys / coordinates
     const uint32_t tn = min(kSortTile, a.n - tb);
-    for (uint32_t q = t; q < tn; q += kSortThreads) {
-        const uint32_t k = skey[q];
-        const uint32_t d = (k >> shift) & 255u;
-        const uint32_t pos = gpos[d] + (q - toff[d]);
-        if (pos < a.n) {   // holds unless the records changed during the sort
-            a.kout[pos] = k;
-            a.vout[pos] = sval[q];
+    constexpr int Q = kSortLaneItems;
+    uint32_t kq[Q], vq[Q], pq[Q];
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+        const uint32_t q = t + u * kSortThreads;
+        kq[u] = 0u;
+        vq[u] = 0u;
+        pq[u] = 0xffffffffu;
+        if (q < tn) {
+            kq[u] = skey[q];
+            vq[u] = sval[q];
+            const uint32_t d = (kq[u] >> shift) & 255u;
+            pq[u] = gpos[d] + (q - toff[d]);
+        }
+    }
+    float4 rq[Q];
+    if (LAST) {
+#pragma unroll
+        for (int u = 0; u < Q; u++) rq[u] = a.rec[pq[u] < a.n ? vq[u] : 0u];
+    }
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+        const uint32_t pos = pq[u];
+        if (pos < a.n) {   // holds for every staged item unless the records changed
+            a.kout[pos] = kq[u];
+            a.vout[pos] = vq[u];
+            if (LAST) {
+                const float4 r = rq[u];
+                a.out.sorted[pos] = r;
+                a.out.skeys[pos] = bin_key(a.out.key, r.z);
+                const double x = a.out.axis == 1 ? sin((double)r.w) : (double)r.w;
+                const float xh = (float)x;
+                a.out.coord[pos] = make_float2(xh, (float)(x - (double)xh));
+            }
         }
     }
 }
@@ -236,9 +271,10 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
     *order_out = b.vals[0];
     *img_out = b.keys[0];
     if (n == 0) {
-        if (b.zero_count)
+        if (b.zero_count || b.fill_count)
             hipLaunchKernelGGL(k_sort_hist<true>, dim3(1), dim3(256), 0, s, records, nullptr, 0u,
-                               0, b.counts, 1u, b.zero, b.zero_count);
+                               0, b.counts, 1u, b.zero, b.zero_count, b.fill, b.fill_value,
+                               b.fill_count);
         return hipGetLastError();
     }
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
@@ -249,6 +285,7 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
     a.counts = b.counts;
     a.totals = b.totals;
     a.tiles = tiles;
+    a.out = b.out;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
     for (int pass = 0; pass < 4; pass++) {
         const int shift = 8 * pass;
@@ -259,16 +296,19 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
         a.vout = b.vals[cur ^ 1];
         if (pass == 0)
             hipLaunchKernelGGL(k_sort_hist<true>, dim3(tiles), dim3(256), 0, s, records, nullptr, n,
-                               shift, b.counts, tiles, b.zero, b.zero_count);
+                               shift, b.counts, tiles, b.zero, b.zero_count, b.fill,
+                               b.fill_value, b.fill_count);
         else
             hipLaunchKernelGGL(k_sort_hist<false>, dim3(tiles), dim3(256), 0, s, records,
                                (const uint32_t*)b.keys[cur], n, shift, b.counts, tiles,
-                               (uint32_t*)nullptr, (size_t)0);
+                               (uint32_t*)nullptr, (size_t)0, (uint32_t*)nullptr, 0u, (size_t)0);
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, b.counts, tiles, b.totals);
         if (pass == 0)
-            hipLaunchKernelGGL(k_sort_scatter<true>, dim3(tiles), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, a);
+        else if (pass < 3)
+            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, a);
         else
-            hipLaunchKernelGGL(k_sort_scatter<false>, dim3(tiles), dim3(256), 0, s, a);
+            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, a);
         cur ^= 1;
     }
     *order_out = b.vals[cur];

@@ -43,15 +43,28 @@ inline size_t sort_count_words(size_t capacity)
 {
     return 256 * ((capacity + kSortTile - 1) / kSortTile);
 }
+// Written by the last pass for every sorted position i: the record, its
+// Binner key (Binner.cu:24-49) and its bearing-axis coordinate as (hi, lo).
+struct SortOutputs {
+    float4* sorted;
+    uint32_t* skeys;
+    float2* coord;
+    KeyParams key;
+    int axis;             // 1: coordinate = sin(bearing) in FP64, else the bearing
+};
 struct SortBuffers {
     uint32_t* keys[2];
     uint32_t* vals[2];
     uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
     uint32_t* totals;     // [256] digit totals of the current pass
     uint32_t tiles_cap;
-    // cleared by the first sort kernel (before any later kernel reads them)
+    // prepared by the first sort kernel (before any later kernel reads them)
     uint32_t* zero;       // zero[0..zero_count) = 0 (Binner segments)
     size_t zero_count;
+    uint32_t* fill;       // fill[0..fill_count) = fill_value (grid row table)
+    uint32_t fill_value;
+    size_t fill_count;
+    SortOutputs out;
 };
 // Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
 // buffers that hold the permutation and the sorted travel images.
@@ -59,25 +72,32 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
                        const uint32_t** order_out, const uint32_t** img_out,
                        hipStream_t s);
 
-// ---- gather sorted records + keys + segments (Binner.cu:24-80) --------------
-struct GatherArgs {
-    const float4* rec;
-    const uint32_t* order;
+// G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
+// row_start[rho + 1] = first sorted sample whose G row is >= rho, rho in [-1, GH + 1].
+struct RowMap {
+    float range0, inv_dr, off_r;
+    int fr, GH;
+    uint32_t* row_start;  // GH + 3 entries, pre-filled with n; nullptr: no row table
+};
+
+// ---- segments + windows (+ the grid row table) after the sort ----------------
+// k_bounds: segments (Binner.cu:51-80; seg zeroed beforehand) and the first
+// sample of every G row; k_windows: do_make_bins (Binner.cu:82-106), with one
+// extra workgroup closing the row table (suffix minimum).
+struct BinArgs {
+    const uint32_t* skeys;
     const uint32_t* img;
     uint32_t n;
-    KeyParams key;
-    float4* sorted;
-    float2* coord;        // bearing-axis coordinate as (hi, lo) floats
-    uint32_t* skeys;
-    uint2* seg;           // must be zeroed beforehand
-    int axis;
+    uint32_t N;
+    int ov;
+    uint2* seg;
+    uint2* win;
+    RowMap rows;
 };
-hipError_t launch_gather(const GatherArgs& a, hipStream_t s);
+hipError_t launch_bins(const BinArgs& a, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
-// ---- windows (Binner.cu:82-136) ---------------------------------------------
-hipError_t launch_windows(const uint2* seg, uint32_t N, int ov, uint2* win, hipStream_t s);
 
 // ---- plain FP32 PSF convolution (SensorInstance{1D,2D}.cu) ------------------
 struct ConvArgs {

@@ -92,6 +92,7 @@ struct Sensor {
     int check_plain_limits() const;
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();
+    int grid_row_map(RowMap* m);    // grid (MFMA) form row table and G-row map
     int compute(int form, hipStream_t s);
     int compute_grid_form(int form, hipStream_t s);
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
