@@ -1,0 +1,91 @@
+"""Golden fixtures (tests/golden/*.npz, written by tests/golden/make_golden.py):
+complete small cases with the oracle's expected streamed records, Binner
+chain (bit-exact integers) and FP64 plain-form ping.
+
+CPU tests pin the oracle to its own fixtures (any change to the oracle's
+arithmetic shows here) and check structural invariants of the stored results;
+GPU tests run the HIP path through the C ABI on the same inputs."""
+import numpy as np
+import pytest
+
+import oracle as O
+from golden import fixtures as G
+
+PLAIN_TOL = 1e-5
+MFMA_TOL = 1e-3
+
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_oracle_reproduces_golden(name):
+    cfg, hits, exp = G.load(name)
+    rec64 = O.stream(cfg, hits)
+    assert np.array_equal(rec64, exp["records64"])
+    b = O.binning(cfg, exp["records32"])
+    for k in ("order", "keys"):
+        assert np.array_equal(b[k].astype(np.uint32), exp[k]), k
+    assert np.array_equal(b["segments"].astype(np.uint32), exp["segments"])
+    assert np.array_equal(b["windows"].astype(np.uint32), exp["windows"])
+    assert b["overlap"] == int(exp["overlap"])
+    ping = O.conv(cfg, b["sorted"], b["windows"])
+    assert O.rel_l2(ping, exp["ping"]) <= 1e-12
+
+
+@pytest.mark.parametrize("name", G.NAMES)
+def test_golden_invariants(name):
+    cfg, hits, exp = G.load(name)
+    rec = exp["records32"]
+    order, keys, seg, win = exp["order"], exp["keys"], exp["segments"], exp["windows"]
+    n, N = len(order), cfg.range_count
+    # the order is a permutation, stable in travel
+    assert np.array_equal(np.sort(order), np.arange(n))
+    t = rec[order, 2]
+    img = np.where(t < 0, ~t.view(np.uint32), t.view(np.uint32) | 0x80000000)
+    assert np.all(np.diff(img.astype(np.int64)) >= 0)
+    tie = np.diff(img.astype(np.int64)) == 0
+    assert np.all(np.diff(order.astype(np.int64))[tie] > 0), "equal travels keep input order"
+    # in-range keys are non-decreasing and each bin's samples form its segment
+    inr = keys != 0xFFFFFFFF
+    assert np.all(np.diff(keys[inr].astype(np.int64)) >= 0)
+    seg = seg.reshape(N, 2)
+    cnt = np.bincount(keys[inr], minlength=N)
+    assert np.array_equal(seg[:, 1] - seg[:, 0], cnt)
+    for k in np.nonzero(cnt)[0][:50]:
+        assert np.all(keys[seg[k, 0]:seg[k, 1]] == k)
+    # windows: brute force of the empty-window rule on the stored segments
+    ov = int(exp["overlap"])
+    win = win.reshape(N, 2)
+    ne = seg[:, 1] != seg[:, 0]
+    for r in range(N):
+        lo, hi = max(0, r - ov), min(N - 1, r + ov)
+        idx = np.nonzero(ne[lo:hi + 1])[0] + lo
+        if len(idx) < 2:
+            assert tuple(win[r]) == (0, 0)
+        else:
+            assert tuple(win[r]) == (seg[idx[0], 0], seg[idx[-1], 1] - seg[idx[0], 0])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", G.NAMES)
+def test_gpu_matches_golden(gpu, name):
+    R = gpu
+    cfg, hits, exp = G.load(name)
+    s = R.make_sensor(cfg)
+    s.set_sample_count(hits.n)
+    # streaming kernel vs the FP64 records
+    s.write_samples(hits.to_device())
+    rec = s.samples()
+    ref = exp["records64"]
+    assert np.array_equal(rec[:, 2], ref[:, 2].astype(np.float32))
+    assert O.rel_l2(rec[:, 0] + 1j * rec[:, 1], ref[:, 0] + 1j * ref[:, 1]) < 2e-6
+    # Binner chain and ping on the fixture's own FP32 records
+    s.upload_samples(exp["records32"])
+    forms = [0] if cfg.kind == 1 else [0, 1, 2]
+    for form in forms:
+        s.compute_output(form)
+        d = s.binner_debug()
+        assert np.array_equal(d["order"], exp["order"]), form
+        assert np.array_equal(d["keys"], exp["keys"]), form
+        assert np.array_equal(d["segments"].ravel(), exp["segments"].ravel()), form
+        assert np.array_equal(d["windows"].ravel(), exp["windows"].ravel()), form
+        err = O.rel_l2(s.output(), exp["ping"])
+        assert err <= (PLAIN_TOL if form == 0 else MFMA_TOL), (form, err)
