@@ -120,6 +120,111 @@ def accuracy(cfg, sensor, plain_ping, form_pings):
     return exact, errs
 
 
+# Algorithmic work per stage and ping (what `roofline.achieved` divides):
+# bytes per sample / per range bin / per grid cell, MFMA flops of the
+# correlation.
+STAGE_BYTES = {
+    "stream": lambda S, cfg, g: 40.0 * S,                    # r 4 + dir 12 + A 8 in, 16 out
+    "sort": lambda S, cfg, g: 32.0 * S,                      # one read + one write of records
+    "bins": lambda S, cfg, g: 4.0 * S + 16.0 * cfg.H,        # keys read, segments + windows
+    "deposit": lambda S, cfg, g: 16.0 * S + 8.0 * g,         # samples in, grid out
+}
+
+
+def grid_cells(cfg) -> float:
+    """Cells of the extended grid the deposit writes (rows and columns padded
+    to 64, plus the PSF halo)."""
+    hp = -(-cfg.H // 64) * 64
+    wp = -(-cfg.W // 64) * 64
+    kb = (cfg.Tb + 15 + 31) // 32
+    return float((hp + cfg.Tr - 1) * (wp - 16 + 32 * kb))
+
+
+def stage_roofline(stage, ms, S, cfg, dtype):
+    if stage == "convolution":
+        flops = 8.0 * cfg.W * cfg.H * cfg.Tr * cfg.Tb
+        ach = flops / (ms * 1e-3) / 1e12
+        peak = PEAK_MFMA_TFLOPS[dtype]
+        return {"stage": stage, "bound": "mfma", "achieved": ach, "peak": peak,
+                "unit": "TFLOP/s", "frac": ach / peak,
+                "algorithmic": f"8*W*H*Tr*Tb = {flops:.4g} flop per launch"}
+    nbytes = STAGE_BYTES[stage](S, cfg, grid_cells(cfg))
+    ach = nbytes / (ms * 1e-3) / 1e9
+    return {"stage": stage, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{nbytes:.4g} B per ping"}
+
+
+def measured_traffic(workload: str, form: str, stage: str):
+    """HBM bytes per ping of a stage from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE per dispatch, scaled by the copy-kernel
+    calibration), or None when no summary covers it."""
+    path = ROOT / "profiles" / f"pmc_traffic_{workload}.json"
+    try:
+        d = json.loads(path.read_text())
+        return float(d["forms"][form]["stages"][stage]["bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def bench_batch(R, configs, stream, steps):
+    """The 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 2D and
+    8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch:
+    MFMA FP16 form for the 2D sensors, plain form for the 1D ones.  Hits are
+    prefixes of two 4M-hit sets (one per scene)."""
+    items = configs.batch_uneven()
+    big = {}
+    sensors, forms, total = [], [], 0
+    for cfg, scene, S, seed in items:
+        key = cfg.kind
+        if key not in big:
+            big[key] = R.generate_hits(scene, seed, 4_000_000,
+                                       nthreads=min(16, os.cpu_count() or 1)).to_device()
+        s = R.make_sensor(cfg)
+        s.set_sample_count(S)
+        sensors.append(s)
+        forms.append(R.FORM_MFMA_FP16 if cfg.kind == 2 else R.FORM_PLAIN_FP32)
+        total += S
+    b = R.Batch(sensors, forms)
+    for i, (cfg, scene, S, seed) in enumerate(items):
+        b.bind_hits(i, big[cfg.kind].prefix(S))
+    b.compute(stream)                       # capture + first replay
+    stream.sync()
+    timer = R.Timer()
+    timer.start(stream)
+    for _ in range(steps):
+        b.compute(stream)
+    timer.stop(stream)
+    stream.sync()
+    ms = timer.elapsed_ms() / steps
+    return {"workload": "64 uneven pings (16x 256x512, 32x 512x1024, 8x 1024x2048, "
+                        "8x 1D 2048), one hipGraph launch", "pings": len(items),
+            "samples": int(total), "ms_per_batch": ms, "pings_per_s": len(items) / (ms * 1e-3),
+            "msamples_per_s": total / (ms * 1e-3) / 1e6, "steps": steps}
+
+
+def bench_large(R, configs, stream, steps):
+    """The 1024 x 4096 ping from 16M samples (63 x 33 PSF): MFMA FP16 after
+    autotuning against one plain-form ping."""
+    cfg, scene, S, seed = configs.workload("fls_1024x4096")
+    dh = R.generate_hits(scene, seed, S, nthreads=min(16, os.cpu_count() or 1)).to_device()
+    s = R.make_sensor(cfg)
+    s.set_sample_count(S)
+    s.write_samples(dh, stream=stream)
+    tile = s.autotune(R.FORM_MFMA_FP16, 5, stream)[0]
+    ms_mfma = time_form(R, s, dh, stream, R.FORM_MFMA_FP16, steps, 1)
+    mf = s.output()
+    ms_plain = time_form(R, s, dh, stream, R.FORM_PLAIN_FP32, 1, 0)
+    pl = s.output()
+    import oracle as O
+    return {"workload": f"fls_1024x4096: {cfg.W} bearings x {cfg.H} ranges, {S} samples, "
+                        f"{cfg.Tr}x{cfg.Tb} PSF", "samples": S, "tile_shape": tile,
+            "mfma_fp16": {"ms_per_ping": ms_mfma, "pings_per_s": 1e3 / ms_mfma,
+                          "msamples_per_s": S / (ms_mfma * 1e-3) / 1e6},
+            "plain_fp32": {"ms_per_ping": ms_plain, "pings_per_s": 1e3 / ms_plain},
+            "mfma_vs_plain_speedup": ms_plain / ms_mfma,
+            "mfma_vs_plain_rel_l2": O.rel_l2(mf, pl)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -128,6 +233,8 @@ def main():
     ap.add_argument("--workload", default="fls_512x1024")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle accuracy check")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the batched and 1024x4096 workloads")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -149,10 +256,7 @@ def main():
 
     forms = {"plain_fp32": R.FORM_PLAIN_FP32, "mfma_bf16": R.FORM_MFMA_BF16,
              "mfma_fp16": R.FORM_MFMA_FP16}
-    ms = {}
-    stages = {}
-    pings = {}
-    tiles = {}
+    ms, stages, pings, tiles = {}, {}, {}, {}
     for name, form in forms.items():
         if form != R.FORM_PLAIN_FP32:
             # pick the correlation tile shape once, outside the timed region
@@ -177,30 +281,32 @@ def main():
     t = ms[best]
     value = 1e3 / t
 
-    # roofline of the dominant kernel of the headline form
+    # rooflines of every stage of the headline form; the dominant one leads
     st = stages[best]
-    dominant = max(st, key=lambda k: st[k])
-    W, H, Tr, Tb = cfg.W, cfg.H, cfg.Tr, cfg.Tb
-    if dominant == "convolution":
-        flops = 8.0 * W * H * Tr * Tb
-        achieved = flops / (st[dominant] * 1e-3) / 1e12
-        roof = {"kernel": "k_correlate (MFMA complex Toeplitz GEMM)", "bound": "mfma",
-                "achieved": achieved, "peak": PEAK_MFMA_TFLOPS[dtype], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_MFMA_TFLOPS[dtype], "traffic": None,
-                "algorithmic": f"8*W*H*Tr*Tb = {flops:.4g} flop per launch"}
-    else:
-        per_unit = {"stream": 40.0, "sort": 32.0, "bins": 32.0, "deposit": 16.0}[dominant]
-        nbytes = per_unit * S
-        achieved = nbytes / (st[dominant] * 1e-3) / 1e9
-        roof = {"kernel": dominant, "bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS,
-                "unit": "GB/s", "frac": achieved / PEAK_HBM_GBS, "traffic": None,
-                "algorithmic": f"{per_unit:.0f} B/sample x {S} samples per launch"}
+    per_stage = {k: stage_roofline(k, v, S, cfg, dtype) for k, v in st.items() if v > 0}
+    dominant = max(per_stage, key=lambda k: st[k])
+    roof = dict(per_stage[dominant])
+    traffic = measured_traffic(args.workload, best, dominant)
+    roof["traffic"] = traffic
+    roof["kernel"] = {"stream": "k_stream", "sort": "k_sort_hist + k_sort_scan + k_sort_scatter "
+                      "(4 passes)", "bins": "k_bounds + k_windows", "deposit": "k_deposit",
+                      "convolution": "k_correlate"}[dominant]
+    roof["measured"] = "HIP events around the stage on the compute stream, mean over " \
+                       f"{max(5, args.steps // 5)} pings"
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, sensor.samples())
         print(f"# cpu baseline {cpu}", file=sys.stderr, flush=True)
 
+    extra = {}
+    if not args.no_extra:
+        extra["batch_uneven"] = bench_batch(R, configs, stream, 3)
+        print(f"# batch {extra['batch_uneven']}", file=sys.stderr, flush=True)
+        extra["fls_1024x4096"] = bench_large(R, configs, stream, 5)
+        print(f"# large {extra['fls_1024x4096']}", file=sys.stderr, flush=True)
+
+    W, H, Tr, Tb = cfg.W, cfg.H, cfg.Tr, cfg.Tb
     out = {
         "metric": "pings/s + Msamples/s, 512x1024 ping, 1M samples; MFMA vs plain at <=1e-3 L2",
         "value": value,
@@ -226,7 +332,9 @@ def main():
         "mfma_vs_plain_speedup": ms["plain_fp32"] / t,
         "binning_bit_exact": exact,
         "roofline": roof,
+        "stage_rooflines": per_stage,
         "cpu_baseline": cpu,
+        "extra": extra or None,
     }
     print(json.dumps(out), flush=True)
 

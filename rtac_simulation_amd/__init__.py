@@ -292,6 +292,15 @@ class DeviceHits:
                      (h.range, h.dir_x, h.dir_y, h.dir_z, h.amplitude)]
         self.struct = Hits(*[b.ptr for b in self.bufs])
 
+    def prefix(self, n: int) -> "DeviceHits":
+        """The first n hits, sharing this object's device buffers (with the
+        counter-based generator these are exactly the hits a run of n gives)."""
+        v = DeviceHits.__new__(DeviceHits)
+        v.n = int(min(n, self.n))
+        v.bufs = self.bufs
+        v.struct = self.struct
+        return v
+
 
 def generate_hits(scene: SceneDesc, seed: int, n: int, first: int = 0,
                   nthreads: int = 8) -> HostHits:
