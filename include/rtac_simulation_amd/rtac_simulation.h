@@ -241,6 +241,14 @@ class SensorInstance : public std::enable_shared_from_this<SensorInstance> {
     {
         check(rtsim_sensor_compute(h_, (int)form, stream));
     }
+    // MFMA forms: time every correlation tile shape, keep the fastest; returns
+    // the shape index (no reference counterpart; synchronises the stream)
+    int autotune(Form form, int iterations = 10, void* stream = nullptr)
+    {
+        int shape = -1;
+        check(rtsim_sensor_autotune(h_, (int)form, iterations, stream, &shape, nullptr));
+        return shape;
+    }
     // ping as host data, row-major [range][bearing]
     std::vector<Complex<float>> output_host(void* stream = nullptr) const
     {
