@@ -4,9 +4,11 @@
  * The reference computes one ping per SensorInstance::compute_output call
  * (SensorInstance.h:67) and synchronises the device after every kernel
  * (Binner.cu:78,118,134, SensorInstance2D.cu:100,117).  A batch computes many
- * sensor instances (mixed 1D / 2D, uneven sizes, per-sensor form) from one
- * hipGraph launch; the graph is captured on the first compute and again only
- * after a sensor's ranges or sample count change.
+ * sensor instances (mixed 1D / 2D, uneven sizes, per-sensor form) together:
+ * each stage of the path (stream, the four sort passes, segments, windows,
+ * the convolutions) is ONE launch covering the workgroups of every ping, and
+ * the ~20 launches are captured into one hipGraph on the first compute and
+ * again only after a sensor's ranges, sample count or tile shape change.
  */
 #ifndef RTSIM_BATCH_H
 #define RTSIM_BATCH_H

@@ -22,10 +22,13 @@ LIB = PKG / "librtsim.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation (see DESIGN.md).
+# -packed-fp32-ops: no v_pk_{add,mul,fma}_f32 generation.
+# -ffp-contract=on: a * b + c is fused only as written in the source, so a
+# kernel body inlined into its single-ping and batched launches rounds the
+# same way in both (the default lets the backend pick which product to fuse).
 COMMON = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
-    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops", "-ffp-contract=on",
     "-Wall", "-Wno-unused-result",
     f"-I{ROOT / 'include'}", f"-I{CSRC}",
 ]

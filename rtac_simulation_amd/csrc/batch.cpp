@@ -1,16 +1,25 @@
 // Batched pings: many sensor instances (mixed 1D / 2D, uneven sizes, either
-// form) computed from one hipGraph launch.  The reference computes one ping
-// per call with a device synchronisation after every kernel
-// (SensorInstance2D.cu:100,117, Binner.cu:78,118,134); here the whole batch
-// (optionally including each ping's streaming kernel) is captured once into a
-// graph and replayed, and re-captured only when a sensor's configuration or
-// sample count changes.
+// form) computed together.  The reference computes one ping per call with a
+// device synchronisation after every kernel (SensorInstance2D.cu:100,117,
+// Binner.cu:78,118,134).  Here every stage of the path runs ONCE for the whole
+// batch: one launch covers the workgroups of all pings (each workgroup finds
+// its ping in a prefix table of workgroup counts and runs the single-ping
+// kernel body with that ping's arguments), so a batch of 64 pings costs the
+// ~20 launches of one ping instead of ~1000 small ones.  The launches are
+// captured once into a hipGraph and replayed; the per-stage descriptor
+// arrays and the graph are rebuilt only when a sensor's configuration,
+// sample count or tile shape changes.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <utility>
 #include <vector>
 
 #include "../../include/rtsim.h"
 #include "../../include/rtsim_batch.h"
+#include "grid_internal.h"
 #include "rtsim_internal.h"
 #include "sensor.h"
 
@@ -25,45 +34,224 @@ struct BatchEntry {
     uint64_t generation = 0;
 };
 
+// Host image of one batched launch: the arguments of the pings it covers and
+// their workgroup prefix, placed in the batch's device buffer.
+template <typename A>
+struct Stage {
+    std::vector<A> args;
+    std::vector<uint32_t> first{0};
+    size_t args_off = 0, first_off = 0;
+    size_t shm = 0;
+    int variant = 0;   // kernel instantiation (correlation: form and shape)
+
+    void add(const A& a, uint32_t blocks)
+    {
+        if (!blocks) return;
+        args.push_back(a);
+        first.push_back(first.back() + blocks);
+    }
+    uint32_t blocks() const { return first.back(); }
+    uint32_t nping() const { return (uint32_t)args.size(); }
+};
+
 struct Batch {
     std::vector<BatchEntry> entries;
     hipStream_t capture_stream = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool dirty = true;
+    void* d_desc = nullptr;
+    size_t desc_cap = 0;
+
+    Stage<StreamArgs> stream;
+    Stage<HistArgs> hist[4];
+    Stage<ScanArgs> scan[4];
+    Stage<ScatterArgs> scatter[4];
+    Stage<BinArgs> bounds, windows;
+    Stage<ConvArgs> plain2d, plain1d;
+    Stage<DepositArgs> deposit;
+    std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape)
 
     ~Batch()
     {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
+        if (d_desc) (void)hipFree(d_desc);
     }
 };
+
+template <typename A>
+static void layout(Stage<A>& st, size_t& off)
+{
+    off = (off + 15) & ~(size_t)15;
+    st.args_off = off;
+    off += sizeof(A) * st.args.size();
+    off = (off + 15) & ~(size_t)15;
+    st.first_off = off;
+    off += sizeof(uint32_t) * st.first.size();
+}
+
+template <typename A>
+static void pack(const Stage<A>& st, std::vector<char>& host)
+{
+    if (!st.args.empty()) std::memcpy(host.data() + st.args_off, st.args.data(), sizeof(A) * st.args.size());
+    std::memcpy(host.data() + st.first_off, st.first.data(), sizeof(uint32_t) * st.first.size());
+}
+
+template <typename A>
+static const A* dargs(const Batch* b, const Stage<A>& st)
+{
+    return reinterpret_cast<const A*>(static_cast<const char*>(b->d_desc) + st.args_off);
+}
+
+template <typename A>
+static BatchMap dmap(const Batch* b, const Stage<A>& st)
+{
+    return BatchMap{reinterpret_cast<const uint32_t*>(static_cast<const char*>(b->d_desc) + st.first_off),
+                    st.nping()};
+}
+
+// Per-stage descriptors of every ping (no device work is queued).
+static int build(Batch* b)
+{
+    b->stream = {};
+    for (int k = 0; k < 4; k++) {
+        b->hist[k] = {};
+        b->scan[k] = {};
+        b->scatter[k] = {};
+    }
+    b->bounds = {};
+    b->windows = {};
+    b->plain2d = {};
+    b->plain1d = {};
+    b->deposit = {};
+    b->corr.clear();
+    std::map<std::pair<int, int>, size_t> corr_index;
+    for (auto& en : b->entries) {
+        Sensor* s = en.sensor;
+        PingPlan p;
+        int rc = s->plan(en.form, &p);
+        if (rc) return rc;
+        if (en.has_hits && en.nhits) {
+            StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
+            b->stream.add(a, stream_blocks(a));
+        }
+        for (int k = 0; k < 4; k++) {
+            b->hist[k].add(p.sort.hist[k], k == 0 ? p.sort.hist_blocks : p.sort.pass_blocks);
+            b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks ? 256u : 0u);
+            b->scatter[k].add(p.sort.scatter[k], p.sort.pass_blocks);
+        }
+        b->bounds.add(p.bins, bounds_blocks(p.bins));
+        b->windows.add(p.bins, windows_blocks(p.bins));
+        if (!p.grid) {
+            if (s->kind == RTSIM_SENSOR_2D) {
+                b->plain2d.add(p.conv, conv2d_plain_blocks(p.conv));
+                b->plain2d.shm = std::max(b->plain2d.shm, plain2d_lds_bytes(p.conv));
+            } else {
+                b->plain1d.add(p.conv, conv1d_plain_blocks(p.conv));
+            }
+        } else {
+            b->deposit.add(p.dep, deposit_blocks(p.dep));
+            b->deposit.shm = std::max(b->deposit.shm, deposit_lds_bytes(p.dep));
+            const std::pair<int, int> key{en.form, p.corr.shape};
+            auto it = corr_index.find(key);
+            if (it == corr_index.end()) {
+                it = corr_index.emplace(key, b->corr.size()).first;
+                b->corr.emplace_back();
+                b->corr.back().variant = en.form;
+            }
+            Stage<CorrArgs>& cs = b->corr[it->second];
+            cs.add(p.corr, correlate_blocks(p.corr));
+            cs.shm = std::max(cs.shm, correlate_lds_bytes(p.corr));
+            s->last_corr = p.corr;
+        }
+        s->order_ptr = p.sort.order;
+        s->img_ptr = p.sort.img;
+        en.generation = s->generation;
+    }
+    // one device buffer for every descriptor array and prefix table
+    size_t off = 0;
+    layout(b->stream, off);
+    for (int k = 0; k < 4; k++) {
+        layout(b->hist[k], off);
+        layout(b->scan[k], off);
+        layout(b->scatter[k], off);
+    }
+    layout(b->bounds, off);
+    layout(b->windows, off);
+    layout(b->plain2d, off);
+    layout(b->plain1d, off);
+    layout(b->deposit, off);
+    for (auto& c : b->corr) layout(c, off);
+    if (off > b->desc_cap) {
+        if (b->d_desc) (void)hipFree(b->d_desc);
+        b->d_desc = nullptr;
+        b->desc_cap = 0;
+        hipError_t e = hipMalloc(&b->d_desc, off);
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(batch descriptors)");
+        b->desc_cap = off;
+    }
+    std::vector<char> host(off, 0);
+    pack(b->stream, host);
+    for (int k = 0; k < 4; k++) {
+        pack(b->hist[k], host);
+        pack(b->scan[k], host);
+        pack(b->scatter[k], host);
+    }
+    pack(b->bounds, host);
+    pack(b->windows, host);
+    pack(b->plain2d, host);
+    pack(b->plain1d, host);
+    pack(b->deposit, host);
+    for (auto& c : b->corr) pack(c, host);
+    hipError_t e = hipMemcpy(b->d_desc, host.data(), off, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(batch descriptors)");
+    return RTSIM_OK;
+}
+
+// The batch's launches, stage after stage (each covers every ping).
+static hipError_t enqueue(const Batch* b, hipStream_t s)
+{
+    hipError_t e;
+#define RT_TRY(x) if ((e = (x)) != hipSuccess) return e
+    RT_TRY(launch_stream_batch(dargs(b, b->stream), dmap(b, b->stream), b->stream.blocks(), s));
+    for (int k = 0; k < 4; k++) {
+        RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]),
+                                      b->hist[k].blocks(), s));
+        RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
+        RT_TRY(launch_sort_scatter_batch(k, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
+                                         b->scatter[k].blocks(), s));
+    }
+    RT_TRY(launch_bounds_batch(dargs(b, b->bounds), dmap(b, b->bounds), b->bounds.blocks(), s));
+    RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
+    RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
+                                     b->plain2d.shm, s));
+    RT_TRY(launch_conv1d_plain_batch(dargs(b, b->plain1d), dmap(b, b->plain1d), b->plain1d.blocks(), s));
+    RT_TRY(launch_deposit_batch(dargs(b, b->deposit), dmap(b, b->deposit), b->deposit.blocks(),
+                                b->deposit.shm, s));
+    for (const auto& c : b->corr)
+        RT_TRY(launch_correlate_batch(c.variant == RTSIM_FORM_MFMA_FP16, c.args.empty() ? 0 : c.args[0].shape,
+                                      dargs(b, c), dmap(b, c), c.blocks(), c.shm, s));
+#undef RT_TRY
+    return hipSuccess;
+}
 
 static int capture(Batch* b)
 {
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+    int rc = build(b);
+    if (rc) return rc;
+    (void)hipGetLastError();
     hipError_t e = hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return hip_fail(e, "hipStreamBeginCapture");
-    int rc = RTSIM_OK;
-    for (auto& en : b->entries) {
-        Sensor* s = en.sensor;
-        if (en.has_hits && en.nhits) {
-            StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
-            if ((e = launch_stream(a, b->capture_stream)) != hipSuccess) {
-                rc = hip_fail(e, "stream kernel");
-                break;
-            }
-        }
-        if ((rc = s->compute(en.form, b->capture_stream))) break;
-        en.generation = s->generation;
-    }
+    const hipError_t le = enqueue(b, b->capture_stream);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(b->capture_stream, &g);
-    if (rc) {
+    if (le != hipSuccess) {
         if (g) (void)hipGraphDestroy(g);
-        return rc;
+        return hip_fail(le, "batched launch");
     }
     if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
     b->graph = g;
@@ -88,6 +276,11 @@ int rtsim_batch_create(const rtsim_sensor* sensors, const int* forms, uint32_t c
     for (uint32_t i = 0; i < count; i++) {
         Sensor* s = reinterpret_cast<Sensor*>(sensors[i]);
         if (!s) { delete b; return fail(RTSIM_ERR_INVALID, "sensor %u is NULL", i); }
+        for (const auto& en : b->entries)
+            if (en.sensor == s) {
+                delete b;
+                return fail(RTSIM_ERR_INVALID, "sensor %u appears twice in the batch", i);
+            }
         int form = forms ? forms[i] : RTSIM_FORM_PLAIN_FP32;
         // resolve form-specific state now: no allocation may happen under capture
         int rc = s->prepare(form);

@@ -276,10 +276,9 @@ int Sensor::check_plain_limits() const
     return RTSIM_OK;
 }
 
-int Sensor::compute(int form, hipStream_t s)
+int Sensor::plan(int form, PingPlan* p)
 {
     const uint32_t n = (uint32_t)count;
-    (void)hipGetLastError();   // start from a clean error state: launches are checked below
     const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
     int rc = RTSIM_OK;
     if (form == RTSIM_FORM_PLAIN_FP32) {
@@ -287,12 +286,13 @@ int Sensor::compute(int form, hipStream_t s)
     } else if (!grid) {
         return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
     }
+    p->form = form;
+    p->grid = grid;
     RowMap rows{};
     if (grid && (rc = grid_row_map(&rows))) return rc;   // also checks the grid form applies
 
-    // ---- sort (Receiver.cu:28-34); the last pass gathers the records, their
-    //      keys (Binner.cu:24-49) and bearing coordinates ----------------------
-    stage_begin(1, s);
+    // sort (Receiver.cu:28-34); the last pass gathers the records, their keys
+    // (Binner.cu:24-49) and bearing coordinates
     SortBuffers sb;
     sb.keys[0] = d_kA;
     sb.keys[1] = d_kB;
@@ -311,39 +311,55 @@ int Sensor::compute(int form, hipStream_t s)
     sb.out.coord = d_coord;
     sb.out.key = key;
     sb.out.axis = axis;
-    const uint32_t* order = nullptr;
-    const uint32_t* img = nullptr;
-    HIP_TRY(launch_sort(d_samples, n, sb, &order, &img, s));
-    order_ptr = order;
-    img_ptr = img;
-    stage_end(s);
+    hipError_t e = make_sort_plan(d_samples, n, sb, &p->sort);
+    if (e != hipSuccess) return hip_fail(e, "sort plan");
 
-    // ---- Binner::compute (Binner.h:96-102): segments, windows ---------------
-    stage_begin(2, s);
-    BinArgs ba;
+    // Binner::compute (Binner.h:96-102): segments, windows
+    BinArgs& ba = p->bins;
     ba.skeys = d_skeys;
-    ba.img = img;
+    ba.img = p->sort.img;
     ba.n = n;
     ba.N = N;
     ba.ov = overlap;
     ba.seg = d_seg;
     ba.win = d_win;
     ba.rows = rows;
-    HIP_TRY(launch_bins(ba, s));
-    stage_end(s);
 
-    // ---- PSF convolution ------------------------------------------------
-    if (form == RTSIM_FORM_PLAIN_FP32) {
-        ConvArgs c = conv_args();
+    if (grid) return grid_plan(form, &p->dep, &p->corr);
+    p->conv = conv_args();
+    return RTSIM_OK;
+}
+
+int Sensor::compute(int form, hipStream_t s)
+{
+    (void)hipGetLastError();   // start from a clean error state: launches are checked below
+    PingPlan p;
+    int rc = plan(form, &p);
+    if (rc) return rc;
+    stage_begin(1, s);
+    HIP_TRY(launch_sort_plan(p.sort, s));
+    order_ptr = p.sort.order;
+    img_ptr = p.sort.img;
+    stage_end(s);
+    stage_begin(2, s);
+    HIP_TRY(launch_bins(p.bins, s));
+    stage_end(s);
+    if (!p.grid) {
         stage_begin(4, s);
-        if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(c, s));
-        else HIP_TRY(launch_conv1d_plain(c, s));
+        if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(p.conv, s));
+        else HIP_TRY(launch_conv1d_plain(p.conv, s));
         stage_end(s);
     } else {
-        rc = compute_grid_form(form, s);   // stages 3 (deposit) and 4 (correlation)
+        stage_begin(3, s);
+        HIP_TRY(launch_deposit(p.dep, s));
+        stage_end(s);
+        stage_begin(4, s);
+        HIP_TRY(launch_correlate(p.corr, form == RTSIM_FORM_MFMA_FP16, s));
+        stage_end(s);
+        last_corr = p.corr;
     }
-    if (rc == RTSIM_OK) computed = true;
-    return rc;
+    computed = true;
+    return RTSIM_OK;
 }
 
 ConvArgs Sensor::conv_args() const

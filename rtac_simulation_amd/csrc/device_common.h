@@ -71,4 +71,22 @@ __device__ __forceinline__ uint32_t lane_id()
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// Batched launch: the ping of this workgroup (largest p with first[p] <=
+// blockIdx.x, so pings without workgroups are skipped), its local workgroup
+// index and its workgroup count.
+__device__ __forceinline__ uint32_t batch_locate(const BatchMap& m, uint32_t& block,
+                                                 uint32_t& nblocks)
+{
+    const uint32_t bx = blockIdx.x;
+    uint32_t lo = 0, hi = m.nping;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (m.first[mid] <= bx) lo = mid;
+        else hi = mid;
+    }
+    block = bx - m.first[lo];
+    nblocks = m.first[lo + 1] - m.first[lo];
+    return lo;
+}
+
 }  // namespace rtsim

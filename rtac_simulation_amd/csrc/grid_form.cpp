@@ -271,13 +271,13 @@ int Sensor::grid_row_map(RowMap* m)
     return RTSIM_OK;
 }
 
-int Sensor::compute_grid_form(int form, hipStream_t s)
+int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
 {
     int rc = ensure_grid();
     if (rc) return rc;
     GridState* g = static_cast<GridState*>(grid_state);
 
-    DepositArgs d;
+    DepositArgs& d = *dp;
     d.sorted = d_sorted;
     d.coord = d_coord;
     d.skeys = d_skeys;
@@ -295,12 +295,8 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     d.bear0_lo = g->bear0_lo;
     d.inv_db = g->inv_db;
     d.off_b = g->off_b;
-    stage_begin(3, s);
-    hipError_t err = launch_deposit(d, s);
-    stage_end(s);
-    if (err != hipSuccess) return hip_fail(err, "deposit");
 
-    CorrArgs c;
+    CorrArgs& c = *cp;
     c.grid = g->d_grid;
     c.toep = g->d_toep[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
     c.win = d_win;
@@ -317,11 +313,6 @@ int Sensor::compute_grid_form(int form, hipStream_t s)
     c.BL = g->BL;
     c.TCH = g->tch[g->shape];
     c.tap_scale = g->tap_scale;
-    stage_begin(4, s);
-    err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);
-    stage_end(s);
-    if (err != hipSuccess) return hip_fail(err, "correlate");
-    last_corr = c;
     return RTSIM_OK;
 }
 

@@ -6,6 +6,8 @@
 #include <cstddef>
 #include <cstdint>
 
+#include "rtsim_internal.h"
+
 namespace rtsim {
 
 constexpr int kCorrRows = 64;     // largest tile height of the correlation shapes
@@ -48,6 +50,15 @@ struct CorrArgs {
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s);
 hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s);
+// batched launches (args are device arrays, one entry per ping; every ping of
+// a correlation launch uses the same form and tile shape)
+uint32_t deposit_blocks(const DepositArgs& a);
+size_t deposit_lds_bytes(const DepositArgs& a);
+hipError_t launch_deposit_batch(const DepositArgs* args, BatchMap m, uint32_t blocks, size_t shm,
+                                hipStream_t s);
+uint32_t correlate_blocks(const CorrArgs& a);
+hipError_t launch_correlate_batch(bool fp16, int shape, const CorrArgs* args, BatchMap m,
+                                  uint32_t blocks, size_t shm, hipStream_t s);
 size_t correlate_lds_bytes(const CorrArgs& a);
 // Tap chunk for a.shape: all Tr taps when that fits kCorrLdsTarget, else
 // the largest chunk that does, else the largest that fits max_lds; 0 when

@@ -49,10 +49,12 @@ __device__ __forceinline__ float directivity_lookup(const StreamArgs& p, float l
     return top + b * (bot - top);
 }
 
-__global__ __launch_bounds__(256) void k_stream(StreamArgs p)
+constexpr uint32_t kStreamMaxBlocks = 256 * 64;
+
+__device__ __forceinline__ void stream_body(const StreamArgs& p, uint32_t blk, uint32_t nblk)
 {
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.n; i += stride) {
+    const uint32_t stride = nblk * 256;
+    for (uint32_t i = blk * 256 + threadIdx.x; i < p.n; i += stride) {
         float r = p.range[i];
         float4 o;
         if (r < 1.0e-4f) {
@@ -86,12 +88,32 @@ __global__ __launch_bounds__(256) void k_stream(StreamArgs p)
     }
 }
 
+__global__ __launch_bounds__(256) void k_stream(StreamArgs p) { stream_body(p, blockIdx.x, gridDim.x); }
+
+__global__ __launch_bounds__(256) void k_stream_batch(const StreamArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t q = batch_locate(m, blk, nblk);
+    stream_body(args[q], blk, nblk);
+}
+
+uint32_t stream_blocks(const StreamArgs& a)
+{
+    const uint32_t blocks = (a.n + 255) / 256;
+    return blocks > kStreamMaxBlocks ? kStreamMaxBlocks : blocks;
+}
+
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s)
 {
     if (a.n == 0) return hipSuccess;
-    uint32_t blocks = (a.n + 255) / 256;
-    if (blocks > 256 * 64) blocks = 256 * 64;
-    hipLaunchKernelGGL(k_stream, dim3(blocks), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_stream, dim3(stream_blocks(a)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(256), 0, s, args, m);
     return hipGetLastError();
 }
 
@@ -116,9 +138,9 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 // ---------------------------------------------------------------------------
 // segments (Binner.cu:51-80) and the grid row table, over the sorted keys
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_bounds(BinArgs a)
+__device__ __forceinline__ void bounds_body(const BinArgs& a, uint32_t blk)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blk * 256 + threadIdx.x;
     if (i >= a.n) return;
     const uint32_t k = a.skeys[i];
     if (k != kOutOfRange) {
@@ -144,10 +166,11 @@ __global__ __launch_bounds__(256) void k_bounds(BinArgs a)
 // workgroup gives every row the minimum over itself and the rows above it
 // (rows without samples then start where the next row that has some starts)
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_windows(BinArgs a, uint32_t win_blocks)
+__device__ __forceinline__ void windows_body(const BinArgs& a, uint32_t blk)
 {
-    if (blockIdx.x < win_blocks) {
-        const int tid = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t win_blocks = (a.N + 255) / 256;
+    if (blk < win_blocks) {
+        const int tid = blk * 256 + threadIdx.x;
         if (tid >= (int)a.N) return;
         const uint2* seg = a.seg;
         int istart = max(0, tid - a.ov);
@@ -184,12 +207,47 @@ __global__ __launch_bounds__(256) void k_windows(BinArgs a, uint32_t win_blocks)
     }
 }
 
+__global__ __launch_bounds__(256) void k_bounds(BinArgs a) { bounds_body(a, blockIdx.x); }
+__global__ __launch_bounds__(256) void k_windows(BinArgs a) { windows_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_bounds_batch(const BinArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    bounds_body(args[p], blk);
+}
+
+__global__ __launch_bounds__(256) void k_windows_batch(const BinArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    windows_body(args[p], blk);
+}
+
+uint32_t bounds_blocks(const BinArgs& a) { return (a.n + 255) / 256; }
+uint32_t windows_blocks(const BinArgs& a)
+{
+    return (a.N + 255) / 256 + (a.rows.row_start ? 1u : 0u);
+}
+
 hipError_t launch_bins(const BinArgs& a, hipStream_t s)
 {
-    if (a.n) hipLaunchKernelGGL(k_bounds, dim3((a.n + 255) / 256), dim3(256), 0, s, a);
-    const uint32_t wb = (a.N + 255) / 256;
-    const uint32_t blocks = wb + (a.rows.row_start ? 1u : 0u);
-    if (blocks) hipLaunchKernelGGL(k_windows, dim3(blocks), dim3(256), 0, s, a, wb);
+    if (bounds_blocks(a)) hipLaunchKernelGGL(k_bounds, dim3(bounds_blocks(a)), dim3(256), 0, s, a);
+    if (windows_blocks(a)) hipLaunchKernelGGL(k_windows, dim3(windows_blocks(a)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_bounds_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_bounds_batch, dim3(blocks), dim3(256), 0, s, args, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_windows_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_windows_batch, dim3(blocks), dim3(256), 0, s, args, m);
     return hipGetLastError();
 }
 

@@ -141,7 +141,7 @@ __device__ __forceinline__ void copy_toeplitz(const CorrArgs& a, uint16_t* B, in
 }
 
 template <bool FP16, int WR, int NT>
-__global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
+__device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by)
 {
     constexpr int WC = 4 / WR;
     constexpr int ROWS = 16 * WR, COLS = 16 * NT * WC;
@@ -149,8 +149,8 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int wr = wave % WR, wc = wave / WR;
     const int m = lane & 15, kg = lane >> 4;
-    const int r0 = blockIdx.y * ROWS;
-    const int c0 = blockIdx.x * COLS;
+    const int r0 = by * ROWS;
+    const int c0 = bx * COLS;
     const int AR = ROWS + a.Tr - 1;
     const int AC = COLS - 16 + 32 * a.KB;   // columns [c0, c0 + COLS - 16 + 32 KB)
     const int PR = AC / 2;                  // float4 loads per A row
@@ -314,6 +314,23 @@ __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
     }
 }
 
+template <bool FP16, int WR, int NT>
+__global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
+{
+    correlate_body<FP16, WR, NT>(a, blockIdx.x, blockIdx.y);
+}
+
+template <bool FP16, int WR, int NT>
+__global__ __launch_bounds__(256) void k_correlate_batch(const CorrArgs* __restrict__ args, BatchMap m)
+{
+    constexpr int COLS = 16 * NT * (4 / WR);
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    const CorrArgs& a = args[p];
+    const int gx = (a.W + COLS - 1) / COLS;
+    correlate_body<FP16, WR, NT>(a, (int)blk % gx, (int)blk / gx);
+}
+
 // tile shapes: {WR, NT}; rows = 16 WR, cols = 16 NT (4 / WR):
 // 64x32, 32x32, 32x64, 64x16
 static const int kShapes[kCorrShapes][2] = {{4, 2}, {2, 1}, {2, 2}, {4, 1}};
@@ -360,6 +377,34 @@ static void launch_shape(const CorrArgs& a, dim3 grid, size_t shm, hipStream_t s
     }
 }
 
+template <bool FP16>
+static void launch_shape_batch(int shape, const CorrArgs* args, BatchMap m, dim3 grid, size_t shm,
+                               hipStream_t s)
+{
+    switch (shape) {
+    case 0: hipLaunchKernelGGL((k_correlate_batch<FP16, 4, 2>), grid, dim3(256), shm, s, args, m); break;
+    case 1: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 1>), grid, dim3(256), shm, s, args, m); break;
+    case 2: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 2>), grid, dim3(256), shm, s, args, m); break;
+    default: hipLaunchKernelGGL((k_correlate_batch<FP16, 4, 1>), grid, dim3(256), shm, s, args, m); break;
+    }
+}
+
+uint32_t correlate_blocks(const CorrArgs& a)
+{
+    int rows, cols;
+    corr_tile(a.shape, &rows, &cols);
+    return (uint32_t)(((a.W + cols - 1) / cols) * ((a.H + rows - 1) / rows));
+}
+
+hipError_t launch_correlate_batch(bool fp16, int shape, const CorrArgs* args, BatchMap m,
+                                  uint32_t blocks, size_t shm, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    if (fp16) launch_shape_batch<true>(shape, args, m, dim3(blocks), shm, s);
+    else launch_shape_batch<false>(shape, args, m, dim3(blocks), shm, s);
+    return hipGetLastError();
+}
+
 hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s)
 {
     int rows, cols;
@@ -381,6 +426,14 @@ void init_corr_attributes(int max_lds)
     set_dynamic_lds_limit((const void*)k_correlate<false, 2, 1>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<false, 2, 2>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<false, 4, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 4, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 4, 1>, max_lds);
 }
 
 }  // namespace rtsim

@@ -37,12 +37,11 @@ __device__ __forceinline__ float grid_y(const DepositArgs& a, float t)
 // non-finite value yields a NaN row.
 constexpr int kDepositU = 4;   // samples per thread whose loads are in flight together
 
-__global__ __launch_bounds__(256) void k_deposit(DepositArgs a)
+__device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
 {
     extern __shared__ long long sacc[];               // [2][GW]: re plane, im plane
     __shared__ float red[4];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int rho = blockIdx.x;
     const int GW = a.GW;
     unsigned long long* sre = reinterpret_cast<unsigned long long*>(sacc);
     unsigned long long* sim = sre + GW;
@@ -136,16 +135,36 @@ __global__ __launch_bounds__(256) void k_deposit(DepositArgs a)
         row[c] = make_float2(ldexpf((float)sacc[c], -k), ldexpf((float)sacc[GW + c], -k));
 }
 
+__global__ __launch_bounds__(256) void k_deposit(DepositArgs a) { deposit_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_deposit_batch(const DepositArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    deposit_body(args[p], (int)blk);
+}
+
+uint32_t deposit_blocks(const DepositArgs& a) { return (uint32_t)a.GH; }
+size_t deposit_lds_bytes(const DepositArgs& a) { return sizeof(long long) * 2 * a.GW; }
+
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
 {
-    const size_t shm = sizeof(long long) * 2 * a.GW;
-    hipLaunchKernelGGL(k_deposit, dim3(a.GH), dim3(256), shm, s, a);
+    hipLaunchKernelGGL(k_deposit, dim3(a.GH), dim3(256), deposit_lds_bytes(a), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_deposit_batch(const DepositArgs* args, BatchMap m, uint32_t blocks, size_t shm,
+                                hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_deposit_batch, dim3(blocks), dim3(256), shm, s, args, m);
     return hipGetLastError();
 }
 
 void init_grid_attributes(int max_lds)
 {
     set_dynamic_lds_limit((const void*)k_deposit, max_lds);
+    set_dynamic_lds_limit((const void*)k_deposit_batch, max_lds);
     init_corr_attributes(max_lds);
 }
 

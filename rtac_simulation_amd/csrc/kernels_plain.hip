@@ -51,7 +51,7 @@ size_t plain2d_lds_bytes(const ConvArgs& a)
 
 // One workgroup per output range row.  LDS holds the row accumulator, the
 // bearing coordinates and (when it fits) the PSF table.
-__global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
+__device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, int r)
 {
     extern __shared__ float smem[];
     const int W = a.W;
@@ -59,7 +59,6 @@ __global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
     float2* bear = reinterpret_cast<float2*>(smem + 2 * W); // W
     // PSF table in LDS when it fits, else read through the caches
     const float2* psf = a.psf_in_lds ? bear + W : a.psf;
-    const int r = blockIdx.x;
     const int t = threadIdx.x;
     for (int b = t; b < W; b += blockDim.x) {
         acc[2 * b] = 0.f;
@@ -101,6 +100,18 @@ __global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a)
     for (int b = t; b < W; b += blockDim.x) out[b] = make_float2(acc[2 * b], acc[2 * b + 1]);
 }
 
+__global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a) { conv2d_plain_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_conv2d_plain_batch(const ConvArgs* __restrict__ args,
+                                                            BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    conv2d_plain_body(args[p], (int)blk);
+}
+
+uint32_t conv2d_plain_blocks(const ConvArgs& a) { return a.W ? (uint32_t)a.H : 0u; }
+
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
 {
     if (a.H == 0 || a.W == 0) return hipSuccess;
@@ -109,17 +120,26 @@ hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, size_t shm,
+                                     hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_conv2d_plain_batch, dim3(blocks), dim3(256), shm, s, args, m);
+    return hipGetLastError();
+}
+
 void init_plain_attributes(int max_lds)
 {
     set_dynamic_lds_limit((const void*)k_conv2d_plain, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch, max_lds);
 }
 
 // 1D: one wave per output range bin, FP32 partial sums reduced across the
 // wave (the reference reduces a 512-thread block in shared memory).
-__global__ __launch_bounds__(256) void k_conv1d_plain(ConvArgs a)
+__device__ __forceinline__ void conv1d_plain_body(const ConvArgs& a, uint32_t blk)
 {
     const int lane = threadIdx.x & 63;
-    const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int r = (int)blk * 4 + (threadIdx.x >> 6);
     if (r >= a.H) return;
     const uint2 win = a.win[r];
     const float2 rc = a.range_c[r];
@@ -140,10 +160,30 @@ __global__ __launch_bounds__(256) void k_conv1d_plain(ConvArgs a)
     if (lane == 0) a.out[r] = make_float2(ar, ai);
 }
 
+__global__ __launch_bounds__(256) void k_conv1d_plain(ConvArgs a) { conv1d_plain_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_conv1d_plain_batch(const ConvArgs* __restrict__ args,
+                                                            BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    conv1d_plain_body(args[p], blk);
+}
+
+uint32_t conv1d_plain_blocks(const ConvArgs& a) { return (uint32_t)(a.H + 3) / 4; }
+
 hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s)
 {
     if (a.H == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_conv1d_plain, dim3((a.H + 3) / 4), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_conv1d_plain, dim3(conv1d_plain_blocks(a)), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_conv1d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
+                                     hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_conv1d_plain_batch, dim3(blocks), dim3(256), 0, s, args, m);
     return hipGetLastError();
 }
 

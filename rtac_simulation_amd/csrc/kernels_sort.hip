@@ -74,43 +74,52 @@ __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
 
 // ---- per-tile digit counts --------------------------------------------------
 template <bool FIRST>
-__global__ __launch_bounds__(256) void k_sort_hist(const float4* __restrict__ rec,
-                                                   const uint32_t* __restrict__ kin, uint32_t n,
-                                                   int shift, uint32_t* __restrict__ counts,
-                                                   uint32_t tiles, uint32_t* __restrict__ zero,
-                                                   size_t zero_count, uint32_t* __restrict__ fill,
-                                                   uint32_t fill_value, size_t fill_count)
+__device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, uint32_t nblk)
 {
     __shared__ uint32_t h[256];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     h[t] = 0u;
     if (FIRST) {
-        for (size_t i = (size_t)blockIdx.x * 256 + t; i < zero_count; i += (size_t)gridDim.x * 256)
-            zero[i] = 0u;
-        for (size_t i = (size_t)blockIdx.x * 256 + t; i < fill_count; i += (size_t)gridDim.x * 256)
-            fill[i] = fill_value;
+        for (size_t i = (size_t)blk * 256 + t; i < a.zero_count; i += (size_t)nblk * 256)
+            a.zero[i] = 0u;
+        for (size_t i = (size_t)blk * 256 + t; i < a.fill_count; i += (size_t)nblk * 256)
+            a.fill[i] = a.fill_value;
     }
-    if (n == 0) return;
-    const uint32_t wbase = blockIdx.x * kSortTile + w * (64 * kSortLaneItems);
+    if (a.n == 0 || blk >= a.tiles) return;
+    const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems];
-    load_tile<FIRST>(rec, kin, nullptr, n, wbase, lane, key, nullptr);
+    load_tile<FIRST>(a.rec, a.kin, nullptr, a.n, wbase, lane, key, nullptr);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++)
-        if (wbase + j * 64 + lane < n) atomicAdd(&h[(key[j] >> shift) & 255u], 1u);
+        if (wbase + j * 64 + lane < a.n) atomicAdd(&h[(key[j] >> a.shift) & 255u], 1u);
     __syncthreads();
-    counts[(size_t)t * tiles + blockIdx.x] = h[t];
+    a.counts[(size_t)t * a.tiles + blk] = h[t];
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sort_hist(HistArgs a)
+{
+    sort_hist_body<FIRST>(a, blockIdx.x, gridDim.x);
+}
+
+template <bool FIRST>
+__global__ __launch_bounds__(256) void k_sort_hist_batch(const HistArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    sort_hist_body<FIRST>(args[p], blk, nblk);
 }
 
 // ---- one workgroup per digit: exclusive scan over tiles, digit total ---------
 constexpr int kScanPer = 32;   // tiles per thread: up to 8192 tiles (16.7M items)
 
-__global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ counts, uint32_t tiles,
-                                                   uint32_t* __restrict__ totals)
+__device__ __forceinline__ void sort_scan_body(const ScanArgs& a, uint32_t digit)
 {
     __shared__ uint32_t ws[4];
     const uint32_t t = threadIdx.x;
-    uint32_t* row = counts + (size_t)blockIdx.x * tiles;
+    const uint32_t tiles = a.tiles;
+    uint32_t* row = a.counts + (size_t)digit * tiles;
     const uint32_t per = (tiles + 255) / 256;
     uint32_t v[kScanPer];
     uint32_t s = 0;
@@ -128,26 +137,21 @@ __global__ __launch_bounds__(256) void k_sort_scan(uint32_t* __restrict__ counts
         if (k < (int)per && idx < tiles) row[idx] = run;
         run += v[k];
     }
-    if (t == 0) totals[blockIdx.x] = total;
+    if (t == 0) a.totals[digit] = total;
+}
+
+__global__ __launch_bounds__(256) void k_sort_scan(ScanArgs a) { sort_scan_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(256) void k_sort_scan_batch(const ScanArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    sort_scan_body(args[p], blk);
 }
 
 // ---- rank + stage + scatter ----------------------------------------------------
-struct ScatterArgs {
-    const float4* rec;        // pass 0 reads the records themselves
-    const uint32_t* kin;
-    const uint32_t* vin;
-    uint32_t* kout;
-    uint32_t* vout;
-    uint32_t n;
-    int shift;
-    const uint32_t* counts;   // scanned [digit][tile]
-    const uint32_t* totals;
-    uint32_t tiles;
-    SortOutputs out;          // last pass only
-};
-
 template <bool FIRST, bool LAST>
-__global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
+__device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t tile)
 {
     __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
     __shared__ uint32_t toff[256];               // digit run offsets in the staging tile
@@ -156,7 +160,6 @@ __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t tile = blockIdx.x;
     const uint32_t tb = tile * kSortTile;
     const int shift = a.shift;
 
@@ -265,54 +268,121 @@ __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
     }
 }
 
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
+{
+    sort_scatter_body<FIRST, LAST>(a, blockIdx.x);
+}
+
+template <bool FIRST, bool LAST>
+__global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* __restrict__ args,
+                                                            BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    sort_scatter_body<FIRST, LAST>(args[p], blk);
+}
+
+hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p)
+{
+    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
+    if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
+    int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
+    for (int pass = 0; pass < 4; pass++) {
+        HistArgs& h = p->hist[pass];
+        h.rec = records;
+        h.kin = b.keys[cur];
+        h.n = n;
+        h.shift = 8 * pass;
+        h.counts = b.counts;
+        h.tiles = tiles;
+        h.zero = pass == 0 ? b.zero : nullptr;
+        h.zero_count = pass == 0 ? b.zero_count : 0;
+        h.fill = pass == 0 ? b.fill : nullptr;
+        h.fill_value = b.fill_value;
+        h.fill_count = pass == 0 ? b.fill_count : 0;
+        p->scan[pass] = ScanArgs{b.counts, tiles, b.totals};
+        ScatterArgs& c = p->scatter[pass];
+        c.rec = records;
+        c.kin = b.keys[cur];
+        c.vin = b.vals[cur];
+        c.kout = b.keys[cur ^ 1];
+        c.vout = b.vals[cur ^ 1];
+        c.n = n;
+        c.shift = 8 * pass;
+        c.counts = b.counts;
+        c.totals = b.totals;
+        c.tiles = tiles;
+        c.out = b.out;
+        cur ^= 1;
+    }
+    const bool clears = b.zero_count || b.fill_count;
+    p->hist_blocks = n ? tiles : (clears ? 1u : 0u);
+    p->pass_blocks = n ? tiles : 0u;
+    p->order = b.vals[n ? cur : 0];
+    p->img = b.keys[n ? cur : 0];
+    return hipSuccess;
+}
+
+hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
+{
+    for (int pass = 0; pass < 4; pass++) {
+        const uint32_t hb = pass == 0 ? p.hist_blocks : p.pass_blocks;
+        if (hb) {
+            if (pass == 0)
+                hipLaunchKernelGGL(k_sort_hist<true>, dim3(hb), dim3(256), 0, s, p.hist[pass]);
+            else
+                hipLaunchKernelGGL(k_sort_hist<false>, dim3(hb), dim3(256), 0, s, p.hist[pass]);
+        }
+        if (!p.pass_blocks) continue;
+        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, p.scan[pass]);
+        const dim3 g(p.pass_blocks);
+        if (pass == 0)
+            hipLaunchKernelGGL((k_sort_scatter<true, false>), g, dim3(256), 0, s, p.scatter[pass]);
+        else if (pass < 3)
+            hipLaunchKernelGGL((k_sort_scatter<false, false>), g, dim3(256), 0, s, p.scatter[pass]);
+        else
+            hipLaunchKernelGGL((k_sort_scatter<false, true>), g, dim3(256), 0, s, p.scatter[pass]);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
                        const uint32_t** order_out, const uint32_t** img_out, hipStream_t s)
 {
-    *order_out = b.vals[0];
-    *img_out = b.keys[0];
-    if (n == 0) {
-        if (b.zero_count || b.fill_count)
-            hipLaunchKernelGGL(k_sort_hist<true>, dim3(1), dim3(256), 0, s, records, nullptr, 0u,
-                               0, b.counts, 1u, b.zero, b.zero_count, b.fill, b.fill_value,
-                               b.fill_count);
-        return hipGetLastError();
-    }
-    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
-    ScatterArgs a;
-    a.rec = records;
-    a.n = n;
-    a.counts = b.counts;
-    a.totals = b.totals;
-    a.tiles = tiles;
-    a.out = b.out;
-    int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
-    for (int pass = 0; pass < 4; pass++) {
-        const int shift = 8 * pass;
-        a.shift = shift;
-        a.kin = b.keys[cur];
-        a.vin = b.vals[cur];
-        a.kout = b.keys[cur ^ 1];
-        a.vout = b.vals[cur ^ 1];
-        if (pass == 0)
-            hipLaunchKernelGGL(k_sort_hist<true>, dim3(tiles), dim3(256), 0, s, records, nullptr, n,
-                               shift, b.counts, tiles, b.zero, b.zero_count, b.fill,
-                               b.fill_value, b.fill_count);
-        else
-            hipLaunchKernelGGL(k_sort_hist<false>, dim3(tiles), dim3(256), 0, s, records,
-                               (const uint32_t*)b.keys[cur], n, shift, b.counts, tiles,
-                               (uint32_t*)nullptr, (size_t)0, (uint32_t*)nullptr, 0u, (size_t)0);
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, b.counts, tiles, b.totals);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_sort_scatter<true, false>), dim3(tiles), dim3(256), 0, s, a);
-        else if (pass < 3)
-            hipLaunchKernelGGL((k_sort_scatter<false, false>), dim3(tiles), dim3(256), 0, s, a);
-        else
-            hipLaunchKernelGGL((k_sort_scatter<false, true>), dim3(tiles), dim3(256), 0, s, a);
-        cur ^= 1;
-    }
-    *order_out = b.vals[cur];
-    *img_out = b.keys[cur];
+    SortPlan p;
+    hipError_t e = make_sort_plan(records, n, b, &p);
+    if (e != hipSuccess) return e;
+    *order_out = p.order;
+    *img_out = p.img;
+    return launch_sort_plan(p, s);
+}
+
+hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
+                                  uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    if (first_pass) hipLaunchKernelGGL(k_sort_hist_batch<true>, dim3(blocks), dim3(256), 0, s, args, m);
+    else hipLaunchKernelGGL(k_sort_hist_batch<false>, dim3(blocks), dim3(256), 0, s, args, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_sort_scan_batch, dim3(blocks), dim3(256), 0, s, args, m);
+    return hipGetLastError();
+}
+
+hipError_t launch_sort_scatter_batch(int pass, const ScatterArgs* args, BatchMap m, uint32_t blocks,
+                                     hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    const dim3 g(blocks);
+    if (pass == 0) hipLaunchKernelGGL((k_sort_scatter_batch<true, false>), g, dim3(256), 0, s, args, m);
+    else if (pass < 3)
+        hipLaunchKernelGGL((k_sort_scatter_batch<false, false>), g, dim3(256), 0, s, args, m);
+    else hipLaunchKernelGGL((k_sort_scatter_batch<false, true>), g, dim3(256), 0, s, args, m);
     return hipGetLastError();
 }
 

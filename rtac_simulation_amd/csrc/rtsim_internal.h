@@ -17,6 +17,16 @@ struct KeyParams {
     float a, b, lower, upper;
 };
 
+// ---- batched launches ---------------------------------------------------------
+// One launch covers the work of many pings: first[p] is the first workgroup of
+// ping p (exclusive prefix of workgroups per ping, nping + 1 entries) and each
+// workgroup finds its ping by binary search, then runs the single-ping body
+// with that ping's arguments and its local workgroup index.
+struct BatchMap {
+    const uint32_t* first;
+    uint32_t nping;
+};
+
 // ---- streaming kernel: ReceiverView::set_sample + raygen phase -------------
 struct StreamArgs {
     const float* range;
@@ -35,6 +45,8 @@ struct StreamArgs {
     float dir_xs, dir_ys;
 };
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
+uint32_t stream_blocks(const StreamArgs& a);
+hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
 constexpr uint32_t kSortTile = 2048;     // items per sort workgroup (256 lanes x 8)
@@ -66,11 +78,62 @@ struct SortBuffers {
     size_t fill_count;
     SortOutputs out;
 };
+// One pass of the sort: per-tile digit counts, their scan, the scatter.
+struct HistArgs {
+    const float4* rec;        // pass 0 reads the records themselves
+    const uint32_t* kin;
+    uint32_t n;
+    int shift;
+    uint32_t* counts;         // [256 digits][tiles]
+    uint32_t tiles;
+    uint32_t* zero;           // pass 0 only: cleared / filled arrays
+    size_t zero_count;
+    uint32_t* fill;
+    uint32_t fill_value;
+    size_t fill_count;
+};
+struct ScanArgs {
+    uint32_t* counts;
+    uint32_t tiles;
+    uint32_t* totals;
+};
+struct ScatterArgs {
+    const float4* rec;
+    const uint32_t* kin;
+    const uint32_t* vin;
+    uint32_t* kout;
+    uint32_t* vout;
+    uint32_t n;
+    int shift;
+    const uint32_t* counts;   // scanned [digit][tile]
+    const uint32_t* totals;
+    uint32_t tiles;
+    SortOutputs out;          // last pass only
+};
+// Arguments and workgroup counts of the 12 launches of one sort.
+struct SortPlan {
+    HistArgs hist[4];
+    ScanArgs scan[4];
+    ScatterArgs scatter[4];
+    uint32_t hist_blocks;     // per pass (pass 0: >= 1 when it clears arrays)
+    uint32_t pass_blocks;     // scan: 256 per pass, scatter: tiles (0 when n = 0)
+    const uint32_t* order;    // permutation after the sort
+    const uint32_t* img;      // sorted travel images
+};
+hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p);
+hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
 // Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
 // buffers that hold the permutation and the sorted travel images.
 hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
                        const uint32_t** order_out, const uint32_t** img_out,
                        hipStream_t s);
+// batched sort passes (args are device arrays, one entry per ping)
+hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
+                                  uint32_t blocks, hipStream_t s);
+hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks,
+                                  hipStream_t s);
+hipError_t launch_sort_scatter_batch(int pass, const ScatterArgs* args, BatchMap m,
+                                     uint32_t blocks, hipStream_t s);
 
 // G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
 // row_start[rho + 1] = first sorted sample whose G row is >= rho, rho in [-1, GH + 1].
@@ -95,6 +158,10 @@ struct BinArgs {
     RowMap rows;
 };
 hipError_t launch_bins(const BinArgs& a, hipStream_t s);
+uint32_t bounds_blocks(const BinArgs& a);
+uint32_t windows_blocks(const BinArgs& a);
+hipError_t launch_bounds_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
+hipError_t launch_windows_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);
 
@@ -131,5 +198,11 @@ inline void set_dynamic_lds_limit(const void* fn, int max_lds)
 }
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s);
 hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
+uint32_t conv2d_plain_blocks(const ConvArgs& a);
+uint32_t conv1d_plain_blocks(const ConvArgs& a);
+hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
+                                     size_t shm, hipStream_t s);
+hipError_t launch_conv1d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
+                                     hipStream_t s);
 
 }  // namespace rtsim

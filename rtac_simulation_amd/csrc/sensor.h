@@ -21,6 +21,18 @@ const char* fail_message();
 KeyParams make_key(uint32_t N, float lo, float hi, float margin);
 int make_overlap(float margin, float lo, float hi, uint32_t N);
 
+// Everything one ping launches, without launching it (a batch gathers the
+// plans of many pings into per-stage launches).
+struct PingPlan {
+    int form = 0;
+    bool grid = false;
+    SortPlan sort{};
+    BinArgs bins{};
+    ConvArgs conv{};
+    DepositArgs dep{};
+    CorrArgs corr{};
+};
+
 struct StageEvent {
     int stage;
     hipEvent_t start, stop;
@@ -94,7 +106,8 @@ struct Sensor {
     int ensure_grid();
     int grid_row_map(RowMap* m);    // grid (MFMA) form row table and G-row map
     int compute(int form, hipStream_t s);
-    int compute_grid_form(int form, hipStream_t s);
+    int grid_plan(int form, DepositArgs* d, CorrArgs* c);
+    int plan(int form, PingPlan* p);   // validates the form; queues nothing
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
     ConvArgs conv_args() const;

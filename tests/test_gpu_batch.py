@@ -82,3 +82,58 @@ def test_batch_rejects_unsupported_form(gpu):
     with pytest.raises(R.RtsimError) as e:
         R.Batch([s], [1])
     assert e.value.code == R.ERR_UNSUPPORTED
+
+
+def test_batched_launches_many_uneven_pings(gpu):
+    """Twelve pings of uneven sizes, both MFMA forms on different tile shapes
+    (so the correlation runs as several batched launches), plain 2D and 1D,
+    and an empty sensor: every stage runs once for the whole batch and must
+    reproduce the per-sensor computes exactly (the plain 2D form accumulates
+    rows with unordered FP32 atomics, so it is held to FP32 rounding)."""
+    R = gpu
+    specs = []
+    for k, (W, H, n, form, shape) in enumerate([
+            (64, 128, 30_000, 1, 0), (40, 96, 12_000, 2, 1), (96, 200, 50_000, 2, 2),
+            (48, 96, 8_000, 0, None), (128, 256, 70_000, 1, 3), (32, 64, 0, 2, 1),
+            (80, 160, 25_000, 2, 1), (56, 112, 15_000, 1, 1), (72, 150, 40_000, 0, None),
+            (64, 128, 5_000, 2, 0)]):
+        cfg = configs._fls(W, H, 0.1, 30.0, 1.2e6, configs.psf_31x17(), f"u{k}")
+        specs.append((cfg, configs.fls_scene(), n, form, shape))
+    c1, sc1, _, _ = configs.workload("single_beam_38k")
+    specs.append((c1, sc1, 9_000, 0, None))
+    specs.append((c1, sc1, 3_000, 0, None))
+    sensors, hits, forms = [], [], []
+    for k, (cfg, scene, n, form, shape) in enumerate(specs):
+        s = R.make_sensor(cfg)
+        s.set_sample_count(n)
+        if shape is not None:
+            s.set_tile_shape(shape)
+        h = R.generate_hits(scene, 60 + k, max(n, 1)).to_device().prefix(n)
+        sensors.append(s)
+        hits.append(h)
+        forms.append(form)
+    eager = []
+    for s, h, f in zip(sensors, hits, forms):
+        if h.n:
+            s.write_samples(h)
+        s.compute_output(f)
+        eager.append((s.output().copy(), s.binner_debug()))
+    b = R.Batch(sensors, forms)
+    for k, h in enumerate(hits):
+        if h.n:
+            b.bind_hits(k, h)
+    stream = R.Stream()
+    for rep in range(2):
+        for s in sensors:
+            if s.sample_count():
+                s.upload_samples(np.zeros((s.sample_count(), 4), np.float32))
+        b.compute(stream)
+        stream.sync()
+        for k, ((ping, dbg), s) in enumerate(zip(eager, sensors)):
+            d = s.binner_debug()
+            for key in ("order", "keys", "segments", "windows"):
+                assert np.array_equal(d[key], dbg[key]), (rep, k, key)
+            if forms[k] == 0 and specs[k][0].kind == 2:
+                assert O.rel_l2(s.output(), ping) < 1e-6, (rep, k)
+            else:
+                assert np.array_equal(s.output().view(np.uint32), ping.view(np.uint32)), (rep, k)
