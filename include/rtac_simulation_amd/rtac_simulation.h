@@ -27,6 +27,7 @@
 #include "../rtsim_batch.h"
 #include "../rtsim_config.h"
 #include "../rtsim_scene.h"
+#include "../rtsim_sink.h"
 
 namespace rtac {
 namespace simulation {
@@ -377,6 +378,49 @@ class Batch {
    private:
     std::vector<SensorInstance::Ptr> sensors_;
     rtsim_batch h_ = nullptr;
+};
+
+// ---------------------------------------------------------------------------
+// FileSink (sinks/FileSink.h): appends pings to a ping file (rtsim_sink.h)
+// ---------------------------------------------------------------------------
+class FileSink {
+   public:
+    using Ptr = std::shared_ptr<FileSink>;
+    using ConstPtr = std::shared_ptr<const FileSink>;
+
+    // FileSink::Create(filename, overwrite, sinkName) (FileSink.cpp:15-24)
+    static Ptr Create(const std::string& filename, bool overwrite = false,
+                      const std::string& sinkName = "file-sink")
+    {
+        rtsim_sink h = nullptr;
+        check(rtsim_sink_open(filename.c_str(), overwrite ? 1 : 0, &h));
+        return Ptr(new FileSink(h, filename, sinkName));
+    }
+    ~FileSink()
+    {
+        if (h_) rtsim_sink_close(h_);
+    }
+    FileSink(const FileSink&) = delete;
+    FileSink& operator=(const FileSink&) = delete;
+
+    const std::string& filename() const { return filename_; }
+    const std::string& name() const { return name_; }
+    // FileSink::set_output(sensor) (FileSink.cpp:26-42): 1D and 2D pings
+    void set_output(const SensorInstance::Ptr& sensor, void* stream = nullptr)
+    {
+        check(rtsim_sink_write(h_, sensor->handle(), stream));
+    }
+    uint64_t count() const
+    {
+        uint64_t n = 0;
+        check(rtsim_sink_count(h_, &n));
+        return n;
+    }
+
+   private:
+    FileSink(rtsim_sink h, const std::string& f, const std::string& n) : h_(h), filename_(f), name_(n) {}
+    rtsim_sink h_ = nullptr;
+    std::string filename_, name_;
 };
 
 }  // namespace simulation

@@ -145,6 +145,16 @@ _SIGNATURES = [
     ("rtsim_batch_bind_hits", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Hits), C.c_size_t]),
     ("rtsim_batch_compute", C.c_int, [C.c_void_p, C.c_void_p]),
     ("rtsim_batch_destroy", C.c_int, [C.c_void_p]),
+    # ping files (rtsim_sink.h)
+    ("rtsim_sink_open", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
+    ("rtsim_sink_write", C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("rtsim_sink_write_host", C.c_int,
+     [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.c_float, C.c_float, C.c_void_p,
+      C.c_void_p]),
+    ("rtsim_sink_count", C.c_int, [C.c_void_p, C.POINTER(C.c_uint64)]),
+    ("rtsim_sink_close", C.c_int, [C.c_void_p]),
+    ("rtsim_ping_file_count", C.c_int, [C.c_char_p, C.POINTER(C.c_uint64)]),
+    ("rtsim_ping_file_read", C.c_int, [C.c_char_p, C.c_uint64, C.c_void_p, C.c_void_p, C.c_void_p]),
 ]
 
 _lib = None
@@ -450,6 +460,74 @@ class SensorInstance2D_Complex(SensorInstance):
 
 def make_sensor(cfg: "SensorConfig") -> SensorInstance:
     return (SensorInstance2D_Complex if cfg.kind == SENSOR_2D else SensorInstance1D_Complex)(cfg)
+
+
+class PingHeader(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("scalar", C.c_uint32), ("height", C.c_uint32),
+                ("width", C.c_uint32), ("range_min", C.c_float), ("range_max", C.c_float),
+                ("index", C.c_uint64), ("checksum", C.c_uint64)]
+
+
+class FileSink:
+    """FileSink drop-in (sinks/FileSink.h): appends pings to a ping file
+    (record format in rtsim_sink.h)."""
+
+    def __init__(self, filename, overwrite: bool = False):
+        self.filename = str(filename)
+        h = C.c_void_p()
+        check(lib().rtsim_sink_open(self.filename.encode(), int(overwrite), C.byref(h)))
+        self.handle = h
+
+    @classmethod
+    def Create(cls, filename, overwrite: bool = False) -> "FileSink":
+        return cls(filename, overwrite)
+
+    def set_output(self, sensor: "SensorInstance", stream: "Stream | None" = None):
+        check(lib().rtsim_sink_write(self.handle, sensor.handle, stream.handle if stream else None))
+
+    def write_host(self, kind: int, ping: np.ndarray, range_min: float, range_max: float,
+                   bearings: np.ndarray | None = None):
+        p = np.ascontiguousarray(ping, np.complex64)
+        H, W = (p.shape[0], 1) if p.ndim == 1 else p.shape
+        b = None if bearings is None else np.ascontiguousarray(bearings, np.float32)
+        check(lib().rtsim_sink_write_host(self.handle, kind, H, W, range_min, range_max,
+                                          None if b is None else _ptr(b), _ptr(p.view(np.float32))))
+
+    def count(self) -> int:
+        n = C.c_uint64()
+        check(lib().rtsim_sink_count(self.handle, C.byref(n)))
+        return n.value
+
+    def close(self):
+        if self.handle:
+            lib().rtsim_sink_close(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ping_file_count(path) -> int:
+    n = C.c_uint64()
+    check(lib().rtsim_ping_file_count(str(path).encode(), C.byref(n)))
+    return n.value
+
+
+def read_ping(path, index: int = 0) -> dict:
+    """Record `index` of a ping file: header fields, bearings, data [range][bearing]."""
+    h = PingHeader()
+    check(lib().rtsim_ping_file_read(str(path).encode(), index, C.byref(h), None, None))
+    nb = h.width if h.kind == SENSOR_2D else 0
+    b = np.zeros(max(nb, 1), np.float32)
+    d = np.zeros((h.height, h.width), np.complex64)
+    check(lib().rtsim_ping_file_read(str(path).encode(), index, C.byref(h), _ptr(b),
+                                     _ptr(d.view(np.float32))))
+    return {"kind": h.kind, "height": h.height, "width": h.width, "range_min": h.range_min,
+            "range_max": h.range_max, "index": h.index, "checksum": h.checksum,
+            "bearings": b[:nb], "data": d if h.kind == SENSOR_2D else d[:, 0]}
 
 
 class Batch:

@@ -90,6 +90,12 @@ int main(int argc, char** argv)
             if (!f) throw FileError(argv[6]);
             std::fwrite(ping.data(), sizeof(Complex<float>), ping.size(), f);
             std::fclose(f);
+            if (argc >= 8) {   // FileSink::Create + set_output (FileSink.cpp:15-42)
+                FileSink::Ptr sink = FileSink::Create(argv[7], true);
+                sink->set_output(sensor);
+                std::printf("sink %s pings=%llu\n", sink->filename().c_str(),
+                            (unsigned long long)sink->count());
+            }
             double e = 0;
             for (auto& v : ping) e += std::norm(v);
             std::printf("ping %zu cells energy=%.6e overlap=%d\n", ping.size(), e, b.overlap);

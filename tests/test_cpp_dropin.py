@@ -43,9 +43,12 @@ def test_cpp_ping_equals_python_binding(gpu, tmp_path, form):
     R_ = gpu
     out = tmp_path / "ping.bin"
     n, seed = 40_000, 77
-    r = _run("ping", R_.CONFIG_DIR / "fls_512x1024.yaml", n, seed, form, out)
+    sink = tmp_path / "sink" / "ping.rtpg"
+    r = _run("ping", R_.CONFIG_DIR / "fls_512x1024.yaml", n, seed, form, out, sink)
     assert r.returncode == 0, r.stderr
     cpp = np.fromfile(out, np.complex64).reshape(1024, 512)
+    assert "pings=1" in r.stdout
+    assert np.array_equal(R_.read_ping(sink, 0)["data"], cpp)
     cfg = R_.config_from_yaml(R_.CONFIG_DIR / "fls_512x1024.yaml")
     s = R_.make_sensor(cfg)
     s.set_sample_count(n)
