@@ -16,9 +16,15 @@
 // RTSIM_ERR_IO -> FileError, others -> std::runtime_error).
 #pragma once
 
+#include <array>
+#include <cmath>
 #include <complex>
 #include <cstdint>
+#include <cstdlib>
+#include <deque>
+#include <fstream>
 #include <memory>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -232,6 +238,8 @@ class SensorInstance : public std::enable_shared_from_this<SensorInstance> {
         check(rtsim_sensor_samples(h_, &p));
         return p;
     }
+    // receiver pose (Receiver.h:56): row-major rotation, directions map as R^T dir
+    void set_pose(const std::array<float, 9>& rotation) { check(rtsim_sensor_set_pose(h_, rotation.data())); }
     // ReceiverView::set_sample over n device-resident hits (Receiver.h:51-60)
     void set_samples(const rtsim_hits& hits, size_t n, size_t offset = 0, void* stream = nullptr)
     {
@@ -421,6 +429,195 @@ class FileSink {
     FileSink(rtsim_sink h, const std::string& f, const std::string& n) : h_(h), filename_(f), name_(n) {}
     rtsim_sink h_ = nullptr;
     std::string filename_, name_;
+};
+
+// ---------------------------------------------------------------------------
+// Poses and pose sources (PoseSource.h:11-87, PoseSource.cpp:7-30)
+// ---------------------------------------------------------------------------
+struct Pose {
+    float x = 0.f, y = 0.f, z = 0.f;              // translation (m)
+    float qw = 1.f, qx = 0.f, qy = 0.f, qz = 0.f;  // orientation, unit quaternion
+
+    // row-major rotation of the (normalised) Hamilton quaternion
+    std::array<float, 9> rotation_matrix() const
+    {
+        double w = qw, a = qx, b = qy, c = qz;
+        const double n = std::sqrt(w * w + a * a + b * b + c * c);
+        if (n > 0) { w /= n; a /= n; b /= n; c /= n; }
+        return {(float)(1 - 2 * (b * b + c * c)), (float)(2 * (a * b - w * c)), (float)(2 * (a * c + w * b)),
+                (float)(2 * (a * b + w * c)), (float)(1 - 2 * (a * a + c * c)), (float)(2 * (b * c - w * a)),
+                (float)(2 * (a * c - w * b)), (float)(2 * (b * c + w * a)), (float)(1 - 2 * (a * a + b * b))};
+    }
+    // one row of a pose CSV: "quat,tx,ty,tz,qw,qx,qy,qz" (examples/pose_source.csv)
+    static Pose decode_string(const std::string& line, char delimiter = ',')
+    {
+        std::vector<std::string> f;
+        std::stringstream ss(line);
+        std::string item;
+        while (std::getline(ss, item, delimiter)) f.push_back(item);
+        if (f.size() != 8 || f[0] != "quat")
+            throw FileError("pose '" + line + "' is not quat,tx,ty,tz,qw,qx,qy,qz");
+        Pose p;
+        float* v[7] = {&p.x, &p.y, &p.z, &p.qw, &p.qx, &p.qy, &p.qz};
+        for (int k = 0; k < 7; k++) {
+            char* end = nullptr;
+            *v[k] = std::strtof(f[k + 1].c_str(), &end);
+            if (end == f[k + 1].c_str()) throw FileError("bad number in pose '" + line + "'");
+        }
+        return p;
+    }
+};
+
+class PoseSource {
+   public:
+    using Ptr = std::shared_ptr<PoseSource>;
+    using ConstPtr = std::shared_ptr<const PoseSource>;
+    virtual ~PoseSource() = default;
+    virtual int size() const = 0;        // may be -1 when undefined
+    virtual int remaining() const = 0;
+    virtual Pose next_pose() = 0;
+};
+
+class PoseSourceStatic : public PoseSource {
+   public:
+    using Ptr = std::shared_ptr<PoseSourceStatic>;
+    static Ptr Create(const Pose& pose = Pose()) { return Ptr(new PoseSourceStatic(pose)); }
+    void set_pose(const Pose& pose) { pose_ = pose; }
+    int size() const override { return -1; }
+    int remaining() const override { return -1; }
+    Pose next_pose() override { return pose_; }
+
+   protected:
+    explicit PoseSourceStatic(const Pose& pose) : pose_(pose) {}
+    Pose pose_;
+};
+
+class PoseSourceList : public PoseSource {
+   public:
+    using Ptr = std::shared_ptr<PoseSourceList>;
+    static Ptr Create(const std::deque<Pose>& poses) { return Ptr(new PoseSourceList(poses)); }
+    // PoseSourceList::CreateFromCSV (PoseSource.cpp:7-30): '#' lines are skipped
+    static Ptr CreateFromCSV(const std::string& path, unsigned int skipLines = 0, char delimiter = ',')
+    {
+        std::ifstream file(path);
+        if (!file.is_open()) throw FileError(" : could not open '" + path + "'");
+        std::string line;
+        for (unsigned int i = 0; i < skipLines && std::getline(file, line); i++) {}
+        std::deque<Pose> poses;
+        while (std::getline(file, line)) {
+            if (line.empty() || line[0] == '#') continue;
+            poses.push_back(Pose::decode_string(line, delimiter));
+        }
+        return Create(poses);
+    }
+    int size() const override { return (int)poses_.size(); }
+    int remaining() const override { return (int)poses_.size() - current_; }
+    Pose next_pose() override { return poses_[current_++]; }
+
+   protected:
+    explicit PoseSourceList(const std::deque<Pose>& poses) : poses_(poses) {}
+    std::deque<Pose> poses_;
+    int current_ = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Simulation loop: OptixSimulation1::run (OptixSimulation.cpp:56-92) with the
+// ray caster (out of scope here) replaced by the seeded synthetic scene
+// generator: per ping, the next receiver pose, `samples` hits of the scene
+// (seed + ping index), compute_output, then every sink.  run_batch() does
+// `in_flight` pings at once through a Batch (batched launches in one graph).
+// ---------------------------------------------------------------------------
+class SyntheticSimulation {
+   public:
+    SyntheticSimulation(const std::string& sensorConfig, const rtsim_scene_desc& scene, size_t samples,
+                        Form form, unsigned int inFlight = 1, uint64_t seed = 1)
+        : scene_(scene), samples_(samples), form_(form), seed_(seed)
+    {
+        if (inFlight < 1) inFlight = 1;
+        for (unsigned int k = 0; k < inFlight; k++) {
+            Slot sl;
+            sl.sensor = SensorFactory::Make(sensorConfig);
+            sl.sensor->set_sample_count((unsigned int)samples);
+            for (int a = 0; a < 5; a++) {
+                const size_t bytes = sizeof(float) * (a == 4 ? 2 : 1) * (samples ? samples : 1);
+                check(rtsim_device_alloc(&sl.dev[a], bytes));
+            }
+            slots_.push_back(std::move(sl));
+        }
+    }
+    ~SyntheticSimulation()
+    {
+        for (auto& sl : slots_)
+            for (auto p : sl.dev)
+                if (p) rtsim_device_free(p);
+    }
+    SyntheticSimulation(const SyntheticSimulation&) = delete;
+    SyntheticSimulation& operator=(const SyntheticSimulation&) = delete;
+
+    void set_receiver_poses(const PoseSource::Ptr& poses) { poses_ = poses; }
+    void add_sink(const FileSink::Ptr& sink) { sinks_.push_back(sink); }
+    uint64_t pings() const { return ping_; }
+    const SensorInstance::Ptr& receiver(unsigned int slot = 0) const { return slots_[slot].sensor; }
+
+    // one ping (OptixSimulation1::run); false when the pose source is exhausted
+    bool run() { return run_batch(1) == 1; }
+
+    // up to `count` pings (at most in_flight) as one batch; returns pings done
+    size_t run_batch(size_t count = (size_t)-1)
+    {
+        if (!poses_) throw std::runtime_error("Receiver pose source was not set");
+        std::vector<SensorInstance::Ptr> active;
+        std::vector<Form> forms;
+        for (auto& sl : slots_) {
+            if (active.size() >= count || poses_->remaining() == 0) break;
+            sl.sensor->set_pose(poses_->next_pose().rotation_matrix());
+            load_hits(sl, seed_ + ping_ + active.size());
+            active.push_back(sl.sensor);
+            forms.push_back(form_);
+        }
+        if (active.empty()) return 0;
+        if (active.size() == 1) {
+            Slot& sl = slots_[0];
+            sl.sensor->set_samples(sl.hits(), samples_);
+            sl.sensor->compute_output(form_);
+        } else {
+            Batch batch(active, forms);
+            for (size_t k = 0; k < active.size(); k++) batch.bind_hits((uint32_t)k, &slots_[k].hits_, samples_);
+            batch.compute();
+        }
+        for (const auto& s : active)
+            for (const auto& sink : sinks_) sink->set_output(s);
+        ping_ += active.size();
+        return active.size();
+    }
+
+   private:
+    struct Slot {
+        SensorInstance::Ptr sensor;
+        void* dev[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+        rtsim_hits hits_{};
+        const rtsim_hits& hits() const { return hits_; }
+    };
+    void load_hits(Slot& sl, uint64_t seed)
+    {
+        std::vector<float> r(samples_), dx(samples_), dy(samples_), dz(samples_), a(2 * samples_);
+        check(rtsim_generate_hits(&scene_, seed, 0, samples_, r.data(), dx.data(), dy.data(), dz.data(),
+                                  a.data(), 8));
+        const std::vector<float>* host[5] = {&r, &dx, &dy, &dz, &a};
+        for (int k = 0; k < 5; k++)
+            if (!host[k]->empty())
+                check(rtsim_memcpy_h2d(sl.dev[k], host[k]->data(), sizeof(float) * host[k]->size()));
+        sl.hits_ = rtsim_hits{(const float*)sl.dev[0], (const float*)sl.dev[1], (const float*)sl.dev[2],
+                              (const float*)sl.dev[3], (const float*)sl.dev[4]};
+    }
+
+    rtsim_scene_desc scene_;
+    size_t samples_;
+    Form form_;
+    uint64_t seed_, ping_ = 0;
+    std::vector<Slot> slots_;
+    PoseSource::Ptr poses_;
+    std::vector<FileSink::Ptr> sinks_;
 };
 
 }  // namespace simulation

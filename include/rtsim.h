@@ -117,6 +117,11 @@ int rtsim_sensor_destroy(rtsim_sensor h);
 int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
                             uint32_t range_count, float margin);
 
+/* Receiver pose (Receiver.h:56, SensorInstance pose set by the simulation loop,
+ * OptixSimulation.cpp:76-81): row-major rotation R of the receiver; the
+ * streaming kernel maps hit directions to the receiver frame as R^T dir.     */
+int rtsim_sensor_set_pose(rtsim_sensor h, const float* rotation);
+
 /* SensorInstance{1D,2D}::set_sample_count / sample_count
  * (SensorInstance1D.h:47-48, SensorInstance2D.h:59-60).                            */
 int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count);

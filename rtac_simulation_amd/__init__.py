@@ -97,6 +97,7 @@ _SIGNATURES = [
     ("rtsim_sensor_autotune", C.c_int,
      [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("rtsim_sensor_set_tile_shape", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_sensor_set_pose", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("rtsim_sensor_tile_shape", C.c_int,
      [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     ("rtsim_sensor_output", C.c_int,
@@ -396,6 +397,11 @@ class SensorInstance:
                                           stream.handle if stream else None, C.byref(sh),
                                           C.byref(ms)))
         return sh.value, ms.value
+
+    def set_pose(self, rotation: np.ndarray):
+        """Receiver rotation R (3 x 3, row-major); directions map as R^T dir."""
+        r = np.ascontiguousarray(np.asarray(rotation, np.float32).reshape(9))
+        check(lib().rtsim_sensor_set_pose(self.handle, r.ctypes.data_as(C.POINTER(C.c_float))))
 
     def set_tile_shape(self, shape: int):
         check(lib().rtsim_sensor_set_tile_shape(self.handle, shape))

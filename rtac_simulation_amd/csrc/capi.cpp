@@ -537,6 +537,17 @@ int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
     return s->configure_ranges();
 }
 
+int rtsim_sensor_set_pose(rtsim_sensor h, const float* rotation)
+{
+    GET_SENSOR(h);
+    if (!rotation) return fail(RTSIM_ERR_INVALID, "rotation is NULL");
+    for (int k = 0; k < 9; k++)
+        if (!(rotation[k] == rotation[k])) return fail(RTSIM_ERR_INVALID, "rotation has a NaN");
+    std::memcpy(s->R, rotation, sizeof(s->R));
+    s->generation++;   // batched stream launches hold the rotation
+    return RTSIM_OK;
+}
+
 int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count)
 {
     GET_SENSOR(h);

@@ -4,9 +4,15 @@
 //
 //   dropin_example info <config.yaml>
 //       prints the parsed description (host only, no GPU)
-//   dropin_example ping <config.yaml> <samples> <seed> <form> <out.bin>
+//   dropin_example ping <config.yaml> <samples> <seed> <form> <out.bin> [sink.rtpg]
 //       computes one ping from seeded synthetic hits and writes it as
-//       complex64 [range][bearing]
+//       complex64 [range][bearing] (and through FileSink)
+//   dropin_example poses <poses.csv>
+//       parses a pose CSV (host only) and prints each rotation
+//   dropin_example drive <config.yaml> <poses.csv> <samples> <seed> <form> <in_flight>
+//                        <out.rtpg>
+//       runs the simulation loop over every pose, in batches of in_flight
+//       pings, into one ping file
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -61,6 +67,33 @@ int main(int argc, char** argv)
                         d.kind, d.range_count, d.range_min, d.range_max, d.bearing_count,
                         d.psf_range_taps, d.psf_bearing_taps, d.psf_range_span,
                         d.psf_bearing_span, d.margin, d.dir_width, d.dir_height, d.frequency);
+            return 0;
+        }
+        if (mode == "poses") {
+            PoseSourceList::Ptr poses = PoseSourceList::CreateFromCSV(argv[2]);
+            std::printf("poses=%d\n", poses->size());
+            while (poses->remaining() > 0) {
+                const Pose p = poses->next_pose();
+                const std::array<float, 9> R = p.rotation_matrix();
+                std::printf("%.6f %.6f %.6f", p.x, p.y, p.z);
+                for (float v : R) std::printf(" %.7f", v);
+                std::printf("\n");
+            }
+            return 0;
+        }
+        if (mode == "drive" && argc >= 9) {
+            SensorInfo::Ptr info = SensorInfoFactory::Make(argv[2]);
+            const rtsim_scene_desc scene = scene_for(*info);
+            SyntheticSimulation sim(argv[2], scene, std::strtoull(argv[4], nullptr, 10),
+                                    static_cast<Form>(std::atoi(argv[6])),
+                                    (unsigned)std::atoi(argv[7]), std::strtoull(argv[5], nullptr, 10));
+            sim.set_receiver_poses(PoseSourceList::CreateFromCSV(argv[3]));
+            FileSink::Ptr sink = FileSink::Create(argv[8], true);
+            sim.add_sink(sink);
+            size_t rounds = 0;
+            while (sim.run_batch() > 0) rounds++;
+            std::printf("pings=%llu rounds=%zu file=%llu\n", (unsigned long long)sim.pings(), rounds,
+                        (unsigned long long)sink->count());
             return 0;
         }
         if (mode == "ping" && argc >= 7) {

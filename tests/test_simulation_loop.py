@@ -1,0 +1,71 @@
+"""Pose sources and the simulation loop of the C++ drop-in (PoseSource.h,
+OptixSimulation.cpp:56-92 with the ray caster replaced by the synthetic scene
+generator), through tests/cpp/dropin_example.cpp."""
+import numpy as np
+import pytest
+from scipy.spatial.transform import Rotation
+
+import rtac_simulation_amd as R
+from rtac_simulation_amd import build as B
+from rtac_simulation_amd import configs
+
+from test_cpp_dropin import GOLDEN, _run
+
+POSES = GOLDEN / "pose_source.csv"
+
+
+def _poses_from_csv(path):
+    rows = [l.split(",") for l in path.read_text().splitlines() if l and not l.startswith("#")]
+    return np.array([[float(v) for v in r[1:]] for r in rows])
+
+
+def test_pose_csv_rotations_match_scipy():
+    r = _run("poses", POSES)
+    assert r.returncode == 0, r.stderr
+    lines = r.stdout.splitlines()
+    ref = _poses_from_csv(POSES)
+    assert lines[0] == f"poses={len(ref)}"
+    got = np.array([[float(v) for v in l.split()] for l in lines[1:]])
+    assert got.shape == (len(ref), 12)
+    assert np.allclose(got[:, :3], ref[:, :3], atol=1e-5)
+    Rm = got[:, 3:].reshape(-1, 3, 3)
+    # quaternions in the file are (qw, qx, qy, qz); scipy takes scalar last
+    exp = Rotation.from_quat(ref[:, [4, 5, 6, 3]]).as_matrix()
+    assert np.abs(Rm - exp).max() < 2e-6
+    assert np.abs(np.einsum("nij,nkj->nik", Rm, Rm) - np.eye(3)).max() < 1e-5
+    assert np.allclose(np.linalg.det(Rm), 1.0, atol=1e-5)
+
+
+def test_bad_pose_rows_raise_file_error(tmp_path):
+    p = tmp_path / "bad.csv"
+    p.write_text("# quat,tx,ty,tz,qw,qx,qy,qz\nquat,1,2,3,1,0,0\n")
+    assert _run("poses", p).returncode == 4
+    assert _run("poses", tmp_path / "missing.csv").returncode == 4
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("in_flight", [1, 4])
+def test_drive_equals_per_ping_computes(gpu, tmp_path, in_flight):
+    R_ = gpu
+    poses = tmp_path / "poses.csv"
+    lines = POSES.read_text().splitlines()
+    poses.write_text("\n".join(lines[:7]) + "\n")          # header + 6 poses
+    out = tmp_path / "drive.rtpg"
+    n, seed, form = 20_000, 11, 2
+    cfgp = R_.CONFIG_DIR / "fls_512x1024.yaml"
+    r = _run("drive", cfgp, poses, n, seed, form, in_flight, out)
+    assert r.returncode == 0, r.stderr
+    assert "pings=6" in r.stdout and R_.ping_file_count(out) == 6
+    cfg = R_.config_from_yaml(cfgp)
+    s = R_.make_sensor(cfg)
+    s.set_sample_count(n)
+    q = _poses_from_csv(poses)
+    for k in range(6):
+        Rk = Rotation.from_quat(q[k, [4, 5, 6, 3]]).as_matrix().astype(np.float32)
+        s.set_pose(Rk)
+        s.write_samples(R_.generate_hits(configs.fls_scene(), seed + k, n).to_device())
+        s.compute_output(form)
+        ping = R_.read_ping(out, k)["data"]
+        # rotations from float32 quaternion math on both sides agree to ~1 ulp
+        from oracle import rel_l2
+        assert rel_l2(ping, s.output()) < 1e-5, k
