@@ -66,7 +66,8 @@ struct Batch {
     Stage<StreamArgs> stream;
     Stage<HistArgs> hist[4];
     Stage<ScanArgs> scan[4];
-    Stage<ScatterArgs> scatter[4];
+    Stage<ScatterArgs> scatter[4];        // passes before a ping's last
+    Stage<ScatterArgs> scatter_last[4];   // a ping's last pass (its sort may take 1-4)
     Stage<BinArgs> bounds, windows;
     Stage<ConvArgs> plain2d, plain1d;
     Stage<DepositArgs> deposit;
@@ -120,6 +121,7 @@ static int build(Batch* b)
         b->hist[k] = {};
         b->scan[k] = {};
         b->scatter[k] = {};
+        b->scatter_last[k] = {};
     }
     b->bounds = {};
     b->windows = {};
@@ -137,10 +139,11 @@ static int build(Batch* b)
             StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
             b->stream.add(a, stream_blocks(a));
         }
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < p.sort.passes; k++) {
             b->hist[k].add(p.sort.hist[k], k == 0 ? p.sort.hist_blocks : p.sort.pass_blocks);
             b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks ? 256u : 0u);
-            b->scatter[k].add(p.sort.scatter[k], p.sort.pass_blocks);
+            (k == p.sort.passes - 1 ? b->scatter_last[k] : b->scatter[k])
+                .add(p.sort.scatter[k], p.sort.pass_blocks);
         }
         b->bounds.add(p.bins, bounds_blocks(p.bins));
         b->windows.add(p.bins, windows_blocks(p.bins));
@@ -177,6 +180,7 @@ static int build(Batch* b)
         layout(b->hist[k], off);
         layout(b->scan[k], off);
         layout(b->scatter[k], off);
+        layout(b->scatter_last[k], off);
     }
     layout(b->bounds, off);
     layout(b->windows, off);
@@ -198,6 +202,7 @@ static int build(Batch* b)
         pack(b->hist[k], host);
         pack(b->scan[k], host);
         pack(b->scatter[k], host);
+        pack(b->scatter_last[k], host);
     }
     pack(b->bounds, host);
     pack(b->windows, host);
@@ -220,8 +225,10 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
         RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]),
                                       b->hist[k].blocks(), s));
         RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
-        RT_TRY(launch_sort_scatter_batch(k, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
+        RT_TRY(launch_sort_scatter_batch(k == 0, false, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
                                          b->scatter[k].blocks(), s));
+        RT_TRY(launch_sort_scatter_batch(k == 0, true, dargs(b, b->scatter_last[k]),
+                                         dmap(b, b->scatter_last[k]), b->scatter_last[k].blocks(), s));
     }
     RT_TRY(launch_bounds_batch(dargs(b, b->bounds), dmap(b, b->bounds), b->bounds.blocks(), s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));

@@ -292,7 +292,8 @@ int Sensor::plan(int form, PingPlan* p)
     if (grid && (rc = grid_row_map(&rows))) return rc;   // also checks the grid form applies
 
     // sort (Receiver.cu:28-34); the last pass gathers the records, their keys
-    // (Binner.cu:24-49) and bearing coordinates
+    // (Binner.cu:24-49) and bearing coordinates.  The plain form sorts by
+    // travel; the grid form by the grouping key (GroupKey).
     SortBuffers sb;
     sb.keys[0] = d_kA;
     sb.keys[1] = d_kB;
@@ -311,13 +312,25 @@ int Sensor::plan(int form, PingPlan* p)
     sb.out.coord = d_coord;
     sb.out.key = key;
     sb.out.axis = axis;
-    hipError_t e = make_sort_plan(d_samples, n, sb, &p->sort);
+    // the grid form only needs samples grouped by bin and G row (its deposit
+    // sums in exact fixed point, independent of the order inside a group)
+    GroupKey gk{};
+    if (grid) {
+        gk.on = 1;
+        gk.rows = rows;
+        gk.key = key;
+        if (group_key_max(gk) >= (1u << 24))
+            return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
+                        rows.GH, N);
+    }
+    hipError_t e = make_sort_plan(d_samples, n, sb, &gk, &p->sort);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
 
     // Binner::compute (Binner.h:96-102): segments, windows
     BinArgs& ba = p->bins;
     ba.skeys = d_skeys;
     ba.img = p->sort.img;
+    ba.img_is_row = gk.on;
     ba.n = n;
     ba.N = N;
     ba.ov = overlap;

@@ -59,6 +59,29 @@ __device__ __forceinline__ int g_row_of(const RowMap& m, float t)
     return r > (float)(m.GH + 2) ? m.GH + 2 : (int)r;
 }
 
+// Bin slot of a travel: 0 below the Binner's range, bin_key + 1 inside it,
+// n + 1 above it or NaN.  Non-decreasing in travel (the truncated fma is, and
+// each clamp compares it with one fixed bound).
+__device__ __forceinline__ uint32_t bin_slot(const KeyParams& k, float t)
+{
+    if (t != t) return k.n + 1;
+    float v = __builtin_fmaf(k.a, t, k.b);
+    if (v != v) return k.n + 1;
+    int res;
+    if (v >= 2147483648.0f)       res = INT_MAX;
+    else if (v <= -2147483648.0f) res = INT_MIN;
+    else                          res = (int)v;
+    if (res < 0) return ((float)res >= k.lower) ? 1u : 0u;
+    if ((uint32_t)res >= k.n) return ((float)res <= k.upper) ? k.n : k.n + 1;
+    return (uint32_t)res + 1;
+}
+
+// Grouping sort key K = (G row + 2) + bin slot of a travel.
+__device__ __forceinline__ uint32_t group_key(const GroupKey& g, float t)
+{
+    return (uint32_t)(g_row_of(g.rows, t) + 2) + bin_slot(g.key, t);
+}
+
 // Number of set bits of `mask` in lanes strictly below the calling lane.
 __device__ __forceinline__ uint32_t popc_below(uint64_t mask)
 {

@@ -155,8 +155,11 @@ __device__ __forceinline__ void bounds_body(const BinArgs& a, uint32_t blk)
     }
     if (a.rows.row_start) {
         // first sample of each G row present (the table holds n elsewhere)
-        const int r = g_row_of(a.rows, image_travel(a.img[i]));
-        const int rp = i == 0 ? -3 : g_row_of(a.rows, image_travel(a.img[i - 1]));
+        auto row = [&](uint32_t j) {
+            return a.img_is_row ? (int)a.img[j] - 2 : g_row_of(a.rows, image_travel(a.img[j]));
+        };
+        const int r = row(i);
+        const int rp = i == 0 ? -3 : row(i - 1);
         if (r != rp && r >= -1 && r <= a.rows.GH + 1) a.rows.row_start[r + 1] = i;
     }
 }

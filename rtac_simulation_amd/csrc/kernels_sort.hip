@@ -19,7 +19,10 @@
 // Items of a tile are ordered (wave, round, lane) = ascending index, ranks
 // count earlier rounds, lower lanes and earlier waves, tiles are scanned in
 // order: every pass is stable, so the result is the stable order of the
-// travel images.
+// travel images.  The grid (MFMA) form sorts by the grouping key K of
+// GroupKey instead (G row + bin slot, below 2^16 for grids up to ~32k range
+// rows), which takes two passes: same keys, segments, windows and G-row
+// table, order permuted only inside runs of equal (bin, G row).
 #include "device_common.h"
 #include "rtsim_internal.h"
 
@@ -55,15 +58,16 @@ __device__ __forceinline__ uint32_t sort_block_scan(uint32_t x, uint32_t* ws, ui
 template <bool FIRST>
 __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
                                           const uint32_t* __restrict__ kin,
-                                          const uint32_t* __restrict__ vin, uint32_t n,
-                                          uint32_t wbase, uint32_t lane, uint32_t* key,
+                                          const uint32_t* __restrict__ vin, const GroupKey& gk,
+                                          uint32_t n, uint32_t wbase, uint32_t lane, uint32_t* key,
                                           uint32_t* val)
 {
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++) {
         const uint32_t i = min(wbase + j * 64 + lane, n - 1);
         if (FIRST) {
-            key[j] = travel_image(rec[i].z);
+            const float t = rec[i].z;
+            key[j] = gk.on ? group_key(gk, t) : travel_image(t);
             if (val) val[j] = i;
         } else {
             key[j] = kin[i];
@@ -88,7 +92,7 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems];
-    load_tile<FIRST>(a.rec, a.kin, nullptr, a.n, wbase, lane, key, nullptr);
+    load_tile<FIRST>(a.rec, a.kin, nullptr, a.gk, a.n, wbase, lane, key, nullptr);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++)
@@ -171,7 +175,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 
     const uint32_t wbase = tb + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems], val[kSortLaneItems], lrank[kSortLaneItems];
-    load_tile<FIRST>(a.rec, a.kin, a.vin, a.n, wbase, lane, key, val);
+    load_tile<FIRST>(a.rec, a.kin, a.vin, a.gk, a.n, wbase, lane, key, val);
     __syncthreads();
 
     // stable ranks inside each wave
@@ -254,10 +258,11 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     for (int u = 0; u < Q; u++) {
         const uint32_t pos = pq[u];
         if (pos < a.n) {   // holds for every staged item unless the records changed
-            a.kout[pos] = kq[u];
             a.vout[pos] = vq[u];
+            if (!LAST || !a.gk.on) a.kout[pos] = kq[u];
             if (LAST) {
                 const float4 r = rq[u];
+                if (a.gk.on) a.kout[pos] = (uint32_t)(g_row_of(a.gk.rows, r.z) + 2);
                 a.out.sorted[pos] = r;
                 a.out.skeys[pos] = bin_key(a.out.key, r.z);
                 const double x = a.out.axis == 1 ? sin((double)r.w) : (double)r.w;
@@ -283,10 +288,19 @@ __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* _
     sort_scatter_body<FIRST, LAST>(args[p], blk);
 }
 
-hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p)
+hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
+                          const GroupKey* gk, SortPlan* p)
 {
     const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
     if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
+    GroupKey g{};
+    if (gk && gk->on) g = *gk;
+    int passes = 4;
+    if (g.on) {
+        const uint32_t kmax = group_key_max(g);
+        passes = kmax < 256u ? 1 : kmax < 65536u ? 2 : kmax < (1u << 24) ? 3 : 4;
+    }
+    p->passes = passes;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
     for (int pass = 0; pass < 4; pass++) {
         HistArgs& h = p->hist[pass];
@@ -294,6 +308,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         h.kin = b.keys[cur];
         h.n = n;
         h.shift = 8 * pass;
+        h.gk = g;
         h.counts = b.counts;
         h.tiles = tiles;
         h.zero = pass == 0 ? b.zero : nullptr;
@@ -313,8 +328,9 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         c.counts = b.counts;
         c.totals = b.totals;
         c.tiles = tiles;
+        c.gk = g;
         c.out = b.out;
-        cur ^= 1;
+        if (pass < passes) cur ^= 1;
     }
     const bool clears = b.zero_count || b.fill_count;
     p->hist_blocks = n ? tiles : (clears ? 1u : 0u);
@@ -324,9 +340,16 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     return hipSuccess;
 }
 
+template <bool FIRST>
+static void launch_scatter(bool last, dim3 g, const ScatterArgs& a, hipStream_t s)
+{
+    if (last) hipLaunchKernelGGL((k_sort_scatter<FIRST, true>), g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_sort_scatter<FIRST, false>), g, dim3(256), 0, s, a);
+}
+
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
 {
-    for (int pass = 0; pass < 4; pass++) {
+    for (int pass = 0; pass < p.passes; pass++) {
         const uint32_t hb = pass == 0 ? p.hist_blocks : p.pass_blocks;
         if (hb) {
             if (pass == 0)
@@ -337,12 +360,9 @@ hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
         if (!p.pass_blocks) continue;
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, p.scan[pass]);
         const dim3 g(p.pass_blocks);
-        if (pass == 0)
-            hipLaunchKernelGGL((k_sort_scatter<true, false>), g, dim3(256), 0, s, p.scatter[pass]);
-        else if (pass < 3)
-            hipLaunchKernelGGL((k_sort_scatter<false, false>), g, dim3(256), 0, s, p.scatter[pass]);
-        else
-            hipLaunchKernelGGL((k_sort_scatter<false, true>), g, dim3(256), 0, s, p.scatter[pass]);
+        const bool last = pass == p.passes - 1;
+        if (pass == 0) launch_scatter<true>(last, g, p.scatter[pass], s);
+        else launch_scatter<false>(last, g, p.scatter[pass], s);
     }
     return hipGetLastError();
 }
@@ -351,7 +371,7 @@ hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
                        const uint32_t** order_out, const uint32_t** img_out, hipStream_t s)
 {
     SortPlan p;
-    hipError_t e = make_sort_plan(records, n, b, &p);
+    hipError_t e = make_sort_plan(records, n, b, nullptr, &p);
     if (e != hipSuccess) return e;
     *order_out = p.order;
     *img_out = p.img;
@@ -374,15 +394,18 @@ hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blo
     return hipGetLastError();
 }
 
-hipError_t launch_sort_scatter_batch(int pass, const ScatterArgs* args, BatchMap m, uint32_t blocks,
-                                     hipStream_t s)
+hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, const ScatterArgs* args,
+                                     BatchMap m, uint32_t blocks, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
     const dim3 g(blocks);
-    if (pass == 0) hipLaunchKernelGGL((k_sort_scatter_batch<true, false>), g, dim3(256), 0, s, args, m);
-    else if (pass < 3)
-        hipLaunchKernelGGL((k_sort_scatter_batch<false, false>), g, dim3(256), 0, s, args, m);
-    else hipLaunchKernelGGL((k_sort_scatter_batch<false, true>), g, dim3(256), 0, s, args, m);
+    if (first_pass) {
+        if (last_pass) hipLaunchKernelGGL((k_sort_scatter_batch<true, true>), g, dim3(256), 0, s, args, m);
+        else hipLaunchKernelGGL((k_sort_scatter_batch<true, false>), g, dim3(256), 0, s, args, m);
+    } else {
+        if (last_pass) hipLaunchKernelGGL((k_sort_scatter_batch<false, true>), g, dim3(256), 0, s, args, m);
+        else hipLaunchKernelGGL((k_sort_scatter_batch<false, false>), g, dim3(256), 0, s, args, m);
+    }
     return hipGetLastError();
 }
 

@@ -48,6 +48,29 @@ hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
 uint32_t stream_blocks(const StreamArgs& a);
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
+// G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
+// row_start[rho + 1] = first sorted sample whose G row is >= rho, rho in [-1, GH + 1].
+struct RowMap {
+    float range0, inv_dr, off_r;
+    int fr, GH;
+    uint32_t* row_start;  // GH + 3 entries, pre-filled with n; nullptr: no row table
+};
+
+// Sort key of the grid (MFMA) form.  K = (G row + 2) + bin slot, where the
+// bin slot is 0 below the Binner's range, key + 1 inside it and n + 1 above
+// it (NaN included).  Both terms are non-decreasing in travel, so samples of
+// equal K share their bin and their G row, and the stable order of K is the
+// travel order up to a permutation inside each run of equal (bin, G row):
+// keys, segments, windows and the G-row table come out exactly as after the
+// travel sort, in ceil(bits(K) / 8) passes instead of 4.
+struct GroupKey {
+    int on;               // 0: sort by travel image
+    RowMap rows;
+    KeyParams key;
+};
+// largest K of a grouping (the sort runs as many 8-bit passes as it needs)
+inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }
+
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
 constexpr uint32_t kSortTile = 2048;     // items per sort workgroup (256 lanes x 8)
 // per-tile digit counts of one pass for `capacity` items
@@ -84,6 +107,7 @@ struct HistArgs {
     const uint32_t* kin;
     uint32_t n;
     int shift;
+    GroupKey gk;              // pass 0 key: K when gk.on, else the travel image
     uint32_t* counts;         // [256 digits][tiles]
     uint32_t tiles;
     uint32_t* zero;           // pass 0 only: cleared / filled arrays
@@ -108,19 +132,23 @@ struct ScatterArgs {
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;
+    GroupKey gk;              // as HistArgs; with gk.on the last pass writes G row + 2 as its key
     SortOutputs out;          // last pass only
 };
-// Arguments and workgroup counts of the 12 launches of one sort.
+// Arguments and workgroup counts of the 3 launches per pass of one sort.
 struct SortPlan {
+    int passes;               // 4 for travel images, fewer for a grouping key
     HistArgs hist[4];
     ScanArgs scan[4];
     ScatterArgs scatter[4];
     uint32_t hist_blocks;     // per pass (pass 0: >= 1 when it clears arrays)
     uint32_t pass_blocks;     // scan: 256 per pass, scatter: tiles (0 when n = 0)
     const uint32_t* order;    // permutation after the sort
-    const uint32_t* img;      // sorted travel images
+    const uint32_t* img;      // sorted travel images (grouping: G row + 2 per sorted sample)
 };
-hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p);
+// gk == nullptr or !gk->on: sort by travel
+hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
+                          const GroupKey* gk, SortPlan* p);
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
 // Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
 // buffers that hold the permutation and the sorted travel images.
@@ -132,16 +160,8 @@ hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMa
                                   uint32_t blocks, hipStream_t s);
 hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks,
                                   hipStream_t s);
-hipError_t launch_sort_scatter_batch(int pass, const ScatterArgs* args, BatchMap m,
-                                     uint32_t blocks, hipStream_t s);
-
-// G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
-// row_start[rho + 1] = first sorted sample whose G row is >= rho, rho in [-1, GH + 1].
-struct RowMap {
-    float range0, inv_dr, off_r;
-    int fr, GH;
-    uint32_t* row_start;  // GH + 3 entries, pre-filled with n; nullptr: no row table
-};
+hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, const ScatterArgs* args,
+                                     BatchMap m, uint32_t blocks, hipStream_t s);
 
 // ---- segments + windows (+ the grid row table) after the sort ----------------
 // k_bounds: segments (Binner.cu:51-80; seg zeroed beforehand) and the first
@@ -149,7 +169,8 @@ struct RowMap {
 // extra workgroup closing the row table (suffix minimum).
 struct BinArgs {
     const uint32_t* skeys;
-    const uint32_t* img;
+    const uint32_t* img;      // sorted travel images, or G row + 2 when img_is_row
+    int img_is_row;
     uint32_t n;
     uint32_t N;
     int ov;

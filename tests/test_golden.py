@@ -83,7 +83,10 @@ def test_gpu_matches_golden(gpu, name):
     for form in forms:
         s.compute_output(form)
         d = s.binner_debug()
-        assert np.array_equal(d["order"], exp["order"]), form
+        # the grid (MFMA) forms sort by the grouping key: same keys, segments
+        # and windows, order permuted inside runs of equal (bin, G row)
+        order = exp["order"] if form == 0 else O.group_order(cfg, exp["records32"][:, 2])
+        assert np.array_equal(d["order"], order), form
         assert np.array_equal(d["keys"], exp["keys"]), form
         assert np.array_equal(d["segments"].ravel(), exp["segments"].ravel()), form
         assert np.array_equal(d["windows"].ravel(), exp["windows"].ravel()), form

@@ -167,3 +167,43 @@ def test_mfma_bitwise_reproducible(gpu, form):
     s.compute_output(form)
     b = s.output().view(np.uint32)
     assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_grouped_binning_full_size(gpu, form):
+    """The grid form sorts by the grouping key (bin slot + G row): keys,
+    segments and windows bit-exact against the oracle's travel-sorted Binner,
+    the order equal to the oracle's grouped order (stable by bin slot, G row),
+    the sorted records their permutation."""
+    R = gpu
+    cfg, scene, S, seed = configs.workload("fls_512x1024")
+    hits = R.generate_hits(scene, seed, S)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    rec = s.samples()
+    b = O.binning(cfg, rec)
+    d = s.binner_debug()
+    og = O.group_order(cfg, rec[:, 2])
+    assert np.array_equal(d["order"], og), "grouped order"
+    for key in ("keys", "segments", "windows"):
+        assert np.array_equal(d[key], b[key]), key
+    assert np.array_equal(s.samples(sorted=True).view(np.uint32), rec[og].view(np.uint32))
+
+
+@pytest.mark.parametrize("form", FORMS)
+def test_mfma_ping_independent_of_sample_order(gpu, form):
+    """Permuting the input records changes the order inside every (bin, G row)
+    run but not the ping: the deposit sums in exact fixed point, so the two
+    pings agree bit for bit."""
+    R = gpu
+    cfg = configs._fls(128, 256, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "perm")
+    hits = R.generate_hits(configs.fls_scene(), 27, 150_000)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    a = s.output().view(np.uint32).copy()
+    rec = s.samples()
+    perm = np.random.default_rng(3).permutation(rec.shape[0])
+    s.upload_samples(np.ascontiguousarray(rec[perm]))
+    s.compute_output(form)
+    assert not np.array_equal(s.binner_debug()["order"], O.group_order(cfg, rec[:, 2]))
+    assert np.array_equal(s.output().view(np.uint32), a)
