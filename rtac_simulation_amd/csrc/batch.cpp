@@ -66,9 +66,10 @@ struct Batch {
     Stage<StreamArgs> stream;
     Stage<HistArgs> hist[4];
     Stage<ScanArgs> scan[4];
-    Stage<ScatterArgs> scatter[4];        // passes before a ping's last
-    Stage<ScatterArgs> scatter_last[4];   // a ping's last pass (its sort may take 1-4)
-    Stage<BinArgs> bounds, windows;
+    // [pass][last pass of its ping][records move with the keys]: a ping's
+    // sort takes 1-4 passes, in one of two scatter variants
+    Stage<ScatterArgs> scatter[4][2][2];
+    Stage<BinArgs> windows;                // plain pings (the grid form's are in its deposit)
     Stage<ConvArgs> plain2d, plain1d;
     Stage<DepositArgs> deposit;
     std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape)
@@ -120,10 +121,9 @@ static int build(Batch* b)
     for (int k = 0; k < 4; k++) {
         b->hist[k] = {};
         b->scan[k] = {};
-        b->scatter[k] = {};
-        b->scatter_last[k] = {};
+        for (auto& sl : b->scatter[k])
+            for (auto& sv : sl) sv = {};
     }
-    b->bounds = {};
     b->windows = {};
     b->plain2d = {};
     b->plain1d = {};
@@ -142,12 +142,11 @@ static int build(Batch* b)
         for (int k = 0; k < p.sort.passes; k++) {
             b->hist[k].add(p.sort.hist[k], k == 0 ? p.sort.hist_blocks : p.sort.pass_blocks);
             b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks ? 256u : 0u);
-            (k == p.sort.passes - 1 ? b->scatter_last[k] : b->scatter[k])
+            b->scatter[k][k == p.sort.passes - 1][p.sort.scatter[k].move ? 1 : 0]
                 .add(p.sort.scatter[k], p.sort.pass_blocks);
         }
-        b->bounds.add(p.bins, bounds_blocks(p.bins));
-        b->windows.add(p.bins, windows_blocks(p.bins));
         if (!p.grid) {
+            b->windows.add(p.bins, windows_blocks(p.bins));
             if (s->kind == RTSIM_SENSOR_2D) {
                 b->plain2d.add(p.conv, conv2d_plain_blocks(p.conv));
                 b->plain2d.shm = std::max(b->plain2d.shm, plain2d_lds_bytes(p.conv));
@@ -170,7 +169,6 @@ static int build(Batch* b)
             s->last_corr = p.corr;
         }
         s->order_ptr = p.sort.order;
-        s->img_ptr = p.sort.img;
         en.generation = s->generation;
     }
     // one device buffer for every descriptor array and prefix table
@@ -179,10 +177,9 @@ static int build(Batch* b)
     for (int k = 0; k < 4; k++) {
         layout(b->hist[k], off);
         layout(b->scan[k], off);
-        layout(b->scatter[k], off);
-        layout(b->scatter_last[k], off);
+        for (auto& sl : b->scatter[k])
+            for (auto& sv : sl) layout(sv, off);
     }
-    layout(b->bounds, off);
     layout(b->windows, off);
     layout(b->plain2d, off);
     layout(b->plain1d, off);
@@ -201,10 +198,9 @@ static int build(Batch* b)
     for (int k = 0; k < 4; k++) {
         pack(b->hist[k], host);
         pack(b->scan[k], host);
-        pack(b->scatter[k], host);
-        pack(b->scatter_last[k], host);
+        for (const auto& sl : b->scatter[k])
+            for (const auto& sv : sl) pack(sv, host);
     }
-    pack(b->bounds, host);
     pack(b->windows, host);
     pack(b->plain2d, host);
     pack(b->plain1d, host);
@@ -225,12 +221,13 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
         RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]),
                                       b->hist[k].blocks(), s));
         RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
-        RT_TRY(launch_sort_scatter_batch(k == 0, false, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
-                                         b->scatter[k].blocks(), s));
-        RT_TRY(launch_sort_scatter_batch(k == 0, true, dargs(b, b->scatter_last[k]),
-                                         dmap(b, b->scatter_last[k]), b->scatter_last[k].blocks(), s));
+        for (int last = 0; last < 2; last++)
+            for (int mv = 0; mv < 2; mv++) {
+                const Stage<ScatterArgs>& st = b->scatter[k][last][mv];
+                RT_TRY(launch_sort_scatter_batch(k == 0, last == 1, mv == 1, dargs(b, st), dmap(b, st),
+                                                 st.blocks(), s));
+            }
     }
-    RT_TRY(launch_bounds_batch(dargs(b, b->bounds), dmap(b, b->bounds), b->bounds.blocks(), s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
                                      b->plain2d.shm, s));

@@ -140,6 +140,7 @@ void Sensor::free_samples()
 {
     dev_free(d_samples);
     dev_free(d_sorted);
+    dev_free(d_tmp);
     dev_free(d_coord);
     dev_free(d_kA);
     dev_free(d_kB);
@@ -159,6 +160,7 @@ int Sensor::reserve(size_t count)
     int rc;
     if ((rc = dev_alloc(&d_samples, count))) return rc;
     if ((rc = dev_alloc(&d_sorted, count))) return rc;
+    if ((rc = dev_alloc(&d_tmp, count))) return rc;
     if ((rc = dev_alloc(&d_coord, count))) return rc;
     if ((rc = dev_alloc(&d_kA, count))) return rc;
     if ((rc = dev_alloc(&d_kB, count))) return rc;
@@ -302,8 +304,9 @@ int Sensor::plan(int form, PingPlan* p)
     sb.counts = d_counts;
     sb.totals = d_totals;
     sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
-    sb.zero = reinterpret_cast<uint32_t*>(d_seg);   // Binner segments start zeroed
-    sb.zero_count = 2 * (size_t)N;
+    sb.tmp = d_tmp;
+    sb.seg_init = d_seg;                              // Binner segments start at (~0, 0)
+    sb.seg_count = N;
     sb.fill = rows.row_start;                         // grid row table starts at n
     sb.fill_value = n;
     sb.fill_count = rows.row_start ? (size_t)rows.GH + 3 : 0;
@@ -312,6 +315,7 @@ int Sensor::plan(int form, PingPlan* p)
     sb.out.coord = d_coord;
     sb.out.key = key;
     sb.out.axis = axis;
+    sb.out.seg = d_seg;
     // the grid form only needs samples grouped by bin and G row (its deposit
     // sums in exact fixed point, independent of the order inside a group)
     GroupKey gk{};
@@ -326,19 +330,19 @@ int Sensor::plan(int form, PingPlan* p)
     hipError_t e = make_sort_plan(d_samples, n, sb, &gk, &p->sort);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
 
-    // Binner::compute (Binner.h:96-102): segments, windows
+    // Binner::compute (Binner.h:96-102): the sort bounded the segments, the
+    // windows follow (the grid form computes them inside its deposit launch)
     BinArgs& ba = p->bins;
-    ba.skeys = d_skeys;
-    ba.img = p->sort.img;
-    ba.img_is_row = gk.on;
-    ba.n = n;
     ba.N = N;
     ba.ov = overlap;
     ba.seg = d_seg;
     ba.win = d_win;
-    ba.rows = rows;
 
-    if (grid) return grid_plan(form, &p->dep, &p->corr);
+    if (grid) {
+        if ((rc = grid_plan(form, &p->dep, &p->corr))) return rc;
+        p->dep.bins = ba;
+        return RTSIM_OK;
+    }
     p->conv = conv_args();
     return RTSIM_OK;
 }
@@ -352,12 +356,11 @@ int Sensor::compute(int form, hipStream_t s)
     stage_begin(1, s);
     HIP_TRY(launch_sort_plan(p.sort, s));
     order_ptr = p.sort.order;
-    img_ptr = p.sort.img;
-    stage_end(s);
-    stage_begin(2, s);
-    HIP_TRY(launch_bins(p.bins, s));
     stage_end(s);
     if (!p.grid) {
+        stage_begin(2, s);
+        HIP_TRY(launch_windows(p.bins, s));
+        stage_end(s);
         stage_begin(4, s);
         if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(p.conv, s));
         else HIP_TRY(launch_conv1d_plain(p.conv, s));

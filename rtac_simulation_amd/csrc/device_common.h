@@ -20,12 +20,6 @@ __device__ __forceinline__ uint32_t travel_image(float t)
     return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
 }
 
-__device__ __forceinline__ float image_travel(uint32_t img)
-{
-    uint32_t u = (img & 0x80000000u) ? (img & 0x7fffffffu) : ~img;
-    return __uint_as_float(u);
-}
-
 // Binner::Key::operator() (Binner.h:47-53): k = (int)fmaf(a, t, b) truncated
 // toward zero (saturating), clamps compare the index with the margin bounds.
 // NaN travel is out of range.
@@ -80,6 +74,31 @@ __device__ __forceinline__ uint32_t bin_slot(const KeyParams& k, float t)
 __device__ __forceinline__ uint32_t group_key(const GroupKey& g, float t)
 {
     return (uint32_t)(g_row_of(g.rows, t) + 2) + bin_slot(g.key, t);
+}
+
+// do_make_bins (Binner.cu:82-106) for output bin blk * 256 + threadIdx.x.
+// A segment is empty when x >= y: the sort's last pass leaves (~0, 0) on keys
+// it never writes.  Each thread writes its own empty segment back as (0, 0),
+// the reference's memset value (Binner.cu:56-58); other threads may read that
+// entry in either state, and both read as empty.
+__device__ __forceinline__ void windows_body(const BinArgs& a, uint32_t blk)
+{
+    const int tid = blk * 256 + threadIdx.x;
+    if (tid >= (int)a.N) return;
+    const uint2* seg = a.seg;
+    auto empty = [&](int k) {
+        const uint2 s = seg[k];
+        return s.x >= s.y;
+    };
+    int istart = max(0, tid - a.ov);
+    int iend = min((int)a.N - 1, tid + a.ov);
+    while (istart < iend && empty(istart)) istart++;
+    while (iend > istart && empty(iend)) iend--;
+    if (istart >= iend)
+        a.win[tid] = make_uint2(0u, 0u);
+    else
+        a.win[tid] = make_uint2(seg[istart].x, seg[iend].y - seg[istart].x);
+    if (empty(tid)) a.seg[tid] = make_uint2(0u, 0u);
 }
 
 // Number of set bits of `mask` in lanes strictly below the calling lane.

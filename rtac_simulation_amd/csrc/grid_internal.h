@@ -25,7 +25,8 @@ struct DepositArgs {
     const float2* coord;       // bearing-axis coordinate (hi, lo) per sorted sample
     const uint32_t* skeys;     // OutOfRange samples are skipped
     uint32_t n;
-    uint32_t* row_start;       // GH + 3 entries (built after the sort, see RowMap)
+    const uint32_t* row_start; // GH + 3 entries: first sample of each G row present, else n
+    BinArgs bins;              // the launch's last workgroups compute the windows
     float2* grid;              // [GH][GW]
     int GH, GW;
     int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)

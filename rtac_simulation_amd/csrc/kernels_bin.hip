@@ -3,8 +3,9 @@
 // Reference semantics (rtac_simulation source tree):
 //   k_stream   : RayCaster.cu:62-74 (phase, spreading) + Receiver.h:51-60
 //                (set_sample) + Directivity.h:30-34 (lookup) + Sample.h:144-160
-//   sort       : Receiver.cu:9-42 (thrust::sort by travel)
-//   k_gather   : Binner.cu:24-49 (keys) + Binner.cu:51-80 (segments)
+//   sort       : Receiver.cu:9-42 (thrust::sort by travel), whose last pass
+//                writes the keys (Binner.cu:24-49) and bounds the segments
+//                (Binner.cu:51-80), kernels_sort.hip
 //   k_windows  : Binner.cu:82-136 (views with the empty-window rule)
 #include "device_common.h"
 
@@ -29,11 +30,12 @@ __device__ __forceinline__ int mirror_index(int i, int n)
 }
 
 // tex2D<float> with linear filtering, mirror addressing, normalised
-// coordinates (Directivity.cpp:109-113), weights in full FP32.
-__device__ __forceinline__ float directivity_lookup(const StreamArgs& p, float lx, float ly,
+// coordinates (Directivity.cpp:109-113), weights in full FP32; `az` is
+// atan2(ly, lx).
+__device__ __forceinline__ float directivity_lookup(const StreamArgs& p, float az, float lx,
                                                    float lz)
 {
-    float u = mirror_coord(p.dir_xs * atan2f(ly, lx));
+    float u = mirror_coord(p.dir_xs * az);
     float v = mirror_coord(p.dir_ys * atan2f(lz, lx));
     float xb = u * (float)p.dir_w - 0.5f, yb = v * (float)p.dir_h - 0.5f;
     float fi = floorf(xb), fj = floorf(yb);
@@ -51,40 +53,65 @@ __device__ __forceinline__ float directivity_lookup(const StreamArgs& p, float l
 
 constexpr uint32_t kStreamMaxBlocks = 256 * 64;
 
+// set_sample (Receiver.h:51-60) of one ray hit
+__device__ __forceinline__ float4 stream_sample(const StreamArgs& p, float r, float d0, float d1,
+                                                float d2, float2 A)
+{
+    if (r < 1.0e-4f) {
+        // set_null_sample (Receiver.h:62-65): value 0, travel -1
+        return make_float4(0.f, 0.f, -1.f, 0.f);
+    }
+    float lx = p.R[0] * d0 + p.R[3] * d1 + p.R[6] * d2;
+    float ly = p.R[1] * d0 + p.R[4] * d1 + p.R[7] * d2;
+    float lz = p.R[2] * d0 + p.R[5] * d1 + p.R[8] * d2;
+    const float az = atan2f(ly, lx);
+    float D = p.dir ? directivity_lookup(p, az, lx, lz) : 1.0f;
+    // round-trip phase 2 k r reduced modulo one wavelength in FP64
+    double q = (double)r * p.k2;
+    q -= floor(q);
+    float sn, cs;
+    sincospif(2.0f * (float)q, &sn, &cs);
+    float vr = A.x * cs - A.y * sn;
+    float vi = A.x * sn + A.y * cs;
+    if (p.spreading) {
+        float g = 1.0f / (r * r);
+        vr *= g;
+        vi *= g;
+    }
+    float4 o;
+    o.x = vr * D;
+    o.y = vi * D;
+    o.z = r;
+    o.w = p.is2d ? -az : 0.0f;   // atan2(-ly, lx): atan2 is odd in y
+    return o;
+}
+
+// Each thread streams kStreamU hits (loads of all of them in flight before
+// any is computed); workgroups stride over the hits.  One hit per thread
+// measured fastest (2 and 4 are 5-10 % slower: fewer resident waves hide
+// the sample's dependent table lookups and FP64 phase less well).
+constexpr int kStreamU = 1;
+
 __device__ __forceinline__ void stream_body(const StreamArgs& p, uint32_t blk, uint32_t nblk)
 {
-    const uint32_t stride = nblk * 256;
-    for (uint32_t i = blk * 256 + threadIdx.x; i < p.n; i += stride) {
-        float r = p.range[i];
-        float4 o;
-        if (r < 1.0e-4f) {
-            // set_null_sample (Receiver.h:62-65): value 0, travel -1
-            o = make_float4(0.f, 0.f, -1.f, 0.f);
-        } else {
-            float d0 = p.dx[i], d1 = p.dy[i], d2 = p.dz[i];
-            float lx = p.R[0] * d0 + p.R[3] * d1 + p.R[6] * d2;
-            float ly = p.R[1] * d0 + p.R[4] * d1 + p.R[7] * d2;
-            float lz = p.R[2] * d0 + p.R[5] * d1 + p.R[8] * d2;
-            float D = p.dir ? directivity_lookup(p, lx, ly, lz) : 1.0f;
-            // round-trip phase 2 k r reduced modulo one wavelength in FP64
-            double q = (double)r * p.k2;
-            q -= floor(q);
-            float sn, cs;
-            sincospif(2.0f * (float)q, &sn, &cs);
-            float2 A = p.amp[i];
-            float vr = A.x * cs - A.y * sn;
-            float vi = A.x * sn + A.y * cs;
-            if (p.spreading) {
-                float g = 1.0f / (r * r);
-                vr *= g;
-                vi *= g;
-            }
-            o.x = vr * D;
-            o.y = vi * D;
-            o.z = r;
-            o.w = p.is2d ? atan2f(-ly, lx) : 0.0f;
+    constexpr uint32_t span = 256 * kStreamU;
+    for (uint32_t base = blk * span; base < p.n; base += nblk * span) {
+        float r[kStreamU], d0[kStreamU], d1[kStreamU], d2[kStreamU];
+        float2 A[kStreamU];
+#pragma unroll
+        for (int u = 0; u < kStreamU; u++) {
+            const uint32_t i = min(base + u * 256 + threadIdx.x, p.n - 1);
+            r[u] = p.range[i];
+            d0[u] = p.dx[i];
+            d1[u] = p.dy[i];
+            d2[u] = p.dz[i];
+            A[u] = p.amp[i];
         }
-        p.out[i] = o;
+#pragma unroll
+        for (int u = 0; u < kStreamU; u++) {
+            const uint32_t i = base + u * 256 + threadIdx.x;
+            if (i < p.n) p.out[i] = stream_sample(p, r[u], d0[u], d1[u], d2[u], A[u]);
+        }
     }
 }
 
@@ -99,7 +126,7 @@ __global__ __launch_bounds__(256) void k_stream_batch(const StreamArgs* __restri
 
 uint32_t stream_blocks(const StreamArgs& a)
 {
-    const uint32_t blocks = (a.n + 255) / 256;
+    const uint32_t blocks = (a.n + 256 * kStreamU - 1) / (256 * kStreamU);
     return blocks > kStreamMaxBlocks ? kStreamMaxBlocks : blocks;
 }
 
@@ -136,89 +163,10 @@ hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_
 }
 
 // ---------------------------------------------------------------------------
-// segments (Binner.cu:51-80) and the grid row table, over the sorted keys
+// windows: do_make_bins (Binner.cu:82-106); the segments (Binner.cu:51-80)
+// were bounded by the sort's last pass (kernels_sort.hip)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void bounds_body(const BinArgs& a, uint32_t blk)
-{
-    const uint32_t i = blk * 256 + threadIdx.x;
-    if (i >= a.n) return;
-    const uint32_t k = a.skeys[i];
-    if (k != kOutOfRange) {
-        if (i == 0)
-            a.seg[0].x = 0;  // Binner.cu:56-58 (equal to the zero fill)
-        else if (a.skeys[i - 1] != k)
-            a.seg[k].x = i;
-        if (i == a.n - 1)
-            a.seg[k].y = a.n;
-        else if (a.skeys[i + 1] != k)
-            a.seg[k].y = i + 1;
-    }
-    if (a.rows.row_start) {
-        // first sample of each G row present (the table holds n elsewhere)
-        auto row = [&](uint32_t j) {
-            return a.img_is_row ? (int)a.img[j] - 2 : g_row_of(a.rows, image_travel(a.img[j]));
-        };
-        const int r = row(i);
-        const int rp = i == 0 ? -3 : row(i - 1);
-        if (r != rp && r >= -1 && r <= a.rows.GH + 1) a.rows.row_start[r + 1] = i;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// windows: do_make_bins (Binner.cu:82-106); with a row table, the last
-// workgroup gives every row the minimum over itself and the rows above it
-// (rows without samples then start where the next row that has some starts)
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void windows_body(const BinArgs& a, uint32_t blk)
-{
-    const uint32_t win_blocks = (a.N + 255) / 256;
-    if (blk < win_blocks) {
-        const int tid = blk * 256 + threadIdx.x;
-        if (tid >= (int)a.N) return;
-        const uint2* seg = a.seg;
-        int istart = max(0, tid - a.ov);
-        int iend = min((int)a.N - 1, tid + a.ov);
-        while (istart < iend && seg[istart].y == seg[istart].x) istart++;
-        while (iend > istart && seg[iend].y == seg[iend].x) iend--;
-        if (istart >= iend)
-            a.win[tid] = make_uint2(0u, 0u);
-        else
-            a.win[tid] = make_uint2(seg[istart].x, seg[iend].y - seg[istart].x);
-        return;
-    }
-    // reverse inclusive min-scan of the GH + 3 row-table entries
-    __shared__ uint32_t part[256];
-    uint32_t* rs = a.rows.row_start;
-    const int count = a.rows.GH + 3;
-    const int t = threadIdx.x;
-    const int per = (count + 255) / 256;
-    const int beg = t * per, end = min(beg + per, count);
-    uint32_t m = 0xffffffffu;
-    for (int k = end - 1; k >= beg; k--) m = min(m, rs[k]);
-    part[t] = m;
-    __syncthreads();
-    for (int o = 1; o < 256; o <<= 1) {   // suffix minimum of the chunk minima
-        const uint32_t v = t + o < 256 ? part[t + o] : 0xffffffffu;
-        __syncthreads();
-        part[t] = min(part[t], v);
-        __syncthreads();
-    }
-    uint32_t run = t + 1 < 256 ? part[t + 1] : 0xffffffffu;
-    for (int k = end - 1; k >= beg; k--) {
-        run = min(run, rs[k]);
-        rs[k] = run;
-    }
-}
-
-__global__ __launch_bounds__(256) void k_bounds(BinArgs a) { bounds_body(a, blockIdx.x); }
 __global__ __launch_bounds__(256) void k_windows(BinArgs a) { windows_body(a, blockIdx.x); }
-
-__global__ __launch_bounds__(256) void k_bounds_batch(const BinArgs* __restrict__ args, BatchMap m)
-{
-    uint32_t blk, nblk;
-    const uint32_t p = batch_locate(m, blk, nblk);
-    bounds_body(args[p], blk);
-}
 
 __global__ __launch_bounds__(256) void k_windows_batch(const BinArgs* __restrict__ args, BatchMap m)
 {
@@ -227,23 +175,11 @@ __global__ __launch_bounds__(256) void k_windows_batch(const BinArgs* __restrict
     windows_body(args[p], blk);
 }
 
-uint32_t bounds_blocks(const BinArgs& a) { return (a.n + 255) / 256; }
-uint32_t windows_blocks(const BinArgs& a)
-{
-    return (a.N + 255) / 256 + (a.rows.row_start ? 1u : 0u);
-}
+uint32_t windows_blocks(const BinArgs& a) { return (a.N + 255) / 256; }
 
-hipError_t launch_bins(const BinArgs& a, hipStream_t s)
+hipError_t launch_windows(const BinArgs& a, hipStream_t s)
 {
-    if (bounds_blocks(a)) hipLaunchKernelGGL(k_bounds, dim3(bounds_blocks(a)), dim3(256), 0, s, a);
     if (windows_blocks(a)) hipLaunchKernelGGL(k_windows, dim3(windows_blocks(a)), dim3(256), 0, s, a);
-    return hipGetLastError();
-}
-
-hipError_t launch_bounds_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
-{
-    if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_bounds_batch, dim3(blocks), dim3(256), 0, s, args, m);
     return hipGetLastError();
 }
 

@@ -25,6 +25,32 @@ __device__ __forceinline__ float grid_y(const DepositArgs& a, float t)
     return grid_y_of(t, a.range0, a.inv_dr, a.off_r);
 }
 
+// First entry below n at index >= j0 of the G-row table (rows without samples
+// hold n), and the same from j0 + 2 (n when there is none): the first samples
+// of G rows >= j0 - 1 and >= j0 + 1.  Each wave scans 64 entries per step.
+__device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ marks, int count,
+                                           uint32_t n, int j0, uint32_t& s0, uint32_t& s1)
+{
+    const uint32_t lane = lane_id();
+    s0 = n;
+    s1 = n;
+    bool need0 = true;
+    for (int base = j0; base < count; base += 64) {
+        const int j = base + (int)lane;
+        const uint32_t m = j < count ? marks[j] : n;
+        const uint64_t hit = __ballot(m < n);
+        if (need0 && hit) {
+            s0 = __shfl(m, __ffsll((long long)hit) - 1, 64);
+            need0 = false;
+        }
+        const uint64_t hit1 = base == j0 ? hit & ~3ull : hit;
+        if (hit1) {
+            s1 = __shfl(m, __ffsll((long long)hit1) - 1, 64);
+            break;
+        }
+    }
+}
+
 // One workgroup per G row rho.  Its samples are the sorted run whose G row
 // is rho - 1 or rho (each sample's range weights go to rows gy and gy + 1);
 // each adds w_y w_x v to two columns of the row.  The row is accumulated in
@@ -46,8 +72,8 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
     unsigned long long* sre = reinterpret_cast<unsigned long long*>(sacc);
     unsigned long long* sim = sre + GW;
     for (int k = t; k < 2 * GW; k += 256) sacc[k] = 0;
-    const uint32_t s0 = a.row_start[rho];            // first sample with G row >= rho - 1
-    const uint32_t s1 = a.row_start[rho + 2];        // first sample with G row >= rho + 1
+    uint32_t s0, s1;   // first samples with G row >= rho - 1 and >= rho + 1
+    run_bounds(a.row_start, a.GH + 3, a.n, rho, s0, s1);
     constexpr int U = kDepositU;
 
     // ---- pass 1: largest component magnitude of the run -------------------
@@ -135,21 +161,29 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
         row[c] = make_float2(ldexpf((float)sacc[c], -k), ldexpf((float)sacc[GW + c], -k));
 }
 
-__global__ __launch_bounds__(256) void k_deposit(DepositArgs a) { deposit_body(a, blockIdx.x); }
+// workgroups [0, GH) deposit G rows, the rest compute the Binner windows
+// (do_make_bins, Binner.cu:82-106) for the correlation's empty-row mask
+__device__ __forceinline__ void deposit_or_windows(const DepositArgs& a, uint32_t blk)
+{
+    if (blk < (uint32_t)a.GH) deposit_body(a, (int)blk);
+    else windows_body(a.bins, blk - (uint32_t)a.GH);
+}
+
+__global__ __launch_bounds__(256) void k_deposit(DepositArgs a) { deposit_or_windows(a, blockIdx.x); }
 
 __global__ __launch_bounds__(256) void k_deposit_batch(const DepositArgs* __restrict__ args, BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    deposit_body(args[p], (int)blk);
+    deposit_or_windows(args[p], blk);
 }
 
-uint32_t deposit_blocks(const DepositArgs& a) { return (uint32_t)a.GH; }
+uint32_t deposit_blocks(const DepositArgs& a) { return (uint32_t)a.GH + windows_blocks(a.bins); }
 size_t deposit_lds_bytes(const DepositArgs& a) { return sizeof(long long) * 2 * a.GW; }
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_deposit, dim3(a.GH), dim3(256), deposit_lds_bytes(a), s, a);
+    hipLaunchKernelGGL(k_deposit, dim3(deposit_blocks(a)), dim3(256), deposit_lds_bytes(a), s, a);
     return hipGetLastError();
 }
 

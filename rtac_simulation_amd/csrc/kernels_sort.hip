@@ -84,8 +84,8 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     h[t] = 0u;
     if (FIRST) {
-        for (size_t i = (size_t)blk * 256 + t; i < a.zero_count; i += (size_t)nblk * 256)
-            a.zero[i] = 0u;
+        for (size_t i = (size_t)blk * 256 + t; i < a.seg_count; i += (size_t)nblk * 256)
+            a.seg_init[i] = make_uint2(0xffffffffu, 0u);
         for (size_t i = (size_t)blk * 256 + t; i < a.fill_count; i += (size_t)nblk * 256)
             a.fill[i] = a.fill_value;
     }
@@ -154,7 +154,10 @@ __global__ __launch_bounds__(256) void k_sort_scan_batch(const ScanArgs* __restr
 }
 
 // ---- rank + stage + scatter ----------------------------------------------------
-template <bool FIRST, bool LAST>
+// MOVE: the records travel with their keys through every pass (staged in LDS,
+// written in digit runs), so the last pass reads them in order; otherwise the
+// last pass gathers them by sample index.
+template <bool FIRST, bool LAST, bool MOVE>
 __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t tile)
 {
     __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
@@ -163,6 +166,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     __shared__ uint32_t ws[4], ws2[4];
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
+    __shared__ float4 srec[MOVE ? kSortTile : 1];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t tb = tile * kSortTile;
     const int shift = a.shift;
@@ -175,7 +179,23 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 
     const uint32_t wbase = tb + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems], val[kSortLaneItems], lrank[kSortLaneItems];
-    load_tile<FIRST>(a.rec, a.kin, a.vin, a.gk, a.n, wbase, lane, key, val);
+    float4 rv[MOVE ? kSortLaneItems : 1];
+    if (MOVE && FIRST) {
+#pragma unroll
+        for (int j = 0; j < kSortLaneItems; j++) {
+            const uint32_t i = min(wbase + j * 64 + lane, a.n - 1);
+            rv[j] = a.rec[i];
+            key[j] = a.gk.on ? group_key(a.gk, rv[j].z) : travel_image(rv[j].z);
+            val[j] = i;
+        }
+    } else {
+        load_tile<FIRST>(a.rec, a.kin, a.vin, a.gk, a.n, wbase, lane, key, val);
+        if (MOVE) {
+#pragma unroll
+            for (int j = 0; j < kSortLaneItems; j++)
+                rv[j] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
+        }
+    }
     __syncthreads();
 
     // stable ranks inside each wave
@@ -227,6 +247,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             const uint32_t p = wcnt[w][d] + lrank[j];
             skey[p] = key[j];
             sval[p] = val[j];
+            if (MOVE) srec[p] = rv[j];
         }
     }
     __syncthreads();
@@ -250,42 +271,88 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         }
     }
     float4 rq[Q];
-    if (LAST) {
+    uint32_t bq[Q];
+    if (MOVE) {
+#pragma unroll
+        for (int u = 0; u < Q; u++) rq[u] = srec[min(t + u * kSortThreads, kSortTile - 1)];
+    } else if (LAST) {
 #pragma unroll
         for (int u = 0; u < Q; u++) rq[u] = a.rec[pq[u] < a.n ? vq[u] : 0u];
+    }
+    if (LAST) {
+#pragma unroll
+        for (int u = 0; u < Q; u++) bq[u] = bin_key(a.out.key, rq[u].z);
     }
 #pragma unroll
     for (int u = 0; u < Q; u++) {
         const uint32_t pos = pq[u];
         if (pos < a.n) {   // holds for every staged item unless the records changed
+            if (!LAST) a.kout[pos] = kq[u];   // the sorted keys themselves are not kept
             a.vout[pos] = vq[u];
-            if (!LAST || !a.gk.on) a.kout[pos] = kq[u];
+            if (MOVE && !LAST) a.rout[pos] = rq[u];
             if (LAST) {
                 const float4 r = rq[u];
-                if (a.gk.on) a.kout[pos] = (uint32_t)(g_row_of(a.gk.rows, r.z) + 2);
                 a.out.sorted[pos] = r;
-                a.out.skeys[pos] = bin_key(a.out.key, r.z);
+                a.out.skeys[pos] = bq[u];
                 const double x = a.out.axis == 1 ? sin((double)r.w) : (double)r.w;
                 const float xh = (float)x;
                 a.out.coord[pos] = make_float2(xh, (float)(x - (double)xh));
             }
         }
     }
+    if (!LAST) return;
+
+    // run bounds: the staged tile holds one run per digit (its items land at
+    // consecutive positions); inside a run, an item whose key (G row) differs
+    // from its predecessor's starts a key (row) run, one whose key differs
+    // from its successor's ends one.  The Binner keys are non-decreasing over
+    // the sorted order, so the smallest start and the largest end over all
+    // workgroups bound each segment.
+    uint32_t* rows = a.gk.on ? a.gk.rows.row_start : nullptr;
+    __syncthreads();   // every thread has read its staged keys and values
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+        const uint32_t q = t + u * kSortThreads;
+        if (pq[u] < a.n) {
+            sval[q] = bq[u];
+            if (rows) skey[q] = (uint32_t)g_row_of(a.gk.rows, rq[u].z);
+        }
+    }
+    __syncthreads();
+    uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+        const uint32_t pos = pq[u];
+        if (pos >= a.n) continue;
+        const uint32_t q = t + u * kSortThreads;
+        const uint32_t d = (kq[u] >> shift) & 255u;
+        const uint32_t r0 = toff[d], r1 = d == 255u ? tn : toff[d + 1];
+        const uint32_t k = bq[u];
+        if (k != kOutOfRange) {
+            if (q == r0 || sval[q - 1] != k) atomicMin(&seg[2 * k], pos);
+            if (q + 1 == r1 || sval[q + 1] != k) atomicMax(&seg[2 * k + 1], pos + 1);
+        }
+        if (rows) {
+            const int g = (int)skey[q];
+            if ((q == r0 || (int)skey[q - 1] != g) && g >= -1 && g <= a.gk.rows.GH + 1)
+                atomicMin(&rows[g + 1], pos);
+        }
+    }
 }
 
-template <bool FIRST, bool LAST>
+template <bool FIRST, bool LAST, bool MOVE>
 __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
 {
-    sort_scatter_body<FIRST, LAST>(a, blockIdx.x);
+    sort_scatter_body<FIRST, LAST, MOVE>(a, blockIdx.x);
 }
 
-template <bool FIRST, bool LAST>
+template <bool FIRST, bool LAST, bool MOVE>
 __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* __restrict__ args,
                                                             BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    sort_scatter_body<FIRST, LAST>(args[p], blk);
+    sort_scatter_body<FIRST, LAST, MOVE>(args[p], blk);
 }
 
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
@@ -295,10 +362,14 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
     GroupKey g{};
     if (gk && gk->on) g = *gk;
+    // 8-bit digits: 4 passes for travel images, ceil(bits(K) / 8) for a
+    // grouping key (its last pass then sees few digit values, so it writes
+    // long runs of consecutive positions)
     int passes = 4;
     if (g.on) {
-        const uint32_t kmax = group_key_max(g);
-        passes = kmax < 256u ? 1 : kmax < 65536u ? 2 : kmax < (1u << 24) ? 3 : 4;
+        int bits = 1;
+        while (bits < 32 && (group_key_max(g) >> bits)) bits++;
+        passes = (bits + 7) / 8;
     }
     p->passes = passes;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
@@ -311,8 +382,8 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         h.gk = g;
         h.counts = b.counts;
         h.tiles = tiles;
-        h.zero = pass == 0 ? b.zero : nullptr;
-        h.zero_count = pass == 0 ? b.zero_count : 0;
+        h.seg_init = pass == 0 ? b.seg_init : nullptr;
+        h.seg_count = pass == 0 ? b.seg_count : 0;
         h.fill = pass == 0 ? b.fill : nullptr;
         h.fill_value = b.fill_value;
         h.fill_count = pass == 0 ? b.fill_count : 0;
@@ -330,21 +401,25 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         c.tiles = tiles;
         c.gk = g;
         c.out = b.out;
+        // grouping: the records move through every pass, ending in out.sorted
+        // (the pass before writes tmp, the one before that out.sorted, ...)
+        c.move = g.on;
+        c.rin = pass == 0 ? records : ((passes - pass) % 2 ? b.tmp : b.out.sorted);
+        c.rout = (passes - 1 - pass) % 2 ? b.tmp : b.out.sorted;
         if (pass < passes) cur ^= 1;
     }
-    const bool clears = b.zero_count || b.fill_count;
+    const bool clears = b.seg_count || b.fill_count;
     p->hist_blocks = n ? tiles : (clears ? 1u : 0u);
     p->pass_blocks = n ? tiles : 0u;
     p->order = b.vals[n ? cur : 0];
-    p->img = b.keys[n ? cur : 0];
     return hipSuccess;
 }
 
-template <bool FIRST>
+template <bool FIRST, bool MOVE>
 static void launch_scatter(bool last, dim3 g, const ScatterArgs& a, hipStream_t s)
 {
-    if (last) hipLaunchKernelGGL((k_sort_scatter<FIRST, true>), g, dim3(256), 0, s, a);
-    else hipLaunchKernelGGL((k_sort_scatter<FIRST, false>), g, dim3(256), 0, s, a);
+    if (last) hipLaunchKernelGGL((k_sort_scatter<FIRST, true, MOVE>), g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_sort_scatter<FIRST, false, MOVE>), g, dim3(256), 0, s, a);
 }
 
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
@@ -361,21 +436,16 @@ hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
         hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, p.scan[pass]);
         const dim3 g(p.pass_blocks);
         const bool last = pass == p.passes - 1;
-        if (pass == 0) launch_scatter<true>(last, g, p.scatter[pass], s);
-        else launch_scatter<false>(last, g, p.scatter[pass], s);
+        const ScatterArgs& c = p.scatter[pass];
+        if (pass == 0) {
+            if (c.move) launch_scatter<true, true>(last, g, c, s);
+            else launch_scatter<true, false>(last, g, c, s);
+        } else {
+            if (c.move) launch_scatter<false, true>(last, g, c, s);
+            else launch_scatter<false, false>(last, g, c, s);
+        }
     }
     return hipGetLastError();
-}
-
-hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
-                       const uint32_t** order_out, const uint32_t** img_out, hipStream_t s)
-{
-    SortPlan p;
-    hipError_t e = make_sort_plan(records, n, b, nullptr, &p);
-    if (e != hipSuccess) return e;
-    *order_out = p.order;
-    *img_out = p.img;
-    return launch_sort_plan(p, s);
 }
 
 hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
@@ -394,17 +464,24 @@ hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blo
     return hipGetLastError();
 }
 
-hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, const ScatterArgs* args,
+template <bool FIRST, bool MOVE>
+static void launch_scatter_batch(bool last, dim3 g, const ScatterArgs* args, BatchMap m, hipStream_t s)
+{
+    if (last) hipLaunchKernelGGL((k_sort_scatter_batch<FIRST, true, MOVE>), g, dim3(256), 0, s, args, m);
+    else hipLaunchKernelGGL((k_sort_scatter_batch<FIRST, false, MOVE>), g, dim3(256), 0, s, args, m);
+}
+
+hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, bool move, const ScatterArgs* args,
                                      BatchMap m, uint32_t blocks, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
     const dim3 g(blocks);
     if (first_pass) {
-        if (last_pass) hipLaunchKernelGGL((k_sort_scatter_batch<true, true>), g, dim3(256), 0, s, args, m);
-        else hipLaunchKernelGGL((k_sort_scatter_batch<true, false>), g, dim3(256), 0, s, args, m);
+        if (move) launch_scatter_batch<true, true>(last_pass, g, args, m, s);
+        else launch_scatter_batch<true, false>(last_pass, g, args, m, s);
     } else {
-        if (last_pass) hipLaunchKernelGGL((k_sort_scatter_batch<false, true>), g, dim3(256), 0, s, args, m);
-        else hipLaunchKernelGGL((k_sort_scatter_batch<false, false>), g, dim3(256), 0, s, args, m);
+        if (move) launch_scatter_batch<false, true>(last_pass, g, args, m, s);
+        else launch_scatter_batch<false, false>(last_pass, g, args, m, s);
     }
     return hipGetLastError();
 }

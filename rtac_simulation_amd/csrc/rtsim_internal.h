@@ -49,7 +49,8 @@ uint32_t stream_blocks(const StreamArgs& a);
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
 // G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
-// row_start[rho + 1] = first sorted sample whose G row is >= rho, rho in [-1, GH + 1].
+// row_start[rho + 1] = first sorted sample whose G row is rho, rho in [-1, GH + 1]
+// (n when the row has none).
 struct RowMap {
     float range0, inv_dr, off_r;
     int fr, GH;
@@ -80,12 +81,19 @@ inline size_t sort_count_words(size_t capacity)
 }
 // Written by the last pass for every sorted position i: the record, its
 // Binner key (Binner.cu:24-49) and its bearing-axis coordinate as (hi, lo).
+// The last pass also bounds the runs of equal keys (Binner.cu:51-80): each
+// workgroup lowers seg[k].x to the first and raises seg[k].y past the last
+// position of every key run it writes (atomic min / max; the runs of one key
+// are contiguous, so the result is the segment), and with a grouping key it
+// lowers row_start[rho + 1] to the first position of every G row rho it
+// writes (rows without samples keep the fill value n).
 struct SortOutputs {
     float4* sorted;
     uint32_t* skeys;
     float2* coord;
     KeyParams key;
     int axis;             // 1: coordinate = sin(bearing) in FP64, else the bearing
+    uint2* seg;           // [key.n], initialised to (~0, 0) by the first pass
 };
 struct SortBuffers {
     uint32_t* keys[2];
@@ -93,9 +101,10 @@ struct SortBuffers {
     uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
     uint32_t* totals;     // [256] digit totals of the current pass
     uint32_t tiles_cap;
+    float4* tmp;          // records between passes when they move with their keys
     // prepared by the first sort kernel (before any later kernel reads them)
-    uint32_t* zero;       // zero[0..zero_count) = 0 (Binner segments)
-    size_t zero_count;
+    uint2* seg_init;      // seg_init[0..seg_count) = (~0, 0) (Binner segments, min / max)
+    size_t seg_count;
     uint32_t* fill;       // fill[0..fill_count) = fill_value (grid row table)
     uint32_t fill_value;
     size_t fill_count;
@@ -110,8 +119,8 @@ struct HistArgs {
     GroupKey gk;              // pass 0 key: K when gk.on, else the travel image
     uint32_t* counts;         // [256 digits][tiles]
     uint32_t tiles;
-    uint32_t* zero;           // pass 0 only: cleared / filled arrays
-    size_t zero_count;
+    uint2* seg_init;          // pass 0 only: initialised / filled arrays
+    size_t seg_count;
     uint32_t* fill;
     uint32_t fill_value;
     size_t fill_count;
@@ -132,7 +141,10 @@ struct ScatterArgs {
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;
-    GroupKey gk;              // as HistArgs; with gk.on the last pass writes G row + 2 as its key
+    GroupKey gk;              // as HistArgs
+    int move;                 // records move with their keys (grouping key)
+    const float4* rin;        // move: records in the order of kin (pass 0: the input)
+    float4* rout;             // move, not the last pass: records in the order of kout
     SortOutputs out;          // last pass only
 };
 // Arguments and workgroup counts of the 3 launches per pass of one sort.
@@ -144,44 +156,30 @@ struct SortPlan {
     uint32_t hist_blocks;     // per pass (pass 0: >= 1 when it clears arrays)
     uint32_t pass_blocks;     // scan: 256 per pass, scatter: tiles (0 when n = 0)
     const uint32_t* order;    // permutation after the sort
-    const uint32_t* img;      // sorted travel images (grouping: G row + 2 per sorted sample)
 };
 // gk == nullptr or !gk->on: sort by travel
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
                           const GroupKey* gk, SortPlan* p);
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
-// Sorts records[0..n) by travel.  On return *order_out / *img_out point to the
-// buffers that hold the permutation and the sorted travel images.
-hipError_t launch_sort(const float4* records, uint32_t n, const SortBuffers& b,
-                       const uint32_t** order_out, const uint32_t** img_out,
-                       hipStream_t s);
 // batched sort passes (args are device arrays, one entry per ping)
 hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
                                   uint32_t blocks, hipStream_t s);
 hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks,
                                   hipStream_t s);
-hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, const ScatterArgs* args,
+hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, bool move, const ScatterArgs* args,
                                      BatchMap m, uint32_t blocks, hipStream_t s);
 
-// ---- segments + windows (+ the grid row table) after the sort ----------------
-// k_bounds: segments (Binner.cu:51-80; seg zeroed beforehand) and the first
-// sample of every G row; k_windows: do_make_bins (Binner.cu:82-106), with one
-// extra workgroup closing the row table (suffix minimum).
+// ---- windows after the sort ------------------------------------------------------
+// do_make_bins (Binner.cu:82-106) over the segments the sort's last pass
+// bounded; every thread also writes its own empty segment back as (0, 0).
 struct BinArgs {
-    const uint32_t* skeys;
-    const uint32_t* img;      // sorted travel images, or G row + 2 when img_is_row
-    int img_is_row;
-    uint32_t n;
     uint32_t N;
     int ov;
     uint2* seg;
     uint2* win;
-    RowMap rows;
 };
-hipError_t launch_bins(const BinArgs& a, hipStream_t s);
-uint32_t bounds_blocks(const BinArgs& a);
+hipError_t launch_windows(const BinArgs& a, hipStream_t s);
 uint32_t windows_blocks(const BinArgs& a);
-hipError_t launch_bounds_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_windows_batch(const BinArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
 hipError_t launch_fill_u32(uint32_t* p, uint32_t value, size_t count, hipStream_t s);

@@ -65,11 +65,11 @@ struct Sensor {
     bool computed = false;
     float4* d_samples = nullptr;  // receivedSamples_ (unsorted)
     float4* d_sorted = nullptr;   // receivedSamples_ after sort()
+    float4* d_tmp = nullptr;      // records between sort passes (grid form)
     float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
     uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_vA = nullptr, *d_vB = nullptr;
     uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
     const uint32_t* order_ptr = nullptr;  // sort permutation (after compute)
-    const uint32_t* img_ptr = nullptr;    // sorted travel images (after compute)
     uint32_t* d_totals = nullptr; // sort: digit totals of a pass
     uint32_t* d_counts = nullptr; // sort: per-tile digit counts (capacity-sized)
 
