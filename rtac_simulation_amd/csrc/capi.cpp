@@ -291,7 +291,8 @@ int Sensor::plan(int form, PingPlan* p)
     p->form = form;
     p->grid = grid;
     RowMap rows{};
-    if (grid && (rc = grid_row_map(&rows))) return rc;   // also checks the grid form applies
+    uint32_t* vmax = nullptr;
+    if (grid && (rc = grid_row_map(&rows, &vmax))) return rc;   // also checks the grid form applies
 
     // sort (Receiver.cu:28-34); the last pass gathers the records, their keys
     // (Binner.cu:24-49) and bearing coordinates.  The plain form sorts by
@@ -323,6 +324,7 @@ int Sensor::plan(int form, PingPlan* p)
         gk.on = 1;
         gk.rows = rows;
         gk.key = key;
+        gk.vmax = vmax;
         if (group_key_max(gk) >= (1u << 24))
             return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
                         rows.GH, N);

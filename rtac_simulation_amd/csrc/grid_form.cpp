@@ -97,6 +97,7 @@ struct GridState {
     float tap_scale = 1.f;
     float2* d_grid = nullptr;
     uint32_t* d_row_start = nullptr;
+    uint32_t* d_vmax = nullptr;          // deposit scale (set by the sort)
     uint16_t* d_toep[2] = {nullptr, nullptr};   // Toeplitz tap rows, [0] bf16, [1] fp16
     size_t lds = 0;
 
@@ -104,6 +105,7 @@ struct GridState {
     {
         if (d_grid) (void)hipFree(d_grid);
         if (d_row_start) (void)hipFree(d_row_start);
+        if (d_vmax) (void)hipFree(d_vmax);
         for (auto p : d_toep)
             if (p) (void)hipFree(p);
     }
@@ -233,6 +235,8 @@ static int prepare_grid(Sensor* s, GridState* g)
         return hip_fail(err, "hipMalloc(grid)");
     if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
         return hip_fail(err, "hipMalloc(row_start)");
+    if ((err = hipMalloc((void**)&g->d_vmax, sizeof(uint32_t) * kVmaxSlots)) != hipSuccess)
+        return hip_fail(err, "hipMalloc(vmax)");
     for (int d = 0; d < 2; d++) {
         if ((err = hipMalloc((void**)&g->d_toep[d], sizeof(uint16_t) * tsize)) != hipSuccess)
             return hip_fail(err, "hipMalloc(toeplitz)");
@@ -257,7 +261,7 @@ int Sensor::ensure_grid()
     return RTSIM_OK;
 }
 
-int Sensor::grid_row_map(RowMap* m)
+int Sensor::grid_row_map(RowMap* m, uint32_t** vmax)
 {
     int rc = ensure_grid();
     if (rc) return rc;
@@ -268,6 +272,7 @@ int Sensor::grid_row_map(RowMap* m)
     m->fr = g->fr;
     m->GH = g->GH;
     m->row_start = g->d_row_start;
+    if (vmax) *vmax = g->d_vmax;
     return RTSIM_OK;
 }
 
@@ -283,6 +288,7 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     d.skeys = d_skeys;
     d.n = (uint32_t)count;
     d.row_start = g->d_row_start;
+    d.vmax = g->d_vmax;
     d.grid = g->d_grid;
     d.GH = g->GH;
     d.GW = g->GW;

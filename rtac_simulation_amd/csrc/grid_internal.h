@@ -26,6 +26,7 @@ struct DepositArgs {
     const uint32_t* skeys;     // OutOfRange samples are skipped
     uint32_t n;
     const uint32_t* row_start; // GH + 3 entries: first sample of each G row present, else n
+    const uint32_t* vmax;      // kVmaxSlots words whose maximum bounds |re|, |im| (FP32 bits)
     BinArgs bins;              // the launch's last workgroups compute the windows
     float2* grid;              // [GH][GW]
     int GH, GW;

@@ -56,61 +56,34 @@ __device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ marks, i
 // each adds w_y w_x v to two columns of the row.  The row is accumulated in
 // LDS as 64-bit fixed point: integer adds are exact and independent of their
 // order, so the grid is bit-reproducible (and integer LDS atomics run at the
-// rate of plain LDS traffic, unlike FP32 ones).  A first pass finds M, the
-// largest |re|, |im| of the run's in-range samples; products are scaled by
-// 2^k with (run length) M 2^k < 2^62, so no partial sum can overflow.  Each
-// product is rounded once, in FP32, then summed exactly.  A run holding a
-// non-finite value yields a NaN row.
+// rate of plain LDS traffic, unlike FP32 ones).  With M the largest finite
+// |re|, |im| over the ping's in-range samples (found by the sort), products
+// are scaled by 2^k with (run length) M 2^k < 2^62, so no partial sum can
+// overflow; each product is rounded once, in FP32, then summed exactly.  A
+// run holding a non-finite in-range value yields a NaN row.
 constexpr int kDepositU = 4;   // samples per thread whose loads are in flight together
 
 __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
 {
     extern __shared__ long long sacc[];               // [2][GW]: re plane, im plane
-    __shared__ float red[4];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int t = threadIdx.x;
     const int GW = a.GW;
     unsigned long long* sre = reinterpret_cast<unsigned long long*>(sacc);
     unsigned long long* sim = sre + GW;
     for (int k = t; k < 2 * GW; k += 256) sacc[k] = 0;
     uint32_t s0, s1;   // first samples with G row >= rho - 1 and >= rho + 1
     run_bounds(a.row_start, a.GH + 3, a.n, rho, s0, s1);
-    constexpr int U = kDepositU;
-
-    // ---- pass 1: largest component magnitude of the run -------------------
-    float mx = 0.f;
-    for (uint32_t sb = s0; sb < s1; sb += U * 256) {
-        uint32_t key[U];
-        float2 v[U];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t s = min(sb + u * 256 + t, s1 - 1);
-            key[u] = a.skeys[s];
-            const float4 r = a.sorted[s];
-            v[u] = make_float2(r.x, r.y);
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const float m = fmaxf(fabsf(v[u].x), fabsf(v[u].y));
-            // NaN propagates as +inf so the row is flagged below
-            const float mm = m == m ? m : INFINITY;
-            if (key[u] != kOutOfRange) mx = fmaxf(mx, mm);
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) red[wave] = mx;
-    __syncthreads();                                  // also orders the LDS clear
-    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    float2* row = a.grid + (size_t)rho * GW;
-    if (!(mx <= 3.402823466e38f)) {
-        for (int c = t; c < GW; c += 256) row[c] = make_float2(NAN, NAN);
-        return;
-    }
+    float M = __uint_as_float(a.vmax[threadIdx.x & (kVmaxSlots - 1)]);
+    for (int o = kVmaxSlots / 2; o > 0; o >>= 1) M = fmaxf(M, __shfl_xor(M, o, 64));
     int em = 0;
-    frexpf(mx, &em);                                  // mx < 2^em (em = 0 for mx = 0)
+    frexpf(M, &em);                                   // M < 2^em (em = 0 for M = 0)
     const int ec = 32 - __clz(max(s1 - s0, 1u));     // run length < 2^ec
     const int k = 62 - em - ec;
+    constexpr int U = kDepositU;
+    __syncthreads();                                  // LDS cleared
+    int bad = 0;
+    float2* row = a.grid + (size_t)rho * GW;
 
-    // ---- pass 2: fixed-point deposit -----------------------------------------
     for (uint32_t sb = s0; sb < s1; sb += U * 256) {
         uint32_t key[U];
         float4 rec[U];
@@ -137,6 +110,8 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
             // this row receives weight 1 - wy1 from gy == rho, wy1 from gy == rho - 1
             const float wy = gy == rho ? 1.0f - wy1 : (gy == rho - 1 ? wy1 : 0.0f);
             const bool ok = sb + u * 256 + t < s1 && key[u] != kOutOfRange;
+            bad |= ok && !(fmaxf(fabsf(rec[u].x), fabsf(rec[u].y)) <= 3.402823466e38f &&
+                           rec[u].x == rec[u].x && rec[u].y == rec[u].y);
             wx[u] = x - fx;
             col[u] = ok ? (int)fx + a.fb : -2;
             vr[u] = wy * rec[u].x;
@@ -156,7 +131,10 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
             }
         }
     }
-    __syncthreads();
+    if (__syncthreads_or(bad)) {
+        for (int c = t; c < GW; c += 256) row[c] = make_float2(NAN, NAN);
+        return;
+    }
     for (int c = t; c < GW; c += 256)
         row[c] = make_float2(ldexpf((float)sacc[c], -k), ldexpf((float)sacc[GW + c], -k));
 }

@@ -88,6 +88,7 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
             a.seg_init[i] = make_uint2(0xffffffffu, 0u);
         for (size_t i = (size_t)blk * 256 + t; i < a.fill_count; i += (size_t)nblk * 256)
             a.fill[i] = a.fill_value;
+        if (a.gk.on && blk == 0 && t < (uint32_t)kVmaxSlots) a.gk.vmax[t] = 0u;
     }
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
@@ -196,6 +197,21 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
                 rv[j] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
         }
     }
+    if (LAST && MOVE && a.gk.on) {
+        // largest finite component of the in-range samples (non-negative
+        // floats order like their bit patterns); issued first so that the
+        // atomic's round trip overlaps the rest of the pass
+        float m = 0.f;
+#pragma unroll
+        for (int j = 0; j < kSortLaneItems; j++) {
+            const float c = fmaxf(fabsf(rv[j].x), fabsf(rv[j].y));
+            if (wbase + j * 64 + lane < a.n && bin_key(a.out.key, rv[j].z) != kOutOfRange &&
+                c <= 3.402823466e38f)
+                m = fmaxf(m, c);
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (lane == 0) atomicMax(&a.gk.vmax[(tile * kSortWaves + w) % kVmaxSlots], __float_as_uint(m));
+    }
     __syncthreads();
 
     // stable ranks inside each wave
@@ -282,6 +298,43 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     if (LAST) {
 #pragma unroll
         for (int u = 0; u < Q; u++) bq[u] = bin_key(a.out.key, rq[u].z);
+        // run bounds, before the stores so that the atomics' round trips
+        // overlap them: the staged tile holds one run per digit (its items
+        // land at consecutive positions); inside a run, an item whose key (G
+        // row) differs from its predecessor's starts a key (row) run, one
+        // whose key differs from its successor's ends one.  The Binner keys
+        // are non-decreasing over the sorted order, so the smallest start and
+        // the largest end over all workgroups bound each segment.
+        uint32_t* rows = a.gk.on ? a.gk.rows.row_start : nullptr;
+        __syncthreads();   // every thread has read its staged keys and values
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const uint32_t q = t + u * kSortThreads;
+            if (pq[u] < a.n) {
+                sval[q] = bq[u];
+                if (rows) skey[q] = (uint32_t)g_row_of(a.gk.rows, rq[u].z);
+            }
+        }
+        __syncthreads();
+        uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
+#pragma unroll
+        for (int u = 0; u < Q; u++) {
+            const uint32_t pos = pq[u];
+            if (pos >= a.n) continue;
+            const uint32_t q = t + u * kSortThreads;
+            const uint32_t d = (kq[u] >> shift) & 255u;
+            const uint32_t r0 = toff[d], r1 = d == 255u ? tn : toff[d + 1];
+            const uint32_t k = bq[u];
+            if (k != kOutOfRange) {
+                if (q == r0 || sval[q - 1] != k) atomicMin(&seg[2 * k], pos);
+                if (q + 1 == r1 || sval[q + 1] != k) atomicMax(&seg[2 * k + 1], pos + 1);
+            }
+            if (rows) {
+                const int g = (int)skey[q];
+                if ((q == r0 || (int)skey[q - 1] != g) && g >= -1 && g <= a.gk.rows.GH + 1)
+                    atomicMin(&rows[g + 1], pos);
+            }
+        }
     }
 #pragma unroll
     for (int u = 0; u < Q; u++) {
@@ -298,44 +351,6 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
                 const float xh = (float)x;
                 a.out.coord[pos] = make_float2(xh, (float)(x - (double)xh));
             }
-        }
-    }
-    if (!LAST) return;
-
-    // run bounds: the staged tile holds one run per digit (its items land at
-    // consecutive positions); inside a run, an item whose key (G row) differs
-    // from its predecessor's starts a key (row) run, one whose key differs
-    // from its successor's ends one.  The Binner keys are non-decreasing over
-    // the sorted order, so the smallest start and the largest end over all
-    // workgroups bound each segment.
-    uint32_t* rows = a.gk.on ? a.gk.rows.row_start : nullptr;
-    __syncthreads();   // every thread has read its staged keys and values
-#pragma unroll
-    for (int u = 0; u < Q; u++) {
-        const uint32_t q = t + u * kSortThreads;
-        if (pq[u] < a.n) {
-            sval[q] = bq[u];
-            if (rows) skey[q] = (uint32_t)g_row_of(a.gk.rows, rq[u].z);
-        }
-    }
-    __syncthreads();
-    uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
-#pragma unroll
-    for (int u = 0; u < Q; u++) {
-        const uint32_t pos = pq[u];
-        if (pos >= a.n) continue;
-        const uint32_t q = t + u * kSortThreads;
-        const uint32_t d = (kq[u] >> shift) & 255u;
-        const uint32_t r0 = toff[d], r1 = d == 255u ? tn : toff[d + 1];
-        const uint32_t k = bq[u];
-        if (k != kOutOfRange) {
-            if (q == r0 || sval[q - 1] != k) atomicMin(&seg[2 * k], pos);
-            if (q + 1 == r1 || sval[q + 1] != k) atomicMax(&seg[2 * k + 1], pos + 1);
-        }
-        if (rows) {
-            const int g = (int)skey[q];
-            if ((q == r0 || (int)skey[q - 1] != g) && g >= -1 && g <= a.gk.rows.GH + 1)
-                atomicMin(&rows[g + 1], pos);
         }
     }
 }

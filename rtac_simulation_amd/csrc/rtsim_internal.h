@@ -64,10 +64,17 @@ struct RowMap {
 // travel order up to a permutation inside each run of equal (bin, G row):
 // keys, segments, windows and the G-row table come out exactly as after the
 // travel sort, in ceil(bits(K) / 8) passes instead of 4.
+// With a grouping key the last pass also raises vmax[0..kVmaxSlots) (FP32
+// bits, set to 0 by the first pass; each wave raises one slot, so that no
+// address takes more than a few dozen atomics) to the largest finite
+// |re|, |im| of the in-range samples: the deposit's fixed-point scale is the
+// maximum over the slots.
+constexpr int kVmaxSlots = 64;
 struct GroupKey {
     int on;               // 0: sort by travel image
     RowMap rows;
     KeyParams key;
+    uint32_t* vmax;       // kVmaxSlots words
 };
 // largest K of a grouping (the sort runs as many 8-bit passes as it needs)
 inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }

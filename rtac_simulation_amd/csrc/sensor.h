@@ -104,7 +104,7 @@ struct Sensor {
     int check_plain_limits() const;
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();
-    int grid_row_map(RowMap* m);    // grid (MFMA) form row table and G-row map
+    int grid_row_map(RowMap* m, uint32_t** vmax = nullptr);   // grid form row table, G-row map
     int compute(int form, hipStream_t s);
     int grid_plan(int form, DepositArgs* d, CorrArgs* c);
     int plan(int form, PingPlan* p);   // validates the form; queues nothing
