@@ -3,6 +3,7 @@
 // deposit + correlation launches.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -51,14 +52,6 @@ float host_f16_f32(uint16_t u)
     return (float)h;
 }
 
-// LDS row stride (elements) of the A planes for tile shape `sh`
-int shape_stride(int sh, int KB)
-{
-    int rows, cols;
-    corr_tile(sh, &rows, &cols);
-    return cols - 16 + 32 * KB + 8;
-}
-
 // Tuned tile shapes, cached per shape class for the whole process (a
 // sensor of the same grid and PSF size starts from the tuned shape).
 struct TuneKey {
@@ -88,17 +81,17 @@ struct GridState {
     bool ready = false;
     int status = RTSIM_OK;
     std::string why;
-    int Hp = 0, Wp = 0, GH = 0, GW = 0, KB = 0, fr = 0, fb = 0, BL = 0;
-    int shape = 1;                       // correlation tile shape in use
+    int Hp = 0, Wp = 0, GH = 0, GHa = 0, GW = 0, fr = 0, fb = 0;
+    int KR = 0, L = 0;                   // correlation K blocks, tap column length
+    int shape = 0;                       // correlation tile shape in use
     bool fits[kCorrShapes] = {false, false, false, false};
-    int tch[kCorrShapes] = {0, 0, 0, 0};  // tap chunk per shape
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
     float bear0_hi = 0, bear0_lo = 0;
     float tap_scale = 1.f;
     float2* d_grid = nullptr;
     uint32_t* d_row_start = nullptr;
     uint32_t* d_vmax = nullptr;          // deposit scale (set by the sort)
-    uint16_t* d_toep[2] = {nullptr, nullptr};   // Toeplitz tap rows, [0] bf16, [1] fp16
+    uint16_t* d_toep[2] = {nullptr, nullptr};   // tap columns, [0] bf16, [1] fp16
     size_t lds = 0;
 
     ~GridState()
@@ -155,10 +148,14 @@ static int prepare_grid(Sensor* s, GridState* g)
 
     g->Hp = (int)((s->N + kCorrRows - 1) / kCorrRows) * kCorrRows;
     g->Wp = (int)((s->W + kCorrCols - 1) / kCorrCols) * kCorrCols;
-    g->KB = ((int)s->Tb + 15 + 31) / 32;
+    g->KR = (31 + (int)s->Tr + 15) / 16;
+    g->L = 16 * g->KR + 32;
     g->GH = g->Hp + (int)s->Tr - 1;
-    g->GW = g->Wp - 16 + 32 * g->KB;   // Wp rounded to the widest tile: every shape fits
-    g->BL = 32 * g->KB + 16;
+    // the correlation stages 16 KR rows per 32-row tile: rows past GH exist and
+    // stay zero (the taps there are zero too, but the data must be finite)
+    g->GHa = std::max(g->GH, g->Hp - 32 + 16 * g->KR);
+    // columns the correlation stages for its last workgroup (even, read in pairs)
+    g->GW = g->Wp + (int)s->Tb - 1 + (((int)s->Tb - 1) & 1);
     g->fr = ((int)s->Tr - 1) / 2;
     g->fb = ((int)s->Tb - 1) / 2;
     g->off_r = (float)(0.5 * (s->Tr - 1) - g->fr);
@@ -174,19 +171,18 @@ static int prepare_grid(Sensor* s, GridState* g)
             hipSuccess || max_lds <= 0)
         max_lds = 64 * 1024;
     max_lds -= 64;   // the kernel's static LDS
-    // tile shapes whose LDS fits (with the tap chunk each can hold);
-    // default: the 32x32 tile when it fits
+    // tile shapes whose LDS fits; default: the first that does (32x32)
     g->shape = -1;
     for (int sh = 0; sh < kCorrShapes; sh++) {
         CorrArgs probe{};
         probe.Tr = (int)s->Tr;
         probe.Tb = (int)s->Tb;
         probe.shape = sh;
-        probe.AS = shape_stride(sh, g->KB);
-        probe.BL = g->BL;
-        g->tch[sh] = correlate_chunk(probe, (size_t)max_lds);
-        g->fits[sh] = g->tch[sh] > 0;
-        if (g->fits[sh] && (g->shape < 0 || sh == 1)) g->shape = sh;
+        probe.KR = g->KR;
+        probe.L = g->L;
+        probe.RS = corr_row_stride(sh, g->KR);
+        g->fits[sh] = correlate_lds_bytes(probe) <= (size_t)max_lds;
+        if (g->fits[sh] && g->shape < 0) g->shape = sh;
     }
     if (g->shape < 0)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: no tile shape fits %d B of LDS for %ux%u taps",
@@ -199,20 +195,21 @@ static int prepare_grid(Sensor* s, GridState* g)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
 
     // taps: power-of-two scale so max |T| in [0.5, 1), split hi/lo, laid out
-    // as Toeplitz rows [shift c][plane p][tap i][x] = T_p[i][x + c - 15]
-    // (zero outside [0, Tb)), planes re_hi, re_lo, im_hi, im_lo
+    // as padded tap columns [shift c][plane p][bearing tap j][x] =
+    // T_p[x + c - 31][j] (zero outside [0, Tr)), planes re_hi, re_lo, im_hi, im_lo
     const size_t T = (size_t)s->Tr * s->Tb;
     float mx = 0.f;
     for (size_t k = 0; k < 2 * T; k++) mx = std::fmax(mx, std::fabs(s->psf[k]));
     int e = 0;
     if (mx > 0.f) std::frexp(mx, &e);
     g->tap_scale = std::ldexp(1.0f, -e);
-    const size_t BL = (size_t)g->BL, Tr = s->Tr;
-    const size_t tsize = 2 * 4 * Tr * BL;
+    const size_t L = (size_t)g->L, Tb = s->Tb;
+    const size_t cs = (size_t)corr_copy_stride_host((int)Tb, (int)L);
+    const size_t tsize = 2 * cs;
     std::vector<uint16_t> toep[2];
     for (int d = 0; d < 2; d++) toep[d].assign(tsize, 0);
-    for (size_t i = 0; i < Tr; i++) {
-        for (size_t j = 0; j < s->Tb; j++) {
+    for (size_t i = 0; i < s->Tr; i++) {
+        for (size_t j = 0; j < Tb; j++) {
             for (int c = 0; c < 2; c++) {               // re, im
                 const float v = s->psf[2 * (i * s->Tb + j) + c] * g->tap_scale;
                 uint16_t planes[2][2];
@@ -223,16 +220,19 @@ static int prepare_grid(Sensor* s, GridState* g)
                 for (int d = 0; d < 2; d++)
                     for (int hl = 0; hl < 2; hl++)
                         for (size_t sh = 0; sh < 2; sh++) {
-                            const size_t x = j + 15 - sh;   // x + sh - 15 = j
+                            const size_t x = i + 31 - sh;   // x + sh - 31 = i
                             const size_t p = 2 * c + hl;
-                            toep[d][((sh * 4 + p) * Tr + i) * BL + x] = planes[d][hl];
+                            toep[d][sh * cs + (p * Tb + j) * L + x] = planes[d][hl];
                         }
             }
         }
     }
     hipError_t err;
-    if ((err = hipMalloc((void**)&g->d_grid, sizeof(float2) * (size_t)g->GH * g->GW)) != hipSuccess)
+    if ((err = hipMalloc((void**)&g->d_grid, sizeof(float2) * (size_t)g->GHa * g->GW)) != hipSuccess)
         return hip_fail(err, "hipMalloc(grid)");
+    // rows [GH, GHa) are read by the correlation and never written
+    if ((err = hipMemset(g->d_grid, 0, sizeof(float2) * (size_t)g->GHa * g->GW)) != hipSuccess)
+        return hip_fail(err, "hipMemset(grid)");
     if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
         return hip_fail(err, "hipMalloc(row_start)");
     if ((err = hipMalloc((void**)&g->d_vmax, sizeof(uint32_t) * kVmaxSlots)) != hipSuccess)
@@ -313,11 +313,10 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     c.Tb = (int)Tb;
     c.GH = g->GH;
     c.GW = g->GW;
-    c.KB = g->KB;
+    c.KR = g->KR;
+    c.L = g->L;
     c.shape = g->shape;
-    c.AS = shape_stride(g->shape, g->KB);
-    c.BL = g->BL;
-    c.TCH = g->tch[g->shape];
+    c.RS = corr_row_stride(g->shape, g->KR);
     c.tap_scale = g->tap_scale;
     return RTSIM_OK;
 }
@@ -342,8 +341,7 @@ int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, f
         if (!g->fits[sh]) continue;
         CorrArgs c = last_corr;
         c.shape = sh;
-        c.AS = shape_stride(sh, g->KB);
-        c.TCH = g->tch[sh];
+        c.RS = corr_row_stride(sh, g->KR);
         hipError_t err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);   // warm
         (void)hipEventRecord(e0, s);
         for (int it = 0; it < iterations && err == hipSuccess; it++)

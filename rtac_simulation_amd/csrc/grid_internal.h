@@ -12,11 +12,14 @@ namespace rtsim {
 
 constexpr int kCorrRows = 64;     // largest tile height of the correlation shapes
 constexpr int kCorrCols = 64;     // largest tile width of the correlation shapes
-constexpr int kCorrShapes = 4;    // {64x32, 32x32, 32x64, 64x16} (rows x bearings)
-constexpr size_t kCorrLdsTarget = 80 * 1024;   // two workgroups per CU
+constexpr int kCorrShapes = 4;    // {32x32, 32x64, 64x32, 64x64} (rows x bearings)
 
 // rows x bearings of correlation tile shape `shape`
 void corr_tile(int shape, int* rows, int* cols);
+// LDS row stride (16-bit elements per staged grid column) of `shape`
+int corr_row_stride(int shape, int KR);
+// stride (16-bit elements) between the two shifted copies of the tap table
+int corr_copy_stride_host(int Tb, int L);
 
 // Extended grid G: rho = grid row + fr, kappa = grid column + fb, so that
 // out(r, b) = sum T[i][j] G[r + i][b + j].
@@ -36,17 +39,16 @@ struct DepositArgs {
 };
 
 struct CorrArgs {
-    const float2* grid;        // [GH][GW]
-    const uint16_t* toep;      // Toeplitz tap rows [2 shifts][4 planes][Tr][BL]
+    const float2* grid;        // [GH + zero rows][GW]
+    const uint16_t* toep;      // tap columns [2 shifts][4 planes][Tb][L]
     const uint2* win;          // empty window -> masked row
     float2* out;               // [H][W]
     int H, W, Tr, Tb;
     int GH, GW;
-    int KB;                    // 32-wide K blocks per 16-wide N tile
+    int KR;                    // 16-row K blocks per 32-row output tile: ceil((31 + Tr) / 16)
+    int L;                     // length of one padded tap column (elements): 16 KR + 32
     int shape;                 // tile shape index (corr_tile)
-    int AS;                    // LDS row stride of the A planes (elements)
-    int BL;                    // length of one padded tap row (elements)
-    int TCH;                   // range taps whose Toeplitz rows are resident in LDS at once
+    int RS;                    // LDS row stride of the staged grid columns (corr_row_stride)
     float tap_scale;           // power of two applied to the taps
 };
 
@@ -62,10 +64,6 @@ uint32_t correlate_blocks(const CorrArgs& a);
 hipError_t launch_correlate_batch(bool fp16, int shape, const CorrArgs* args, BatchMap m,
                                   uint32_t blocks, size_t shm, hipStream_t s);
 size_t correlate_lds_bytes(const CorrArgs& a);
-// Tap chunk for a.shape: all Tr taps when that fits kCorrLdsTarget, else
-// the largest chunk that does, else the largest that fits max_lds; 0 when
-// even one tap row does not fit.
-int correlate_chunk(const CorrArgs& a, size_t max_lds);
 void init_corr_attributes(int max_lds);
 
 }  // namespace rtsim

@@ -1,36 +1,47 @@
 // Grid-form correlation on MFMA: out(r, b) = sum_{i < Tr, j < Tb} T[i][j] G[r + i][b + j]
 // (the extended grid G comes from k_deposit, kernels_grid.hip).
 //
-// MFMA mapping (v_mfma_f32_16x16x32_{bf16,f16}, wave64; lane l holds
-// A[l % 16][8 (l / 16) + p], B[8 (l / 16) + p][l % 16], D[4 (l / 16) + e][l % 16]):
-//   M = 16 output range rows, N = 16 output bearings, K = 32 grid columns.
-//   For range tap i, A[m][k] = G[r0 + m + i][c + k] and the Toeplitz
-//   B_i[k][n] = T[i][k - n] (zero outside [0, Tb)), so out += A B_i.
-//   Complex products use the real embedding (re = ArBr - AiBi,
-//   im = ArBi + AiBr); every real product is split hi+lo,
-//   X Y ~ XhYh + XlYh + XhYl, with FP32 accumulation.
+// For each bearing tap j the range taps form a Toeplitz GEMM:
+//   out(r0 + m, b0 + n) += sum_k A_j[m][k] B_j[k][n],
+//   A_j[m][k] = T[k - m][j] (zero outside [0, Tr)),  B_j[k][n] = G[r0 + k][b0 + n + j],
+// on v_mfma_f32_32x32x16_{bf16,f16} (wave64): M = 32 output rows, N = 32
+// bearings, K = 16 grid rows per instruction, KR = ceil((31 + Tr) / 16) K blocks
+// per tap column.  Lane l holds A[l % 32][8 (l / 32) + p], B[8 (l / 32) + p][l % 32]
+// and D[8 (e / 4) + 4 (l / 32) + e % 4][l % 32] (element e of 16).  On gfx950 this
+// instruction sustains 1.7x the flop rate of v_mfma_f32_16x16x32 (measured with
+// independent chains: 2.21 vs 1.32 PFLOP/s in FP16), which pays for the wider
+// Toeplitz band (31 taps over 64 K rows).
+// Complex products use the real embedding (re = ArBr - AiBi, im = ArBi + AiBr);
+// every real product is split hi+lo, X Y ~ XhYh + XlYh + XhYl, with FP32
+// accumulation in six independent chains (so that no MFMA waits for the
+// previous one's result).
 // Scaling: each workgroup scales its G tile by a power of two (max |G| in
-// [2^13, 2^14)), found from the staged values themselves, and the taps
-// carry a host power of two, so FP16 operands stay in range; the epilogue
-// divides both out exactly.  Rows whose reference window is empty
-// (Binner.cu:96-97) are written as zero.
+// [2^13, 2^14)), found from the staged values themselves, and the taps carry a
+// host power of two, so FP16 operands stay in range; the epilogue divides both
+// out exactly.  Rows whose reference window is empty (Binner.cu:96-97) are
+// written as zero.
 //
-// Tile shapes (template WR x WC waves, NT 16-wide N tiles per wave): a
-// workgroup computes 16 WR output rows x 16 NT WC bearings with 4 waves;
-// the shape is chosen per sensor by timing (Sensor::autotune).
+// Tile shapes (template TR x TC output tiles of 32 x 32 per workgroup, JS =
+// 4 / (TR TC) waves per tile, each taking an equal share of the tile's Tb KR
+// (tap, K block) steps; partial sums are added through LDS): chosen per sensor
+// by timing (Sensor::autotune).
 // LDS (16-bit elements):
-//   A: 4 planes (re_hi, re_lo, im_hi, im_lo) x AR rows x AS stride
-//   B: the Toeplitz rows of TCH range taps, copied from the prebuilt table
-//      [2 shifts][4 planes][Tr][BL] (shift c holds Tpad[x + c],
-//      Tpad[x] = T[x - 15]); the two shifts keep every lane's 8-element B
-//      fragment 4-byte aligned.
-// Staging: every thread issues all its grid loads (float4 = two complex
-// cells) and the B copy at once, so a workgroup pays one memory round trip
-// before its MFMA loop.  Per K step a wave reads its A fragments
-// (ds_read_b128) and the shared B fragment with pointer increments only;
-// fragments are double-buffered so the next step's reads overlap the
-// current step's MFMAs.
+//   G tile, transposed: [4 planes re_hi, re_lo, im_hi, im_lo][column][row],
+//     row stride RS = (rows rounded to 16) + 8, so that the 16-byte B fragment
+//     reads of 8 consecutive lanes (consecutive columns) fall in distinct banks;
+//   tap table, copied from the prebuilt [2 shifts][4 planes][Tb][L] (shift c
+//     holds Tpad[x + c], Tpad[x] = T[x - 31][j]; the shifts are
+//     corr_copy_stride apart): lane (m, h) of K block kb reads
+//     Tpad[16 kb + 8 h - m + 31 + p], p < 8, from the shift that makes the
+//     read 4-byte aligned.
+// Staging: every thread issues all its grid loads (2 x 2 cells per item, two
+// float4 loads; each column's two rows pack into one 32-bit LDS word per
+// plane) and the table copy at once,
+// so a workgroup pays one memory round trip before its MFMA loop, where every
+// step's fragments are read while the previous step's MFMAs run.
 #include "device_common.h"
+
+#include <algorithm>
 #include "grid_internal.h"
 
 namespace rtsim {
@@ -38,9 +49,10 @@ namespace rtsim {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int kRegCells = 16;   // float4 grid loads per thread held in registers
+constexpr int kRegQuads = 8;    // staged 2 x 2 cell items per thread held in registers
+constexpr int kChains = 6;      // independent accumulators (3 re, 3 im)
 
 __device__ __forceinline__ uint16_t bf16_rne(float x)
 {
@@ -65,56 +77,36 @@ __device__ __forceinline__ void split_hi_lo(float x, uint16_t& hi, uint16_t& lo)
 }
 
 template <bool FP16>
-__device__ __forceinline__ f32x4 mfma(u32x4 a, u32x4 b, f32x4 c)
+__device__ __forceinline__ f32x16 mfma(u32x4 a, u32x4 b, f32x16 c)
 {
     if (FP16)
-        return __builtin_amdgcn_mfma_f32_16x16x32_f16(__builtin_bit_cast(f16x8, a),
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(__builtin_bit_cast(f16x8, a),
                                                       __builtin_bit_cast(f16x8, b), c, 0, 0, 0);
     else
-        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a),
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8, a),
                                                        __builtin_bit_cast(bf16x8, b), c, 0, 0, 0);
 }
 
-template <int NT>
-struct Frags {
-    u32x4 a[NT][4];  // [n tile][plane]
-    u32x4 b[4];      // [plane]
-};
-
-template <bool FP16, int NT>
-__device__ __forceinline__ void mma_step(const Frags<NT>& f, f32x4 (&re)[NT], f32x4 (&im)[NT])
+// One K step: A = taps (planes a_hi, a_lo, b_hi, b_lo of T = a + ib), B = grid
+// (c_hi, c_lo, d_hi, d_lo of G = c + id); acc[0..2] collect re, acc[3..5] im.
+template <bool FP16>
+__device__ __forceinline__ void mma_step(const u32x4 (&A)[4], const u32x4 (&B)[4],
+                                         f32x16 (&acc)[kChains])
 {
     const u32x4 sgn = {0x80008000u, 0x80008000u, 0x80008000u, 0x80008000u};
-    const u32x4 nbh = f.b[2] ^ sgn, nbl = f.b[3] ^ sgn;
-    // re += Ar Br - Ai Bi, im += Ar Bi + Ai Br; chains interleaved across n
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][0], f.b[0], re[n]); im[n] = mfma<FP16>(f.a[n][0], f.b[2], im[n]); }
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][1], f.b[0], re[n]); im[n] = mfma<FP16>(f.a[n][1], f.b[2], im[n]); }
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][0], f.b[1], re[n]); im[n] = mfma<FP16>(f.a[n][0], f.b[3], im[n]); }
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][2], nbh, re[n]); im[n] = mfma<FP16>(f.a[n][2], f.b[0], im[n]); }
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][3], nbh, re[n]); im[n] = mfma<FP16>(f.a[n][3], f.b[0], im[n]); }
-#pragma unroll
-    for (int n = 0; n < NT; n++) { re[n] = mfma<FP16>(f.a[n][2], nbl, re[n]); im[n] = mfma<FP16>(f.a[n][2], f.b[1], im[n]); }
-}
-
-template <int NT>
-__device__ __forceinline__ void load_frags(Frags<NT>& f, const uint16_t* aptr, int plane_stride,
-                                           const uint16_t* bptr, int bplane_stride)
-{
-#pragma unroll
-    for (int p = 0; p < 4; p++) {
-        const uint32_t* src = reinterpret_cast<const uint32_t*>(bptr + p * bplane_stride);
-        f.b[p] = (u32x4){src[0], src[1], src[2], src[3]};
-    }
-#pragma unroll
-    for (int n = 0; n < NT; n++)
-#pragma unroll
-        for (int p = 0; p < 4; p++)
-            f.a[n][p] = *reinterpret_cast<const u32x4*>(aptr + p * plane_stride + 16 * n);
+    const u32x4 nbh = A[2] ^ sgn, nbl = A[3] ^ sgn;
+    acc[0] = mfma<FP16>(A[0], B[0], acc[0]);   // ah ch
+    acc[3] = mfma<FP16>(A[0], B[2], acc[3]);   // ah dh
+    acc[1] = mfma<FP16>(A[1], B[0], acc[1]);   // al ch
+    acc[4] = mfma<FP16>(A[1], B[2], acc[4]);   // al dh
+    acc[2] = mfma<FP16>(A[0], B[1], acc[2]);   // ah cl
+    acc[5] = mfma<FP16>(A[0], B[3], acc[5]);   // ah dl
+    acc[0] = mfma<FP16>(nbh, B[2], acc[0]);    // -bh dh
+    acc[3] = mfma<FP16>(A[2], B[0], acc[3]);   // bh ch
+    acc[1] = mfma<FP16>(nbl, B[2], acc[1]);    // -bl dh
+    acc[4] = mfma<FP16>(A[3], B[0], acc[4]);   // bl ch
+    acc[2] = mfma<FP16>(nbh, B[3], acc[2]);    // -bh dl
+    acc[5] = mfma<FP16>(A[2], B[1], acc[5]);   // bh cl
 }
 
 // q = floor(k / d) for 0 <= k < 2^20, d >= 1: (k + 0.5) / d lies at least
@@ -125,80 +117,100 @@ __device__ __forceinline__ int small_div(int k, float inv_d)
     return (int)(((float)k + 0.5f) * inv_d);
 }
 
-// Copies the Toeplitz rows of range taps [i0, i0 + tc) into LDS B: for each
-// (shift, plane) run a contiguous tc * BL elements, moved as uint4.
-__device__ __forceinline__ void copy_toeplitz(const CorrArgs& a, uint16_t* B, int i0, int tc, int t)
+// staged grid columns of a workgroup of TC 32-column tiles: 32 TC + Tb - 1,
+// rounded up to even (the grid is read two columns at a time)
+__host__ __device__ inline int corr_cols(int TC, int Tb)
 {
-    const int n16 = tc * a.BL / 8;                 // uint4 per (shift, plane) run
-    const float inv_n16 = 1.0f / (float)n16;
-    const uint4* src = reinterpret_cast<const uint4*>(a.toep);
-    uint4* dst = reinterpret_cast<uint4*>(B);
-    const int run_src = a.Tr * a.BL / 8, run_dst = a.TCH * a.BL / 8, off = i0 * a.BL / 8;
-    for (int k = t; k < 8 * n16; k += 256) {
-        const int cp = small_div(k, inv_n16), o = k - cp * n16;
-        dst[cp * run_dst + o] = src[cp * run_src + off + o];
-    }
+    const int c = 32 * TC + Tb - 1;
+    return c + (c & 1);
 }
 
-template <bool FP16, int WR, int NT>
+// Stride (elements) between the two shifted copies of the tap table, padded to
+// 16 dwords modulo the 32 LDS banks: lanes 2k and 2k + 1 read the same dword
+// offset of the two copies, which then falls in distinct banks.
+__host__ __device__ inline int corr_copy_stride(int Tb, int L)
+{
+    const int dw = (4 * Tb * L / 2 + 31) / 32 * 32 + 16;
+    return 2 * dw;
+}
+
+template <int TR, int TC>
+struct CorrGeom {
+    static constexpr int JS = 4 / (TR * TC);   // waves per output tile
+};
+
+template <bool FP16, int TR, int TC>
 __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by)
 {
-    constexpr int WC = 4 / WR;
-    constexpr int ROWS = 16 * WR, COLS = 16 * NT * WC;
-    extern __shared__ uint16_t lds[];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int wr = wave % WR, wc = wave / WR;
-    const int m = lane & 15, kg = lane >> 4;
-    const int r0 = by * ROWS;
-    const int c0 = bx * COLS;
-    const int AR = ROWS + a.Tr - 1;
-    const int AC = COLS - 16 + 32 * a.KB;   // columns [c0, c0 + COLS - 16 + 32 KB)
-    const int PR = AC / 2;                  // float4 loads per A row
-    const int total = AR * PR;
-    const float inv_pr = 1.0f / (float)PR;
-    const int AS = a.AS;
-    const int bps = a.TCH * a.BL;           // B plane stride (elements)
-    uint16_t* A = lds;
-    uint16_t* B = lds + 4 * AR * AS;
+    constexpr int JS = CorrGeom<TR, TC>::JS;
+    extern __shared__ __align__(16) uint16_t lds[];
     __shared__ float red[4];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int tile = wave / JS, js = wave % JS;   // this wave's output tile and tap subset
+    const int tr = tile / TC, tc = tile % TC;
+    const int r0 = by * 32 * TR, c0 = bx * 32 * TC;
+    const int KR = a.KR, RS = a.RS, L = a.L;
+    const int SR = 32 * (TR - 1) + 16 * KR;      // staged grid rows [r0, r0 + SR)
+    const int CW = corr_cols(TC, a.Tb);          // staged grid columns [c0, c0 + CW)
+    const int gps = CW * RS;                     // G plane stride (elements)
+    uint16_t* Gt = lds;
+    uint16_t* Tt = lds + 4 * gps;                // tap table [2][4][Tb][L]
+    const int cs = corr_copy_stride(a.Tb, L);   // elements between the two shifted copies
+    const int tsize = 2 * cs;
 
-    // window sizes of this lane's output rows, for the epilogue mask
-    uint32_t wsz[4];
+    // window sizes of this lane's output rows (D rows 8 (q / 4) + 4 hh + q % 4),
+    // for the epilogue mask
+    const int hh = lane >> 5, col = lane & 31;
+    uint32_t wsz[16];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-        const int rr = r0 + 16 * wr + 4 * kg + e;
-        wsz[e] = a.win[min(rr, a.H - 1)].y;
+    for (int q = 0; q < 16; q++) {
+        const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
+        wsz[q] = a.win[min(rr, a.H - 1)].y;
     }
 
-    // ---- grid tile loads: all in flight together ------------------------------
+    // ---- staging loads: 2 x 2 cells (rows r, r + 1, columns c, c + 1) per item,
+    // all in flight ---------------------------------------------------------------
+    const int CQ = CW / 2;                       // column pairs
+    const int nquads = (SR / 2) * CQ;
+    const float inv_cq = 1.0f / (float)CQ;
     const float4* gtile = reinterpret_cast<const float4*>(a.grid + (size_t)r0 * a.GW + c0);
-    const int grow4 = a.GW / 2;             // float4 per grid row
-    const bool in_regs = total <= kRegCells * 256;
-    float4 v[kRegCells];
+    const int grow4 = a.GW / 2;                  // float4 per grid row
+    const bool in_regs = nquads <= kRegQuads * 256;
+    float4 v0[kRegQuads], v1[kRegQuads];
     float mx = 0.f;
+    auto quad_of = [&](int k, int& rp, int& cq) {
+        rp = small_div(k, inv_cq);
+        cq = k - rp * CQ;
+    };
+    auto quad_max = [](const float4& g0, const float4& g1) {
+        return fmaxf(fmaxf(fmaxf(fabsf(g0.x), fabsf(g0.y)), fmaxf(fabsf(g0.z), fabsf(g0.w))),
+                     fmaxf(fmaxf(fabsf(g1.x), fabsf(g1.y)), fmaxf(fabsf(g1.z), fabsf(g1.w))));
+    };
     if (in_regs) {
-        const int nreg = (total + 255) / 256;
 #pragma unroll
-        for (int j = 0; j < kRegCells; j++) {
-            if (j < nreg) {
-                const int k = min(t + 256 * j, total - 1);
-                const int rr = small_div(k, inv_pr);
-                v[j] = gtile[rr * grow4 + (k - rr * PR)];
-            }
+        for (int u = 0; u < kRegQuads; u++) {
+            const int k = min(t + 256 * u, nquads - 1);
+            int rp, cq;
+            quad_of(k, rp, cq);
+            v0[u] = gtile[(size_t)(2 * rp) * grow4 + cq];
+            v1[u] = gtile[(size_t)(2 * rp + 1) * grow4 + cq];
         }
-        copy_toeplitz(a, B, 0, min(a.TCH, a.Tr), t);
+    }
+    {   // tap table: a contiguous copy, uint4 at a time
+        const uint4* src = reinterpret_cast<const uint4*>(a.toep);
+        uint4* dst = reinterpret_cast<uint4*>(Tt);
+        for (int k = t; k < tsize / 8; k += 256) dst[k] = src[k];
+    }
+    if (in_regs) {
 #pragma unroll
-        for (int j = 0; j < kRegCells; j++) {
-            if (j < nreg && t + 256 * j < total)
-                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v[j].x), fabsf(v[j].y)),
-                                     fmaxf(fabsf(v[j].z), fabsf(v[j].w))));
-        }
+        for (int u = 0; u < kRegQuads; u++)
+            if (t + 256 * u < nquads) mx = fmaxf(mx, quad_max(v0[u], v1[u]));
     } else {
-        copy_toeplitz(a, B, 0, min(a.TCH, a.Tr), t);
-        for (int k = t; k < total; k += 256) {
-            const int rr = small_div(k, inv_pr);
-            const float4 g = gtile[rr * grow4 + (k - rr * PR)];
-            mx = fmaxf(mx, fmaxf(fmaxf(fabsf(g.x), fabsf(g.y)), fmaxf(fabsf(g.z), fabsf(g.w))));
+        for (int k = t; k < nquads; k += 256) {
+            int rp, cq;
+            quad_of(k, rp, cq);
+            mx = fmaxf(mx, quad_max(gtile[(size_t)(2 * rp) * grow4 + cq],
+                                    gtile[(size_t)(2 * rp + 1) * grow4 + cq]));
         }
     }
     for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
@@ -206,8 +218,8 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by
     __syncthreads();
     mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
     if (!(mx > 0.f)) {
-        for (int k = t; k < ROWS * COLS; k += 256) {
-            const int rr = r0 + k / COLS, cc = c0 + k % COLS;
+        for (int k = t; k < 32 * TR * 32 * TC; k += 256) {
+            const int rr = r0 + k / (32 * TC), cc = c0 + k % (32 * TC);
             if (rr < a.H && cc < a.W) a.out[(size_t)rr * a.W + cc] = make_float2(0.f, 0.f);
         }
         return;
@@ -216,164 +228,178 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by
     frexpf(mx, &e);                                       // mx in [2^(e-1), 2^e)
     const float scale = ldexpf(1.0f, min(14 - e, 126));   // mx * scale in [2^13, 2^14)
 
-    // ---- A: scaled, hi/lo split, two complex cells per 32-bit LDS word pair ----
+    // ---- G planes: scaled, hi/lo split, rows (2 rp, 2 rp + 1) in one word ------
     {
-        const int ps = AR * AS / 2;   // plane stride in words
-        auto put = [&](int k, const float4& g) {
-            const int rr = small_div(k, inv_pr), cp = k - rr * PR;
+        const int wps = gps / 2;   // plane stride in 32-bit words
+        // column c of rows (2 rp, 2 rp + 1) holds cells g0, g1
+        auto put_col = [&](uint32_t* base, float2 g0, float2 g1) {
             uint16_t h0, l0, h1, l1, h2, l2, h3, l3;
-            split_hi_lo<FP16>(g.x * scale, h0, l0);   // re, column 2cp
-            split_hi_lo<FP16>(g.z * scale, h1, l1);   // re, column 2cp + 1
-            split_hi_lo<FP16>(g.y * scale, h2, l2);   // im, column 2cp
-            split_hi_lo<FP16>(g.w * scale, h3, l3);   // im, column 2cp + 1
-            uint32_t* base = reinterpret_cast<uint32_t*>(A + rr * AS) + cp;
+            split_hi_lo<FP16>(g0.x * scale, h0, l0);   // re, row 2 rp
+            split_hi_lo<FP16>(g1.x * scale, h1, l1);   // re, row 2 rp + 1
+            split_hi_lo<FP16>(g0.y * scale, h2, l2);   // im, row 2 rp
+            split_hi_lo<FP16>(g1.y * scale, h3, l3);   // im, row 2 rp + 1
             base[0] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-            base[ps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-            base[2 * ps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
-            base[3 * ps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
+            base[wps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+            base[2 * wps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
+            base[3 * wps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
+        };
+        auto put = [&](int k, const float4& g0, const float4& g1) {
+            int rp, cq;
+            quad_of(k, rp, cq);
+            uint32_t* base = reinterpret_cast<uint32_t*>(Gt + 2 * cq * RS) + rp;
+            put_col(base, make_float2(g0.x, g0.y), make_float2(g1.x, g1.y));
+            put_col(base + RS / 2, make_float2(g0.z, g0.w), make_float2(g1.z, g1.w));
         };
         if (in_regs) {
 #pragma unroll
-            for (int j = 0; j < kRegCells; j++)
-                if (t + 256 * j < total) put(t + 256 * j, v[j]);
+            for (int u = 0; u < kRegQuads; u++)
+                if (t + 256 * u < nquads) put(t + 256 * u, v0[u], v1[u]);
         } else {
-            for (int k = t; k < total; k += 256) {
-                const int rr = small_div(k, inv_pr);
-                put(k, gtile[rr * grow4 + (k - rr * PR)]);
+            for (int k = t; k < nquads; k += 256) {
+                int rp, cq;
+                quad_of(k, rp, cq);
+                put(k, gtile[(size_t)(2 * rp) * grow4 + cq], gtile[(size_t)(2 * rp + 1) * grow4 + cq]);
             }
         }
     }
+    __syncthreads();
 
-    f32x4 re[NT], im[NT];
+    // ---- MFMA loop over this wave's taps j = js, js + JS, ... and K blocks ------
+    f32x16 acc[kChains];
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
-        re[n] = (f32x4){0.f, 0.f, 0.f, 0.f};
-        im[n] = re[n];
-    }
-    // per-lane fragment origins (advanced by pointer increments)
-    const int cpy = (15 - m) & 1;                       // parity of 32q + 8kg + 15 - m
-    const uint16_t* a_lane = A + (16 * wr + m) * AS + 16 * NT * wc + 8 * kg;
-    const uint16_t* b_lane = B + cpy * 4 * bps + (8 * kg + 15 - m - cpy);
-
-    for (int i0 = 0; i0 < a.Tr; i0 += a.TCH) {
-        const int tc = min(a.TCH, a.Tr - i0);
-        if (i0 > 0) {
-            __syncthreads();   // previous chunk consumed
-            copy_toeplitz(a, B, i0, tc, t);
+    for (int c = 0; c < kChains; c++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[c][q] = 0.f;
+    const int m = lane & 31, h = lane >> 5;
+    // A: lane (m, h) reads Tpad[16 kb + 8 h - m + 31 + p] of tap column j
+    const int s0 = 8 * h - m + 31;
+    const int cpy = s0 & 1;                              // 16 kb keeps the parity
+    const uint16_t* a_lane = Tt + cpy * cs + (s0 - cpy);
+    const int aps = a.Tb * L;                            // A plane stride (elements)
+    // B: lane (n, h) reads rows 32 tr + 16 kb + 8 h + p of column 32 tc + n + j
+    const uint16_t* b_lane = Gt + (32 * tc + m) * RS + 32 * tr + 8 * h;
+    // the tile's Tb KR steps (tap j, K block kb), split evenly over its JS waves
+    const int all = a.Tb * KR;
+    const int sbeg = js * all / JS, nsteps = (js + 1) * all / JS - sbeg;
+    const float inv_kr = 1.0f / (float)KR;
+    auto load = [&](int step, u32x4 (&A)[4], u32x4 (&B)[4]) {
+        const int st = sbeg + step;
+        const int j = small_div(st, inv_kr), kb = st - j * KR;
+        const uint16_t* ap = a_lane + j * L + 16 * kb;
+        const uint16_t* bp = b_lane + j * RS + 16 * kb;
+#pragma unroll
+        for (int p = 0; p < 4; p++) {
+            const uint32_t* src = reinterpret_cast<const uint32_t*>(ap + p * aps);
+            A[p] = (u32x4){src[0], src[1], src[2], src[3]};
+            B[p] = *reinterpret_cast<const u32x4*>(bp + p * gps);
         }
-        __syncthreads();       // A (first chunk) and B staged
-
-        // K steps (tap ii, K block q) of this chunk, ping-pong fragments
-        const int nsteps = tc * a.KB;
-        const uint16_t* ap = a_lane + i0 * AS;
-        const uint16_t* bp = b_lane;
-        int q = 0;
-        auto advance = [&]() {
-            if (++q == a.KB) {
-                q = 0;
-                ap += AS - 32 * (a.KB - 1);
-                bp += a.BL - 32 * (a.KB - 1);
-            } else {
-                ap += 32;
-                bp += 32;
-            }
-        };
-        Frags<NT> f0, f1;
-        load_frags<NT>(f0, ap, AR * AS, bp, bps);
-        advance();
+    };
+    if (nsteps > 0) {
+        u32x4 A0[4], B0[4], A1[4], B1[4];
+        load(0, A0, B0);
         for (int step = 0; step < nsteps; step += 2) {
+            load(min(step + 1, nsteps - 1), A1, B1);
+            mma_step<FP16>(A0, B0, acc);
             if (step + 1 < nsteps) {
-                load_frags<NT>(f1, ap, AR * AS, bp, bps);
-                advance();
+                load(min(step + 2, nsteps - 1), A0, B0);
+                mma_step<FP16>(A1, B1, acc);
             }
-            mma_step<FP16, NT>(f0, re, im);
-            if (step + 1 < nsteps) {
-                if (step + 2 < nsteps) {
-                    load_frags<NT>(f0, ap, AR * AS, bp, bps);
-                    advance();
-                }
-                mma_step<FP16, NT>(f1, re, im);
+        }
+    }
+    f32x16 re = acc[0] + acc[1] + acc[2];
+    f32x16 im = acc[3] + acc[4] + acc[5];
+
+    // ---- partial sums of the JS waves of a tile, through LDS ------------------
+    if (JS > 1) {
+        __syncthreads();                                  // every wave done with Gt / Tt
+        float* part = reinterpret_cast<float*>(lds);      // [wave][32 values][64 lanes]
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            part[(wave * 32 + q) * 64 + lane] = re[q];
+            part[(wave * 32 + 16 + q) * 64 + lane] = im[q];
+        }
+        __syncthreads();
+        // wave js of the tile sums elements [16 js / JS, 16 (js + 1) / JS)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (q * JS / 16 != js) continue;
+            float sr = 0.f, si = 0.f;
+#pragma unroll
+            for (int w2 = 0; w2 < JS; w2++) {
+                sr += part[((tile * JS + w2) * 32 + q) * 64 + lane];
+                si += part[((tile * JS + w2) * 32 + 16 + q) * 64 + lane];
             }
+            re[q] = sr;
+            im[q] = si;
         }
     }
 
     // ---- epilogue: undo scales, empty-window mask, store ---------------------
     const float inv = 1.0f / (scale * a.tap_scale);
+    const int cc = c0 + 32 * tc + col;
 #pragma unroll
-    for (int n = 0; n < NT; n++) {
-        const int cc = c0 + 16 * (NT * wc + n) + m;
-#pragma unroll
-        for (int e2 = 0; e2 < 4; e2++) {
-            const int rr = r0 + 16 * wr + 4 * kg + e2;
-            if (rr < a.H && cc < a.W) {
-                float2 v2 = make_float2(re[n][e2] * inv, im[n][e2] * inv);
-                if (wsz[e2] == 0) v2 = make_float2(0.f, 0.f);
-                a.out[(size_t)rr * a.W + cc] = v2;
-            }
+    for (int q = 0; q < 16; q++) {
+        if (JS > 1 && q * JS / 16 != js) continue;
+        const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
+        if (rr < a.H && cc < a.W) {
+            float2 v2 = make_float2(re[q] * inv, im[q] * inv);
+            if (wsz[q] == 0) v2 = make_float2(0.f, 0.f);
+            a.out[(size_t)rr * a.W + cc] = v2;
         }
     }
 }
 
-template <bool FP16, int WR, int NT>
+template <bool FP16, int TR, int TC>
 __global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
 {
-    correlate_body<FP16, WR, NT>(a, blockIdx.x, blockIdx.y);
+    correlate_body<FP16, TR, TC>(a, blockIdx.x, blockIdx.y);
 }
 
-template <bool FP16, int WR, int NT>
+template <bool FP16, int TR, int TC>
 __global__ __launch_bounds__(256) void k_correlate_batch(const CorrArgs* __restrict__ args, BatchMap m)
 {
-    constexpr int COLS = 16 * NT * (4 / WR);
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const CorrArgs& a = args[p];
-    const int gx = (a.W + COLS - 1) / COLS;
-    correlate_body<FP16, WR, NT>(a, (int)blk % gx, (int)blk / gx);
+    const int gx = (a.W + 32 * TC - 1) / (32 * TC);
+    correlate_body<FP16, TR, TC>(a, (int)blk % gx, (int)blk / gx);
 }
 
-// tile shapes: {WR, NT}; rows = 16 WR, cols = 16 NT (4 / WR):
-// 64x32, 32x32, 32x64, 64x16
-static const int kShapes[kCorrShapes][2] = {{4, 2}, {2, 1}, {2, 2}, {4, 1}};
+// tile shapes {TR, TC}: rows = 32 TR, cols = 32 TC; 4 / (TR TC) waves per tile
+static const int kShapes[kCorrShapes][2] = {{1, 1}, {1, 2}, {2, 1}, {2, 2}};
 
 void corr_tile(int shape, int* rows, int* cols)
 {
-    const int WR = kShapes[shape][0], NT = kShapes[shape][1];
-    *rows = 16 * WR;
-    *cols = 16 * NT * (4 / WR);
+    *rows = 32 * kShapes[shape][0];
+    *cols = 32 * kShapes[shape][1];
+}
+
+int corr_copy_stride_host(int Tb, int L) { return corr_copy_stride(Tb, L); }
+
+int corr_row_stride(int shape, int KR)
+{
+    const int rows = 32 * (kShapes[shape][0] - 1) + 16 * KR;   // a multiple of 16
+    return rows + 8;
 }
 
 size_t correlate_lds_bytes(const CorrArgs& a)
 {
     int rows, cols;
     corr_tile(a.shape, &rows, &cols);
-    const int AR = rows + a.Tr - 1;
-    return sizeof(uint16_t) * ((size_t)4 * AR * a.AS + (size_t)8 * a.TCH * a.BL);
-}
-
-int correlate_chunk(const CorrArgs& a, size_t max_lds)
-{
-    CorrArgs c = a;
-    auto largest = [&](size_t budget) {
-        int lo = 0;
-        for (int tc = a.Tr; tc >= 1; tc--) {
-            c.TCH = tc;
-            if (correlate_lds_bytes(c) <= budget) { lo = tc; break; }
-        }
-        return lo;
-    };
-    int tc = largest(kCorrLdsTarget);
-    if (tc == 0) tc = largest(max_lds);
-    return tc;
+    const size_t gt = (size_t)4 * corr_cols(cols / 32, a.Tb) * a.RS;
+    const size_t tt = (size_t)2 * corr_copy_stride(a.Tb, a.L);
+    const size_t part = (size_t)4 * 32 * 64 * 2;   // partial sums (floats, as 16-bit units)
+    return sizeof(uint16_t) * std::max(gt + tt, part);
 }
 
 template <bool FP16>
 static void launch_shape(const CorrArgs& a, dim3 grid, size_t shm, hipStream_t s)
 {
     switch (a.shape) {
-    case 0: hipLaunchKernelGGL((k_correlate<FP16, 4, 2>), grid, dim3(256), shm, s, a); break;
-    case 1: hipLaunchKernelGGL((k_correlate<FP16, 2, 1>), grid, dim3(256), shm, s, a); break;
-    case 2: hipLaunchKernelGGL((k_correlate<FP16, 2, 2>), grid, dim3(256), shm, s, a); break;
-    default: hipLaunchKernelGGL((k_correlate<FP16, 4, 1>), grid, dim3(256), shm, s, a); break;
+    case 0: hipLaunchKernelGGL((k_correlate<FP16, 1, 1>), grid, dim3(256), shm, s, a); break;
+    case 1: hipLaunchKernelGGL((k_correlate<FP16, 1, 2>), grid, dim3(256), shm, s, a); break;
+    case 2: hipLaunchKernelGGL((k_correlate<FP16, 2, 1>), grid, dim3(256), shm, s, a); break;
+    default: hipLaunchKernelGGL((k_correlate<FP16, 2, 2>), grid, dim3(256), shm, s, a); break;
     }
 }
 
@@ -382,10 +408,10 @@ static void launch_shape_batch(int shape, const CorrArgs* args, BatchMap m, dim3
                                hipStream_t s)
 {
     switch (shape) {
-    case 0: hipLaunchKernelGGL((k_correlate_batch<FP16, 4, 2>), grid, dim3(256), shm, s, args, m); break;
-    case 1: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 1>), grid, dim3(256), shm, s, args, m); break;
-    case 2: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 2>), grid, dim3(256), shm, s, args, m); break;
-    default: hipLaunchKernelGGL((k_correlate_batch<FP16, 4, 1>), grid, dim3(256), shm, s, args, m); break;
+    case 0: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 1>), grid, dim3(256), shm, s, args, m); break;
+    case 1: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 2>), grid, dim3(256), shm, s, args, m); break;
+    case 2: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 1>), grid, dim3(256), shm, s, args, m); break;
+    default: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 2>), grid, dim3(256), shm, s, args, m); break;
     }
 }
 
@@ -418,22 +444,22 @@ hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s)
 
 void init_corr_attributes(int max_lds)
 {
-    set_dynamic_lds_limit((const void*)k_correlate<true, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 2>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<true, 2, 1>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<true, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<true, 4, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 2>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<false, 2, 1>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate<false, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 4, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 2>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 1>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 4, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 4, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 1>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 2>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 1>, max_lds);
     set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 4, 1>, max_lds);
 }
 
 }  // namespace rtsim
