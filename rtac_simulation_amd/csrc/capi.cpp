@@ -142,10 +142,9 @@ void Sensor::free_samples()
     dev_free(d_sorted);
     dev_free(d_tmp);
     dev_free(d_coord);
-    dev_free(d_kA);
-    dev_free(d_kB);
-    dev_free(d_vA);
-    dev_free(d_vB);
+    dev_free(d_kv[0]);
+    dev_free(d_kv[1]);
+    dev_free(d_order);
     dev_free(d_skeys);
     dev_free(d_counts);
     capacity = 0;
@@ -162,10 +161,9 @@ int Sensor::reserve(size_t count)
     if ((rc = dev_alloc(&d_sorted, count))) return rc;
     if ((rc = dev_alloc(&d_tmp, count))) return rc;
     if ((rc = dev_alloc(&d_coord, count))) return rc;
-    if ((rc = dev_alloc(&d_kA, count))) return rc;
-    if ((rc = dev_alloc(&d_kB, count))) return rc;
-    if ((rc = dev_alloc(&d_vA, count))) return rc;
-    if ((rc = dev_alloc(&d_vB, count))) return rc;
+    if ((rc = dev_alloc(&d_kv[0], count))) return rc;
+    if ((rc = dev_alloc(&d_kv[1], count))) return rc;
+    if ((rc = dev_alloc(&d_order, count))) return rc;
     if ((rc = dev_alloc(&d_skeys, count))) return rc;
     if ((rc = dev_alloc(&d_counts, sort_count_words(count)))) return rc;
     capacity = count;
@@ -298,10 +296,9 @@ int Sensor::plan(int form, PingPlan* p)
     // (Binner.cu:24-49) and bearing coordinates.  The plain form sorts by
     // travel; the grid form by the grouping key (GroupKey).
     SortBuffers sb;
-    sb.keys[0] = d_kA;
-    sb.keys[1] = d_kB;
-    sb.vals[0] = d_vA;
-    sb.vals[1] = d_vB;
+    sb.kv[0] = d_kv[0];
+    sb.kv[1] = d_kv[1];
+    sb.order = d_order;
     sb.counts = d_counts;
     sb.totals = d_totals;
     sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);

@@ -57,8 +57,7 @@ __device__ __forceinline__ uint32_t sort_block_scan(uint32_t x, uint32_t* ws, ui
 
 template <bool FIRST>
 __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
-                                          const uint32_t* __restrict__ kin,
-                                          const uint32_t* __restrict__ vin, const GroupKey& gk,
+                                          const uint2* __restrict__ kvin, const GroupKey& gk,
                                           uint32_t n, uint32_t wbase, uint32_t lane, uint32_t* key,
                                           uint32_t* val)
 {
@@ -70,8 +69,9 @@ __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
             key[j] = gk.on ? group_key(gk, t) : travel_image(t);
             if (val) val[j] = i;
         } else {
-            key[j] = kin[i];
-            if (val) val[j] = vin[i];
+            const uint2 kv = kvin[i];
+            key[j] = kv.x;
+            if (val) val[j] = kv.y;
         }
     }
 }
@@ -93,7 +93,7 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems];
-    load_tile<FIRST>(a.rec, a.kin, nullptr, a.gk, a.n, wbase, lane, key, nullptr);
+    load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, nullptr);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++)
@@ -190,7 +190,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             val[j] = i;
         }
     } else {
-        load_tile<FIRST>(a.rec, a.kin, a.vin, a.gk, a.n, wbase, lane, key, val);
+        load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
         if (MOVE) {
 #pragma unroll
             for (int j = 0; j < kSortLaneItems; j++)
@@ -340,8 +340,8 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     for (int u = 0; u < Q; u++) {
         const uint32_t pos = pq[u];
         if (pos < a.n) {   // holds for every staged item unless the records changed
-            if (!LAST) a.kout[pos] = kq[u];   // the sorted keys themselves are not kept
-            a.vout[pos] = vq[u];
+            if (LAST) a.order[pos] = vq[u];   // the sorted keys themselves are not kept
+            else a.kvout[pos] = make_uint2(kq[u], vq[u]);
             if (MOVE && !LAST) a.rout[pos] = rq[u];
             if (LAST) {
                 const float4 r = rq[u];
@@ -391,7 +391,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     for (int pass = 0; pass < 4; pass++) {
         HistArgs& h = p->hist[pass];
         h.rec = records;
-        h.kin = b.keys[cur];
+        h.kvin = b.kv[cur];
         h.n = n;
         h.shift = 8 * pass;
         h.gk = g;
@@ -405,10 +405,9 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         p->scan[pass] = ScanArgs{b.counts, tiles, b.totals};
         ScatterArgs& c = p->scatter[pass];
         c.rec = records;
-        c.kin = b.keys[cur];
-        c.vin = b.vals[cur];
-        c.kout = b.keys[cur ^ 1];
-        c.vout = b.vals[cur ^ 1];
+        c.kvin = b.kv[cur];
+        c.kvout = b.kv[cur ^ 1];
+        c.order = b.order;
         c.n = n;
         c.shift = 8 * pass;
         c.counts = b.counts;
@@ -426,7 +425,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     const bool clears = b.seg_count || b.fill_count;
     p->hist_blocks = n ? tiles : (clears ? 1u : 0u);
     p->pass_blocks = n ? tiles : 0u;
-    p->order = b.vals[n ? cur : 0];
+    p->order = b.order;
     return hipSuccess;
 }
 

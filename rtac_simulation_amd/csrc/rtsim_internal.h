@@ -103,8 +103,8 @@ struct SortOutputs {
     uint2* seg;           // [key.n], initialised to (~0, 0) by the first pass
 };
 struct SortBuffers {
-    uint32_t* keys[2];
-    uint32_t* vals[2];
+    uint2* kv[2];         // (key, sample index) ping-pong between passes
+    uint32_t* order;      // the permutation, written by the last pass
     uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
     uint32_t* totals;     // [256] digit totals of the current pass
     uint32_t tiles_cap;
@@ -120,7 +120,7 @@ struct SortBuffers {
 // One pass of the sort: per-tile digit counts, their scan, the scatter.
 struct HistArgs {
     const float4* rec;        // pass 0 reads the records themselves
-    const uint32_t* kin;
+    const uint2* kvin;        // later passes: (key, index) of the previous pass
     uint32_t n;
     int shift;
     GroupKey gk;              // pass 0 key: K when gk.on, else the travel image
@@ -139,10 +139,9 @@ struct ScanArgs {
 };
 struct ScatterArgs {
     const float4* rec;
-    const uint32_t* kin;
-    const uint32_t* vin;
-    uint32_t* kout;
-    uint32_t* vout;
+    const uint2* kvin;        // (key, index) in the previous pass's order (not pass 0)
+    uint2* kvout;             // not the last pass: (key, index) in this pass's order
+    uint32_t* order;          // last pass: the permutation
     uint32_t n;
     int shift;
     const uint32_t* counts;   // scanned [digit][tile]
@@ -150,8 +149,8 @@ struct ScatterArgs {
     uint32_t tiles;
     GroupKey gk;              // as HistArgs
     int move;                 // records move with their keys (grouping key)
-    const float4* rin;        // move: records in the order of kin (pass 0: the input)
-    float4* rout;             // move, not the last pass: records in the order of kout
+    const float4* rin;        // move: records in the order of kvin (pass 0: the input)
+    float4* rout;             // move, not the last pass: records in the order of kvout
     SortOutputs out;          // last pass only
 };
 // Arguments and workgroup counts of the 3 launches per pass of one sort.

@@ -67,7 +67,8 @@ struct Sensor {
     float4* d_sorted = nullptr;   // receivedSamples_ after sort()
     float4* d_tmp = nullptr;      // records between sort passes (grid form)
     float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
-    uint32_t *d_kA = nullptr, *d_kB = nullptr, *d_vA = nullptr, *d_vB = nullptr;
+    uint2* d_kv[2] = {nullptr, nullptr};  // sort: (key, index) between passes
+    uint32_t* d_order = nullptr;  // sort permutation
     uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
     const uint32_t* order_ptr = nullptr;  // sort permutation (after compute)
     uint32_t* d_totals = nullptr; // sort: digit totals of a pass
