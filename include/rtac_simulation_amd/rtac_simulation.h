@@ -521,6 +521,62 @@ class PoseSourceList : public PoseSource {
 };
 
 // ---------------------------------------------------------------------------
+// Emitter (Emitter.h:67-115, Emitter.cu:21-80): ray directions (a polar fan or
+// a caller's list, 3 floats per ray), their initial values D(dir) from a
+// directivity table (normalised coordinates, scale 1/pi), a frequency and a
+// pose; rays leave along pose.rotation * dir.
+// ---------------------------------------------------------------------------
+class Emitter {
+   public:
+    using Ptr = std::shared_ptr<Emitter>;
+    using ConstPtr = std::shared_ptr<const Emitter>;
+
+    static std::vector<float> generate_polar_directions(float resolution, float bearingAperture,
+                                                        float elevationAperture)
+    {
+        size_t n = 0;
+        check(rtsim_emitter_polar_directions(resolution, bearingAperture, elevationAperture, nullptr,
+                                             0, &n));
+        std::vector<float> d(3 * n);
+        check(rtsim_emitter_polar_directions(resolution, bearingAperture, elevationAperture, d.data(),
+                                             n, &n));
+        return d;
+    }
+    static Ptr Create(const std::vector<float>& rayDirs, const DirectivityTable& directivity,
+                      float frequency, const Pose& pose = Pose())
+    {
+        return Ptr(new Emitter(rayDirs, directivity, frequency, pose));
+    }
+    // resolution and apertures in degrees, as Emitter::Create
+    static Ptr Create(float resolution, float bearingAperture, float elevationAperture,
+                      const DirectivityTable& directivity, float frequency, const Pose& pose = Pose())
+    {
+        return Create(generate_polar_directions(resolution, bearingAperture, elevationAperture),
+                      directivity, frequency, pose);
+    }
+    unsigned int size() const { return ray_count(); }
+    unsigned int ray_count() const { return (unsigned int)(directions_.size() / 3); }
+    float frequency() const { return frequency_; }
+    const Pose& pose() const { return pose_; }
+    Pose& pose() { return pose_; }
+    const std::vector<float>& directions() const { return directions_; }
+    const std::vector<float>& initial_values() const { return values_; }   // complex interleaved
+
+   protected:
+    Emitter(const std::vector<float>& dirs, const DirectivityTable& d, float frequency, const Pose& pose)
+        : directions_(dirs), values_(2 * (dirs.size() / 3)), frequency_(frequency), pose_(pose)
+    {
+        const float inv_pi = (float)(1.0 / M_PI);
+        check(rtsim_emitter_initial_values(directions_.data(), directions_.size() / 3,
+                                           d.data.empty() ? nullptr : d.data.data(), d.width, d.height,
+                                           inv_pi, inv_pi, values_.data()));
+    }
+    std::vector<float> directions_, values_;
+    float frequency_;
+    Pose pose_;
+};
+
+// ---------------------------------------------------------------------------
 // Simulation loop: OptixSimulation1::run (OptixSimulation.cpp:56-92) with the
 // ray caster (out of scope here) replaced by the seeded synthetic scene
 // generator: per ping, the next receiver pose, `samples` hits of the scene
@@ -555,6 +611,9 @@ class SyntheticSimulation {
     SyntheticSimulation& operator=(const SyntheticSimulation&) = delete;
 
     void set_receiver_poses(const PoseSource::Ptr& poses) { poses_ = poses; }
+    // rays of an emitter instead of the scene's random fan (the scene must be
+    // SEABED); ray i uses the emitter's ray i mod ray_count
+    void set_emitter(const Emitter::Ptr& emitter) { emitter_ = emitter; }
     void add_sink(const FileSink::Ptr& sink) { sinks_.push_back(sink); }
     uint64_t pings() const { return ping_; }
     const SensorInstance::Ptr& receiver(unsigned int slot = 0) const { return slots_[slot].sensor; }
@@ -601,8 +660,16 @@ class SyntheticSimulation {
     void load_hits(Slot& sl, uint64_t seed)
     {
         std::vector<float> r(samples_), dx(samples_), dy(samples_), dz(samples_), a(2 * samples_);
-        check(rtsim_generate_hits(&scene_, seed, 0, samples_, r.data(), dx.data(), dy.data(), dz.data(),
-                                  a.data(), 8));
+        if (emitter_) {
+            const auto rot = emitter_->pose().rotation_matrix();
+            check(rtsim_generate_emitter_hits(&scene_, seed, emitter_->directions().data(),
+                                              emitter_->initial_values().data(), emitter_->ray_count(),
+                                              rot.data(), 0, samples_, r.data(), dx.data(), dy.data(),
+                                              dz.data(), a.data(), 8));
+        } else {
+            check(rtsim_generate_hits(&scene_, seed, 0, samples_, r.data(), dx.data(), dy.data(),
+                                      dz.data(), a.data(), 8));
+        }
         const std::vector<float>* host[5] = {&r, &dx, &dy, &dz, &a};
         for (int k = 0; k < 5; k++)
             if (!host[k]->empty())
@@ -617,6 +684,7 @@ class SyntheticSimulation {
     uint64_t seed_, ping_ = 0;
     std::vector<Slot> slots_;
     PoseSource::Ptr poses_;
+    Emitter::Ptr emitter_;
     std::vector<FileSink::Ptr> sinks_;
 };
 

@@ -135,6 +135,15 @@ _SIGNATURES = [
       C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
     ("rtsim_directivity_uniform", C.c_int,
      [C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    ("rtsim_emitter_polar_directions", C.c_int,
+     [C.c_float, C.c_float, C.c_float, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("rtsim_emitter_initial_values", C.c_int,
+     [C.c_void_p, C.c_size_t, C.c_void_p, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
+      C.c_void_p]),
+    ("rtsim_generate_emitter_hits", C.c_int,
+     [C.POINTER(SceneDesc), C.c_uint64, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
+      C.c_uint64, C.c_size_t, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+      C.c_int]),
     ("rtsim_info_from_yaml", C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     ("rtsim_info_desc", C.c_int, [C.c_void_p, C.POINTER(C.POINTER(SensorDesc))]),
     ("rtsim_info_destroy", C.c_int, [C.c_void_p]),
@@ -319,6 +328,51 @@ def generate_hits(scene: SceneDesc, seed: int, n: int, first: int = 0,
     check(lib().rtsim_generate_hits(C.byref(scene), seed, first, n, _ptr(h.range),
                                     _ptr(h.dir_x), _ptr(h.dir_y), _ptr(h.dir_z),
                                     _ptr(h.amplitude), nthreads))
+    return h
+
+
+def emitter_polar_directions(resolution_deg: float, bearing_aperture_deg: float,
+                             elevation_aperture_deg: float) -> np.ndarray:
+    """Emitter::generate_polar_directions (Emitter.cu:59-80): (rays, 3) float32."""
+    cnt = C.c_size_t()
+    check(lib().rtsim_emitter_polar_directions(resolution_deg, bearing_aperture_deg,
+                                               elevation_aperture_deg, None, 0, C.byref(cnt)))
+    out = np.zeros((cnt.value, 3), np.float32)
+    check(lib().rtsim_emitter_polar_directions(resolution_deg, bearing_aperture_deg,
+                                               elevation_aperture_deg, _ptr(out), cnt.value,
+                                               C.byref(cnt)))
+    return out
+
+
+def emitter_initial_values(dirs: np.ndarray, table: np.ndarray | None = None,
+                           xscale: float = 1.0 / np.pi, yscale: float = 1.0 / np.pi) -> np.ndarray:
+    """Emitter::load_initial_values (Emitter.cu:38-55): D(dir) per ray, complex64."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    out = np.zeros(d.shape[0], np.complex64)
+    if table is None:
+        check(lib().rtsim_emitter_initial_values(_ptr(d), d.shape[0], None, 0, 0, 0.0, 0.0,
+                                                 _ptr(out)))
+    else:
+        t = np.ascontiguousarray(table, np.float32)
+        check(lib().rtsim_emitter_initial_values(_ptr(d), d.shape[0], _ptr(t), t.shape[1],
+                                                 t.shape[0], xscale, yscale, _ptr(out)))
+    return out
+
+
+def generate_emitter_hits(scene: SceneDesc, seed: int, dirs: np.ndarray, values: np.ndarray,
+                          n: int, first: int = 0, rotation: np.ndarray | None = None,
+                          nthreads: int = 8) -> HostHits:
+    """Hits of an emitter's rays (ray i along rotation @ dirs[i % rays], value values[i % rays])."""
+    d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+    v = np.ascontiguousarray(values, np.complex64)
+    if v.size != d.shape[0]:
+        raise ValueError("one initial value per emitter ray")
+    rot = None if rotation is None else np.ascontiguousarray(rotation, np.float32).reshape(9)
+    h = HostHits(n)
+    check(lib().rtsim_generate_emitter_hits(C.byref(scene), seed, _ptr(d), _ptr(v), d.shape[0],
+                                            None if rot is None else _ptr(rot), first, n,
+                                            _ptr(h.range), _ptr(h.dir_x), _ptr(h.dir_y),
+                                            _ptr(h.dir_z), _ptr(h.amplitude), nthreads))
     return h
 
 

@@ -1,6 +1,8 @@
 // Seeded synthetic ray-hit generator (replaces the OptiX/GL ray casters,
-// RayCaster.cu:17-81 / SimulationGL.cu:82-132, which are out of scope) and the
-// host-side directivity table builders of Directivity (Directivity.cpp:35-166).
+// RayCaster.cu:17-81 / SimulationGL.cu:82-132, which are out of scope), the
+// Emitter's ray fan and initial values (Emitter.cu:38-80) with hits of its
+// rays, and the host-side directivity table builders of Directivity
+// (Directivity.cpp:35-166).
 //
 // The generator is counter based: sample i of a scene depends only on
 // (seed, i), so any sub-range [first, first + n) can be produced on its own,
@@ -135,6 +137,94 @@ void gen_range(const rtsim_scene_desc* sc, uint64_t seed, uint64_t first, size_t
     }
 }
 
+// Hits of emitter rays: ray i along d = rot * dirs[i mod ndirs] against the
+// scene's planes (with the per-ray height noise) and spheres, closest hit.
+void gen_emitter_range(const rtsim_scene_desc* sc, uint64_t seed, const float* dirs,
+                       const float* values, size_t ndirs, const double* rot, uint64_t first,
+                       size_t begin, size_t end, const std::vector<Sphere>& spheres, float* range,
+                       float* dx, float* dy, float* dz, float* amp)
+{
+    for (size_t k = begin; k < end; k++) {
+        const uint64_t ray = first + k;
+        const size_t q = (size_t)(ray % ndirs);
+        Rng g = stream_for(seed, ray, 0xe3177e5ull);
+        const double mag = sc->rayleigh_sigma * std::sqrt(-2.0 * std::log(1.0 - g.unit()));
+        const double ph = 2.0 * M_PI * g.unit();
+        const double eta = sc->height_noise > 0 ? sc->height_noise * g.normal() : 0.0;
+        const double e[3] = {dirs[3 * q], dirs[3 * q + 1], dirs[3 * q + 2]};
+        double d[3];
+        for (int r = 0; r < 3; r++) d[r] = rot[3 * r] * e[0] + rot[3 * r + 1] * e[1] + rot[3 * r + 2] * e[2];
+        double best = 0.0, gain = 0.0;
+        if (d[0] == d[0] && d[1] == d[1] && d[2] == d[2]) {
+            for (uint32_t pl = 0; pl < sc->plane_count && pl < 4; pl++) {
+                const float* P = sc->planes[pl];
+                const double nd = P[0] * d[0] + P[1] * d[1] + P[2] * d[2];
+                if (nd >= -1e-9) continue;
+                const double t = -((double)P[3] + eta) / nd;
+                if (t > 0 && (best == 0.0 || t < best)) {
+                    best = t;
+                    gain = std::pow(-nd, sc->incidence_power);
+                }
+            }
+            for (const Sphere& S : spheres) {
+                // |t d - c|^2 = R^2, nearest positive root
+                const double dc = d[0] * S.c[0] + d[1] * S.c[1] + d[2] * S.c[2];
+                const double cc = S.c[0] * S.c[0] + S.c[1] * S.c[1] + S.c[2] * S.c[2];
+                const double disc = dc * dc - (cc - S.radius * S.radius);
+                if (disc < 0) continue;
+                const double t = dc - std::sqrt(disc);
+                if (!(t > 0) || (best != 0.0 && t >= best)) continue;
+                double nrm[3];
+                for (int r = 0; r < 3; r++) nrm[r] = (t * d[r] - S.c[r]) / S.radius;
+                const double ci = std::max(0.0, -(nrm[0] * d[0] + nrm[1] * d[1] + nrm[2] * d[2]));
+                best = t;
+                gain = S.gain * std::pow(ci, sc->incidence_power);
+            }
+        }
+        const double vr = values ? values[2 * q] : 1.0, vi = values ? values[2 * q + 1] : 0.0;
+        const double ar = gain * mag * std::cos(ph), ai = gain * mag * std::sin(ph);
+        range[k] = (float)best;
+        dx[k] = (float)d[0];
+        dy[k] = (float)d[1];
+        dz[k] = (float)d[2];
+        amp[2 * k] = best > 0 ? (float)(vr * ar - vi * ai) : 0.0f;
+        amp[2 * k + 1] = best > 0 ? (float)(vr * ai + vi * ar) : 0.0f;
+    }
+}
+
+// Host restatement of the directivity lookup of the streaming kernel
+// (DirectivityView, Directivity.h:30-34; tex2D linear filter, mirror
+// addressing, normalised coordinates), in the same FP32 operations.
+float mirror_coord_host(float x)
+{
+    const float f = std::floor(x), fr = x - f;
+    return ((int)f & 1) ? 1.0f - fr : fr;
+}
+
+int mirror_index_host(int i, int n)
+{
+    if (i < 0) i = -1 - i;
+    if (i >= n) i = 2 * n - 1 - i;
+    return std::min(std::max(i, 0), n - 1);
+}
+
+float directivity_host(const float* t, int W, int H, float xs, float ys, float x, float y, float z)
+{
+    const float u = mirror_coord_host(xs * std::atan2(y, x));
+    const float v = mirror_coord_host(ys * std::atan2(z, x));
+    const float xb = u * (float)W - 0.5f, yb = v * (float)H - 0.5f;
+    const float fi = std::floor(xb), fj = std::floor(yb);
+    const float a = xb - fi, b = yb - fj;
+    const int i0 = (int)fi, j0 = (int)fj;
+    const int c0 = mirror_index_host(i0, W), c1 = mirror_index_host(i0 + 1, W);
+    const int r0 = mirror_index_host(j0, H), r1 = mirror_index_host(j0 + 1, H);
+    const float t00 = t[r0 * W + c0], t10 = t[r0 * W + c1];
+    const float t01 = t[r1 * W + c0], t11 = t[r1 * W + c1];
+    const float top = t00 + a * (t10 - t00);
+    const float bot = t01 + a * (t11 - t01);
+    return top + b * (bot - top);
+}
+
 }  // namespace
 }  // namespace rtsim
 
@@ -162,6 +252,91 @@ int rtsim_generate_hits(const rtsim_scene_desc* scene, uint64_t seed, uint64_t f
         if (b >= e) break;
         pool.emplace_back(gen_range, scene, seed, first, b, e, std::cref(spheres), range, dir_x,
                           dir_y, dir_z, amplitude);
+    }
+    for (auto& th : pool) th.join();
+    return RTSIM_OK;
+}
+
+// Emitter::generate_polar_directions (Emitter.cu:59-80) in the reference's
+// FP32 arithmetic (degrees converted as `x *= M_PI / 180.0f`).
+int rtsim_emitter_polar_directions(float resolution_deg, float bearing_aperture_deg,
+                                   float elevation_aperture_deg, float* dirs, size_t capacity,
+                                   size_t* count)
+{
+    if (!count) return fail(RTSIM_ERR_INVALID, "count is NULL");
+    if (!(resolution_deg > 0.f) || !(bearing_aperture_deg >= 0.f) || !(elevation_aperture_deg >= 0.f))
+        return fail(RTSIM_ERR_INVALID, "emitter fan: resolution must be > 0, apertures >= 0");
+    float resolution = resolution_deg, bearing = bearing_aperture_deg, elevation = elevation_aperture_deg;
+    resolution *= M_PI / 180.0f;
+    bearing *= M_PI / 180.0f;
+    elevation *= M_PI / 180.0f;
+    const float hrows = elevation / resolution;
+    if (!(hrows < 1.0e6f)) return fail(RTSIM_ERR_INVALID, "emitter fan too large");
+    const int H = (int)hrows + 1;
+    size_t total = 0;
+    for (int h = 0; h < H; h++) {
+        const float phi = elevation * (((float)h) / (H - 1) - 0.5f);
+        const float wr = std::fabs(std::cos(phi)) * bearing / resolution;
+        // H == 1 makes phi NaN (0 / 0); the reference's (int) of NaN is INT_MIN
+        // on x86, so its row loop runs zero times: no rays
+        const int W = wr == wr ? (int)wr + 1 : 0;
+        if ((double)total + W > 1.0e9) return fail(RTSIM_ERR_INVALID, "emitter fan too large");
+        if (dirs) {
+            for (int w = 0; w < W; w++) {
+                if (total + w >= capacity)
+                    return fail(RTSIM_ERR_INVALID, "capacity below the fan's ray count");
+                const float theta = bearing * (((float)w) / (W - 1) - 0.5f);
+                float* d = dirs + 3 * (total + w);
+                d[0] = std::cos(theta) * std::cos(phi);
+                d[1] = std::sin(theta) * std::cos(phi);
+                d[2] = std::sin(phi);
+            }
+        }
+        total += (size_t)W;
+    }
+    *count = total;
+    return RTSIM_OK;
+}
+
+int rtsim_emitter_initial_values(const float* dirs, size_t n, const float* table, uint32_t width,
+                                 uint32_t height, float xscale, float yscale, float* values)
+{
+    if (n && (!dirs || !values)) return fail(RTSIM_ERR_INVALID, "NULL array");
+    if (table && (width < 1 || height < 1)) return fail(RTSIM_ERR_INVALID, "directivity table size");
+    for (size_t i = 0; i < n; i++) {
+        const float* d = dirs + 3 * i;
+        values[2 * i] = table ? directivity_host(table, (int)width, (int)height, xscale, yscale,
+                                                 d[0], d[1], d[2])
+                              : 1.0f;
+        values[2 * i + 1] = 0.0f;
+    }
+    return RTSIM_OK;
+}
+
+int rtsim_generate_emitter_hits(const rtsim_scene_desc* scene, uint64_t seed, const float* dirs,
+                                const float* values, size_t ndirs, const float* rotation,
+                                uint64_t first, size_t n, float* range, float* dir_x,
+                                float* dir_y, float* dir_z, float* amplitude, int nthreads)
+{
+    if (!scene) return fail(RTSIM_ERR_INVALID, "scene is NULL");
+    if (scene->kind != RTSIM_SCENE_SEABED)
+        return fail(RTSIM_ERR_INVALID, "emitter rays need a SEABED scene (planes / spheres)");
+    if (!dirs || ndirs == 0) return fail(RTSIM_ERR_INVALID, "emitter has no rays");
+    if (n && (!range || !dir_x || !dir_y || !dir_z || !amplitude))
+        return fail(RTSIM_ERR_INVALID, "NULL output array");
+    double rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    if (rotation)
+        for (int k = 0; k < 9; k++) rot[k] = rotation[k];
+    std::vector<Sphere> spheres = make_spheres(scene, seed);
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    size_t chunk = (n + nthreads - 1) / nthreads;
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; t++) {
+        size_t b = t * chunk, e = std::min(n, b + chunk);
+        if (b >= e) break;
+        pool.emplace_back(gen_emitter_range, scene, seed, dirs, values, ndirs, rot, first, b, e,
+                          std::cref(spheres), range, dir_x, dir_y, dir_z, amplitude);
     }
     for (auto& th : pool) th.join();
     return RTSIM_OK;

@@ -10,9 +10,10 @@
 //   dropin_example poses <poses.csv>
 //       parses a pose CSV (host only) and prints each rotation
 //   dropin_example drive <config.yaml> <poses.csv> <samples> <seed> <form> <in_flight>
-//                        <out.rtpg>
+//                        <out.rtpg> [emitter]
 //       runs the simulation loop over every pose, in batches of in_flight
-//       pings, into one ping file
+//       pings, into one ping file; with `emitter`, the rays are those of an
+//       Emitter (0.5 deg polar fan, 130 x 20 deg, 20 mm disk directivity)
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -88,6 +89,11 @@ int main(int argc, char** argv)
                                     static_cast<Form>(std::atoi(argv[6])),
                                     (unsigned)std::atoi(argv[7]), std::strtoull(argv[5], nullptr, 10));
             sim.set_receiver_poses(PoseSourceList::CreateFromCSV(argv[3]));
+            if (argc >= 10 && std::string(argv[9]) == "emitter") {
+                const float f = info->desc().frequency, lambda = 1500.0f / f;
+                sim.set_emitter(Emitter::Create(0.5f, 130.0f, 20.0f,
+                                                Directivity::disk_antenna(0.02f, lambda), f));
+            }
             FileSink::Ptr sink = FileSink::Create(argv[8], true);
             sim.add_sink(sink);
             size_t rounds = 0;

@@ -69,3 +69,38 @@ def test_drive_equals_per_ping_computes(gpu, tmp_path, in_flight):
         # rotations from float32 quaternion math on both sides agree to ~1 ulp
         from oracle import rel_l2
         assert rel_l2(ping, s.output()) < 1e-5, k
+
+
+@pytest.mark.gpu
+def test_drive_with_emitter_equals_per_ping_computes(gpu, tmp_path):
+    """The loop with an Emitter's rays (Emitter::Create with a polar fan and a
+    disk directivity) equals per-ping computes of the same emitter hits."""
+    R_ = gpu
+    poses = tmp_path / "poses.csv"
+    lines = POSES.read_text().splitlines()
+    poses.write_text("\n".join(lines[:4]) + "\n")          # header + 3 poses
+    out = tmp_path / "drive_emitter.rtpg"
+    n, seed, form = 20_000, 5, 2
+    cfgp = R_.CONFIG_DIR / "fls_512x1024.yaml"
+    r = _run("drive", cfgp, poses, n, seed, form, 2, out, "emitter")
+    assert r.returncode == 0, r.stderr
+    assert "pings=3" in r.stdout and R_.ping_file_count(out) == 3
+    cfg = R_.config_from_yaml(cfgp)
+    lam = np.float32(1500.0) / np.float32(cfg.frequency)
+    dirs = R_.emitter_polar_directions(0.5, 130.0, 20.0)
+    vals = R_.emitter_initial_values(dirs, R_.directivity_disk(0.02, float(lam), True, 8))
+    sc = configs.fls_scene()
+    s = R_.make_sensor(cfg)
+    s.set_sample_count(n)
+    q = _poses_from_csv(poses)
+    from oracle import rel_l2
+    for k in range(3):
+        Rk = Rotation.from_quat(q[k, [4, 5, 6, 3]]).as_matrix().astype(np.float32)
+        s.set_pose(Rk)
+        h = R_.generate_emitter_hits(sc, seed + k, dirs, vals, n)
+        assert (h.range > 0).mean() > 0.3
+        s.write_samples(h.to_device())
+        s.compute_output(form)
+        ping = R_.read_ping(out, k)["data"]
+        assert np.abs(ping).max() > 0
+        assert rel_l2(ping, s.output()) < 1e-5, k
