@@ -121,7 +121,8 @@ __global__ __launch_bounds__(256) void k_stream_batch(const StreamArgs* __restri
 {
     uint32_t blk, nblk;
     const uint32_t q = batch_locate(m, blk, nblk);
-    stream_body(args[q], blk, nblk);
+    const StreamArgs a = args[q];   // a private copy: the body's stores cannot alias it
+    stream_body(a, blk, nblk);
 }
 
 uint32_t stream_blocks(const StreamArgs& a)
@@ -172,7 +173,8 @@ __global__ __launch_bounds__(256) void k_windows_batch(const BinArgs* __restrict
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    windows_body(args[p], blk);
+    const BinArgs a = args[p];
+    windows_body(a, blk);
 }
 
 uint32_t windows_blocks(const BinArgs& a) { return (a.N + 255) / 256; }

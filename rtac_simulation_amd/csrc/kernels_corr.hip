@@ -360,7 +360,7 @@ __global__ __launch_bounds__(256) void k_correlate_batch(const CorrArgs* __restr
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    const CorrArgs& a = args[p];
+    const CorrArgs a = args[p];   // a private copy: the body's stores cannot alias it
     const int gx = (a.W + 32 * TC - 1) / (32 * TC);
     correlate_body<FP16, TR, TC>(a, (int)blk % gx, (int)blk / gx);
 }

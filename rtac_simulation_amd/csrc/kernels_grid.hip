@@ -153,7 +153,8 @@ __global__ __launch_bounds__(256) void k_deposit_batch(const DepositArgs* __rest
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    deposit_or_windows(args[p], blk);
+    const DepositArgs a = args[p];   // a private copy: the body's stores cannot alias it
+    deposit_or_windows(a, blk);
 }
 
 uint32_t deposit_blocks(const DepositArgs& a) { return (uint32_t)a.GH + windows_blocks(a.bins); }

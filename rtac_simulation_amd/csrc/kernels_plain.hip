@@ -107,7 +107,8 @@ __global__ __launch_bounds__(256) void k_conv2d_plain_batch(const ConvArgs* __re
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    conv2d_plain_body(args[p], (int)blk);
+    const ConvArgs a = args[p];   // a private copy: the body's stores cannot alias it
+    conv2d_plain_body(a, (int)blk);
 }
 
 uint32_t conv2d_plain_blocks(const ConvArgs& a) { return a.W ? (uint32_t)a.H : 0u; }
@@ -167,7 +168,8 @@ __global__ __launch_bounds__(256) void k_conv1d_plain_batch(const ConvArgs* __re
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    conv1d_plain_body(args[p], blk);
+    const ConvArgs a = args[p];
+    conv1d_plain_body(a, blk);
 }
 
 uint32_t conv1d_plain_blocks(const ConvArgs& a) { return (uint32_t)(a.H + 3) / 4; }

@@ -113,7 +113,8 @@ __global__ __launch_bounds__(256) void k_sort_hist_batch(const HistArgs* __restr
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    sort_hist_body<FIRST>(args[p], blk, nblk);
+    const HistArgs a = args[p];   // a private copy: the body's stores cannot alias it
+    sort_hist_body<FIRST>(a, blk, nblk);
 }
 
 // ---- one workgroup per digit: exclusive scan over tiles, digit total ---------
@@ -151,7 +152,8 @@ __global__ __launch_bounds__(256) void k_sort_scan_batch(const ScanArgs* __restr
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    sort_scan_body(args[p], blk);
+    const ScanArgs a = args[p];
+    sort_scan_body(a, blk);
 }
 
 // ---- rank + stage + scatter ----------------------------------------------------
@@ -286,9 +288,11 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             pq[u] = gpos[d] + (q - toff[d]);
         }
     }
-    float4 rq[Q];
+    // the last pass holds its records (keys, bounds and stores use them); an
+    // earlier pass reads each one from LDS at its store
+    float4 rq[LAST ? Q : 1];
     uint32_t bq[Q];
-    if (MOVE) {
+    if (LAST && MOVE) {
 #pragma unroll
         for (int u = 0; u < Q; u++) rq[u] = srec[min(t + u * kSortThreads, kSortTile - 1)];
     } else if (LAST) {
@@ -342,7 +346,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         if (pos < a.n) {   // holds for every staged item unless the records changed
             if (LAST) a.order[pos] = vq[u];   // the sorted keys themselves are not kept
             else a.kvout[pos] = make_uint2(kq[u], vq[u]);
-            if (MOVE && !LAST) a.rout[pos] = rq[u];
+            if (MOVE && !LAST) a.rout[pos] = srec[t + u * kSortThreads];
             if (LAST) {
                 const float4 r = rq[u];
                 a.out.sorted[pos] = r;
