@@ -193,7 +193,9 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         }
     } else {
         load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
-        if (MOVE) {
+        // a middle pass reads its records at the staging step instead (held
+        // across the ranking they would spill)
+        if (MOVE && LAST) {
 #pragma unroll
             for (int j = 0; j < kSortLaneItems; j++)
                 rv[j] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
@@ -265,7 +267,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             const uint32_t p = wcnt[w][d] + lrank[j];
             skey[p] = key[j];
             sval[p] = val[j];
-            if (MOVE) srec[p] = rv[j];
+            if (MOVE) srec[p] = (FIRST || LAST) ? rv[j] : a.rin[wbase + j * 64 + lane];
         }
     }
     __syncthreads();
