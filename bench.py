@@ -3,8 +3,10 @@
 Workload: the 512 x 1024 ping (512 bearings uniform in sin over +-65 deg x
 1024 range bins over [0.1, 30] m) from 1 000 000 synthetic seabed samples,
 non-separable 31 x 17 complex PSF.  One step = one ping from ray hits already
-resident in HBM: streaming kernel (directivity x exp(i 2kr)) -> sort by
-travel -> Binner (keys, segments, windows) -> PSF convolution.  Every form is
+resident in HBM: streaming kernel (directivity x exp(i 2kr)) -> sort (by
+travel for the plain form, by the grouping key of bin and grid row for the
+MFMA forms) with the Binner's keys and segments -> windows -> PSF
+convolution (plain gather, or deposit + MFMA correlation).  Every form is
 timed over the same K steps with HIP events on the compute stream; the
 headline value is the MFMA form's pings/s.
 
@@ -132,18 +134,17 @@ def accuracy(cfg, sensor, plain_ping, form_pings, dbgs):
 STAGE_BYTES = {
     "stream": lambda S, cfg, g: 40.0 * S,                    # r 4 + dir 12 + A 8 in, 16 out
     "sort": lambda S, cfg, g: 32.0 * S,                      # one read + one write of records
-    "bins": lambda S, cfg, g: 4.0 * S + 16.0 * cfg.H,        # keys read, segments + windows
+    "bins": lambda S, cfg, g: 16.0 * cfg.H,                  # segments read, windows written
     "deposit": lambda S, cfg, g: 16.0 * S + 8.0 * g,         # samples in, grid out
 }
 
 
 def grid_cells(cfg) -> float:
     """Cells of the extended grid the deposit writes (rows and columns padded
-    to 64, plus the PSF halo)."""
+    to 64, plus the PSF halo; columns even)."""
     hp = -(-cfg.H // 64) * 64
     wp = -(-cfg.W // 64) * 64
-    kb = (cfg.Tb + 15 + 31) // 32
-    return float((hp + cfg.Tr - 1) * (wp - 16 + 32 * kb))
+    return float((hp + cfg.Tr - 1) * (wp + cfg.Tb - 1 + ((cfg.Tb - 1) & 1)))
 
 
 def stage_roofline(stage, ms, S, cfg, dtype):
@@ -158,6 +159,27 @@ def stage_roofline(stage, ms, S, cfg, dtype):
     ach = nbytes / (ms * 1e-3) / 1e9
     return {"stage": stage, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{nbytes:.4g} B per ping"}
+
+
+def copy_calibration(nbytes: int = 256 << 20, repeats: int = 3):
+    """hipMemcpy device-to-device of `nbytes`, `repeats` times: a kernel of known
+    traffic (reads and writes nbytes each) for calibrating the PMC byte counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    for p in (a, b):
+        if hip.hipMalloc(C.byref(p), C.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed")
+    if hip.hipMemset(a, 1, C.c_size_t(nbytes)) != 0:
+        raise RuntimeError("hipMemset failed")
+    for _ in range(repeats):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) != 0:   # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    if hip.hipDeviceSynchronize() != 0:
+        raise RuntimeError("hipDeviceSynchronize failed")
+    for p in (a, b):
+        hip.hipFree(p)
+    print(f"copied {nbytes} bytes x{repeats}", flush=True)
 
 
 def measured_traffic(workload: str, form: str, stage: str):
@@ -239,9 +261,16 @@ def main():
     ap.add_argument("--workload", default="fls_512x1024")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-check", action="store_true", help="skip the oracle accuracy check")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB device to device three times (one hipMemcpy "
+                         "dispatch each): the reference kernel for scaling FETCH_SIZE / "
+                         "WRITE_SIZE counters to bytes")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched and 1024x4096 workloads")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
               "sharding of this path is a design note, not a build target", file=sys.stderr)
