@@ -78,12 +78,13 @@ def cpu_baseline(cfg, records, budget_s=20.0):
     t0 = time.perf_counter()
     srt, win = O.cpu_binning(cfg, records, threads)
     t_bin = time.perf_counter() - t0
-    # rows spread over the range axis; grow the sample until ~budget/4
+    # rows spread over the range axis; grow the sample until about half the
+    # budget has been spent on gathers
     rows_total = 0
     t_rows = 0.0
     step = 64
     start = 3
-    while t_rows < budget_s / 4 and rows_total < cfg.H:
+    while t_rows < budget_s / 2 and rows_total < cfg.H:
         rows = list(range(start, cfg.H, step))
         t0 = time.perf_counter()
         for r in rows:
@@ -327,7 +328,7 @@ def main():
     roof["traffic"] = traffic
     roof["kernel"] = {"stream": "k_stream",
                       "sort": "k_sort_hist + k_sort_scan + k_sort_scatter (2 grouping-key passes)",
-                      "bins": "k_bounds + k_windows", "deposit": "k_deposit",
+                      "bins": "k_windows", "deposit": "k_deposit",
                       "convolution": "k_correlate"}[dominant]
     roof["measured"] = "HIP events around the stage on the compute stream, mean over " \
                        f"{max(5, args.steps // 5)} pings"
