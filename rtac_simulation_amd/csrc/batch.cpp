@@ -168,7 +168,6 @@ static int build(Batch* b)
             cs.shm = std::max(cs.shm, correlate_lds_bytes(p.corr));
             s->last_corr = p.corr;
         }
-        s->order_ptr = p.sort.order;
         en.generation = s->generation;
     }
     // one device buffer for every descriptor array and prefix table
@@ -339,7 +338,10 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
     }
     hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
-    for (auto& en : b->entries) en.sensor->computed = true;
+    for (auto& en : b->entries) {
+        en.sensor->computed = true;
+        en.sensor->computed_form = en.form;
+    }
     return RTSIM_OK;
 }
 

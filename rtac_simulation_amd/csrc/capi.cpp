@@ -146,6 +146,7 @@ void Sensor::free_samples()
     dev_free(d_kv[1]);
     dev_free(d_order);
     dev_free(d_skeys);
+    dev_free(d_tmax);
     dev_free(d_counts);
     capacity = 0;
 }
@@ -166,6 +167,7 @@ int Sensor::reserve(size_t count)
     if ((rc = dev_alloc(&d_order, count))) return rc;
     if ((rc = dev_alloc(&d_skeys, count))) return rc;
     if ((rc = dev_alloc(&d_counts, sort_count_words(count)))) return rc;
+    if ((rc = dev_alloc(&d_tmax, sort_count_words(count) / 256))) return rc;
     capacity = count;
     return RTSIM_OK;
 }
@@ -276,7 +278,7 @@ int Sensor::check_plain_limits() const
     return RTSIM_OK;
 }
 
-int Sensor::plan(int form, PingPlan* p)
+int Sensor::plan(int form, PingPlan* p, bool introspect)
 {
     const uint32_t n = (uint32_t)count;
     const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
@@ -298,7 +300,7 @@ int Sensor::plan(int form, PingPlan* p)
     SortBuffers sb;
     sb.kv[0] = d_kv[0];
     sb.kv[1] = d_kv[1];
-    sb.order = d_order;
+    sb.order = introspect ? d_order : nullptr;
     sb.counts = d_counts;
     sb.totals = d_totals;
     sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
@@ -309,8 +311,15 @@ int Sensor::plan(int form, PingPlan* p)
     sb.fill_value = n;
     sb.fill_count = rows.row_start ? (size_t)rows.GH + 3 : 0;
     sb.out.sorted = d_sorted;
-    sb.out.skeys = d_skeys;
+    sb.out.skeys = introspect ? d_skeys : nullptr;
     sb.out.coord = d_coord;
+    sb.out.xg = nullptr;
+    if (grid) {   // the deposit reads grid column coordinates (in the same buffer)
+        sb.out.xg = reinterpret_cast<float*>(d_coord);
+        sb.out.xg_b0 = rows.bear0;
+        sb.out.xg_inv = rows.inv_db;
+        sb.out.xg_off = rows.off_b;
+    }
     sb.out.key = key;
     sb.out.axis = axis;
     sb.out.seg = d_seg;
@@ -322,6 +331,7 @@ int Sensor::plan(int form, PingPlan* p)
         gk.rows = rows;
         gk.key = key;
         gk.vmax = vmax;
+        gk.tmax = d_tmax;
         if (group_key_max(gk) >= (1u << 24))
             return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
                         rows.GH, N);
@@ -354,7 +364,6 @@ int Sensor::compute(int form, hipStream_t s)
     if (rc) return rc;
     stage_begin(1, s);
     HIP_TRY(launch_sort_plan(p.sort, s));
-    order_ptr = p.sort.order;
     stage_end(s);
     if (!p.grid) {
         stage_begin(2, s);
@@ -374,6 +383,7 @@ int Sensor::compute(int form, hipStream_t s)
         last_corr = p.corr;
     }
     computed = true;
+    computed_form = form;
     return RTSIM_OK;
 }
 
@@ -641,10 +651,25 @@ int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments
     HIP_TRY(hipStreamSynchronize(st));
     if (!s->computed) return fail(RTSIM_ERR_INVALID, "compute has not run since the last change");
     size_t n = s->count;
+    if ((keys || order) && n) {
+        // compute writes neither the keys nor the permutation (nothing
+        // downstream reads them): sort the same samples again with both kept,
+        // then the windows (which also write empty segments back as (0, 0)).
+        // Both are deterministic, so every array they rewrite (sorted
+        // records, coordinates, segments, windows, row table) gets the same
+        // contents.
+        PingPlan p;
+        int rc = s->plan(s->computed_form, &p, true);
+        if (rc) return rc;
+        (void)hipGetLastError();
+        HIP_TRY(launch_sort_plan(p.sort, st));
+        HIP_TRY(launch_windows(p.bins, st));
+        HIP_TRY(hipStreamSynchronize(st));
+    }
     if (keys && n) HIP_TRY(hipMemcpy(keys, s->d_skeys, 4 * n, hipMemcpyDeviceToHost));
     if (segments) HIP_TRY(hipMemcpy(segments, s->d_seg, 8 * (size_t)s->N, hipMemcpyDeviceToHost));
     if (windows) HIP_TRY(hipMemcpy(windows, s->d_win, 8 * (size_t)s->N, hipMemcpyDeviceToHost));
-    if (order && n) HIP_TRY(hipMemcpy(order, s->order_ptr, 4 * n, hipMemcpyDeviceToHost));
+    if (order && n) HIP_TRY(hipMemcpy(order, s->d_order, 4 * n, hipMemcpyDeviceToHost));
     if (overlap) *overlap = s->overlap;
     return RTSIM_OK;
 }

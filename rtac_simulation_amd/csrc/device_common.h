@@ -37,6 +37,26 @@ __device__ __forceinline__ uint32_t bin_key(const KeyParams& k, float t)
     return (uint32_t)res;
 }
 
+// sin(x) in FP64 for bearing-axis coordinates.  |x| <= pi/2 (every sensor
+// bearing): the Taylor polynomial of degree 19, whose truncation error is
+// below (pi/2)^21 / 21! < 3e-16, in Horner form on x^2 (11 FP64 operations
+// instead of a full range-reduced sine); larger |x| and NaN use sin().
+__device__ __forceinline__ double sin_bearing(double x)
+{
+    if (!(fabs(x) <= 1.5707963267948966)) return sin(x);
+    const double z = x * x;
+    double p = -1.0 / 121645100408832000.0;            // -1/19!
+    p = __builtin_fma(p, z, 1.0 / 355687428096000.0);  //  1/17!
+    p = __builtin_fma(p, z, -1.0 / 1307674368000.0);   // -1/15!
+    p = __builtin_fma(p, z, 1.0 / 6227020800.0);       //  1/13!
+    p = __builtin_fma(p, z, -1.0 / 39916800.0);        // -1/11!
+    p = __builtin_fma(p, z, 1.0 / 362880.0);           //  1/9!
+    p = __builtin_fma(p, z, -1.0 / 5040.0);            // -1/7!
+    p = __builtin_fma(p, z, 1.0 / 120.0);              //  1/5!
+    p = __builtin_fma(p, z, -1.0 / 6.0);               // -1/3!
+    return __builtin_fma(p * z, x, x);
+}
+
 // Grid (MFMA) form: range coordinate of a travel in grid rows, one explicit
 // fma so that the row table and the deposit round identically.
 __device__ __forceinline__ float grid_y_of(float t, float range0, float inv_dr, float off_r)

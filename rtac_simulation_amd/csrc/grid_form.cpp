@@ -86,7 +86,7 @@ struct GridState {
     int shape = 0;                       // correlation tile shape in use
     bool fits[kCorrShapes] = {false, false, false, false};
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
-    float bear0_hi = 0, bear0_lo = 0;
+    double bear0 = 0;
     float tap_scale = 1.f;
     float2* d_grid = nullptr;
     uint32_t* d_row_start = nullptr;
@@ -162,8 +162,7 @@ static int prepare_grid(Sensor* s, GridState* g)
     g->off_b = (float)(0.5 * (s->Tb - 1) - g->fb);
     g->inv_dr = (float)(1.0 / dr);
     g->inv_db = (float)(1.0 / psf_pb);
-    g->bear0_hi = (float)x[0];
-    g->bear0_lo = (float)(x[0] - (double)g->bear0_hi);
+    g->bear0 = x[0];
 
     int dev_id = 0, max_lds = 0;
     (void)hipGetDevice(&dev_id);
@@ -235,7 +234,7 @@ static int prepare_grid(Sensor* s, GridState* g)
         return hip_fail(err, "hipMemset(grid)");
     if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
         return hip_fail(err, "hipMalloc(row_start)");
-    if ((err = hipMalloc((void**)&g->d_vmax, sizeof(uint32_t) * kVmaxSlots)) != hipSuccess)
+    if ((err = hipMalloc((void**)&g->d_vmax, sizeof(uint32_t))) != hipSuccess)
         return hip_fail(err, "hipMalloc(vmax)");
     for (int d = 0; d < 2; d++) {
         if ((err = hipMalloc((void**)&g->d_toep[d], sizeof(uint16_t) * tsize)) != hipSuccess)
@@ -272,8 +271,70 @@ int Sensor::grid_row_map(RowMap* m, uint32_t** vmax)
     m->fr = g->fr;
     m->GH = g->GH;
     m->row_start = g->d_row_start;
+    m->bear0 = g->bear0;
+    m->inv_db = g->inv_db;
+    m->off_b = g->off_b;
     if (vmax) *vmax = g->d_vmax;
     return RTSIM_OK;
+}
+
+// Binner::Key::operator() (Binner.h:47-53) on the host: the same FP32 fused
+// multiply-add, truncation and clamps as bin_key on the device.
+static uint32_t host_bin_key(const KeyParams& k, float t)
+{
+    if (t != t) return kOutOfRange;
+    const float v = std::fmaf(k.a, t, k.b);
+    if (v != v) return kOutOfRange;
+    int res;
+    if (v >= 2147483648.0f)       res = INT_MAX;
+    else if (v <= -2147483648.0f) res = INT_MIN;
+    else                          res = (int)v;
+    if (res < 0) return ((float)res >= k.lower) ? 0u : kOutOfRange;
+    if ((uint32_t)res >= k.n) return ((float)res <= k.upper) ? k.n - 1u : kOutOfRange;
+    return (uint32_t)res;
+}
+
+// float of an order-preserving 32-bit image (the image is bits | 2^31 for
+// x >= 0, ~bits for x < 0)
+static float float_of_image(uint32_t u)
+{
+    const uint32_t bits = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &bits, 4);
+    return f;
+}
+static uint32_t image_of_float(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+// The travels with a Binner key form one interval of floats: the key's
+// integer is non-decreasing in travel and each clamp compares it with one
+// bound, and -inf, +inf and NaN are out of range.  Bisection over the ordered
+// float images from a travel inside the range finds its ends exactly, so the
+// deposit tests `lo <= t && t <= hi` instead of evaluating the key.
+static bool key_travel_interval(const KeyParams& k, float inside, float* lo, float* hi)
+{
+    if (host_bin_key(k, inside) == kOutOfRange) return false;
+    const uint32_t um = image_of_float(inside);
+    uint32_t out = image_of_float(-INFINITY), in = um;   // key(out) is OOR, key(in) is not
+    while (in - out > 1) {
+        const uint32_t mid = out + (in - out) / 2;
+        if (host_bin_key(k, float_of_image(mid)) == kOutOfRange) out = mid;
+        else in = mid;
+    }
+    *lo = float_of_image(in);
+    in = um;
+    out = image_of_float(INFINITY);
+    while (out - in > 1) {
+        const uint32_t mid = in + (out - in) / 2;
+        if (host_bin_key(k, float_of_image(mid)) == kOutOfRange) out = mid;
+        else in = mid;
+    }
+    *hi = float_of_image(in);
+    return true;
 }
 
 int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
@@ -284,8 +345,9 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
 
     DepositArgs& d = *dp;
     d.sorted = d_sorted;
-    d.coord = d_coord;
-    d.skeys = d_skeys;
+    d.xg = reinterpret_cast<const float*>(d_coord);   // written by the sort's last pass
+    if (!key_travel_interval(key, 0.5f * (range_min + range_max), &d.t_lo, &d.t_hi))
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: the range midpoint has no Binner key");
     d.n = (uint32_t)count;
     d.row_start = g->d_row_start;
     d.vmax = g->d_vmax;
@@ -297,10 +359,6 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     d.range0 = range_min;
     d.inv_dr = g->inv_dr;
     d.off_r = g->off_r;
-    d.bear0_hi = g->bear0_hi;
-    d.bear0_lo = g->bear0_lo;
-    d.inv_db = g->inv_db;
-    d.off_b = g->off_b;
 
     CorrArgs& c = *cp;
     c.grid = g->d_grid;

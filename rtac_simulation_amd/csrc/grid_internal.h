@@ -25,17 +25,16 @@ int corr_copy_stride_host(int Tb, int L);
 // out(r, b) = sum T[i][j] G[r + i][b + j].
 struct DepositArgs {
     const float4* sorted;
-    const float2* coord;       // bearing-axis coordinate (hi, lo) per sorted sample
-    const uint32_t* skeys;     // OutOfRange samples are skipped
+    const float* xg;           // grid column coordinate per sorted sample (from the sort)
+    float t_lo, t_hi;          // travels with a Binner key: [t_lo, t_hi] (others are skipped)
     uint32_t n;
     const uint32_t* row_start; // GH + 3 entries: first sample of each G row present, else n
-    const uint32_t* vmax;      // kVmaxSlots words whose maximum bounds |re|, |im| (FP32 bits)
+    const uint32_t* vmax;      // 1 word: the largest finite in-range |re|, |im| (FP32 bits)
     BinArgs bins;              // the launch's last workgroups compute the windows
     float2* grid;              // [GH][GW]
     int GH, GW;
     int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)
     float range0, inv_dr, off_r;        // y = (t - range0) / dr + off_r
-    float bear0_hi, bear0_lo, inv_db, off_b;  // x = (x_s - bear0) / db + off_b
 };
 
 struct CorrArgs {

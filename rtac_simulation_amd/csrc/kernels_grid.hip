@@ -73,8 +73,7 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
     for (int k = t; k < 2 * GW; k += 256) sacc[k] = 0;
     uint32_t s0, s1;   // first samples with G row >= rho - 1 and >= rho + 1
     run_bounds(a.row_start, a.GH + 3, a.n, rho, s0, s1);
-    float M = __uint_as_float(a.vmax[threadIdx.x & (kVmaxSlots - 1)]);
-    for (int o = kVmaxSlots / 2; o > 0; o >>= 1) M = fmaxf(M, __shfl_xor(M, o, 64));
+    const float M = __uint_as_float(a.vmax[0]);
     int em = 0;
     frexpf(M, &em);                                   // M < 2^em (em = 0 for M = 0)
     const int ec = 32 - __clz(max(s1 - s0, 1u));     // run length < 2^ec
@@ -85,15 +84,13 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
     float2* row = a.grid + (size_t)rho * GW;
 
     for (uint32_t sb = s0; sb < s1; sb += U * 256) {
-        uint32_t key[U];
         float4 rec[U];
-        float2 xc[U];
+        float xg[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const uint32_t s = min(sb + u * 256 + t, s1 - 1);
-            key[u] = a.skeys[s];
             rec[u] = a.sorted[s];
-            xc[u] = a.coord[s];
+            xg[u] = a.xg[s];
         }
         // weights for every slot first (no branch between the loads and their
         // use), then the atomics; a slot past the run or out of range gets
@@ -103,13 +100,13 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const float y = grid_y(a, rec[u].z);
-            const float x = ((xc[u].x - a.bear0_hi) + (xc[u].y - a.bear0_lo)) * a.inv_db + a.off_b;
+            const float x = xg[u];
             const float fy = floorf(y), fx = floorf(x);
             const float wy1 = y - fy;
             const int gy = (int)fy + a.fr;
             // this row receives weight 1 - wy1 from gy == rho, wy1 from gy == rho - 1
             const float wy = gy == rho ? 1.0f - wy1 : (gy == rho - 1 ? wy1 : 0.0f);
-            const bool ok = sb + u * 256 + t < s1 && key[u] != kOutOfRange;
+            const bool ok = sb + u * 256 + t < s1 && rec[u].z >= a.t_lo && rec[u].z <= a.t_hi;
             bad |= ok && !(fmaxf(fabsf(rec[u].x), fabsf(rec[u].y)) <= 3.402823466e38f &&
                            rec[u].x == rec[u].x && rec[u].y == rec[u].y);
             wx[u] = x - fx;

@@ -14,7 +14,8 @@
 //                    tile in LDS in digit order and writes every digit run to
 //                    consecutive global positions.  The last pass also moves
 //                    the records to their sorted positions and writes their
-//                    Binner keys and bearing-axis coordinates.
+//                    bearing-axis coordinates (the Binner keys and the
+//                    permutation only when asked for).
 // Kernel boundaries order the passes; no workgroup ever waits for another.
 // Items of a tile are ordered (wave, round, lane) = ascending index, ranks
 // count earlier rounds, lower lanes and earlier waves, tiles are scanned in
@@ -31,6 +32,10 @@ namespace rtsim {
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;                  // 4
 constexpr int kSortLaneItems = (int)kSortTile / kSortThreads;  // 16
+// grouping key: split its bits evenly over the passes (else 8-bit digits)
+constexpr bool kGroupEvenDigits = true;
+// the last pass of a sort whose records move loads them before the ranking
+constexpr bool kLastEarlyRecords = false;
 
 // Inclusive wave scan (wave64) of a 32-bit value.
 __device__ __forceinline__ uint32_t sort_wave_scan(uint32_t x, uint32_t lane)
@@ -88,18 +93,37 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
             a.seg_init[i] = make_uint2(0xffffffffu, 0u);
         for (size_t i = (size_t)blk * 256 + t; i < a.fill_count; i += (size_t)nblk * 256)
             a.fill[i] = a.fill_value;
-        if (a.gk.on && blk == 0 && t < (uint32_t)kVmaxSlots) a.gk.vmax[t] = 0u;
     }
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
     uint32_t key[kSortLaneItems];
-    load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, nullptr);
+    __shared__ float wmax[kSortWaves];
+    if (FIRST && a.gk.on) {
+        // grouping keys, and the largest finite |re|, |im| of the in-range
+        // samples (the record's components share the travel's cache line)
+        float m = 0.f;
+#pragma unroll
+        for (int j = 0; j < kSortLaneItems; j++) {
+            const float4 r = a.rec[min(wbase + j * 64 + lane, a.n - 1)];
+            const uint32_t slot = bin_slot(a.gk.key, r.z);
+            key[j] = (uint32_t)(g_row_of(a.gk.rows, r.z) + 2) + slot;
+            const float c = fmaxf(fabsf(r.x), fabsf(r.y));
+            if (wbase + j * 64 + lane < a.n && slot - 1u < a.gk.key.n && c <= 3.402823466e38f)
+                m = fmaxf(m, c);
+        }
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if (lane == 0) wmax[w] = m;
+    } else {
+        load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, nullptr);
+    }
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++)
-        if (wbase + j * 64 + lane < a.n) atomicAdd(&h[(key[j] >> a.shift) & 255u], 1u);
+        if (wbase + j * 64 + lane < a.n) atomicAdd(&h[(key[j] >> a.shift) & a.mask], 1u);
     __syncthreads();
     a.counts[(size_t)t * a.tiles + blk] = h[t];
+    if (FIRST && a.gk.on && t == 0)
+        a.gk.tmax[blk] = __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
 }
 
 template <bool FIRST>
@@ -144,6 +168,16 @@ __device__ __forceinline__ void sort_scan_body(const ScanArgs& a, uint32_t digit
         run += v[k];
     }
     if (t == 0) a.totals[digit] = total;
+    if (a.tmax && digit == 0) {
+        // deposit scale: the largest tile maximum (non-negative floats)
+        __shared__ float wm[4];
+        float m = 0.f;
+        for (uint32_t i = t; i < tiles; i += 256) m = fmaxf(m, __uint_as_float(a.tmax[i]));
+        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+        if ((t & 63) == 0) wm[t >> 6] = m;
+        __syncthreads();
+        if (t == 0) a.vmax[0] = __float_as_uint(fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3])));
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sort_scan(ScanArgs a) { sort_scan_body(a, blockIdx.x); }
@@ -154,6 +188,13 @@ __global__ __launch_bounds__(256) void k_sort_scan_batch(const ScanArgs* __restr
     const uint32_t p = batch_locate(m, blk, nblk);
     const ScanArgs a = args[p];
     sort_scan_body(a, blk);
+}
+
+// Binner key from the value the last pass compares: a bin slot (grouping key)
+// or the key itself
+__device__ __forceinline__ uint32_t binner_key(bool grp, uint32_t c, uint32_t n)
+{
+    return grp ? (c - 1u < n ? c - 1u : kOutOfRange) : c;
 }
 
 // ---- rank + stage + scatter ----------------------------------------------------
@@ -170,9 +211,11 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
     __shared__ float4 srec[MOVE ? kSortTile : 1];
+    __shared__ uint32_t scmp[LAST ? kSortTile : 1];   // last pass: bin slot / Binner key
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t tb = tile * kSortTile;
     const int shift = a.shift;
+    const uint32_t mask = a.mask;
 
     // global run start of digit t: digit base (scan of the totals) + earlier tiles
     const uint32_t tot = a.totals[t];
@@ -193,28 +236,13 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         }
     } else {
         load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
-        // a middle pass reads its records at the staging step instead (held
-        // across the ranking they would spill)
-        if (MOVE && LAST) {
+        // later passes read their records at the staging step instead (held
+        // across the ranking they spill)
+        if (MOVE && LAST && kLastEarlyRecords) {
 #pragma unroll
             for (int j = 0; j < kSortLaneItems; j++)
                 rv[j] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
         }
-    }
-    if (LAST && MOVE && a.gk.on) {
-        // largest finite component of the in-range samples (non-negative
-        // floats order like their bit patterns); issued first so that the
-        // atomic's round trip overlaps the rest of the pass
-        float m = 0.f;
-#pragma unroll
-        for (int j = 0; j < kSortLaneItems; j++) {
-            const float c = fmaxf(fabsf(rv[j].x), fabsf(rv[j].y));
-            if (wbase + j * 64 + lane < a.n && bin_key(a.out.key, rv[j].z) != kOutOfRange &&
-                c <= 3.402823466e38f)
-                m = fmaxf(m, c);
-        }
-        for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
-        if (lane == 0) atomicMax(&a.gk.vmax[(tile * kSortWaves + w) % kVmaxSlots], __float_as_uint(m));
     }
     __syncthreads();
 
@@ -222,10 +250,10 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++) {
         const bool ok = wbase + j * 64 + lane < a.n;
-        const uint32_t d = (key[j] >> shift) & 255u;
+        const uint32_t d = (key[j] >> shift) & mask;
         uint64_t peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < 8; b++) {
+        for (int b = 0; b < 8; b++) {   // bits above dbits are zero: no-op ballots
             const uint32_t bit = (d >> b) & 1u;
             const uint64_t m = __ballot(bit);
             peers &= bit ? m : ~m;
@@ -263,11 +291,12 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 #pragma unroll
     for (int j = 0; j < kSortLaneItems; j++) {
         if (wbase + j * 64 + lane < a.n) {
-            const uint32_t d = (key[j] >> shift) & 255u;
+            const uint32_t d = (key[j] >> shift) & mask;
             const uint32_t p = wcnt[w][d] + lrank[j];
             skey[p] = key[j];
             sval[p] = val[j];
-            if (MOVE) srec[p] = (FIRST || LAST) ? rv[j] : a.rin[wbase + j * 64 + lane];
+            if (MOVE)
+                srec[p] = (FIRST || (LAST && kLastEarlyRecords)) ? rv[j] : a.rin[wbase + j * 64 + lane];
         }
     }
     __syncthreads();
@@ -286,14 +315,14 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         if (q < tn) {
             kq[u] = skey[q];
             vq[u] = sval[q];
-            const uint32_t d = (kq[u] >> shift) & 255u;
+            const uint32_t d = (kq[u] >> shift) & mask;
             pq[u] = gpos[d] + (q - toff[d]);
         }
     }
     // the last pass holds its records (keys, bounds and stores use them); an
     // earlier pass reads each one from LDS at its store
     float4 rq[LAST ? Q : 1];
-    uint32_t bq[Q];
+    uint32_t cq[LAST ? Q : 1];
     if (LAST && MOVE) {
 #pragma unroll
         for (int u = 0; u < Q; u++) rq[u] = srec[min(t + u * kSortThreads, kSortTile - 1)];
@@ -302,25 +331,27 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         for (int u = 0; u < Q; u++) rq[u] = a.rec[pq[u] < a.n ? vq[u] : 0u];
     }
     if (LAST) {
-#pragma unroll
-        for (int u = 0; u < Q; u++) bq[u] = bin_key(a.out.key, rq[u].z);
-        // run bounds, before the stores so that the atomics' round trips
-        // overlap them: the staged tile holds one run per digit (its items
-        // land at consecutive positions); inside a run, an item whose key (G
-        // row) differs from its predecessor's starts a key (row) run, one
-        // whose key differs from its successor's ends one.  The Binner keys
-        // are non-decreasing over the sorted order, so the smallest start and
-        // the largest end over all workgroups bound each segment.
-        uint32_t* rows = a.gk.on ? a.gk.rows.row_start : nullptr;
-        __syncthreads();   // every thread has read its staged keys and values
+        // Binner key of every held item.  A grouping key K = (G row + 2) +
+        // bin slot decodes with the G row of the travel (the same function
+        // of the same float as in the first pass); the travel sort evaluates
+        // the key.  cq is what neighbours compare: the bin slot (grouping)
+        // or the Binner key.
+        const bool grp = a.gk.on;
 #pragma unroll
         for (int u = 0; u < Q; u++) {
-            const uint32_t q = t + u * kSortThreads;
-            if (pq[u] < a.n) {
-                sval[q] = bq[u];
-                if (rows) skey[q] = (uint32_t)g_row_of(a.gk.rows, rq[u].z);
-            }
+            cq[u] = grp ? kq[u] - (uint32_t)(g_row_of(a.gk.rows, rq[u].z) + 2)
+                        : bin_key(a.out.key, rq[u].z);
+            if (pq[u] < a.n) scmp[t + u * kSortThreads] = cq[u];
         }
+        // run bounds, before the stores so that the atomics' round trips
+        // overlap them: the staged tile holds one run per digit (its items
+        // land at consecutive positions); inside a run, an item whose bin (G
+        // row) differs from its predecessor's starts a key (row) run, one
+        // whose bin differs from its successor's ends one.  The Binner keys
+        // are non-decreasing over the sorted order, so the smallest start and
+        // the largest end over all workgroups bound each segment.  The G row
+        // of a staged neighbour is its K minus (its slot + 2).
+        uint32_t* rows = grp ? a.gk.rows.row_start : nullptr;
         __syncthreads();
         uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
 #pragma unroll
@@ -328,16 +359,17 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             const uint32_t pos = pq[u];
             if (pos >= a.n) continue;
             const uint32_t q = t + u * kSortThreads;
-            const uint32_t d = (kq[u] >> shift) & 255u;
+            const uint32_t d = (kq[u] >> shift) & mask;
             const uint32_t r0 = toff[d], r1 = d == 255u ? tn : toff[d + 1];
-            const uint32_t k = bq[u];
+            const uint32_t k = binner_key(grp, cq[u], a.out.key.n);
             if (k != kOutOfRange) {
-                if (q == r0 || sval[q - 1] != k) atomicMin(&seg[2 * k], pos);
-                if (q + 1 == r1 || sval[q + 1] != k) atomicMax(&seg[2 * k + 1], pos + 1);
+                if (q == r0 || scmp[q - 1] != cq[u]) atomicMin(&seg[2 * k], pos);
+                if (q + 1 == r1 || scmp[q + 1] != cq[u]) atomicMax(&seg[2 * k + 1], pos + 1);
             }
             if (rows) {
-                const int g = (int)skey[q];
-                if ((q == r0 || (int)skey[q - 1] != g) && g >= -1 && g <= a.gk.rows.GH + 1)
+                const int g = (int)(kq[u] - cq[u]) - 2;
+                if ((q == r0 || (int)(skey[q - 1] - scmp[q - 1]) - 2 != g) && g >= -1 &&
+                    g <= a.gk.rows.GH + 1)
                     atomicMin(&rows[g + 1], pos);
             }
         }
@@ -346,16 +378,22 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     for (int u = 0; u < Q; u++) {
         const uint32_t pos = pq[u];
         if (pos < a.n) {   // holds for every staged item unless the records changed
-            if (LAST) a.order[pos] = vq[u];   // the sorted keys themselves are not kept
-            else a.kvout[pos] = make_uint2(kq[u], vq[u]);
+            if (!LAST) a.kvout[pos] = make_uint2(kq[u], vq[u]);
             if (MOVE && !LAST) a.rout[pos] = srec[t + u * kSortThreads];
             if (LAST) {
                 const float4 r = rq[u];
                 a.out.sorted[pos] = r;
-                a.out.skeys[pos] = bq[u];
-                const double x = a.out.axis == 1 ? sin((double)r.w) : (double)r.w;
-                const float xh = (float)x;
-                a.out.coord[pos] = make_float2(xh, (float)(x - (double)xh));
+                // the permutation and the Binner keys only when asked for
+                // (nothing downstream reads them)
+                if (a.order) a.order[pos] = vq[u];
+                if (a.out.skeys) a.out.skeys[pos] = binner_key(a.gk.on, cq[u], a.out.key.n);
+                const double x = a.out.axis == 1 ? sin_bearing((double)r.w) : (double)r.w;
+                if (a.out.xg) {
+                    a.out.xg[pos] = (float)((x - a.out.xg_b0) * (double)a.out.xg_inv) + a.out.xg_off;
+                } else {
+                    const float xh = (float)x;
+                    a.out.coord[pos] = make_float2(xh, (float)(x - (double)xh));
+                }
             }
         }
     }
@@ -383,23 +421,26 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
     GroupKey g{};
     if (gk && gk->on) g = *gk;
-    // 8-bit digits: 4 passes for travel images, ceil(bits(K) / 8) for a
-    // grouping key (its last pass then sees few digit values, so it writes
-    // long runs of consecutive positions)
-    int passes = 4;
+    // 8-bit digits: 4 passes for travel images; ceil(bits(K) / 8) passes for a
+    // grouping key, its bits split evenly over them (a tile's digit runs are
+    // then long in every pass: 2048 items over at most 2^dbits digits)
+    int passes = 4, dbits = 8;
     if (g.on) {
         int bits = 1;
         while (bits < 32 && (group_key_max(g) >> bits)) bits++;
         passes = (bits + 7) / 8;
+        dbits = kGroupEvenDigits ? (bits + passes - 1) / passes : 8;
     }
     p->passes = passes;
+    const uint32_t mask = (1u << dbits) - 1u;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
     for (int pass = 0; pass < 4; pass++) {
         HistArgs& h = p->hist[pass];
         h.rec = records;
         h.kvin = b.kv[cur];
         h.n = n;
-        h.shift = 8 * pass;
+        h.shift = dbits * pass;
+        h.mask = mask;
         h.gk = g;
         h.counts = b.counts;
         h.tiles = tiles;
@@ -408,14 +449,17 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         h.fill = pass == 0 ? b.fill : nullptr;
         h.fill_value = b.fill_value;
         h.fill_count = pass == 0 ? b.fill_count : 0;
-        p->scan[pass] = ScanArgs{b.counts, tiles, b.totals};
+        p->scan[pass] = ScanArgs{b.counts, tiles, b.totals, pass == 0 && g.on ? g.tmax : nullptr,
+                                 pass == 0 && g.on ? g.vmax : nullptr};
         ScatterArgs& c = p->scatter[pass];
         c.rec = records;
         c.kvin = b.kv[cur];
         c.kvout = b.kv[cur ^ 1];
-        c.order = b.order;
+        c.order = pass == passes - 1 ? b.order : nullptr;
         c.n = n;
-        c.shift = 8 * pass;
+        c.shift = dbits * pass;
+        c.mask = mask;
+        c.dbits = dbits;
         c.counts = b.counts;
         c.totals = b.totals;
         c.tiles = tiles;

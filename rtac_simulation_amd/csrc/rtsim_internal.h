@@ -55,6 +55,9 @@ struct RowMap {
     float range0, inv_dr, off_r;
     int fr, GH;
     uint32_t* row_start;  // GH + 3 entries, pre-filled with n; nullptr: no row table
+    // grid column coordinate of a bearing-axis value x: (x - bear0) inv_db + off_b
+    double bear0;
+    float inv_db, off_b;
 };
 
 // Sort key of the grid (MFMA) form.  K = (G row + 2) + bin slot, where the
@@ -64,17 +67,16 @@ struct RowMap {
 // travel order up to a permutation inside each run of equal (bin, G row):
 // keys, segments, windows and the G-row table come out exactly as after the
 // travel sort, in ceil(bits(K) / 8) passes instead of 4.
-// With a grouping key the last pass also raises vmax[0..kVmaxSlots) (FP32
-// bits, set to 0 by the first pass; each wave raises one slot, so that no
-// address takes more than a few dozen atomics) to the largest finite
-// |re|, |im| of the in-range samples: the deposit's fixed-point scale is the
-// maximum over the slots.
-constexpr int kVmaxSlots = 64;
+// With a grouping key the first histogram also writes tmax[tile], the largest
+// finite |re|, |im| of the tile's in-range samples (FP32 bits), and the first
+// scan reduces them into vmax[0]: the deposit's fixed-point scale (no
+// atomics: a few dozen on one address serialise at the end of a kernel).
 struct GroupKey {
     int on;               // 0: sort by travel image
     RowMap rows;
     KeyParams key;
-    uint32_t* vmax;       // kVmaxSlots words
+    uint32_t* vmax;       // 1 word
+    uint32_t* tmax;       // one word per tile
 };
 // largest K of a grouping (the sort runs as many 8-bit passes as it needs)
 inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }
@@ -86,8 +88,10 @@ inline size_t sort_count_words(size_t capacity)
 {
     return 256 * ((capacity + kSortTile - 1) / kSortTile);
 }
-// Written by the last pass for every sorted position i: the record, its
-// Binner key (Binner.cu:24-49) and its bearing-axis coordinate as (hi, lo).
+// Written by the last pass for every sorted position i: the record and its
+// bearing-axis coordinate, as (hi, lo) (coord) or, for the grid form, as a
+// grid column coordinate (xg); when skeys is set, also its Binner key
+// (Binner.cu:24-49).
 // The last pass also bounds the runs of equal keys (Binner.cu:51-80): each
 // workgroup lowers seg[k].x to the first and raises seg[k].y past the last
 // position of every key run it writes (atomic min / max; the runs of one key
@@ -96,15 +100,18 @@ inline size_t sort_count_words(size_t capacity)
 // writes (rows without samples keep the fill value n).
 struct SortOutputs {
     float4* sorted;
-    uint32_t* skeys;
-    float2* coord;
+    uint32_t* skeys;      // may be null (the keys are not needed downstream)
+    float2* coord;        // used when xg is null
+    float* xg;            // (x - xg_b0) xg_inv + xg_off of x = the bearing-axis value
+    double xg_b0;
+    float xg_inv, xg_off;
     KeyParams key;
     int axis;             // 1: coordinate = sin(bearing) in FP64, else the bearing
     uint2* seg;           // [key.n], initialised to (~0, 0) by the first pass
 };
 struct SortBuffers {
     uint2* kv[2];         // (key, sample index) ping-pong between passes
-    uint32_t* order;      // the permutation, written by the last pass
+    uint32_t* order;      // the permutation, written by the last pass (may be null)
     uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
     uint32_t* totals;     // [256] digit totals of the current pass
     uint32_t tiles_cap;
@@ -123,6 +130,7 @@ struct HistArgs {
     const uint2* kvin;        // later passes: (key, index) of the previous pass
     uint32_t n;
     int shift;
+    uint32_t mask;            // digit = (key >> shift) & mask
     GroupKey gk;              // pass 0 key: K when gk.on, else the travel image
     uint32_t* counts;         // [256 digits][tiles]
     uint32_t tiles;
@@ -136,14 +144,18 @@ struct ScanArgs {
     uint32_t* counts;
     uint32_t tiles;
     uint32_t* totals;
+    const uint32_t* tmax;     // first pass of a grouping sort: vmax[0] = max of tmax[0..tiles)
+    uint32_t* vmax;
 };
 struct ScatterArgs {
     const float4* rec;
     const uint2* kvin;        // (key, index) in the previous pass's order (not pass 0)
     uint2* kvout;             // not the last pass: (key, index) in this pass's order
-    uint32_t* order;          // last pass: the permutation
+    uint32_t* order;          // last pass: the permutation (null: not written)
     uint32_t n;
     int shift;
+    uint32_t mask;            // digit = (key >> shift) & mask, mask = 2^dbits - 1
+    int dbits;
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;

@@ -63,16 +63,18 @@ struct Sensor {
     // ---- samples -----------------------------------------------------------
     size_t count = 0, capacity = 0;
     bool computed = false;
+    int computed_form = -1;       // form of the last compute (batched or not)
     float4* d_samples = nullptr;  // receivedSamples_ (unsorted)
     float4* d_sorted = nullptr;   // receivedSamples_ after sort()
     float4* d_tmp = nullptr;      // records between sort passes (grid form)
-    float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
+    float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample;
+                                  // grid form: a float grid column coordinate
     uint2* d_kv[2] = {nullptr, nullptr};  // sort: (key, index) between passes
-    uint32_t* d_order = nullptr;  // sort permutation
-    uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples
-    const uint32_t* order_ptr = nullptr;  // sort permutation (after compute)
+    uint32_t* d_order = nullptr;  // sort permutation (written on request)
+    uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples (written on request)
     uint32_t* d_totals = nullptr; // sort: digit totals of a pass
     uint32_t* d_counts = nullptr; // sort: per-tile digit counts (capacity-sized)
+    uint32_t* d_tmax = nullptr;   // grouping sort: per-tile largest |re|, |im|
 
     // ---- per-configuration device tables -----------------------------------
     uint2* d_seg = nullptr;       // Binner::binIndexes_
@@ -108,7 +110,9 @@ struct Sensor {
     int grid_row_map(RowMap* m, uint32_t** vmax = nullptr);   // grid form row table, G-row map
     int compute(int form, hipStream_t s);
     int grid_plan(int form, DepositArgs* d, CorrArgs* c);
-    int plan(int form, PingPlan* p);   // validates the form; queues nothing
+    // validates the form; queues nothing.  introspect: the sort also writes
+    // the permutation and the Binner keys
+    int plan(int form, PingPlan* p, bool introspect = false);
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
     ConvArgs conv_args() const;
