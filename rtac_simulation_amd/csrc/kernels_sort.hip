@@ -34,8 +34,6 @@ constexpr int kSortWaves = kSortThreads / 64;                  // 4
 constexpr int kSortLaneItems = (int)kSortTile / kSortThreads;  // 16
 // grouping key: split its bits evenly over the passes (else 8-bit digits)
 constexpr bool kGroupEvenDigits = true;
-// the last pass of a sort whose records move loads them before the ranking
-constexpr bool kLastEarlyRecords = false;
 
 // Inclusive wave scan (wave64) of a 32-bit value.
 __device__ __forceinline__ uint32_t sort_wave_scan(uint32_t x, uint32_t lane)
@@ -210,8 +208,17 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     __shared__ uint32_t ws[4], ws2[4];
     __shared__ uint32_t skey[kSortTile];
     __shared__ uint32_t sval[kSortTile];
+    // Records that move: the first pass stages them in digit order from
+    // registers, a middle pass reads them at the staging step, and the last
+    // pass copies them into LDS in input order at its start (their loads
+    // overlap the key loads; held in registers across the ranking they would
+    // spill) and takes each staged item's record by its input slot.
+    constexpr bool RIN = MOVE && LAST && !FIRST;
     __shared__ float4 srec[MOVE ? kSortTile : 1];
-    __shared__ uint32_t scmp[LAST ? kSortTile : 1];   // last pass: bin slot / Binner key
+    // last pass: the input slot of each staged item (RIN), then the value its
+    // neighbours compare (bin slot or Binner key); each entry is read and
+    // rewritten by the thread that owns it
+    __shared__ uint32_t scmp[LAST ? kSortTile : 1];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     const uint32_t tb = tile * kSortTile;
     const int shift = a.shift;
@@ -236,12 +243,10 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         }
     } else {
         load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
-        // later passes read their records at the staging step instead (held
-        // across the ranking they spill)
-        if (MOVE && LAST && kLastEarlyRecords) {
+        if (RIN) {
 #pragma unroll
             for (int j = 0; j < kSortLaneItems; j++)
-                rv[j] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
+                srec[w * (64 * kSortLaneItems) + j * 64 + lane] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
         }
     }
     __syncthreads();
@@ -295,8 +300,8 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
             const uint32_t p = wcnt[w][d] + lrank[j];
             skey[p] = key[j];
             sval[p] = val[j];
-            if (MOVE)
-                srec[p] = (FIRST || (LAST && kLastEarlyRecords)) ? rv[j] : a.rin[wbase + j * 64 + lane];
+            if (RIN) scmp[p] = w * (64 * kSortLaneItems) + j * 64 + lane;
+            else if (MOVE) srec[p] = FIRST ? rv[j] : a.rin[wbase + j * 64 + lane];
         }
     }
     __syncthreads();
@@ -325,7 +330,10 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
     uint32_t cq[LAST ? Q : 1];
     if (LAST && MOVE) {
 #pragma unroll
-        for (int u = 0; u < Q; u++) rq[u] = srec[min(t + u * kSortThreads, kSortTile - 1)];
+        for (int u = 0; u < Q; u++) {
+            const uint32_t q = t + u * kSortThreads;
+            rq[u] = srec[RIN ? (pq[u] < a.n ? scmp[q] : 0u) : min(q, kSortTile - 1)];
+        }
     } else if (LAST) {
 #pragma unroll
         for (int u = 0; u < Q; u++) rq[u] = a.rec[pq[u] < a.n ? vq[u] : 0u];
