@@ -229,3 +229,60 @@ def test_mfma_grouping_sort_three_passes(gpu):
     win[rows] = b["windows"][rows]
     ref = O.conv(cfg, b["sorted"], win)
     assert O.rel_l2(s.output()[rows], ref[rows]) <= MFMA_TOL
+
+
+def _key_interval(k, inside):
+    """Ends of the float interval of travels with a Binner key, by bisection
+    over the ordered float32 images with the oracle's key function."""
+    def img(f):
+        u = int(np.float32(f).view(np.uint32))
+        return (~u & 0xFFFFFFFF) if u & 0x80000000 else (u | 0x80000000)
+
+    def flt(u):
+        b = (u & 0x7FFFFFFF) if u & 0x80000000 else (~u & 0xFFFFFFFF)
+        return float(np.uint32(b).view(np.float32))
+
+    def search(out, inn):
+        while abs(inn - out) > 1:
+            mid = (inn + out) // 2
+            if O.key_eval(k, flt(mid)) == 0xFFFFFFFF:
+                out = mid
+            else:
+                inn = mid
+        return flt(inn)
+    return search(img(-np.inf), img(inside)), search(img(np.inf), img(inside))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("form", [1, 2])
+def test_mfma_key_interval_edges(gpu, form):
+    """Samples exactly at both ends of the in-range travel interval are
+    deposited and their outside neighbours (one float away, 1000x louder,
+    inside the grid's PSF halo) are not: the MFMA ping matches the oracle."""
+    R = gpu
+    cfg = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "edges")
+    k = O.key_make(cfg.range_count, cfg.range_min, cfg.range_max, O.effective_margin(cfg))
+    lo, hi = _key_interval(k, 0.5 * (cfg.range_min + cfg.range_max))
+    assert O.key_eval(k, lo) == 0 and O.key_eval(k, hi) == cfg.range_count - 1
+    below = float(np.nextafter(np.float32(lo), np.float32(-np.inf)))
+    above = float(np.nextafter(np.float32(hi), np.float32(np.inf)))
+    assert O.key_eval(k, below) == 0xFFFFFFFF and O.key_eval(k, above) == 0xFFFFFFFF
+    step = (cfg.range_max - cfg.range_min) / (cfg.range_count - 1)
+    halo = (cfg.Tr - 1) // 2 * step                      # grid rows beyond both range ends
+    assert 0 < cfg.range_min - lo < halo and 0 < hi - cfg.range_max < halo
+    rng = np.random.default_rng(5)
+    n = 4000
+    rec = np.zeros((n + 4, 4), np.float32)
+    rec[:n, 0] = rng.normal(size=n)
+    rec[:n, 1] = rng.normal(size=n)
+    rec[:n, 2] = rng.uniform(cfg.range_min, cfg.range_max, n)
+    rec[:n, 3] = rng.uniform(-1.0, 1.0, n)
+    rec[n:, 2] = [lo, hi, below, above]
+    rec[n:, 0] = [1.0, 1.0, 1000.0, 1000.0]
+    rec[n:, 3] = [0.1, -0.2, 0.1, -0.2]
+    s = R.make_sensor(cfg)
+    s.set_sample_count(rec.shape[0])
+    s.upload_samples(rec)
+    s.compute_output(form)
+    ref, _ = O.reference_ping(cfg, rec)
+    assert O.rel_l2(s.output(), ref) <= MFMA_TOL
