@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <utility>
 #include <vector>
 
@@ -72,7 +73,7 @@ struct Batch {
     Stage<BinArgs> windows;                // plain pings (the grid form's are in its deposit)
     Stage<ConvArgs> plain2d, plain1d;
     Stage<DepositArgs> deposit;
-    std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape)
+    std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape, waves per workgroup)
 
     ~Batch()
     {
@@ -129,7 +130,7 @@ static int build(Batch* b)
     b->plain1d = {};
     b->deposit = {};
     b->corr.clear();
-    std::map<std::pair<int, int>, size_t> corr_index;
+    std::map<std::tuple<int, int, int>, size_t> corr_index;
     for (auto& en : b->entries) {
         Sensor* s = en.sensor;
         PingPlan p;
@@ -156,7 +157,7 @@ static int build(Batch* b)
         } else {
             b->deposit.add(p.dep, deposit_blocks(p.dep));
             b->deposit.shm = std::max(b->deposit.shm, deposit_lds_bytes(p.dep));
-            const std::pair<int, int> key{en.form, p.corr.shape};
+            const std::tuple<int, int, int> key{en.form, p.corr.shape, p.corr.waves8};
             auto it = corr_index.find(key);
             if (it == corr_index.end()) {
                 it = corr_index.emplace(key, b->corr.size()).first;
@@ -235,7 +236,8 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
                                 b->deposit.shm, s));
     for (const auto& c : b->corr)
         RT_TRY(launch_correlate_batch(c.variant == RTSIM_FORM_MFMA_FP16, c.args.empty() ? 0 : c.args[0].shape,
-                                      dargs(b, c), dmap(b, c), c.blocks(), c.shm, s));
+                                      !c.args.empty() && c.args[0].waves8, dargs(b, c), dmap(b, c),
+                                      c.blocks(), c.shm, s));
 #undef RT_TRY
     return hipSuccess;
 }

@@ -88,6 +88,7 @@ struct GridState {
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
     double bear0 = 0;
     float tap_scale = 1.f;
+    int max_lds = 0;                     // device: LDS per workgroup
     float2* d_grid = nullptr;
     uint32_t* d_row_start = nullptr;
     uint32_t* d_vmax = nullptr;          // deposit scale (set by the sort)
@@ -170,6 +171,7 @@ static int prepare_grid(Sensor* s, GridState* g)
             hipSuccess || max_lds <= 0)
         max_lds = 64 * 1024;
     max_lds -= 64;   // the kernel's static LDS
+    g->max_lds = max_lds;
     // tile shapes whose LDS fits; default: the first that does (32x32)
     g->shape = -1;
     for (int sh = 0; sh < kCorrShapes; sh++) {
@@ -376,6 +378,7 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     c.shape = g->shape;
     c.RS = corr_row_stride(g->shape, g->KR);
     c.tap_scale = g->tap_scale;
+    c.waves8 = corr_waves8(c, g->max_lds);
     return RTSIM_OK;
 }
 
@@ -400,6 +403,7 @@ int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, f
         CorrArgs c = last_corr;
         c.shape = sh;
         c.RS = corr_row_stride(sh, g->KR);
+        c.waves8 = corr_waves8(c, g->max_lds);
         hipError_t err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);   // warm
         (void)hipEventRecord(e0, s);
         for (int it = 0; it < iterations && err == hipSuccess; it++)

@@ -49,6 +49,7 @@ struct CorrArgs {
     int shape;                 // tile shape index (corr_tile)
     int RS;                    // LDS row stride of the staged grid columns (corr_row_stride)
     float tap_scale;           // power of two applied to the taps
+    int waves8;                // 1: 8 waves per workgroup (corr_waves8)
 };
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s);
@@ -60,9 +61,12 @@ size_t deposit_lds_bytes(const DepositArgs& a);
 hipError_t launch_deposit_batch(const DepositArgs* args, BatchMap m, uint32_t blocks, size_t shm,
                                 hipStream_t s);
 uint32_t correlate_blocks(const CorrArgs& a);
-hipError_t launch_correlate_batch(bool fp16, int shape, const CorrArgs* args, BatchMap m,
+hipError_t launch_correlate_batch(bool fp16, int shape, bool waves8, const CorrArgs* args, BatchMap m,
                                   uint32_t blocks, size_t shm, hipStream_t s);
 size_t correlate_lds_bytes(const CorrArgs& a);
+// 1 when the shape's LDS leaves room for one workgroup per CU only: the
+// workgroup then has 8 waves (two per SIMD), else 4
+int corr_waves8(const CorrArgs& a, int max_lds);
 void init_corr_attributes(int max_lds);
 
 }  // namespace rtsim

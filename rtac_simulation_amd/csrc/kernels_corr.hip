@@ -22,9 +22,9 @@
 // written as zero.
 //
 // Tile shapes (template TR x TC output tiles of 32 x 32 per workgroup, JS =
-// 4 / (TR TC) waves per tile, each taking an equal share of the tile's Tb KR
-// (tap, K block) steps; partial sums are added through LDS): chosen per sensor
-// by timing (Sensor::autotune).
+// NW / (TR TC) waves per tile for NW = 4 or 8 waves per workgroup, each taking
+// an equal share of the tile's Tb KR (tap, K block) steps; partial sums are
+// added through LDS): chosen per sensor by timing (Sensor::autotune).
 // LDS (16-bit elements):
 //   G tile, transposed: [4 planes re_hi, re_lo, im_hi, im_lo][column][row],
 //     row stride RS = (rows rounded to 16) + 8, so that the 16-byte B fragment
@@ -134,21 +134,30 @@ __host__ __device__ inline int corr_copy_stride(int Tb, int L)
     return 2 * dw;
 }
 
-template <int TR, int TC>
+// NW waves per workgroup: 4, or 8 (W8) when the shape's LDS leaves room for
+// one workgroup per CU only: two waves per SIMD then hide each other's LDS
+// latency in the MFMA loop (one wave alone ran it at about half the MFMA rate)
+template <int TR, int TC, bool W8>
 struct CorrGeom {
-    static constexpr int JS = 4 / (TR * TC);   // waves per output tile
+    static constexpr int NW = W8 ? 8 : 4;
+    static constexpr int NT = 64 * NW;               // threads
+    static constexpr int JS = NW / (TR * TC);        // waves per output tile
 };
 
-template <bool FP16, int TR, int TC>
-__device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by)
+// Output tile tt at (tt % gx, tt / gx) in units of 32 TC bearings x 32 TR rows.
+template <bool FP16, int TR, int TC, bool W8>
+__device__ __forceinline__ void correlate_body(const CorrArgs& a, int tt, int gx)
 {
-    constexpr int JS = CorrGeom<TR, TC>::JS;
+    constexpr int JS = CorrGeom<TR, TC, W8>::JS;
+    constexpr int NW = CorrGeom<TR, TC, W8>::NW;
+    constexpr int NT = CorrGeom<TR, TC, W8>::NT;
     extern __shared__ __align__(16) uint16_t lds[];
-    __shared__ float red[4];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    __shared__ float red[NW];
+    // the wave index in a scalar register: everything derived from it (tile,
+    // tap subset, step range, LDS offsets) stays scalar
+    const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
     const int tile = wave / JS, js = wave % JS;   // this wave's output tile and tap subset
     const int tr = tile / TC, tc = tile % TC;
-    const int r0 = by * 32 * TR, c0 = bx * 32 * TC;
     const int KR = a.KR, RS = a.RS, L = a.L;
     const int SR = 32 * (TR - 1) + 16 * KR;      // staged grid rows [r0, r0 + SR)
     const int CW = corr_cols(TC, a.Tb);          // staged grid columns [c0, c0 + CW)
@@ -157,27 +166,15 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by
     uint16_t* Tt = lds + 4 * gps;                // tap table [2][4][Tb][L]
     const int cs = corr_copy_stride(a.Tb, L);   // elements between the two shifted copies
     const int tsize = 2 * cs;
-
-    // window sizes of this lane's output rows (D rows 8 (q / 4) + 4 hh + q % 4),
-    // for the epilogue mask
     const int hh = lane >> 5, col = lane & 31;
-    uint32_t wsz[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
-        wsz[q] = a.win[min(rr, a.H - 1)].y;
-    }
 
-    // ---- staging loads: 2 x 2 cells (rows r, r + 1, columns c, c + 1) per item,
-    // all in flight ---------------------------------------------------------------
+    // staging: 2 x 2 cells (rows r, r + 1, columns c, c + 1) per item, all in flight
     const int CQ = CW / 2;                       // column pairs
     const int nquads = (SR / 2) * CQ;
     const float inv_cq = 1.0f / (float)CQ;
-    const float4* gtile = reinterpret_cast<const float4*>(a.grid + (size_t)r0 * a.GW + c0);
     const int grow4 = a.GW / 2;                  // float4 per grid row
-    const bool in_regs = nquads <= kRegQuads * 256;
+    const bool in_regs = nquads <= kRegQuads * NT;
     float4 v0[kRegQuads], v1[kRegQuads];
-    float mx = 0.f;
     auto quad_of = [&](int k, int& rp, int& cq) {
         rp = small_div(k, inv_cq);
         cq = k - rp * CQ;
@@ -186,90 +183,25 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by
         return fmaxf(fmaxf(fmaxf(fabsf(g0.x), fabsf(g0.y)), fmaxf(fabsf(g0.z), fabsf(g0.w))),
                      fmaxf(fmaxf(fabsf(g1.x), fabsf(g1.y)), fmaxf(fabsf(g1.z), fabsf(g1.w))));
     };
-    if (in_regs) {
+    const int r0 = (tt / gx) * 32 * TR, c0 = (tt % gx) * 32 * TC;
+    const float4* gtile = reinterpret_cast<const float4*>(a.grid + (size_t)r0 * a.GW + c0);
+    auto load_regs = [&]() {
 #pragma unroll
         for (int u = 0; u < kRegQuads; u++) {
-            const int k = min(t + 256 * u, nquads - 1);
+            const int k = min(t + NT * u, nquads - 1);
             int rp, cq;
             quad_of(k, rp, cq);
             v0[u] = gtile[(size_t)(2 * rp) * grow4 + cq];
             v1[u] = gtile[(size_t)(2 * rp + 1) * grow4 + cq];
         }
-    }
+    };
+    if (in_regs) load_regs();   // before the table copy
     {   // tap table: a contiguous copy, uint4 at a time
         const uint4* src = reinterpret_cast<const uint4*>(a.toep);
         uint4* dst = reinterpret_cast<uint4*>(Tt);
-        for (int k = t; k < tsize / 8; k += 256) dst[k] = src[k];
+        for (int k = t; k < tsize / 8; k += NT) dst[k] = src[k];
     }
-    if (in_regs) {
-#pragma unroll
-        for (int u = 0; u < kRegQuads; u++)
-            if (t + 256 * u < nquads) mx = fmaxf(mx, quad_max(v0[u], v1[u]));
-    } else {
-        for (int k = t; k < nquads; k += 256) {
-            int rp, cq;
-            quad_of(k, rp, cq);
-            mx = fmaxf(mx, quad_max(gtile[(size_t)(2 * rp) * grow4 + cq],
-                                    gtile[(size_t)(2 * rp + 1) * grow4 + cq]));
-        }
-    }
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
-    if (lane == 0) red[wave] = mx;
-    __syncthreads();
-    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-    if (!(mx > 0.f)) {
-        for (int k = t; k < 32 * TR * 32 * TC; k += 256) {
-            const int rr = r0 + k / (32 * TC), cc = c0 + k % (32 * TC);
-            if (rr < a.H && cc < a.W) a.out[(size_t)rr * a.W + cc] = make_float2(0.f, 0.f);
-        }
-        return;
-    }
-    int e;
-    frexpf(mx, &e);                                       // mx in [2^(e-1), 2^e)
-    const float scale = ldexpf(1.0f, min(14 - e, 126));   // mx * scale in [2^13, 2^14)
 
-    // ---- G planes: scaled, hi/lo split, rows (2 rp, 2 rp + 1) in one word ------
-    {
-        const int wps = gps / 2;   // plane stride in 32-bit words
-        // column c of rows (2 rp, 2 rp + 1) holds cells g0, g1
-        auto put_col = [&](uint32_t* base, float2 g0, float2 g1) {
-            uint16_t h0, l0, h1, l1, h2, l2, h3, l3;
-            split_hi_lo<FP16>(g0.x * scale, h0, l0);   // re, row 2 rp
-            split_hi_lo<FP16>(g1.x * scale, h1, l1);   // re, row 2 rp + 1
-            split_hi_lo<FP16>(g0.y * scale, h2, l2);   // im, row 2 rp
-            split_hi_lo<FP16>(g1.y * scale, h3, l3);   // im, row 2 rp + 1
-            base[0] = (uint32_t)h0 | ((uint32_t)h1 << 16);
-            base[wps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
-            base[2 * wps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
-            base[3 * wps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
-        };
-        auto put = [&](int k, const float4& g0, const float4& g1) {
-            int rp, cq;
-            quad_of(k, rp, cq);
-            uint32_t* base = reinterpret_cast<uint32_t*>(Gt + 2 * cq * RS) + rp;
-            put_col(base, make_float2(g0.x, g0.y), make_float2(g1.x, g1.y));
-            put_col(base + RS / 2, make_float2(g0.z, g0.w), make_float2(g1.z, g1.w));
-        };
-        if (in_regs) {
-#pragma unroll
-            for (int u = 0; u < kRegQuads; u++)
-                if (t + 256 * u < nquads) put(t + 256 * u, v0[u], v1[u]);
-        } else {
-            for (int k = t; k < nquads; k += 256) {
-                int rp, cq;
-                quad_of(k, rp, cq);
-                put(k, gtile[(size_t)(2 * rp) * grow4 + cq], gtile[(size_t)(2 * rp + 1) * grow4 + cq]);
-            }
-        }
-    }
-    __syncthreads();
-
-    // ---- MFMA loop over this wave's taps j = js, js + JS, ... and K blocks ------
-    f32x16 acc[kChains];
-#pragma unroll
-    for (int c = 0; c < kChains; c++)
-#pragma unroll
-        for (int q = 0; q < 16; q++) acc[c][q] = 0.f;
     const int m = lane & 31, h = lane >> 5;
     // A: lane (m, h) reads Tpad[16 kb + 8 h - m + 31 + p] of tap column j
     const int s0 = 8 * h - m + 31;
@@ -281,88 +213,186 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int bx, int by
     // the tile's Tb KR steps (tap j, K block kb), split evenly over its JS waves
     const int all = a.Tb * KR;
     const int sbeg = js * all / JS, nsteps = (js + 1) * all / JS - sbeg;
-    const float inv_kr = 1.0f / (float)KR;
-    auto load = [&](int step, u32x4 (&A)[4], u32x4 (&B)[4]) {
-        const int st = sbeg + step;
-        const int j = small_div(st, inv_kr), kb = st - j * KR;
-        const uint16_t* ap = a_lane + j * L + 16 * kb;
-        const uint16_t* bp = b_lane + j * RS + 16 * kb;
+
+    {
+        float mx = 0.f;
+        if (in_regs) {
 #pragma unroll
-        for (int p = 0; p < 4; p++) {
-            const uint32_t* src = reinterpret_cast<const uint32_t*>(ap + p * aps);
-            A[p] = (u32x4){src[0], src[1], src[2], src[3]};
-            B[p] = *reinterpret_cast<const u32x4*>(bp + p * gps);
-        }
-    };
-    if (nsteps > 0) {
-        u32x4 A0[4], B0[4], A1[4], B1[4];
-        load(0, A0, B0);
-        for (int step = 0; step < nsteps; step += 2) {
-            load(min(step + 1, nsteps - 1), A1, B1);
-            mma_step<FP16>(A0, B0, acc);
-            if (step + 1 < nsteps) {
-                load(min(step + 2, nsteps - 1), A0, B0);
-                mma_step<FP16>(A1, B1, acc);
+            for (int u = 0; u < kRegQuads; u++)
+                if (t + NT * u < nquads) mx = fmaxf(mx, quad_max(v0[u], v1[u]));
+        } else {
+            for (int k = t; k < nquads; k += NT) {
+                int rp, cq;
+                quad_of(k, rp, cq);
+                mx = fmaxf(mx, quad_max(gtile[(size_t)(2 * rp) * grow4 + cq],
+                                        gtile[(size_t)(2 * rp + 1) * grow4 + cq]));
             }
         }
-    }
-    f32x16 re = acc[0] + acc[1] + acc[2];
-    f32x16 im = acc[3] + acc[4] + acc[5];
-
-    // ---- partial sums of the JS waves of a tile, through LDS ------------------
-    if (JS > 1) {
-        __syncthreads();                                  // every wave done with Gt / Tt
-        float* part = reinterpret_cast<float*>(lds);      // [wave][32 values][64 lanes]
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o, 64));
+        if (lane == 0) red[wave] = mx;
+        __syncthreads();   // also: the tap table is in LDS
+        mx = red[0];
 #pragma unroll
-        for (int q = 0; q < 16; q++) {
-            part[(wave * 32 + q) * 64 + lane] = re[q];
-            part[(wave * 32 + 16 + q) * 64 + lane] = im[q];
+        for (int w2 = 1; w2 < NW; w2++) mx = fmaxf(mx, red[w2]);
+        const bool zero = !(mx > 0.f);
+        int e;
+        frexpf(zero ? 1.f : mx, &e);                          // mx in [2^(e-1), 2^e)
+        const float scale = ldexpf(1.0f, min(14 - e, 126));   // mx * scale in [2^13, 2^14)
+
+        // ---- G planes: scaled, hi/lo split, rows (2 rp, 2 rp + 1) in one word ------
+        if (!zero) {
+            const int wps = gps / 2;   // plane stride in 32-bit words
+            // column c of rows (2 rp, 2 rp + 1) holds cells g0, g1
+            auto put_col = [&](uint32_t* base, float2 g0, float2 g1) {
+                uint16_t h0, l0, h1, l1, h2, l2, h3, l3;
+                split_hi_lo<FP16>(g0.x * scale, h0, l0);   // re, row 2 rp
+                split_hi_lo<FP16>(g1.x * scale, h1, l1);   // re, row 2 rp + 1
+                split_hi_lo<FP16>(g0.y * scale, h2, l2);   // im, row 2 rp
+                split_hi_lo<FP16>(g1.y * scale, h3, l3);   // im, row 2 rp + 1
+                base[0] = (uint32_t)h0 | ((uint32_t)h1 << 16);
+                base[wps] = (uint32_t)l0 | ((uint32_t)l1 << 16);
+                base[2 * wps] = (uint32_t)h2 | ((uint32_t)h3 << 16);
+                base[3 * wps] = (uint32_t)l2 | ((uint32_t)l3 << 16);
+            };
+            auto put = [&](int k, const float4& g0, const float4& g1) {
+                int rp, cq;
+                quad_of(k, rp, cq);
+                uint32_t* base = reinterpret_cast<uint32_t*>(Gt + 2 * cq * RS) + rp;
+                put_col(base, make_float2(g0.x, g0.y), make_float2(g1.x, g1.y));
+                put_col(base + RS / 2, make_float2(g0.z, g0.w), make_float2(g1.z, g1.w));
+            };
+            if (in_regs) {
+#pragma unroll
+                for (int u = 0; u < kRegQuads; u++)
+                    if (t + NT * u < nquads) put(t + NT * u, v0[u], v1[u]);
+            } else {
+                for (int k = t; k < nquads; k += NT) {
+                    int rp, cq;
+                    quad_of(k, rp, cq);
+                    put(k, gtile[(size_t)(2 * rp) * grow4 + cq],
+                        gtile[(size_t)(2 * rp + 1) * grow4 + cq]);
+                }
+            }
+        }
+        if (zero) {
+            for (int k = t; k < 32 * TR * 32 * TC; k += NT) {
+                const int rr = r0 + k / (32 * TC), cc = c0 + k % (32 * TC);
+                if (rr < a.H && cc < a.W) a.out[(size_t)rr * a.W + cc] = make_float2(0.f, 0.f);
+            }
+            return;
         }
         __syncthreads();
-        // wave js of the tile sums elements [16 js / JS, 16 (js + 1) / JS)
+
+        // ---- MFMA loop over this wave's taps j = js, js + JS, ... and K blocks ------
+        f32x16 acc[kChains];
+#pragma unroll
+        for (int c = 0; c < kChains; c++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[c][q] = 0.f;
+        auto load = [&](int step, u32x4 (&A)[4], u32x4 (&B)[4]) {
+            const int st = sbeg + step;
+            const int j = st / KR, kb = st - j * KR;      // scalar: st is wave-uniform
+            const uint16_t* ap = a_lane + (j * L + 16 * kb);
+            const uint16_t* bp = b_lane + (j * RS + 16 * kb);
+#pragma unroll
+            for (int p = 0; p < 4; p++) {
+                const uint32_t* src = reinterpret_cast<const uint32_t*>(ap + p * aps);
+                A[p] = (u32x4){src[0], src[1], src[2], src[3]};
+                B[p] = *reinterpret_cast<const u32x4*>(bp + p * gps);
+            }
+        };
+        // the next step's LDS reads are interleaved with this step's MFMAs, so
+        // that they complete under them (with one wave per SIMD nothing else
+        // would hide their latency)
+        auto interleave = []() {
+#pragma unroll
+            for (int k = 0; k < 12; k++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);   // one MFMA
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);   // one LDS read
+            }
+        };
+        if (nsteps > 0) {
+            u32x4 A0[4], B0[4], A1[4], B1[4];
+            load(0, A0, B0);
+            for (int step = 0; step < nsteps; step += 2) {
+                load(min(step + 1, nsteps - 1), A1, B1);
+                mma_step<FP16>(A0, B0, acc);
+                interleave();
+                if (step + 1 < nsteps) {
+                    load(min(step + 2, nsteps - 1), A0, B0);
+                    mma_step<FP16>(A1, B1, acc);
+                    interleave();
+                }
+            }
+        }
+        f32x16 re = acc[0] + acc[1] + acc[2];
+        f32x16 im = acc[3] + acc[4] + acc[5];
+        // window sizes of this lane's output rows (D rows 8 (q / 4) + 4 hh + q % 4),
+        // for the epilogue mask (loaded here: held across the MFMA loop, their
+        // registers would be missed there)
+        uint32_t wsz[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) {
-            if (q * JS / 16 != js) continue;
-            float sr = 0.f, si = 0.f;
+            const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
+            wsz[q] = a.win[min(rr, a.H - 1)].y;
+        }
+
+        // ---- partial sums of the JS waves of a tile, through LDS (over Gt) ------
+        if (JS > 1) {
+            __syncthreads();                                  // every wave done with Gt
+            float* part = reinterpret_cast<float*>(lds);      // [wave][32 values][64 lanes]
 #pragma unroll
-            for (int w2 = 0; w2 < JS; w2++) {
-                sr += part[((tile * JS + w2) * 32 + q) * 64 + lane];
-                si += part[((tile * JS + w2) * 32 + 16 + q) * 64 + lane];
+            for (int q = 0; q < 16; q++) {
+                part[(wave * 32 + q) * 64 + lane] = re[q];
+                part[(wave * 32 + 16 + q) * 64 + lane] = im[q];
             }
-            re[q] = sr;
-            im[q] = si;
-        }
-    }
-
-    // ---- epilogue: undo scales, empty-window mask, store ---------------------
-    const float inv = 1.0f / (scale * a.tap_scale);
-    const int cc = c0 + 32 * tc + col;
+            __syncthreads();
+            // wave js of the tile sums elements [16 js / JS, 16 (js + 1) / JS)
 #pragma unroll
-    for (int q = 0; q < 16; q++) {
-        if (JS > 1 && q * JS / 16 != js) continue;
-        const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
-        if (rr < a.H && cc < a.W) {
-            float2 v2 = make_float2(re[q] * inv, im[q] * inv);
-            if (wsz[q] == 0) v2 = make_float2(0.f, 0.f);
-            a.out[(size_t)rr * a.W + cc] = v2;
+            for (int q = 0; q < 16; q++) {
+                if (q * JS / 16 != js) continue;
+                float sr = 0.f, si = 0.f;
+#pragma unroll
+                for (int w2 = 0; w2 < JS; w2++) {
+                    sr += part[((tile * JS + w2) * 32 + q) * 64 + lane];
+                    si += part[((tile * JS + w2) * 32 + 16 + q) * 64 + lane];
+                }
+                re[q] = sr;
+                im[q] = si;
+            }
+        }
+
+        // ---- epilogue: undo scales, empty-window mask, store ---------------------
+        const float inv = 1.0f / (scale * a.tap_scale);
+        const int cc = c0 + 32 * tc + col;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            if (JS > 1 && q * JS / 16 != js) continue;
+            const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
+            if (rr < a.H && cc < a.W) {
+                float2 v2 = make_float2(re[q] * inv, im[q] * inv);
+                if (wsz[q] == 0) v2 = make_float2(0.f, 0.f);
+                a.out[(size_t)rr * a.W + cc] = v2;
+            }
         }
     }
 }
 
-template <bool FP16, int TR, int TC>
-__global__ __launch_bounds__(256) void k_correlate(CorrArgs a)
+template <bool FP16, int TR, int TC, bool W8>
+__global__ __launch_bounds__(W8 ? 512 : 256) void k_correlate(CorrArgs a)
 {
-    correlate_body<FP16, TR, TC>(a, blockIdx.x, blockIdx.y);
+    correlate_body<FP16, TR, TC, W8>(a, (int)blockIdx.x, (a.W + 32 * TC - 1) / (32 * TC));
 }
 
-template <bool FP16, int TR, int TC>
-__global__ __launch_bounds__(256) void k_correlate_batch(const CorrArgs* __restrict__ args, BatchMap m)
+template <bool FP16, int TR, int TC, bool W8>
+__global__ __launch_bounds__(W8 ? 512 : 256) void k_correlate_batch(const CorrArgs* __restrict__ args,
+                                                                  BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const CorrArgs a = args[p];   // a private copy: the body's stores cannot alias it
     const int gx = (a.W + 32 * TC - 1) / (32 * TC);
-    correlate_body<FP16, TR, TC>(a, (int)blk % gx, (int)blk / gx);
+    correlate_body<FP16, TR, TC, W8>(a, (int)blk, gx);
 }
 
 // tile shapes {TR, TC}: rows = 32 TR, cols = 32 TC; 4 / (TR TC) waves per tile
@@ -376,6 +406,13 @@ void corr_tile(int shape, int* rows, int* cols)
 
 int corr_copy_stride_host(int Tb, int L) { return corr_copy_stride(Tb, L); }
 
+int corr_waves8(const CorrArgs& a, int max_lds)
+{
+    CorrArgs b = a;
+    b.waves8 = 0;
+    return 2 * correlate_lds_bytes(b) > (size_t)max_lds ? 1 : 0;
+}
+
 int corr_row_stride(int shape, int KR)
 {
     const int rows = 32 * (kShapes[shape][0] - 1) + 16 * KR;   // a multiple of 16
@@ -388,30 +425,32 @@ size_t correlate_lds_bytes(const CorrArgs& a)
     corr_tile(a.shape, &rows, &cols);
     const size_t gt = (size_t)4 * corr_cols(cols / 32, a.Tb) * a.RS;
     const size_t tt = (size_t)2 * corr_copy_stride(a.Tb, a.L);
-    const size_t part = (size_t)4 * 32 * 64 * 2;   // partial sums (floats, as 16-bit units)
+    const int nw = a.waves8 ? 8 : 4;
+    const size_t part = (size_t)nw * 32 * 64 * 2;   // partial sums (floats, as 16-bit units)
     return sizeof(uint16_t) * std::max(gt + tt, part);
 }
 
-template <bool FP16>
+template <bool FP16, bool W8>
 static void launch_shape(const CorrArgs& a, dim3 grid, size_t shm, hipStream_t s)
 {
     switch (a.shape) {
-    case 0: hipLaunchKernelGGL((k_correlate<FP16, 1, 1>), grid, dim3(256), shm, s, a); break;
-    case 1: hipLaunchKernelGGL((k_correlate<FP16, 1, 2>), grid, dim3(256), shm, s, a); break;
-    case 2: hipLaunchKernelGGL((k_correlate<FP16, 2, 1>), grid, dim3(256), shm, s, a); break;
-    default: hipLaunchKernelGGL((k_correlate<FP16, 2, 2>), grid, dim3(256), shm, s, a); break;
+    case 0: hipLaunchKernelGGL((k_correlate<FP16, 1, 1, W8>), grid, dim3(W8 ? 512 : 256), shm, s, a); break;
+    case 1: hipLaunchKernelGGL((k_correlate<FP16, 1, 2, W8>), grid, dim3(W8 ? 512 : 256), shm, s, a); break;
+    case 2: hipLaunchKernelGGL((k_correlate<FP16, 2, 1, W8>), grid, dim3(W8 ? 512 : 256), shm, s, a); break;
+    default: hipLaunchKernelGGL((k_correlate<FP16, 2, 2, W8>), grid, dim3(W8 ? 512 : 256), shm, s, a); break;
     }
 }
 
-template <bool FP16>
+template <bool FP16, bool W8>
 static void launch_shape_batch(int shape, const CorrArgs* args, BatchMap m, dim3 grid, size_t shm,
                                hipStream_t s)
 {
+    const dim3 block(W8 ? 512 : 256);
     switch (shape) {
-    case 0: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 1>), grid, dim3(256), shm, s, args, m); break;
-    case 1: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 2>), grid, dim3(256), shm, s, args, m); break;
-    case 2: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 1>), grid, dim3(256), shm, s, args, m); break;
-    default: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 2>), grid, dim3(256), shm, s, args, m); break;
+    case 0: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 1, W8>), grid, block, shm, s, args, m); break;
+    case 1: hipLaunchKernelGGL((k_correlate_batch<FP16, 1, 2, W8>), grid, block, shm, s, args, m); break;
+    case 2: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 1, W8>), grid, block, shm, s, args, m); break;
+    default: hipLaunchKernelGGL((k_correlate_batch<FP16, 2, 2, W8>), grid, block, shm, s, args, m); break;
     }
 }
 
@@ -422,44 +461,69 @@ uint32_t correlate_blocks(const CorrArgs& a)
     return (uint32_t)(((a.W + cols - 1) / cols) * ((a.H + rows - 1) / rows));
 }
 
-hipError_t launch_correlate_batch(bool fp16, int shape, const CorrArgs* args, BatchMap m,
+hipError_t launch_correlate_batch(bool fp16, int shape, bool waves8, const CorrArgs* args, BatchMap m,
                                   uint32_t blocks, size_t shm, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    if (fp16) launch_shape_batch<true>(shape, args, m, dim3(blocks), shm, s);
-    else launch_shape_batch<false>(shape, args, m, dim3(blocks), shm, s);
+    const dim3 g(blocks);
+    if (fp16) {
+        if (waves8) launch_shape_batch<true, true>(shape, args, m, g, shm, s);
+        else launch_shape_batch<true, false>(shape, args, m, g, shm, s);
+    } else {
+        if (waves8) launch_shape_batch<false, true>(shape, args, m, g, shm, s);
+        else launch_shape_batch<false, false>(shape, args, m, g, shm, s);
+    }
     return hipGetLastError();
 }
 
 hipError_t launch_correlate(const CorrArgs& a, bool fp16, hipStream_t s)
 {
-    int rows, cols;
-    corr_tile(a.shape, &rows, &cols);
-    dim3 grid((a.W + cols - 1) / cols, (a.H + rows - 1) / rows);
+    const dim3 grid(correlate_blocks(a));
     const size_t shm = correlate_lds_bytes(a);
-    if (fp16) launch_shape<true>(a, grid, shm, s);
-    else launch_shape<false>(a, grid, shm, s);
+    if (fp16) {
+        if (a.waves8) launch_shape<true, true>(a, grid, shm, s);
+        else launch_shape<true, false>(a, grid, shm, s);
+    } else {
+        if (a.waves8) launch_shape<false, true>(a, grid, shm, s);
+        else launch_shape<false, false>(a, grid, shm, s);
+    }
     return hipGetLastError();
 }
 
 void init_corr_attributes(int max_lds)
 {
-    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 2>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 1>, max_lds);
-    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 1, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<true, 2, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 1, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate<false, 2, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 1, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<true, 2, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 1, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 2, false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 1, 2, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 1, true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_correlate_batch<false, 2, 2, true>, max_lds);
 }
 
 }  // namespace rtsim

@@ -286,3 +286,32 @@ def test_mfma_key_interval_edges(gpu, form):
     s.compute_output(form)
     ref, _ = O.reference_ping(cfg, rec)
     assert O.rel_l2(s.output(), ref) <= MFMA_TOL
+
+
+@pytest.mark.gpu
+def test_mfma_persistent_correlation_large_psf(gpu):
+    """A 63 x 33 PSF needs so much LDS per workgroup that one fits per CU:
+    the correlation then runs persistent workgroups that walk several output
+    tiles each (512 tiles of 32 x 32 at the smallest shape), prefetching the
+    next tile's grid cells.  Every shape matches the oracle on sampled rows."""
+    R = gpu
+    cfg = configs._fls(512, 1024, 0.1, 120.0, 750e3, configs.psf_63x33(), "wide_psf")
+    _, scene, _, _ = configs.fls_1024x4096()
+    hits = R.generate_hits(scene, 7, 400_000)
+    s = _sensor(R, cfg, hits)
+    rec = s.samples()
+    b = O.binning(cfg, rec)
+    rows = np.arange(5, cfg.H, 61)
+    win = np.zeros_like(b["windows"])
+    win[rows] = b["windows"][rows]
+    ref = O.conv(cfg, b["sorted"], win)
+    tested = 0
+    for sh in range(4):
+        try:
+            s.set_tile_shape(sh)
+        except R.RtsimError:
+            continue
+        s.compute_output(2)
+        assert O.rel_l2(s.output()[rows], ref[rows]) <= MFMA_TOL, sh
+        tested += 1
+    assert tested >= 2
