@@ -289,11 +289,11 @@ def test_mfma_key_interval_edges(gpu, form):
 
 
 @pytest.mark.gpu
-def test_mfma_persistent_correlation_large_psf(gpu):
+def test_mfma_eight_wave_correlation_large_psf(gpu):
     """A 63 x 33 PSF needs so much LDS per workgroup that one fits per CU:
-    the correlation then runs persistent workgroups that walk several output
-    tiles each (512 tiles of 32 x 32 at the smallest shape), prefetching the
-    next tile's grid cells.  Every shape matches the oracle on sampled rows."""
+    the correlation then runs 8 waves per workgroup (JS doubles, so every
+    wave takes half as many tap steps and the partial sums of 2 to 8 waves
+    meet in LDS).  Every shape matches the oracle on sampled rows."""
     R = gpu
     cfg = configs._fls(512, 1024, 0.1, 120.0, 750e3, configs.psf_63x33(), "wide_psf")
     _, scene, _, _ = configs.fls_1024x4096()
