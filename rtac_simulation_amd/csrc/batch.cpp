@@ -65,7 +65,7 @@ struct Batch {
     size_t desc_cap = 0;
 
     Stage<StreamArgs> stream;
-    Stage<HistArgs> hist[4];
+    Stage<HistArgs> hist[4][2];   // per pass: small tiles, large tiles
     Stage<ScanArgs> scan[4];
     // [pass][last pass of its ping][records move with the keys]: a ping's
     // sort takes 1-4 passes, in one of two scatter variants
@@ -120,7 +120,8 @@ static int build(Batch* b)
 {
     b->stream = {};
     for (int k = 0; k < 4; k++) {
-        b->hist[k] = {};
+        b->hist[k][0] = {};
+        b->hist[k][1] = {};
         b->scan[k] = {};
         for (auto& sl : b->scatter[k])
             for (auto& sv : sl) sv = {};
@@ -141,10 +142,10 @@ static int build(Batch* b)
             b->stream.add(a, stream_blocks(a));
         }
         for (int k = 0; k < p.sort.passes; k++) {
-            b->hist[k].add(p.sort.hist[k], k == 0 ? p.sort.hist_blocks : p.sort.pass_blocks);
-            b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks ? 256u : 0u);
+            b->hist[k][p.sort.tile[k] == kSortTile ? 1 : 0].add(p.sort.hist[k], p.sort.hist_blocks[k]);
+            b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks[k] ? p.sort.hist[k].mask + 1 : 0u);
             b->scatter[k][k == p.sort.passes - 1][p.sort.scatter[k].move ? 1 : 0]
-                .add(p.sort.scatter[k], p.sort.pass_blocks);
+                .add(p.sort.scatter[k], p.sort.pass_blocks[k]);
         }
         if (!p.grid) {
             b->windows.add(p.bins, windows_blocks(p.bins));
@@ -175,7 +176,8 @@ static int build(Batch* b)
     size_t off = 0;
     layout(b->stream, off);
     for (int k = 0; k < 4; k++) {
-        layout(b->hist[k], off);
+        layout(b->hist[k][0], off);
+        layout(b->hist[k][1], off);
         layout(b->scan[k], off);
         for (auto& sl : b->scatter[k])
             for (auto& sv : sl) layout(sv, off);
@@ -196,7 +198,8 @@ static int build(Batch* b)
     std::vector<char> host(off, 0);
     pack(b->stream, host);
     for (int k = 0; k < 4; k++) {
-        pack(b->hist[k], host);
+        pack(b->hist[k][0], host);
+        pack(b->hist[k][1], host);
         pack(b->scan[k], host);
         for (const auto& sl : b->scatter[k])
             for (const auto& sv : sl) pack(sv, host);
@@ -218,8 +221,9 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
 #define RT_TRY(x) if ((e = (x)) != hipSuccess) return e
     RT_TRY(launch_stream_batch(dargs(b, b->stream), dmap(b, b->stream), b->stream.blocks(), s));
     for (int k = 0; k < 4; k++) {
-        RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]),
-                                      b->hist[k].blocks(), s));
+        for (int big = 0; big < 2; big++)
+            RT_TRY(launch_sort_hist_batch(k == 0, big == 1, dargs(b, b->hist[k][big]),
+                                          dmap(b, b->hist[k][big]), b->hist[k][big].blocks(), s));
         RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
         for (int last = 0; last < 2; last++)
             for (int mv = 0; mv < 2; mv++) {

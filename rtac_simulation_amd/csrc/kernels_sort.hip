@@ -31,7 +31,6 @@ namespace rtsim {
 
 constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;                  // 4
-constexpr int kSortLaneItems = (int)kSortTile / kSortThreads;  // 16
 // grouping key: split its bits evenly over the passes (else 8-bit digits)
 constexpr bool kGroupEvenDigits = true;
 
@@ -58,14 +57,14 @@ __device__ __forceinline__ uint32_t sort_block_scan(uint32_t x, uint32_t* ws, ui
     return off + incl - x;
 }
 
-template <bool FIRST>
+template <bool FIRST, int LI>
 __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
                                           const uint2* __restrict__ kvin, const GroupKey& gk,
                                           uint32_t n, uint32_t wbase, uint32_t lane, uint32_t* key,
                                           uint32_t* val)
 {
 #pragma unroll
-    for (int j = 0; j < kSortLaneItems; j++) {
+    for (int j = 0; j < LI; j++) {
         const uint32_t i = min(wbase + j * 64 + lane, n - 1);
         if (FIRST) {
             const float t = rec[i].z;
@@ -79,10 +78,11 @@ __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
     }
 }
 
-// ---- per-tile digit counts --------------------------------------------------
-template <bool FIRST>
+// ---- per-tile digit counts (tiles of TILE items) ------------------------------
+template <bool FIRST, uint32_t TILE>
 __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, uint32_t nblk)
 {
+    constexpr int LI = (int)TILE / kSortThreads;   // items per thread
     __shared__ uint32_t h[256];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
     h[t] = 0u;
@@ -93,15 +93,15 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
             a.fill[i] = a.fill_value;
     }
     if (a.n == 0 || blk >= a.tiles) return;
-    const uint32_t wbase = blk * kSortTile + w * (64 * kSortLaneItems);
-    uint32_t key[kSortLaneItems];
+    const uint32_t wbase = blk * TILE + w * (64 * LI);
+    uint32_t key[LI];
     __shared__ float wmax[kSortWaves];
     if (FIRST && a.gk.on) {
         // grouping keys, and the largest finite |re|, |im| of the in-range
         // samples (the record's components share the travel's cache line)
         float m = 0.f;
 #pragma unroll
-        for (int j = 0; j < kSortLaneItems; j++) {
+        for (int j = 0; j < LI; j++) {
             const float4 r = a.rec[min(wbase + j * 64 + lane, a.n - 1)];
             const uint32_t slot = bin_slot(a.gk.key, r.z);
             key[j] = (uint32_t)(g_row_of(a.gk.rows, r.z) + 2) + slot;
@@ -112,58 +112,67 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
         for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
         if (lane == 0) wmax[w] = m;
     } else {
-        load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, nullptr);
+        load_tile<FIRST, LI>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, nullptr);
     }
     __syncthreads();
 #pragma unroll
-    for (int j = 0; j < kSortLaneItems; j++)
+    for (int j = 0; j < LI; j++)
         if (wbase + j * 64 + lane < a.n) atomicAdd(&h[(key[j] >> a.shift) & a.mask], 1u);
     __syncthreads();
-    a.counts[(size_t)t * a.tiles + blk] = h[t];
+    if (t <= a.mask) a.counts[(size_t)t * a.tiles + blk] = h[t];   // digits beyond the mask: none
     if (FIRST && a.gk.on && t == 0)
         a.gk.tmax[blk] = __float_as_uint(fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3])));
 }
 
-template <bool FIRST>
+template <bool FIRST, uint32_t TILE>
 __global__ __launch_bounds__(256) void k_sort_hist(HistArgs a)
 {
-    sort_hist_body<FIRST>(a, blockIdx.x, gridDim.x);
+    sort_hist_body<FIRST, TILE>(a, blockIdx.x, gridDim.x);
 }
 
-template <bool FIRST>
+template <bool FIRST, uint32_t TILE>
 __global__ __launch_bounds__(256) void k_sort_hist_batch(const HistArgs* __restrict__ args, BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const HistArgs a = args[p];   // a private copy: the body's stores cannot alias it
-    sort_hist_body<FIRST>(a, blk, nblk);
+    sort_hist_body<FIRST, TILE>(a, blk, nblk);
 }
 
 // ---- one workgroup per digit: exclusive scan over tiles, digit total ---------
-constexpr int kScanPer = 32;   // tiles per thread: up to 8192 tiles (16.7M items)
+constexpr int kScanPer = 64;   // tiles per thread: up to 16384 tiles (16.7M items of small tiles)
 
 __device__ __forceinline__ void sort_scan_body(const ScanArgs& a, uint32_t digit)
 {
     __shared__ uint32_t ws[4];
+    __shared__ uint32_t srow[256 * kScanPer];     // the digit's row of tile counts
     const uint32_t t = threadIdx.x;
     const uint32_t tiles = a.tiles;
     uint32_t* row = a.counts + (size_t)digit * tiles;
+    // thread t scans tiles [t per, (t + 1) per): its sum, the block scan, its
+    // prefixes.  Long rows go through LDS (coalesced in and out; strided
+    // global reads of up to 64 words per thread measured 60 % slower), short
+    // ones are read in place (the extra LDS round trip cost more)
     const uint32_t per = (tiles + 255) / 256;
-    uint32_t v[kScanPer];
-    uint32_t s = 0;
-#pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
-        const uint32_t idx = t * per + k;
-        v[k] = (k < (int)per && idx < tiles) ? row[idx] : 0u;
-        s += v[k];
+    const bool staged = per > 8;
+    uint32_t* src = staged ? srow : row;
+    if (staged) {
+        for (uint32_t i = t; i < tiles; i += 256) srow[i] = row[i];
+        __syncthreads();
     }
+    const uint32_t i0 = t * per, i1 = min(i0 + per, tiles);
+    uint32_t s = 0;
+    for (uint32_t i = i0; i < i1; i++) s += src[i];
     uint32_t total;
     uint32_t run = sort_block_scan(s, ws, &total);
-#pragma unroll
-    for (int k = 0; k < kScanPer; k++) {
-        const uint32_t idx = t * per + k;
-        if (k < (int)per && idx < tiles) row[idx] = run;
-        run += v[k];
+    for (uint32_t i = i0; i < i1; i++) {
+        const uint32_t v = src[i];
+        src[i] = run;
+        run += v;
+    }
+    if (staged) {
+        __syncthreads();
+        for (uint32_t i = t; i < tiles; i += 256) row[i] = srow[i];
     }
     if (t == 0) a.totals[digit] = total;
     if (a.tmax && digit == 0) {
@@ -202,58 +211,60 @@ __device__ __forceinline__ uint32_t binner_key(bool grp, uint32_t c, uint32_t n)
 template <bool FIRST, bool LAST, bool MOVE>
 __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t tile)
 {
+    constexpr uint32_t TILE = sort_pass_tile(LAST, MOVE);
+    constexpr int LI = (int)TILE / kSortThreads;   // items per thread
     __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
     __shared__ uint32_t toff[256];               // digit run offsets in the staging tile
     __shared__ uint32_t gpos[256];               // global position of each digit run
     __shared__ uint32_t ws[4], ws2[4];
-    __shared__ uint32_t skey[kSortTile];
-    __shared__ uint32_t sval[kSortTile];
+    __shared__ uint32_t skey[TILE];
+    __shared__ uint32_t sval[TILE];
     // Records that move: the first pass stages them in digit order from
     // registers, a middle pass reads them at the staging step, and the last
     // pass copies them into LDS in input order at its start (their loads
     // overlap the key loads; held in registers across the ranking they would
     // spill) and takes each staged item's record by its input slot.
     constexpr bool RIN = MOVE && LAST && !FIRST;
-    __shared__ float4 srec[MOVE ? kSortTile : 1];
+    __shared__ float4 srec[MOVE ? TILE : 1];
     // last pass: the input slot of each staged item (RIN), then the value its
     // neighbours compare (bin slot or Binner key); each entry is read and
     // rewritten by the thread that owns it
-    __shared__ uint32_t scmp[LAST ? kSortTile : 1];
+    __shared__ uint32_t scmp[LAST ? TILE : 1];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const uint32_t tb = tile * kSortTile;
+    const uint32_t tb = tile * TILE;
     const int shift = a.shift;
     const uint32_t mask = a.mask;
 
     // global run start of digit t: digit base (scan of the totals) + earlier tiles
-    const uint32_t tot = a.totals[t];
-    const uint32_t pre = a.counts[(size_t)t * a.tiles + tile];
+    const uint32_t tot = t <= mask ? a.totals[t] : 0u;
+    const uint32_t pre = t <= mask ? a.counts[(size_t)t * a.tiles + tile] : 0u;
 #pragma unroll
     for (int k = 0; k < kSortWaves; k++) wcnt[k][t] = 0u;
 
-    const uint32_t wbase = tb + w * (64 * kSortLaneItems);
-    uint32_t key[kSortLaneItems], val[kSortLaneItems], lrank[kSortLaneItems];
-    float4 rv[MOVE ? kSortLaneItems : 1];
+    const uint32_t wbase = tb + w * (64 * LI);
+    uint32_t key[LI], val[LI], lrank[LI];
+    float4 rv[MOVE ? LI : 1];
     if (MOVE && FIRST) {
 #pragma unroll
-        for (int j = 0; j < kSortLaneItems; j++) {
+        for (int j = 0; j < LI; j++) {
             const uint32_t i = min(wbase + j * 64 + lane, a.n - 1);
             rv[j] = a.rec[i];
             key[j] = a.gk.on ? group_key(a.gk, rv[j].z) : travel_image(rv[j].z);
             val[j] = i;
         }
     } else {
-        load_tile<FIRST>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
+        load_tile<FIRST, LI>(a.rec, a.kvin, a.gk, a.n, wbase, lane, key, val);
         if (RIN) {
 #pragma unroll
-            for (int j = 0; j < kSortLaneItems; j++)
-                srec[w * (64 * kSortLaneItems) + j * 64 + lane] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
+            for (int j = 0; j < LI; j++)
+                srec[w * (64 * LI) + j * 64 + lane] = a.rin[min(wbase + j * 64 + lane, a.n - 1)];
         }
     }
     __syncthreads();
 
     // stable ranks inside each wave
 #pragma unroll
-    for (int j = 0; j < kSortLaneItems; j++) {
+    for (int j = 0; j < LI; j++) {
         const bool ok = wbase + j * 64 + lane < a.n;
         const uint32_t d = (key[j] >> shift) & mask;
         uint64_t peers = __ballot(ok);
@@ -294,13 +305,13 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 
     // stage the tile in digit order
 #pragma unroll
-    for (int j = 0; j < kSortLaneItems; j++) {
+    for (int j = 0; j < LI; j++) {
         if (wbase + j * 64 + lane < a.n) {
             const uint32_t d = (key[j] >> shift) & mask;
             const uint32_t p = wcnt[w][d] + lrank[j];
             skey[p] = key[j];
             sval[p] = val[j];
-            if (RIN) scmp[p] = w * (64 * kSortLaneItems) + j * 64 + lane;
+            if (RIN) scmp[p] = w * (64 * LI) + j * 64 + lane;
             else if (MOVE) srec[p] = FIRST ? rv[j] : a.rin[wbase + j * 64 + lane];
         }
     }
@@ -308,8 +319,8 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 
     // each digit run of the staged tile goes to consecutive global positions;
     // the last pass also moves the records and writes their keys / coordinates
-    const uint32_t tn = min(kSortTile, a.n - tb);
-    constexpr int Q = kSortLaneItems;
+    const uint32_t tn = min(TILE, a.n - tb);
+    constexpr int Q = LI;
     uint32_t kq[Q], vq[Q], pq[Q];
 #pragma unroll
     for (int u = 0; u < Q; u++) {
@@ -332,7 +343,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 #pragma unroll
         for (int u = 0; u < Q; u++) {
             const uint32_t q = t + u * kSortThreads;
-            rq[u] = srec[RIN ? (pq[u] < a.n ? scmp[q] : 0u) : min(q, kSortTile - 1)];
+            rq[u] = srec[RIN ? (pq[u] < a.n ? scmp[q] : 0u) : min(q, TILE - 1)];
         }
     } else if (LAST) {
 #pragma unroll
@@ -425,8 +436,6 @@ __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* _
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
                           const GroupKey* gk, SortPlan* p)
 {
-    const uint32_t tiles = (n + kSortTile - 1) / kSortTile;
-    if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
     GroupKey g{};
     if (gk && gk->on) g = *gk;
     // 8-bit digits: 4 passes for travel images; ceil(bits(K) / 8) passes for a
@@ -441,8 +450,15 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     }
     p->passes = passes;
     const uint32_t mask = (1u << dbits) - 1u;
+    const bool clears = b.seg_count || b.fill_count;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
     for (int pass = 0; pass < 4; pass++) {
+        const uint32_t tile = sort_pass_tile(pass == passes - 1, g.on);
+        const uint32_t tiles = (n + tile - 1) / tile;
+        if (pass < passes && (tiles > b.tiles_cap || tiles > 256u * kScanPer)) return hipErrorInvalidValue;
+        p->tile[pass] = tile;
+        p->hist_blocks[pass] = n ? tiles : (pass == 0 && clears ? 1u : 0u);
+        p->pass_blocks[pass] = n ? tiles : 0u;
         HistArgs& h = p->hist[pass];
         h.rec = records;
         h.kvin = b.kv[cur];
@@ -480,9 +496,6 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         c.rout = (passes - 1 - pass) % 2 ? b.tmp : b.out.sorted;
         if (pass < passes) cur ^= 1;
     }
-    const bool clears = b.seg_count || b.fill_count;
-    p->hist_blocks = n ? tiles : (clears ? 1u : 0u);
-    p->pass_blocks = n ? tiles : 0u;
     p->order = b.order;
     return hipSuccess;
 }
@@ -494,19 +507,24 @@ static void launch_scatter(bool last, dim3 g, const ScatterArgs& a, hipStream_t 
     else hipLaunchKernelGGL((k_sort_scatter<FIRST, false, MOVE>), g, dim3(256), 0, s, a);
 }
 
+template <uint32_t TILE>
+static void launch_hist(bool first, dim3 g, const HistArgs& a, hipStream_t s)
+{
+    if (first) hipLaunchKernelGGL((k_sort_hist<true, TILE>), g, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL((k_sort_hist<false, TILE>), g, dim3(256), 0, s, a);
+}
+
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
 {
     for (int pass = 0; pass < p.passes; pass++) {
-        const uint32_t hb = pass == 0 ? p.hist_blocks : p.pass_blocks;
+        const uint32_t hb = p.hist_blocks[pass];
         if (hb) {
-            if (pass == 0)
-                hipLaunchKernelGGL(k_sort_hist<true>, dim3(hb), dim3(256), 0, s, p.hist[pass]);
-            else
-                hipLaunchKernelGGL(k_sort_hist<false>, dim3(hb), dim3(256), 0, s, p.hist[pass]);
+            if (p.tile[pass] == kSortTile) launch_hist<kSortTile>(pass == 0, dim3(hb), p.hist[pass], s);
+            else launch_hist<kSortTileSmall>(pass == 0, dim3(hb), p.hist[pass], s);
         }
-        if (!p.pass_blocks) continue;
-        hipLaunchKernelGGL(k_sort_scan, dim3(256), dim3(256), 0, s, p.scan[pass]);
-        const dim3 g(p.pass_blocks);
+        if (!p.pass_blocks[pass]) continue;
+        hipLaunchKernelGGL(k_sort_scan, dim3(p.hist[pass].mask + 1), dim3(256), 0, s, p.scan[pass]);
+        const dim3 g(p.pass_blocks[pass]);
         const bool last = pass == p.passes - 1;
         const ScatterArgs& c = p.scatter[pass];
         if (pass == 0) {
@@ -520,12 +538,20 @@ hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
+hipError_t launch_sort_hist_batch(bool first_pass, bool big_tiles, const HistArgs* args, BatchMap m,
                                   uint32_t blocks, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    if (first_pass) hipLaunchKernelGGL(k_sort_hist_batch<true>, dim3(blocks), dim3(256), 0, s, args, m);
-    else hipLaunchKernelGGL(k_sort_hist_batch<false>, dim3(blocks), dim3(256), 0, s, args, m);
+    const dim3 g(blocks);
+    if (big_tiles) {
+        if (first_pass) hipLaunchKernelGGL((k_sort_hist_batch<true, kSortTile>), g, dim3(256), 0, s, args, m);
+        else hipLaunchKernelGGL((k_sort_hist_batch<false, kSortTile>), g, dim3(256), 0, s, args, m);
+    } else {
+        if (first_pass)
+            hipLaunchKernelGGL((k_sort_hist_batch<true, kSortTileSmall>), g, dim3(256), 0, s, args, m);
+        else
+            hipLaunchKernelGGL((k_sort_hist_batch<false, kSortTileSmall>), g, dim3(256), 0, s, args, m);
+    }
     return hipGetLastError();
 }
 

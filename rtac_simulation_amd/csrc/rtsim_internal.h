@@ -82,11 +82,17 @@ struct GroupKey {
 inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }
 
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
-constexpr uint32_t kSortTile = 2048;     // items per sort workgroup (256 lanes x 8)
+// Items per sort workgroup (256 lanes x 8 or x 4).  The last pass of a sort
+// whose records move uses the large tiles (twice the small tiles' run-bound
+// atomics cost more than the large tiles' lower parallelism), every other
+// pass the small ones (more workgroups hide the latency of their loads).
+constexpr uint32_t kSortTile = 2048;
+constexpr uint32_t kSortTileSmall = 1024;
+constexpr uint32_t sort_pass_tile(bool last, bool move) { return last && move ? kSortTile : kSortTileSmall; }
 // per-tile digit counts of one pass for `capacity` items
 inline size_t sort_count_words(size_t capacity)
 {
-    return 256 * ((capacity + kSortTile - 1) / kSortTile);
+    return 256 * ((capacity + kSortTileSmall - 1) / kSortTileSmall);
 }
 // Written by the last pass for every sorted position i: the record and its
 // bearing-axis coordinate, as (hi, lo) (coord) or, for the grid form, as a
@@ -171,8 +177,9 @@ struct SortPlan {
     HistArgs hist[4];
     ScanArgs scan[4];
     ScatterArgs scatter[4];
-    uint32_t hist_blocks;     // per pass (pass 0: >= 1 when it clears arrays)
-    uint32_t pass_blocks;     // scan: 256 per pass, scatter: tiles (0 when n = 0)
+    uint32_t tile[4];         // items per tile of each pass (sort_pass_tile)
+    uint32_t hist_blocks[4];  // per pass: tiles (pass 0: >= 1 when it clears arrays)
+    uint32_t pass_blocks[4];  // scan: 256 per pass, scatter: tiles (0 when n = 0)
     const uint32_t* order;    // permutation after the sort
 };
 // gk == nullptr or !gk->on: sort by travel
@@ -180,7 +187,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
                           const GroupKey* gk, SortPlan* p);
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
 // batched sort passes (args are device arrays, one entry per ping)
-hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m,
+hipError_t launch_sort_hist_batch(bool first_pass, bool big_tiles, const HistArgs* args, BatchMap m,
                                   uint32_t blocks, hipStream_t s);
 hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks,
                                   hipStream_t s);
