@@ -91,6 +91,7 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
             a.seg_init[i] = make_uint2(0xffffffffu, 0u);
         for (size_t i = (size_t)blk * 256 + t; i < a.fill_count; i += (size_t)nblk * 256)
             a.fill[i] = a.fill_value;
+        if (a.gk.on && blk == 0 && t == 0) a.gk.vmax[0] = 0u;   // the first scan sets it (n > 0)
     }
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * TILE + w * (64 * LI);
