@@ -369,28 +369,48 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         // row) differs from its predecessor's starts a key (row) run, one
         // whose bin differs from its successor's ends one.  The Binner keys
         // are non-decreasing over the sorted order, so the smallest start and
-        // the largest end over all workgroups bound each segment.  The G row
-        // of a staged neighbour is its K minus (its slot + 2).
+        // the largest end over all workgroups bound each segment.  Thread t
+        // examines the staged items q = LI t .. LI t + LI - 1: consecutive,
+        // so their neighbours are its own but at both ends; a digit run
+        // starts where the digit changes (the runs are in digit order).  The
+        // G row of a staged item is its K minus (its slot + 2).
         uint32_t* rows = grp ? a.gk.rows.row_start : nullptr;
         __syncthreads();
         uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
+        const uint32_t q0 = t * (uint32_t)LI;
+        if (q0 < tn) {
+            uint32_t kk[LI + 2], cc[LI + 2];   // items q0 - 1 .. q0 + LI
 #pragma unroll
-        for (int u = 0; u < Q; u++) {
-            const uint32_t pos = pq[u];
-            if (pos >= a.n) continue;
-            const uint32_t q = t + u * kSortThreads;
-            const uint32_t d = (kq[u] >> shift) & mask;
-            const uint32_t r0 = toff[d], r1 = d == 255u ? tn : toff[d + 1];
-            const uint32_t k = binner_key(grp, cq[u], a.out.key.n);
-            if (k != kOutOfRange) {
-                if (q == r0 || scmp[q - 1] != cq[u]) atomicMin(&seg[2 * k], pos);
-                if (q + 1 == r1 || scmp[q + 1] != cq[u]) atomicMax(&seg[2 * k + 1], pos + 1);
+            for (int j = 0; j < LI; j += 4) {
+                const uint4 k4 = *reinterpret_cast<const uint4*>(&skey[q0 + j]);
+                const uint4 c4 = *reinterpret_cast<const uint4*>(&scmp[q0 + j]);
+                kk[j + 1] = k4.x; kk[j + 2] = k4.y; kk[j + 3] = k4.z; kk[j + 4] = k4.w;
+                cc[j + 1] = c4.x; cc[j + 2] = c4.y; cc[j + 3] = c4.z; cc[j + 4] = c4.w;
             }
-            if (rows) {
-                const int g = (int)(kq[u] - cq[u]) - 2;
-                if ((q == r0 || (int)(skey[q - 1] - scmp[q - 1]) - 2 != g) && g >= -1 &&
-                    g <= a.gk.rows.GH + 1)
-                    atomicMin(&rows[g + 1], pos);
+            kk[0] = q0 > 0 ? skey[q0 - 1] : 0u;
+            cc[0] = q0 > 0 ? scmp[q0 - 1] : 0u;
+            kk[LI + 1] = q0 + LI < tn ? skey[q0 + LI] : 0u;
+            cc[LI + 1] = q0 + LI < tn ? scmp[q0 + LI] : 0u;
+#pragma unroll
+            for (int j = 0; j < LI; j++) {
+                const uint32_t q = q0 + j;
+                if (q >= tn) break;
+                const uint32_t d = (kk[j + 1] >> shift) & mask;
+                const bool first = q == 0 || ((kk[j] >> shift) & mask) != d;
+                const bool last = q + 1 == tn || ((kk[j + 2] >> shift) & mask) != d;
+                const uint32_t c = cc[j + 1];
+                const uint32_t k = binner_key(grp, c, a.out.key.n);
+                const bool kstart = k != kOutOfRange && (first || cc[j] != c);
+                const bool kend = k != kOutOfRange && (last || cc[j + 2] != c);
+                const int g = (int)(kk[j + 1] - c) - 2;
+                const bool gstart = rows && g >= -1 && g <= a.gk.rows.GH + 1 &&
+                                    (first || (int)(kk[j] - cc[j]) - 2 != g);
+                if (kstart || kend || gstart) {
+                    const uint32_t pos = gpos[d] + (q - toff[d]);
+                    if (kstart) atomicMin(&seg[2 * k], pos);
+                    if (kend) atomicMax(&seg[2 * k + 1], pos + 1);
+                    if (gstart) atomicMin(&rows[g + 1], pos);
+                }
             }
         }
     }
