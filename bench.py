@@ -247,6 +247,7 @@ def bench_large(R, configs, stream, steps):
     import oracle as O
     return {"workload": f"fls_1024x4096: {cfg.W} bearings x {cfg.H} ranges, {S} samples, "
                         f"{cfg.Tr}x{cfg.Tb} PSF", "samples": S, "tile_shape": tile,
+            "tile_waves": s.tile_waves(),
             "mfma_fp16": {"ms_per_ping": ms_mfma, "pings_per_s": 1e3 / ms_mfma,
                           "msamples_per_s": S / (ms_mfma * 1e-3) / 1e6},
             "plain_fp32": {"ms_per_ping": ms_plain, "pings_per_s": 1e3 / ms_plain},
@@ -297,7 +298,7 @@ def main():
         if form != R.FORM_PLAIN_FP32:
             # pick the correlation tile shape once, outside the timed region
             sensor.write_samples(dhits, stream=stream)
-            tiles[name] = sensor.autotune(form, 20, stream)[0]
+            tiles[name] = [sensor.autotune(form, 20, stream)[0], sensor.tile_waves()]
         ms[name] = time_form(R, sensor, dhits, stream, form, args.steps, args.warmup)
         stages[name] = stage_profile(R, sensor, dhits, stream, form, max(5, args.steps // 5))
         pings[name] = sensor.output()
@@ -362,7 +363,8 @@ def main():
         "config": {"workload": f"{args.workload}: {W} bearings x {H} ranges, {S} samples, "
                                f"{Tr}x{Tb} complex PSF", "samples": S, "bearings": W,
                    "ranges": H, "psf": [Tr, Tb], "form": best,
-                   "tile_shape": tiles.get(best)},
+                   "tile_shape": tiles[best][0] if best in tiles else None,
+                   "tile_waves": tiles[best][1] if best in tiles else None},
         "msamples_per_s": S / (t * 1e-3) / 1e6,
         "forms": {k: {"ms_per_ping": ms[k], "pings_per_s": 1e3 / ms[k],
                       "msamples_per_s": S / (ms[k] * 1e-3) / 1e6,

@@ -150,11 +150,18 @@ int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
  * Synchronises the stream; shape / ms may be NULL.                             */
 int rtsim_sensor_autotune(rtsim_sensor h, int form, int iterations, void* stream, int* shape,
                           float* ms);
-/* Sets / reads the MFMA correlation tile shape (0: 64x32, 1: 32x32, 2: 32x64,
- * 3: 64x16 range rows x bearings).  A shape whose LDS does not fit is
+/* Sets / reads the MFMA correlation tile shape (0: 32x32, 1: 32x64, 2: 64x32,
+ * 3: 64x64 range rows x bearings).  A shape whose LDS does not fit is
  * RTSIM_ERR_UNSUPPORTED.  No reference counterpart (see autotune).            */
 int rtsim_sensor_set_tile_shape(rtsim_sensor h, int shape);
 int rtsim_sensor_tile_shape(rtsim_sensor h, int* shape, int* rows, int* cols);
+/* Sets / reads the waves per MFMA correlation workgroup, 4 or 8 (8 share each
+ * output tile's tap steps among twice as many waves).  Setting a tile shape
+ * resets it to the shape's default: 8 when its LDS leaves room for one
+ * workgroup per CU only, else 4.  A count whose LDS does not fit is
+ * RTSIM_ERR_UNSUPPORTED.  No reference counterpart (see autotune).            */
+int rtsim_sensor_set_tile_waves(rtsim_sensor h, int waves);
+int rtsim_sensor_tile_waves(rtsim_sensor h, int* waves);
 
 /* get_ping()/output() (SensorInstance2D.h:144-150, SensorInstance1D.h:103)         */
 int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height,

@@ -97,6 +97,8 @@ _SIGNATURES = [
     ("rtsim_sensor_autotune", C.c_int,
      [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("rtsim_sensor_set_tile_shape", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_sensor_set_tile_waves", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_sensor_tile_waves", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     ("rtsim_sensor_set_pose", C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
     ("rtsim_sensor_tile_shape", C.c_int,
      [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
@@ -459,6 +461,14 @@ class SensorInstance:
 
     def set_tile_shape(self, shape: int):
         check(lib().rtsim_sensor_set_tile_shape(self.handle, shape))
+
+    def set_tile_waves(self, waves: int):
+        check(lib().rtsim_sensor_set_tile_waves(self.handle, waves))
+
+    def tile_waves(self) -> int:
+        w = C.c_int()
+        check(lib().rtsim_sensor_tile_waves(self.handle, C.byref(w)))
+        return w.value
 
     def tile_shape(self):
         sh, r, c = C.c_int(), C.c_int(), C.c_int()

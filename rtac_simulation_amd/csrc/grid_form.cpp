@@ -84,7 +84,9 @@ struct GridState {
     int Hp = 0, Wp = 0, GH = 0, GHa = 0, GW = 0, fr = 0, fb = 0;
     int KR = 0, L = 0;                   // correlation K blocks, tap column length
     int shape = 0;                       // correlation tile shape in use
-    bool fits[kCorrShapes] = {false, false, false, false};
+    int waves8 = 0;                      // 1: 8 waves per correlation workgroup
+    bool fits[kCorrShapes] = {false, false, false, false};    // 4 waves
+    bool fits8[kCorrShapes] = {false, false, false, false};   // 8 waves
     float inv_dr = 0, off_r = 0, inv_db = 0, off_b = 0;
     double bear0 = 0;
     float tap_scale = 1.f;
@@ -111,6 +113,26 @@ void Sensor::free_grid()
 {
     delete static_cast<GridState*>(grid_state);
     grid_state = nullptr;
+}
+
+// LDS-sizing arguments of a correlation shape
+static CorrArgs shape_probe(const Sensor* s, const GridState* g, int sh, int waves8)
+{
+    CorrArgs probe{};
+    probe.Tr = (int)s->Tr;
+    probe.Tb = (int)s->Tb;
+    probe.shape = sh;
+    probe.KR = g->KR;
+    probe.L = g->L;
+    probe.RS = corr_row_stride(sh, g->KR);
+    probe.waves8 = waves8;
+    return probe;
+}
+
+// 8 waves when the shape's LDS allows one workgroup per CU only (and 8 fit)
+static int default_waves8(const Sensor* s, const GridState* g, int sh)
+{
+    return g->fits8[sh] && corr_waves8(shape_probe(s, g, sh, 0), g->max_lds) ? 1 : 0;
 }
 
 static int prepare_grid(Sensor* s, GridState* g)
@@ -175,22 +197,20 @@ static int prepare_grid(Sensor* s, GridState* g)
     // tile shapes whose LDS fits; default: the first that does (32x32)
     g->shape = -1;
     for (int sh = 0; sh < kCorrShapes; sh++) {
-        CorrArgs probe{};
-        probe.Tr = (int)s->Tr;
-        probe.Tb = (int)s->Tb;
-        probe.shape = sh;
-        probe.KR = g->KR;
-        probe.L = g->L;
-        probe.RS = corr_row_stride(sh, g->KR);
-        g->fits[sh] = correlate_lds_bytes(probe) <= (size_t)max_lds;
+        g->fits[sh] = correlate_lds_bytes(shape_probe(s, g, sh, 0)) <= (size_t)max_lds;
+        g->fits8[sh] = correlate_lds_bytes(shape_probe(s, g, sh, 1)) <= (size_t)max_lds;
         if (g->fits[sh] && g->shape < 0) g->shape = sh;
     }
     if (g->shape < 0)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: no tile shape fits %d B of LDS for %ux%u taps",
                     max_lds, s->Tr, s->Tb);
+    g->waves8 = default_waves8(s, g, g->shape);
     const TuneKey key{(int)s->N, (int)s->W, (int)s->Tr, (int)s->Tb};
-    int cached = tuned_shape(key);
-    if (cached >= 0 && g->fits[cached]) g->shape = cached;
+    const int cached = tuned_shape(key);   // shape + 4 waves8
+    if (cached >= 0 && ((cached >> 2) ? g->fits8 : g->fits)[cached & 3]) {
+        g->shape = cached & 3;
+        g->waves8 = cached >> 2;
+    }
     const size_t dep_lds = sizeof(long long) * 2 * g->GW;
     if (dep_lds > (size_t)max_lds)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
@@ -378,13 +398,14 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     c.shape = g->shape;
     c.RS = corr_row_stride(g->shape, g->KR);
     c.tap_scale = g->tap_scale;
-    c.waves8 = corr_waves8(c, g->max_lds);
+    c.waves8 = g->waves8;
     return RTSIM_OK;
 }
 
-// Times every tile shape that fits (correlation kernel only, HIP events on
-// the stream, after one full compute so G is current) and keeps the fastest;
-// the result is cached for the shape class (H, W, Tr, Tb).
+// Times every tile shape that fits, with 4 and with 8 waves per workgroup
+// (correlation kernel only, HIP events on the stream, after one full compute
+// so G is current) and keeps the fastest; the result is cached for the shape
+// class (H, W, Tr, Tb).
 int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms)
 {
     if (form != RTSIM_FORM_MFMA_BF16 && form != RTSIM_FORM_MFMA_FP16)
@@ -396,14 +417,15 @@ int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, f
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
         return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
-    int best = -1;
+    int best = -1, best8 = 0;
     float bms = 0.f;
-    for (int sh = 0; sh < kCorrShapes; sh++) {
-        if (!g->fits[sh]) continue;
+    for (int v = 0; v < 2 * kCorrShapes; v++) {
+        const int sh = v % kCorrShapes, w8 = v / kCorrShapes;
+        if (!(w8 ? g->fits8 : g->fits)[sh]) continue;
         CorrArgs c = last_corr;
         c.shape = sh;
         c.RS = corr_row_stride(sh, g->KR);
-        c.waves8 = corr_waves8(c, g->max_lds);
+        c.waves8 = w8;
         hipError_t err = launch_correlate(c, form == RTSIM_FORM_MFMA_FP16, s);   // warm
         (void)hipEventRecord(e0, s);
         for (int it = 0; it < iterations && err == hipSuccess; it++)
@@ -418,12 +440,13 @@ int Sensor::autotune(int form, int iterations, hipStream_t s, int* best_shape, f
             return hip_fail(err, "autotune");
         }
         ms /= iterations;
-        if (best < 0 || ms < bms) { best = sh; bms = ms; }
+        if (best < 0 || ms < bms) { best = sh; best8 = w8; bms = ms; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     g->shape = best;
-    store_tuned(TuneKey{(int)N, (int)W, (int)Tr, (int)Tb}, best);
+    g->waves8 = best8;
+    store_tuned(TuneKey{(int)N, (int)W, (int)Tr, (int)Tb}, best + 4 * best8);
     generation++;   // captured graphs hold the old shape
     if (best_shape) *best_shape = best;
     if (best_ms) *best_ms = bms;
@@ -454,7 +477,34 @@ extern "C" int rtsim_sensor_set_tile_shape(rtsim_sensor h, int shape)
     if (!g->fits[shape])
         return fail(RTSIM_ERR_UNSUPPORTED, "tile shape %d does not fit this sensor's LDS budget", shape);
     g->shape = shape;
+    g->waves8 = default_waves8(s, g, shape);
     s->generation++;
+    return RTSIM_OK;
+}
+
+extern "C" int rtsim_sensor_set_tile_waves(rtsim_sensor h, int waves)
+{
+    Sensor* s = reinterpret_cast<Sensor*>(h);
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+    if (waves != 4 && waves != 8) return fail(RTSIM_ERR_INVALID, "waves must be 4 or 8 (got %d)", waves);
+    int rc = s->ensure_grid();
+    if (rc) return rc;
+    GridState* g = static_cast<GridState*>(s->grid_state);
+    if (!(waves == 8 ? g->fits8 : g->fits)[g->shape])
+        return fail(RTSIM_ERR_UNSUPPORTED, "%d waves do not fit tile shape %d's LDS budget", waves, g->shape);
+    g->waves8 = waves == 8 ? 1 : 0;
+    s->generation++;
+    return RTSIM_OK;
+}
+
+extern "C" int rtsim_sensor_tile_waves(rtsim_sensor h, int* waves)
+{
+    Sensor* s = reinterpret_cast<Sensor*>(h);
+    if (!s) return fail(RTSIM_ERR_INVALID, "sensor handle is NULL");
+    if (!waves) return fail(RTSIM_ERR_INVALID, "waves is NULL");
+    int rc = s->ensure_grid();
+    if (rc) return rc;
+    *waves = static_cast<GridState*>(s->grid_state)->waves8 ? 8 : 4;
     return RTSIM_OK;
 }
 

@@ -113,16 +113,19 @@ def test_mfma_dynamic_range(gpu, form):
     assert O.rel_l2(s.output(), ref) <= MFMA_TOL
 
 
+@pytest.mark.parametrize("waves", [4, 8])
 @pytest.mark.parametrize("shape", [0, 1, 2, 3])
 @pytest.mark.parametrize("W,H,psf", [(48, 96, "31x17"), (37, 70, "8x6"), (40, 100, "21x33")])
-def test_mfma_every_tile_shape_vs_oracle(gpu, shape, W, H, psf):
-    """Every correlation tile shape the autotuner may pick restates the gather."""
+def test_mfma_every_tile_shape_vs_oracle(gpu, shape, waves, W, H, psf):
+    """Every correlation tile shape and wave count the autotuner may pick
+    restates the gather."""
     R = gpu
     cfg = configs._fls(W, H, 0.1, 30.0, 1.2e6, _psf(psf), f"t{W}x{H}")
     hits = R.generate_hits(configs.fls_scene(), 23, 30_000)
     s = _sensor(R, cfg, hits)
     s.set_tile_shape(shape)
-    assert s.tile_shape()[0] == shape
+    s.set_tile_waves(waves)
+    assert s.tile_shape()[0] == shape and s.tile_waves() == waves
     ref = None
     for form in FORMS:
         s.compute_output(form)
