@@ -128,12 +128,18 @@ int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count);
 int rtsim_sensor_sample_count(rtsim_sensor h, size_t* count);
 
 /* receiver_view() samples pointer (SensorInstance.cu:41-50, Receiver.h:18-35):
- * the device array of 16-B records a producer may fill directly.                  */
+ * the device array of 16-B records a producer may fill directly.  Records
+ * written through it must be written after this call (it discards what a
+ * previous rtsim_receiver_write_samples prepared for compute).                  */
 int rtsim_sensor_samples(rtsim_sensor h, void** dev_ptr);
 
 /* ReceiverView<T>::set_sample / set_null_sample (Receiver.h:51-65) applied to
  * n hits with the phase/spreading of RayCaster.cu:62-74: writes records
- * [offset, offset + n).                                                           */
+ * [offset, offset + n).  A write of every sample (offset 0, n = sample count)
+ * into a sensor whose MFMA form is prepared (by an earlier MFMA compute or a
+ * tile-shape call) also makes that form's first sort histogram, which the
+ * next MFMA compute then skips; any other write, upload, count or range change
+ * and any compute discard it.                                                     */
 int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t n,
                                  size_t offset, void* stream);
 

@@ -346,6 +346,7 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
     for (auto& en : b->entries) {
         en.sensor->computed = true;
+        en.sensor->hist_fused = false;   // the batch's own sort rewrote the counts
         en.sensor->computed_form = en.form;
     }
     return RTSIM_OK;

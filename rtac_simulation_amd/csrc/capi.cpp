@@ -363,7 +363,9 @@ int Sensor::compute(int form, hipStream_t s)
     int rc = plan(form, &p);
     if (rc) return rc;
     stage_begin(1, s);
-    HIP_TRY(launch_sort_plan(p.sort, s));
+    const bool fused = p.grid && hist_fused;
+    hist_fused = false;   // the sort consumes (and overwrites) the counts
+    HIP_TRY(launch_sort_plan(p.sort, s, fused));
     stage_end(s);
     if (!p.grid) {
         stage_begin(2, s);
@@ -559,6 +561,7 @@ int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
     s->N = range_count;
     s->margin_request = margin;
     s->computed = false;
+    s->hist_fused = false;
     return s->configure_ranges();
 }
 
@@ -582,6 +585,7 @@ int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count)
     if (s->count != count) s->generation++;
     s->count = count;
     s->computed = false;
+    s->hist_fused = false;
     return RTSIM_OK;
 }
 
@@ -598,6 +602,7 @@ int rtsim_sensor_samples(rtsim_sensor h, void** dev_ptr)
     GET_SENSOR(h);
     if (!dev_ptr) return fail(RTSIM_ERR_INVALID, "dev_ptr is NULL");
     *dev_ptr = s->d_samples;
+    s->hist_fused = false;   // the caller may write the samples
     return RTSIM_OK;
 }
 
@@ -613,9 +618,17 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
     hipStream_t st = (hipStream_t)stream;
     StreamArgs a = s->stream_args(*hits, n, offset);
     (void)hipGetLastError();
+    // a write of every sample into a sensor whose grid form is prepared also
+    // makes that form's first sort histogram (one launch fewer per ping)
+    PingPlan p;
+    const bool fuse = offset == 0 && n == s->count && n > 0 && s->grid_ready() &&
+                      s->plan(RTSIM_FORM_MFMA_FP16, &p) == RTSIM_OK && p.sort.passes > 1 &&
+                      p.sort.tile[0] == kSortTileSmall;
     s->stage_begin(0, st);
-    HIP_TRY(launch_stream(a, st));
+    if (fuse) HIP_TRY(launch_stream_hist(a, p.sort.hist[0], st));
+    else HIP_TRY(launch_stream(a, st));
     s->stage_end(st);
+    s->hist_fused = fuse;
     s->computed = false;
     return RTSIM_OK;
 }
@@ -682,6 +695,7 @@ int rtsim_sensor_upload_samples(rtsim_sensor h, const void* host, size_t n, void
     if (n) HIP_TRY(hipMemcpyAsync(s->d_samples, host, 16 * n, hipMemcpyHostToDevice, st));
     HIP_TRY(hipStreamSynchronize(st));
     s->computed = false;
+    s->hist_fused = false;
     return RTSIM_OK;
 }
 

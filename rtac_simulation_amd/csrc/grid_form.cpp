@@ -282,6 +282,11 @@ int Sensor::ensure_grid()
     return RTSIM_OK;
 }
 
+bool Sensor::grid_ready() const
+{
+    return grid_state && static_cast<const GridState*>(grid_state)->ready;
+}
+
 int Sensor::grid_row_map(RowMap* m, uint32_t** vmax)
 {
     int rc = ensure_grid();

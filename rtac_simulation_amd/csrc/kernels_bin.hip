@@ -51,8 +51,6 @@ __device__ __forceinline__ float directivity_lookup(const StreamArgs& p, float a
     return top + b * (bot - top);
 }
 
-constexpr uint32_t kStreamMaxBlocks = 256 * 64;
-
 // set_sample (Receiver.h:51-60) of one ray hit
 __device__ __forceinline__ float4 stream_sample(const StreamArgs& p, float r, float d0, float d1,
                                                 float d2, float2 A)
@@ -86,62 +84,101 @@ __device__ __forceinline__ float4 stream_sample(const StreamArgs& p, float r, fl
     return o;
 }
 
-// Each thread streams kStreamU hits (loads of all of them in flight before
-// any is computed); workgroups stride over the hits.  One hit per thread
-// measured fastest (2 and 4 are 5-10 % slower: fewer resident waves hide
-// the sample's dependent table lookups and FP64 phase less well).
-constexpr int kStreamU = 1;
+// One workgroup per tile of kSortTileSmall hits, one hit per thread (more per
+// thread measured 5-10 % slower: fewer resident waves hide the sample's
+// dependent table lookups less well).  With h (the grid form's first sort
+// histogram) the workgroup also makes that tile's digit counts of the
+// grouping key, its largest finite in-range |re|, |im| and the first
+// histogram's clears, as k_sort_hist's first pass makes them from the same
+// records (a histogram and a maximum: the thread mapping is irrelevant).
+// Every streaming launch (one ping, fused or not, and batched) runs this one
+// body with the histogram switched at run time (a compile-time switch lets the
+// compiler fuse multiply-adds differently, and the records would differ in
+// their last bits between launches).
+constexpr uint32_t kStreamThreads = kSortTileSmall;
+constexpr int kStreamWaves = (int)kStreamThreads / 64;
 
-__device__ __forceinline__ void stream_body(const StreamArgs& p, uint32_t blk, uint32_t nblk)
+__device__ __forceinline__ void stream_body(const StreamArgs& p, const HistArgs& hist, int with_hist,
+                                            uint32_t blk, uint32_t nblk)
 {
-    constexpr uint32_t span = 256 * kStreamU;
-    for (uint32_t base = blk * span; base < p.n; base += nblk * span) {
-        float r[kStreamU], d0[kStreamU], d1[kStreamU], d2[kStreamU];
-        float2 A[kStreamU];
-#pragma unroll
-        for (int u = 0; u < kStreamU; u++) {
-            const uint32_t i = min(base + u * 256 + threadIdx.x, p.n - 1);
-            r[u] = p.range[i];
-            d0[u] = p.dx[i];
-            d1[u] = p.dy[i];
-            d2[u] = p.dz[i];
-            A[u] = p.amp[i];
+    const HistArgs* h = with_hist ? &hist : nullptr;
+    __shared__ uint32_t cnt[256];
+    __shared__ float wmax[kStreamWaves];
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    if (h) {
+        if (t < 256) cnt[t] = 0u;
+        for (size_t i = (size_t)blk * kStreamThreads + t; i < h->seg_count;
+             i += (size_t)nblk * kStreamThreads)
+            h->seg_init[i] = make_uint2(0xffffffffu, 0u);
+        for (size_t i = (size_t)blk * kStreamThreads + t; i < h->fill_count;
+             i += (size_t)nblk * kStreamThreads)
+            h->fill[i] = h->fill_value;
+        if (blk == 0 && t == 0) h->gk.vmax[0] = 0u;
+    }
+    const uint32_t i = blk * kStreamThreads + t;
+    const uint32_t ic = min(i, p.n - 1);
+    const float r = p.range[ic], d0 = p.dx[ic], d1 = p.dy[ic], d2 = p.dz[ic];
+    const float2 A = p.amp[ic];
+    if (h) __syncthreads();   // counts cleared
+    float m = 0.f;
+    if (i < p.n) {
+        const float4 o = stream_sample(p, r, d0, d1, d2, A);
+        p.out[i] = o;
+        if (h) {
+            const uint32_t slot = bin_slot(h->gk.key, o.z);
+            const uint32_t K = (uint32_t)(g_row_of(h->gk.rows, o.z) + 2) + slot;
+            atomicAdd(&cnt[(K >> h->shift) & h->mask], 1u);
+            const float c = fmaxf(fabsf(o.x), fabsf(o.y));
+            if (slot - 1u < h->gk.key.n && c <= 3.402823466e38f) m = fmaxf(m, c);
         }
-#pragma unroll
-        for (int u = 0; u < kStreamU; u++) {
-            const uint32_t i = base + u * 256 + threadIdx.x;
-            if (i < p.n) p.out[i] = stream_sample(p, r[u], d0[u], d1[u], d2[u], A[u]);
-        }
+    }
+    if (!h) return;
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
+    if (lane == 0) wmax[w] = m;
+    __syncthreads();
+    if (t <= h->mask) h->counts[(size_t)t * h->tiles + blk] = cnt[t];
+    if (t == 0) {
+        float mm = wmax[0];
+        for (int k = 1; k < kStreamWaves; k++) mm = fmaxf(mm, wmax[k]);
+        h->gk.tmax[blk] = __float_as_uint(mm);
     }
 }
 
-__global__ __launch_bounds__(256) void k_stream(StreamArgs p) { stream_body(p, blockIdx.x, gridDim.x); }
+__global__ __launch_bounds__(kStreamThreads) void k_stream(StreamArgs p, HistArgs h, int with_hist)
+{
+    stream_body(p, h, with_hist, blockIdx.x, gridDim.x);
+}
 
-__global__ __launch_bounds__(256) void k_stream_batch(const StreamArgs* __restrict__ args, BatchMap m)
+__global__ __launch_bounds__(kStreamThreads) void k_stream_batch(const StreamArgs* __restrict__ args,
+                                                                 BatchMap m, HistArgs h, int with_hist)
 {
     uint32_t blk, nblk;
     const uint32_t q = batch_locate(m, blk, nblk);
     const StreamArgs a = args[q];   // a private copy: the body's stores cannot alias it
-    stream_body(a, blk, nblk);
+    stream_body(a, h, with_hist, blk, nblk);
 }
 
-uint32_t stream_blocks(const StreamArgs& a)
-{
-    const uint32_t blocks = (a.n + 256 * kStreamU - 1) / (256 * kStreamU);
-    return blocks > kStreamMaxBlocks ? kStreamMaxBlocks : blocks;
-}
+uint32_t stream_blocks(const StreamArgs& a) { return (a.n + kStreamThreads - 1) / kStreamThreads; }
 
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s)
 {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stream, dim3(stream_blocks(a)), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_stream, dim3(stream_blocks(a)), dim3(kStreamThreads), 0, s, a, HistArgs{}, 0);
     return hipGetLastError();
 }
 
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(256), 0, s, args, m);
+    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(kStreamThreads), 0, s, args, m, HistArgs{}, 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_stream_hist(const StreamArgs& a, const HistArgs& h, hipStream_t s)
+{
+    if (a.n == 0 || h.tiles == 0) return hipErrorInvalidValue;
+    if (stream_blocks(a) != h.tiles) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stream, dim3(h.tiles), dim3(kStreamThreads), 0, s, a, h, 1);
     return hipGetLastError();
 }
 

@@ -535,10 +535,10 @@ static void launch_hist(bool first, dim3 g, const HistArgs& a, hipStream_t s)
     else hipLaunchKernelGGL((k_sort_hist<false, TILE>), g, dim3(256), 0, s, a);
 }
 
-hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s)
+hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s, bool skip_hist0)
 {
     for (int pass = 0; pass < p.passes; pass++) {
-        const uint32_t hb = p.hist_blocks[pass];
+        const uint32_t hb = pass == 0 && skip_hist0 ? 0u : p.hist_blocks[pass];
         if (hb) {
             if (p.tile[pass] == kSortTile) launch_hist<kSortTile>(pass == 0, dim3(hb), p.hist[pass], s);
             else launch_hist<kSortTileSmall>(pass == 0, dim3(hb), p.hist[pass], s);

@@ -45,6 +45,10 @@ struct StreamArgs {
     float dir_xs, dir_ys;
 };
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
+struct HistArgs;
+// the streaming kernel fused with the grid form's first sort histogram h
+// (tiles of kSortTileSmall samples; a.n == h.n, a.out == h.rec)
+hipError_t launch_stream_hist(const StreamArgs& a, const HistArgs& h, hipStream_t s);
 uint32_t stream_blocks(const StreamArgs& a);
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
@@ -185,7 +189,9 @@ struct SortPlan {
 // gk == nullptr or !gk->on: sort by travel
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
                           const GroupKey* gk, SortPlan* p);
-hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
+// skip_hist0: the first pass's histogram (and its clears) are already made
+// (launch_stream_hist)
+hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s, bool skip_hist0 = false);
 // batched sort passes (args are device arrays, one entry per ping)
 hipError_t launch_sort_hist_batch(bool first_pass, bool big_tiles, const HistArgs* args, BatchMap m,
                                   uint32_t blocks, hipStream_t s);

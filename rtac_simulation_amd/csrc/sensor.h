@@ -63,6 +63,9 @@ struct Sensor {
     // ---- samples -----------------------------------------------------------
     size_t count = 0, capacity = 0;
     bool computed = false;
+    // the last write_samples also made the grid form's first sort histogram
+    // (and its clears) for exactly the resident samples: compute skips it
+    bool hist_fused = false;
     int computed_form = -1;       // form of the last compute (batched or not)
     float4* d_samples = nullptr;  // receivedSamples_ (unsorted)
     float4* d_sorted = nullptr;   // receivedSamples_ after sort()
@@ -107,6 +110,7 @@ struct Sensor {
     int check_plain_limits() const;
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();
+    bool grid_ready() const;         // grid form prepared (no preparation is started)
     int grid_row_map(RowMap* m, uint32_t** vmax = nullptr);   // grid form row table, G-row map
     int compute(int form, hipStream_t s);
     int grid_plan(int form, DepositArgs* d, CorrArgs* c);

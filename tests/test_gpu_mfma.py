@@ -318,3 +318,46 @@ def test_mfma_eight_wave_correlation_large_psf(gpu):
         assert O.rel_l2(s.output()[rows], ref[rows]) <= MFMA_TOL, sh
         tested += 1
     assert tested >= 2
+
+
+@pytest.mark.gpu
+def test_mfma_first_histogram_fused_into_write_samples(gpu):
+    """Once the grid form is prepared, a write of every sample also makes the
+    grouping sort's first histogram; the computes that use it give the same
+    ping (bit for bit) and binning as computes that run the histogram, and
+    writes by other means, partial writes and other forms discard it."""
+    R = gpu
+    cfg = configs._fls(128, 256, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "fused")
+    hits = R.generate_hits(configs.fls_scene(), 29, 120_000)
+    dh = hits.to_device()
+    s = R.make_sensor(cfg)
+    s.set_sample_count(hits.n)
+    s.write_samples(dh)                      # grid form not prepared yet: plain stream kernel
+    s.compute_output(2)
+    ref = s.output().view(np.uint32).copy()
+    b = O.binning(cfg, s.samples())
+    s.compute_output(2)                      # histogram run by the sort itself
+    assert np.array_equal(s.output().view(np.uint32), ref)
+    for form in (2, 1, 2):
+        s.write_samples(dh)                  # fused
+        s.compute_output(form)
+        if form == 2:
+            assert np.array_equal(s.output().view(np.uint32), ref)
+        d = s.binner_debug()
+        for key in ("keys", "segments", "windows"):
+            assert np.array_equal(d[key], b[key]), (form, key)
+    # a fused write followed by the plain form, then the grid form (unfused)
+    s.write_samples(dh)
+    s.compute_output(0)
+    ref_plain, _ = O.reference_ping(cfg, s.samples())
+    assert O.rel_l2(s.output(), ref_plain) <= 1e-5
+    s.compute_output(2)
+    assert np.array_equal(s.output().view(np.uint32), ref)
+    # a partial write (the same hits of the second half) is not fused and
+    # discards the fused histogram of the full write before it
+    s.write_samples(dh)
+    n1 = hits.n // 2
+    tail = R.generate_hits(configs.fls_scene(), 29, hits.n - n1, first=n1).to_device()
+    s.write_samples(tail, offset=n1)
+    s.compute_output(2)
+    assert np.array_equal(s.output().view(np.uint32), ref)

@@ -243,3 +243,20 @@ def test_mfma_binner_edge_cases(gpu, case):
             assert np.abs(out).max() == 0, form
         else:
             assert O.rel_l2(out, ref) <= 1e-3, form
+
+
+def test_fused_stream_kernel_writes_identical_records(gpu):
+    """The streaming kernel fused with the first sort histogram (used once the
+    grid form is prepared) writes the same records, bit for bit, as the
+    plain streaming kernel."""
+    R = gpu
+    cfg = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "fusedrec")
+    h = R.generate_hits(configs.fls_scene(), 60, 30_000).to_device()
+    a = R.make_sensor(cfg)
+    a.set_sample_count(h.n)
+    a.write_samples(h)
+    b = R.make_sensor(cfg)
+    b.set_sample_count(h.n)
+    b.set_tile_shape(0)                 # prepares the grid form: the next write is fused
+    b.write_samples(h)
+    assert np.array_equal(a.samples().view(np.uint32), b.samples().view(np.uint32))
