@@ -103,7 +103,7 @@ __device__ __forceinline__ uint32_t group_key(const GroupKey& g, float t)
 // entry in either state, and both read as empty.
 __device__ __forceinline__ void windows_body(const BinArgs& a, uint32_t blk)
 {
-    const int tid = blk * 256 + threadIdx.x;
+    const int tid = blk * blockDim.x + threadIdx.x;
     if (tid >= (int)a.N) return;
     const uint2* seg = a.seg;
     auto empty = [&](int k) {
