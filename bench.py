@@ -328,7 +328,8 @@ def main():
     traffic = measured_traffic(args.workload, best, dominant)
     roof["traffic"] = traffic
     roof["kernel"] = {"stream": "k_stream",
-                      "sort": "k_sort_hist + k_sort_scan + k_sort_scatter (2 grouping-key passes)",
+                      "sort": "2 grouping-key passes: k_sort_scan + k_sort_scatter each, k_sort_hist "
+                              "for the second (k_stream made the first histogram)",
                       "bins": "k_windows", "deposit": "k_deposit",
                       "convolution": "k_correlate"}[dominant]
     roof["measured"] = "HIP events around the stage on the compute stream, mean over " \
