@@ -135,7 +135,7 @@ static int build(Batch* b)
     for (auto& en : b->entries) {
         Sensor* s = en.sensor;
         PingPlan p;
-        int rc = s->plan(en.form, &p);
+        int rc = s->plan(en.form, &p, false, true);
         if (rc) return rc;
         if (en.has_hits && en.nhits) {
             StreamArgs a = s->stream_args(en.hits, en.nhits, 0);

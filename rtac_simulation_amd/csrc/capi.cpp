@@ -278,7 +278,7 @@ int Sensor::check_plain_limits() const
     return RTSIM_OK;
 }
 
-int Sensor::plan(int form, PingPlan* p, bool introspect)
+int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
 {
     const uint32_t n = (uint32_t)count;
     const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
@@ -336,7 +336,7 @@ int Sensor::plan(int form, PingPlan* p, bool introspect)
             return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
                         rows.GH, N);
     }
-    hipError_t e = make_sort_plan(d_samples, n, sb, &gk, &p->sort);
+    hipError_t e = make_sort_plan(d_samples, n, sb, &gk, &p->sort, batched);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
 
     // Binner::compute (Binner.h:96-102): the sort bounded the segments, the

@@ -209,10 +209,10 @@ __device__ __forceinline__ uint32_t binner_key(bool grp, uint32_t c, uint32_t n)
 // MOVE: the records travel with their keys through every pass (staged in LDS,
 // written in digit runs), so the last pass reads them in order; otherwise the
 // last pass gathers them by sample index.
-template <bool FIRST, bool LAST, bool MOVE>
+template <bool FIRST, bool LAST, bool MOVE, bool BATCHED>
 __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t tile)
 {
-    constexpr uint32_t TILE = sort_pass_tile(LAST, MOVE);
+    constexpr uint32_t TILE = sort_pass_tile(LAST, MOVE, BATCHED);
     constexpr int LI = (int)TILE / kSortThreads;   // items per thread
     __shared__ uint32_t wcnt[kSortWaves][256];   // per-wave digit counts, then staging bases
     __shared__ uint32_t toff[256];               // digit run offsets in the staging tile
@@ -442,7 +442,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 template <bool FIRST, bool LAST, bool MOVE>
 __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
 {
-    sort_scatter_body<FIRST, LAST, MOVE>(a, blockIdx.x);
+    sort_scatter_body<FIRST, LAST, MOVE, false>(a, blockIdx.x);
 }
 
 template <bool FIRST, bool LAST, bool MOVE>
@@ -451,11 +451,11 @@ __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* _
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    sort_scatter_body<FIRST, LAST, MOVE>(args[p], blk);
+    sort_scatter_body<FIRST, LAST, MOVE, true>(args[p], blk);
 }
 
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
-                          const GroupKey* gk, SortPlan* p)
+                          const GroupKey* gk, SortPlan* p, bool batched)
 {
     GroupKey g{};
     if (gk && gk->on) g = *gk;
@@ -474,7 +474,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
     const bool clears = b.seg_count || b.fill_count;
     int cur = 0;   // buffer pair holding the current order (pass 0 reads the records)
     for (int pass = 0; pass < 4; pass++) {
-        const uint32_t tile = sort_pass_tile(pass == passes - 1, g.on);
+        const uint32_t tile = sort_pass_tile(pass == passes - 1, g.on, batched);
         const uint32_t tiles = (n + tile - 1) / tile;
         if (pass < passes && (tiles > b.tiles_cap || tiles > 256u * kScanPer)) return hipErrorInvalidValue;
         p->tile[pass] = tile;

@@ -86,13 +86,20 @@ struct GroupKey {
 inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }
 
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
-// Items per sort workgroup (256 lanes x 8 or x 4).  The last pass of a sort
-// whose records move uses the large tiles (twice the small tiles' run-bound
-// atomics cost more than the large tiles' lower parallelism), every other
-// pass the small ones (more workgroups hide the latency of their loads).
+// Items per sort workgroup (256 lanes x 8 or x 4).  The last pass of one
+// ping's sort whose records move uses the large tiles (at 1M samples its
+// tiles fit the chip's workgroup slots either way, and twice the small
+// tiles' run-bound atomics cost more than the large tiles' lower
+// parallelism), every other pass the small ones (more workgroups hide the
+// latency of their loads).  Batched launches use the small tiles in every
+// pass: a batch queues many workgroups per slot, and the small tiles' five
+// resident workgroups per CU (against two) finish them sooner.
 constexpr uint32_t kSortTile = 2048;
 constexpr uint32_t kSortTileSmall = 1024;
-constexpr uint32_t sort_pass_tile(bool last, bool move) { return last && move ? kSortTile : kSortTileSmall; }
+constexpr uint32_t sort_pass_tile(bool last, bool move, bool batched = false)
+{
+    return last && move && !batched ? kSortTile : kSortTileSmall;
+}
 // per-tile digit counts of one pass for `capacity` items
 inline size_t sort_count_words(size_t capacity)
 {
@@ -186,9 +193,10 @@ struct SortPlan {
     uint32_t pass_blocks[4];  // scan: 256 per pass, scatter: tiles (0 when n = 0)
     const uint32_t* order;    // permutation after the sort
 };
-// gk == nullptr or !gk->on: sort by travel
+// gk == nullptr or !gk->on: sort by travel.  batched: the plan is for the
+// batched launches (their tiles, sort_pass_tile)
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
-                          const GroupKey* gk, SortPlan* p);
+                          const GroupKey* gk, SortPlan* p, bool batched = false);
 // skip_hist0: the first pass's histogram (and its clears) are already made
 // (launch_stream_hist)
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s, bool skip_hist0 = false);

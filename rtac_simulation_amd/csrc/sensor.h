@@ -115,8 +115,9 @@ struct Sensor {
     int compute(int form, hipStream_t s);
     int grid_plan(int form, DepositArgs* d, CorrArgs* c);
     // validates the form; queues nothing.  introspect: the sort also writes
-    // the permutation and the Binner keys
-    int plan(int form, PingPlan* p, bool introspect = false);
+    // the permutation and the Binner keys.  batched: for the batched launches
+    // (their sort tiles)
+    int plan(int form, PingPlan* p, bool introspect = false, bool batched = false);
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
     ConvArgs conv_args() const;
