@@ -180,10 +180,13 @@ int rtsim_sensor_sorted_samples(rtsim_sensor h, const void** dev_ptr);
  * pointer may be NULL.  keys: sorted keys (n); segments: (start, end) per bin
  * (2N); windows: (start, size) per bin (2N); order: sort permutation (n);
  * overlap: binOverlap.  Synchronises the stream.  Compute keeps neither the
- * keys nor the permutation (nothing downstream reads them): asking for either
- * sorts the resident samples again on the stream with both kept (the sort is
- * deterministic, so every array it rewrites keeps its contents).  The samples
- * must be those of the last compute (RTSIM_ERR_INVALID after a change).        */
+ * keys nor the permutation (nothing downstream reads them), and an MFMA-form
+ * compute only marks the bins that hold samples (the windows' emptiness is all
+ * it reads): asking for the keys or the permutation, or for the segments or
+ * windows after an MFMA-form compute, sorts the resident samples again on the
+ * stream with all of them kept (the sort is deterministic, so every array it
+ * rewrites keeps its contents).  The samples must be those of the last
+ * compute (RTSIM_ERR_INVALID after a change).                                  */
 int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments,
                               uint32_t* windows, uint32_t* order, int* overlap,
                               void* stream);

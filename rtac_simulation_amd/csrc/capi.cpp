@@ -323,6 +323,10 @@ int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
     sb.out.key = key;
     sb.out.axis = axis;
     sb.out.seg = d_seg;
+    // the grid form reads the windows' emptiness alone: without introspection
+    // its sort marks the keys that hold samples instead of bounding their
+    // segments (two thirds of its run-bound atomics)
+    sb.out.seg_marks = grid && !introspect ? 1 : 0;
     // the grid form only needs samples grouped by bin and G row (its deposit
     // sums in exact fixed point, independent of the order inside a group)
     GroupKey gk{};
@@ -664,9 +668,11 @@ int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments
     HIP_TRY(hipStreamSynchronize(st));
     if (!s->computed) return fail(RTSIM_ERR_INVALID, "compute has not run since the last change");
     size_t n = s->count;
-    if ((keys || order) && n) {
+    const bool grid = s->computed_form == RTSIM_FORM_MFMA_BF16 || s->computed_form == RTSIM_FORM_MFMA_FP16;
+    if ((keys || order || (grid && (segments || windows))) && n) {
         // compute writes neither the keys nor the permutation (nothing
-        // downstream reads them): sort the same samples again with both kept,
+        // downstream reads them), and the grid form only marks the keys that
+        // hold samples: sort the same samples again with all of them kept,
         // then the windows (which also write empty segments back as (0, 0)).
         // Both are deterministic, so every array they rewrite (sorted
         // records, coordinates, segments, windows, row table) gets the same

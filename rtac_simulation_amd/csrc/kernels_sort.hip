@@ -377,6 +377,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
         uint32_t* rows = grp ? a.gk.rows.row_start : nullptr;
         __syncthreads();
         uint32_t* seg = reinterpret_cast<uint32_t*>(a.out.seg);
+        const bool marks = a.out.seg_marks != 0;
         const uint32_t q0 = t * (uint32_t)LI;
         if (q0 < tn) {
             uint32_t kk[LI + 2], cc[LI + 2];   // items q0 - 1 .. q0 + LI
@@ -405,10 +406,15 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
                 const int g = (int)(kk[j + 1] - c) - 2;
                 const bool gstart = rows && g >= -1 && g <= a.gk.rows.GH + 1 &&
                                     (first || (int)(kk[j] - cc[j]) - 2 != g);
-                if (kstart || kend || gstart) {
+                if (marks ? kstart || gstart : kstart || kend || gstart) {
                     const uint32_t pos = gpos[d] + (q - toff[d]);
-                    if (kstart) atomicMin(&seg[2 * k], pos);
-                    if (kend) atomicMax(&seg[2 * k + 1], pos + 1);
+                    if (marks) {
+                        // every tile holding key k stores the same mark
+                        if (kstart) reinterpret_cast<uint2*>(seg)[k] = make_uint2(0u, 1u);
+                    } else {
+                        if (kstart) atomicMin(&seg[2 * k], pos);
+                        if (kend) atomicMax(&seg[2 * k + 1], pos + 1);
+                    }
                     if (gstart) atomicMin(&rows[g + 1], pos);
                 }
             }

@@ -125,6 +125,9 @@ struct SortOutputs {
     KeyParams key;
     int axis;             // 1: coordinate = sin(bearing) in FP64, else the bearing
     uint2* seg;           // [key.n], initialised to (~0, 0) by the first pass
+    // 1: seg only marks the keys that hold samples, as (0, 1) (a window's
+    // emptiness is all the grid form reads); 0: the Binner segments
+    int seg_marks;
 };
 struct SortBuffers {
     uint2* kv[2];         // (key, sample index) ping-pong between passes
