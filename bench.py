@@ -216,7 +216,7 @@ def bench_batch(R, configs, stream, steps):
     b = R.Batch(sensors, forms)
     for i, (cfg, scene, S, seed) in enumerate(items):
         b.bind_hits(i, big[cfg.kind].prefix(S))
-    b.compute(stream)                       # capture + first replay
+    b.autotune(2, stream)                   # tile shapes per sensor class, outside the timed region
     stream.sync()
     timer = R.Timer()
     timer.start(stream)
@@ -231,41 +231,34 @@ def bench_batch(R, configs, stream, steps):
             "msamples_per_s": total / (ms * 1e-3) / 1e6, "steps": steps}
 
 
-def bench_repeat(R, cfg, dhits, S, tile, stream, pings=8, steps=10):
+def bench_repeat(R, cfg, dhits, S, stream, pings=8, steps=10):
     """`pings` pings of the headline workload from one graph launch (the
     multi-ping driver's path): one sensor instance per ping, all bound to the
-    same resident hits.  A full batch has workgroups enough for the smaller
-    32x32 correlation tiles with 4 waves, so that pair is timed beside the
-    single ping's tuned (shape, waves) and the faster one is reported."""
-    best = None
-    for shape in ([tuple(tile)] if tile else []) + [(0, 4)]:
-        sensors = []
-        for _ in range(pings):
-            s = R.make_sensor(cfg)
-            s.set_sample_count(S)
-            s.set_tile_shape(shape[0])
-            s.set_tile_waves(shape[1])
-            sensors.append(s)
-        b = R.Batch(sensors, [R.FORM_MFMA_FP16] * pings)
-        for i in range(pings):
-            b.bind_hits(i, dhits)
-        b.compute(stream)                   # capture + first replay
-        stream.sync()
-        timer = R.Timer()
-        timer.start(stream)
-        for _ in range(steps):
-            b.compute(stream)
-        timer.stop(stream)
-        stream.sync()
-        ms = timer.elapsed_ms() / steps
-        if best is None or ms < best[0]:
-            best = (ms, shape)
-        del b, sensors
-    ms, shape = best
+    same resident hits, tile shape and wave count from the batch autotuner
+    (a full batch can prefer other tiles than one ping alone)."""
+    sensors = []
+    for _ in range(pings):
+        s = R.make_sensor(cfg)
+        s.set_sample_count(S)
+        sensors.append(s)
+    b = R.Batch(sensors, [R.FORM_MFMA_FP16] * pings)
+    for i in range(pings):
+        b.bind_hits(i, dhits)
+    b.autotune(5, stream)                   # outside the timed region
+    stream.sync()
+    timer = R.Timer()
+    timer.start(stream)
+    for _ in range(steps):
+        b.compute(stream)
+    timer.stop(stream)
+    stream.sync()
+    ms = timer.elapsed_ms() / steps
+    shape = sensors[0].tile_shape()[0]
     return {"workload": f"{pings} pings of {cfg.W}x{cfg.H}, {S} samples each, one hipGraph launch "
-                        "(MFMA FP16)", "pings": pings, "tile_shape": shape[0], "tile_waves": shape[1],
-            "ms_per_batch": ms, "pings_per_s": pings / (ms * 1e-3),
-            "msamples_per_s": pings * S / (ms * 1e-3) / 1e6, "steps": steps}
+                        "(MFMA FP16)", "pings": pings, "tile_shape": shape,
+            "tile_waves": sensors[0].tile_waves(), "ms_per_batch": ms,
+            "pings_per_s": pings / (ms * 1e-3), "msamples_per_s": pings * S / (ms * 1e-3) / 1e6,
+            "steps": steps}
 
 
 def bench_large(R, configs, stream, steps):
@@ -379,7 +372,7 @@ def main():
 
     extra = {}
     if not args.no_extra:
-        extra["batch_same"] = bench_repeat(R, cfg, dhits, S, tiles.get("mfma_fp16"), stream)
+        extra["batch_same"] = bench_repeat(R, cfg, dhits, S, stream)
         print(f"# repeated {extra['batch_same']}", file=sys.stderr, flush=True)
         extra["batch_uneven"] = bench_batch(R, configs, stream, 3)
         print(f"# batch {extra['batch_uneven']}", file=sys.stderr, flush=True)

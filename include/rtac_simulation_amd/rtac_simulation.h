@@ -382,6 +382,14 @@ class Batch {
         check(rtsim_batch_bind_hits(h_, index, hits, n));
     }
     void compute(void* stream = nullptr) { check(rtsim_batch_compute(h_, stream)); }
+    // tile shape and wave count per class of MFMA sensors, tuned on the whole
+    // batch; returns its mean replay time in ms
+    float autotune(int iterations = 5, void* stream = nullptr)
+    {
+        float ms = 0.f;
+        check(rtsim_batch_autotune(h_, iterations, stream, &ms));
+        return ms;
+    }
 
    private:
     std::vector<SensorInstance::Ptr> sensors_;

@@ -34,6 +34,16 @@ int rtsim_batch_create(const rtsim_sensor* sensors, const int* forms, uint32_t c
 int rtsim_batch_bind_hits(rtsim_batch batch, uint32_t index, const rtsim_hits* hits, size_t n);
 /* Queues the whole batch on `stream` (one graph launch).                       */
 int rtsim_batch_compute(rtsim_batch batch, void* stream);
+/* Tile autotuner of the batch (no counterpart in the reference, whose pings
+ * are computed one at a time): for each class of MFMA-form sensors (same
+ * grid and PSF size) in turn, times `iterations` graph replays of the whole
+ * batch with every correlation tile shape and wave count that fits the class,
+ * keeps the fastest (the other classes held at their current choice) and sets
+ * it on those sensors (rtsim_sensor_set_tile_shape / _waves).  A full batch
+ * can prefer other tiles than one ping alone (rtsim_sensor_autotune).  Leaves
+ * every ping computed with the chosen shapes; best_ms (may be NULL) receives
+ * the batch's mean replay time.  Synchronises the stream.                      */
+int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float* best_ms);
 int rtsim_batch_destroy(rtsim_batch batch);
 
 #ifdef __cplusplus

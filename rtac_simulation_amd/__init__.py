@@ -156,6 +156,7 @@ _SIGNATURES = [
      [C.POINTER(C.c_void_p), C.POINTER(C.c_int), C.c_uint32, C.POINTER(C.c_void_p)]),
     ("rtsim_batch_bind_hits", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Hits), C.c_size_t]),
     ("rtsim_batch_compute", C.c_int, [C.c_void_p, C.c_void_p]),
+    ("rtsim_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_float)]),
     ("rtsim_batch_destroy", C.c_int, [C.c_void_p]),
     # ping files (rtsim_sink.h)
     ("rtsim_sink_open", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
@@ -623,6 +624,14 @@ class Batch:
 
     def compute(self, stream: "Stream | None" = None):
         check(lib().rtsim_batch_compute(self.handle, stream.handle if stream else None))
+
+    def autotune(self, iterations: int = 5, stream: "Stream | None" = None) -> float:
+        """Per class of MFMA-form sensors, the fastest tile shape and wave
+        count for the whole batch (set on the sensors); the batch's replay ms."""
+        ms = C.c_float()
+        check(lib().rtsim_batch_autotune(self.handle, iterations, stream.handle if stream else None,
+                                         C.byref(ms)))
+        return ms.value
 
     def __del__(self):
         try:

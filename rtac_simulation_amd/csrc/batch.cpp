@@ -352,6 +352,82 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
     return RTSIM_OK;
 }
 
+// Mean milliseconds of one replay of the batch's graph (captured first when
+// out of date).
+static int time_batch(Batch* b, int iterations, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms)
+{
+    int rc = rtsim_batch_compute(reinterpret_cast<rtsim_batch>(b), st);   // capture + warm
+    if (rc) return rc;
+    hipError_t e = hipEventRecord(e0, st);
+    for (int it = 0; it < iterations && e == hipSuccess; it++) e = hipGraphLaunch(b->exec, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
+    if (e != hipSuccess) return hip_fail(e, "batch autotune");
+    *ms /= (float)iterations;
+    return RTSIM_OK;
+}
+
+int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float* best_ms)
+{
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return fail(RTSIM_ERR_INVALID, "batch is NULL");
+    if (iterations < 1) iterations = 1;
+    hipStream_t st = (hipStream_t)stream;
+    // MFMA-form sensors by class (grid and PSF size); one class at a time,
+    // the others held at their current choice
+    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::vector<Sensor*>> classes;
+    for (auto& en : b->entries)
+        if (en.form == RTSIM_FORM_MFMA_BF16 || en.form == RTSIM_FORM_MFMA_FP16) {
+            Sensor* s = en.sensor;
+            classes[std::make_tuple((uint32_t)s->N, (uint32_t)s->W, (uint32_t)s->Tr, (uint32_t)s->Tb)]
+                .push_back(s);
+        }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    }
+    float bms = 0.f;
+    int rc = time_batch(b, iterations, st, e0, e1, &bms);
+    for (auto it = classes.begin(); rc == RTSIM_OK && it != classes.end(); ++it) {
+        std::vector<Sensor*>& ss = it->second;
+        int best_shape = 0, best_waves = 4;
+        (void)rtsim_sensor_tile_waves(reinterpret_cast<rtsim_sensor>(ss[0]), &best_waves);
+        int r0 = 0, c0 = 0;
+        (void)rtsim_sensor_tile_shape(reinterpret_cast<rtsim_sensor>(ss[0]), &best_shape, &r0, &c0);
+        auto apply = [&](int shape, int waves) {
+            for (Sensor* s : ss) {
+                const rtsim_sensor h = reinterpret_cast<rtsim_sensor>(s);
+                if (rtsim_sensor_set_tile_shape(h, shape) || rtsim_sensor_set_tile_waves(h, waves))
+                    return false;
+            }
+            return true;
+        };
+        for (int v = 0; v < 2 * kCorrShapes && rc == RTSIM_OK; v++) {
+            const int shape = v % kCorrShapes, waves = v < kCorrShapes ? 4 : 8;
+            if (shape == best_shape && waves == best_waves) continue;
+            float ms = 0.f;
+            if (!apply(shape, waves)) continue;   // does not fit some sensor's LDS budget
+            if ((rc = time_batch(b, iterations, st, e0, e1, &ms))) break;
+            if (ms < bms) {
+                bms = ms;
+                best_shape = shape;
+                best_waves = waves;
+            }
+        }
+        if (!apply(best_shape, best_waves) && rc == RTSIM_OK)
+            rc = fail(RTSIM_ERR_HIP, "batch autotune could not restore a tile shape");
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    if (best_ms) *best_ms = bms;
+    // leave every ping consistent with the chosen shapes
+    return rtsim_batch_compute(batch, stream);
+}
+
 int rtsim_batch_destroy(rtsim_batch batch)
 {
     delete reinterpret_cast<Batch*>(batch);

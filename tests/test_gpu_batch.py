@@ -57,6 +57,38 @@ def test_batch_graph_matches_eager(gpu):
             assert O.rel_l2(s.output(), ping) < 1e-6, rep
 
 
+def test_batch_autotune_keeps_batched_equal_to_eager(gpu):
+    """The batch autotuner sets a tile shape per sensor class; the batched
+    pings it leaves must still equal each sensor's own compute with that shape
+    (binning bit-exact, pings to FP32 summation order), and the MFMA pings stay
+    within the form's bound of the oracle."""
+    R = gpu
+    items = _small_batch(R)
+    sensors, hits, forms = [], [], []
+    for k, (cfg, scene, n, form) in enumerate(items):
+        s = R.make_sensor(cfg)
+        s.set_sample_count(n)
+        sensors.append(s)
+        hits.append(R.generate_hits(scene, 60 + k, n).to_device())
+        forms.append(form)
+    b = R.Batch(sensors, forms)
+    for k, h in enumerate(hits):
+        b.bind_hits(k, h)
+    stream = R.Stream()
+    ms = b.autotune(2, stream)
+    assert ms > 0.0
+    stream.sync()
+    batched = [(s.output(), s.binner_debug()["order"]) for s in sensors]
+    for (ping, order), s, h, f, (cfg, _, n, _) in zip(batched, sensors, hits, forms, items):
+        s.write_samples(h)
+        s.compute_output(f)
+        assert np.array_equal(s.binner_debug()["order"], order)
+        assert O.rel_l2(s.output(), ping) < 1e-6
+        if f != R.FORM_PLAIN_FP32:
+            ref, _ = O.reference_ping(cfg, s.samples())
+            assert O.rel_l2(ping, ref) < 1e-3
+
+
 def test_batch_recaptures_after_resize(gpu):
     R = gpu
     cfg = configs._fls(48, 96, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "r")
