@@ -86,6 +86,9 @@ class SensorInfo {
 
     // explicit description (copies every table)
     static Ptr Create(const rtsim_sensor_desc& d) { return Ptr(new SensorInfo(d)); }
+    // the description points into this object: not copyable
+    SensorInfo(const SensorInfo&) = delete;
+    SensorInfo& operator=(const SensorInfo&) = delete;
 
     const rtsim_sensor_desc& desc() const { return desc_; }
     bool is_2d() const { return desc_.kind == RTSIM_SENSOR_2D; }
@@ -107,10 +110,19 @@ class SensorInfo {
             dir_.assign(d.directivity, d.directivity + (size_t)d.dir_width * d.dir_height);
             desc_.directivity = dir_.data();
         }
+        if (d.waveform) {
+            wave_ = *d.waveform;
+            if (wave_.beam) {
+                beam_.assign(wave_.beam, wave_.beam + wave_.beam_taps);
+                wave_.beam = beam_.data();
+            }
+            desc_.waveform = &wave_;
+        }
     }
     rtsim_info info_ = nullptr;
     rtsim_sensor_desc desc_{};
-    std::vector<float> psf_, bearings_, dir_;
+    rtsim_waveform_sine wave_{};
+    std::vector<float> psf_, bearings_, dir_, beam_;
 };
 using SensorInfo2D = SensorInfo;
 
@@ -648,9 +660,16 @@ class SyntheticSimulation {
             sl.sensor->set_samples(sl.hits(), samples_);
             sl.sensor->compute_output(form_);
         } else {
-            Batch batch(active, forms);
-            for (size_t k = 0; k < active.size(); k++) batch.bind_hits((uint32_t)k, &slots_[k].hits_, samples_);
-            batch.compute();
+            // one batch per number of pings in flight, kept across calls: the
+            // same sensors and hit buffers replay its graph (new poses are
+            // rewritten in place, no re-capture)
+            if (!batch_ || batch_size_ != active.size()) {
+                batch_.reset();
+                batch_.reset(new Batch(active, forms));
+                batch_size_ = active.size();
+            }
+            for (size_t k = 0; k < active.size(); k++) batch_->bind_hits((uint32_t)k, &slots_[k].hits_, samples_);
+            batch_->compute();
         }
         for (const auto& s : active)
             for (const auto& sink : sinks_) sink->set_output(s);
@@ -691,6 +710,8 @@ class SyntheticSimulation {
     Form form_;
     uint64_t seed_, ping_ = 0;
     std::vector<Slot> slots_;
+    std::unique_ptr<Batch> batch_;   // declared after the slots: destroyed first
+    size_t batch_size_ = 0;
     PoseSource::Ptr poses_;
     Emitter::Ptr emitter_;
     std::vector<FileSink::Ptr> sinks_;

@@ -56,7 +56,32 @@ extern "C" {
 
 #define RTSIM_KEY_OUT_OF_RANGE 0xFFFFFFFFu /* Binner::Key::OutOfRange */
 
+/* Largest sample count of one sensor: the sort scans at most 16384 tiles of
+ * 1024 samples per pass (2^24 samples); set_sample_count refuses more.       */
+#define RTSIM_MAX_SAMPLES 16777216u
+
 typedef struct rtsim_sensor_s* rtsim_sensor;
+
+/*
+ * Sine waveform recipe: Waveform_Sine (Waveform.h:96-138) times the bearing
+ * taps of the beam directivity, as SensorInstance{1D,2D}::generate_psf_data
+ * (SensorInstance2D.cu:20-33, SensorInstance1D.cu:15-25) tabulates them.
+ * A sensor created with a recipe keeps it: at creation and at every
+ * rtsim_sensor_set_ranges it rebuilds what SensorInstance::set_ranges
+ * (SensorInstance.cu:23-34) rebuilds -- an adaptive waveform's duration
+ * tau = resolution / c (resolution = (max - min) / (count - 1) in FP32), the
+ * range taps exp(i pi n / oversampling) for n < floor((2 f oversampling) tau),
+ * the PSF table wave[h] * beam[w], its range span c tau and (unless a margin
+ * is given) the Binner margin c tau.  A fixed waveform keeps its duration.
+ */
+typedef struct {
+    float frequency;            /* Hz                                              */
+    float duration;             /* s; replaced by resolution / c when not fixed    */
+    int fixed;                  /* Waveform::fixed()                               */
+    uint32_t oversampling;      /* samples per half period (reference default 8)   */
+    const float* beam;          /* bearing taps (real); NULL with beam_taps = 1 = 1 */
+    uint32_t beam_taps;         /* 1 for a 1D sensor                               */
+} rtsim_waveform_sine;
 
 /*
  * Sensor description: what SensorInfo / SensorInfo2D (SensorInfo.h:18-105),
@@ -88,6 +113,10 @@ typedef struct {
     float sound_speed;          /* m/s                                             */
     int spreading;              /* RTSIM_SPREAD_*                                  */
     float rotation[9];          /* receiver pose R, row-major; local = R^T dir     */
+    /* NULL: the PSF taps above are fixed.  Else the taps, their range span and a */
+    /* negative margin's c * tau follow the recipe, and psf, psf_range_taps,       */
+    /* psf_bearing_taps and psf_range_span are ignored (psf_bearing_span is not).  */
+    const rtsim_waveform_sine* waveform;
 } rtsim_sensor_desc;
 
 /* Ray hits handed to the streaming kernel (SoA, device pointers). */
@@ -113,7 +142,9 @@ int rtsim_sensor_create(const rtsim_sensor_desc* desc, rtsim_sensor* out);
 int rtsim_sensor_destroy(rtsim_sensor h);
 
 /* SensorInstance::set_ranges (SensorInstance.h:56-58, SensorInstance.cu:23-34):
- * new output ranges and Binner margin (< 0 keeps c * tau).                         */
+ * new output ranges and Binner margin (< 0 selects c * tau).  A sensor with a
+ * waveform recipe also re-derives the duration (adaptive), the PSF taps and
+ * their range span; explicit taps are kept as given.                            */
 int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
                             uint32_t range_count, float margin);
 
@@ -123,7 +154,8 @@ int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
 int rtsim_sensor_set_pose(rtsim_sensor h, const float* rotation);
 
 /* SensorInstance{1D,2D}::set_sample_count / sample_count
- * (SensorInstance1D.h:47-48, SensorInstance2D.h:59-60).                            */
+ * (SensorInstance1D.h:47-48, SensorInstance2D.h:59-60).  At most
+ * RTSIM_MAX_SAMPLES (RTSIM_ERR_INVALID beyond).                                   */
 int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count);
 int rtsim_sensor_sample_count(rtsim_sensor h, size_t* count);
 

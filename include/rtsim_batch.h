@@ -30,9 +30,14 @@ typedef struct rtsim_batch_s* rtsim_batch;
 int rtsim_batch_create(const rtsim_sensor* sensors, const int* forms, uint32_t count,
                        rtsim_batch* out);
 /* Includes the streaming kernel (rtsim_receiver_write_samples) of sensor
- * `index` in the batch: n must equal its sample count.  hits == NULL unbinds. */
+ * `index` in the batch: n must equal its sample count.  hits == NULL unbinds.
+ * Binding the same arrays again keeps the captured graph.                     */
 int rtsim_batch_bind_hits(rtsim_batch batch, uint32_t index, const rtsim_hits* hits, size_t n);
-/* Queues the whole batch on `stream` (one graph launch).                       */
+/* Queues the whole batch on `stream` (one graph launch).  A receiver pose set
+ * since the last compute (rtsim_sensor_set_pose) is copied into that ping's
+ * streaming arguments on `stream` before the launch, without re-capture; the
+ * graph is re-captured after a sensor's ranges, sample count, tile shape or
+ * bound hit arrays change.                                                    */
 int rtsim_batch_compute(rtsim_batch batch, void* stream);
 /* Tile autotuner of the batch (no counterpart in the reference, whose pings
  * are computed one at a time): for each class of MFMA-form sensors (same
@@ -44,6 +49,7 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream);
  * every ping computed with the chosen shapes; best_ms (may be NULL) receives
  * the batch's mean replay time.  Synchronises the stream.                      */
 int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float* best_ms);
+/* Waits for the batch's last launch, then frees it.                            */
 int rtsim_batch_destroy(rtsim_batch batch);
 
 #ifdef __cplusplus

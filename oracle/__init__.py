@@ -55,6 +55,7 @@ _SIGS = {
                        C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "orc_conv2d": (None, [_P, _P, C.POINTER(Grid), _P, _P, _P, C.c_int]),
     "orc_conv1d": (None, [_P, _P, C.POINTER(Grid), _P, _P, C.c_int]),
+    "orc_conv2d_rows": (None, [_P, _P, C.POINTER(Grid), _P, _P, _P, C.c_size_t, _P, C.c_int]),
     "orc_grid2d": (None, [_P, C.c_size_t, C.POINTER(Key), _P, C.POINTER(Grid), C.c_double,
                           C.c_double, _P, _P]),
     "cpu_binning": (None, [_P, C.c_size_t, C.POINTER(Grid), C.c_float, _P, _P, C.c_int]),
@@ -247,6 +248,22 @@ def conv(cfg, sorted_records: np.ndarray, win: np.ndarray, threads: int | None =
         return out.view(np.complex128).reshape(cfg.H, cfg.W)
     lib().orc_conv1d(_p(srt), _p(w), C.byref(g), _p(taps), _p(out), threads or nthreads())
     return out.view(np.complex128)
+
+
+def conv_rows(cfg, sorted_records: np.ndarray, win: np.ndarray, rows,
+              threads: int | None = None) -> np.ndarray:
+    """FP64 rows (len(rows) x W, complex128) of the 2D plain-form ping: the
+    rows of `conv`, each sample visiting only the bearings in the PSF support."""
+    assert cfg.kind == 2
+    srt = np.ascontiguousarray(sorted_records, np.float32).reshape(-1, 4)
+    w = np.ascontiguousarray(win, np.uint32).reshape(-1)
+    r = np.ascontiguousarray(rows, np.uint32)
+    assert r.size == 0 or int(r.max()) < cfg.H
+    out = np.empty(2 * r.size * cfg.W, np.float64)
+    b = np.ascontiguousarray(cfg.bearings, np.float32)
+    lib().orc_conv2d_rows(_p(srt), _p(w), C.byref(grid(cfg)), _p(b), _p(_taps(cfg)), _p(r), r.size,
+                          _p(out), threads or nthreads())
+    return out.view(np.complex128).reshape(r.size, cfg.W)
 
 
 def grid_form(cfg, records: np.ndarray, win: np.ndarray, bearing0: float, pitch: float):

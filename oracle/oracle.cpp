@@ -370,6 +370,53 @@ void orc_conv2d(const float* sorted, const uint32_t* win, const orc_grid* g,
 }
 
 // --------------------------------------------------------------------------
+// orc_conv2d on a list of output rows (out: nrows x W), for pings whose full
+// gather is too long: each window sample visits only the bearings inside the
+// PSF's bearing support |x_s - x_b| < (Tb + 1)/2 * span_b / Tb (the border
+// address mode makes every other term exactly 0), and every cell still sums
+// its terms in window order, so the rows equal orc_conv2d's.
+// --------------------------------------------------------------------------
+void orc_conv2d_rows(const float* sorted, const uint32_t* win, const orc_grid* g,
+                     const float* bearings, const float* taps, const uint32_t* rows,
+                     size_t nrows, double* out, int nthreads)
+{
+    const long W = g->bearing_count;
+    std::vector<double> bc(W);
+    for (long b = 0; b < W; b++) bc[b] = bearing_coord(g->axis, bearings[b]);
+    // the support bound, widened a little: terms just inside it are evaluated
+    // by orc_psf like any other
+    const double supp = 0.5 * (g->psf_bearing_taps + 1) * (double)g->psf_bearing_span /
+                        g->psf_bearing_taps * (1.0 + 1e-6);
+    const bool ascending = std::is_sorted(bc.begin(), bc.end());
+    parallel_rows(nthreads, 0, (long)nrows, [&](long i0, long i1) {
+        std::vector<double> acc(2 * W);
+        for (long i = i0; i < i1; i++) {
+            const long r = rows[i];
+            const double rr = out_range(g, r);
+            std::fill(acc.begin(), acc.end(), 0.0);
+            const uint32_t st = win[2 * r], sz = win[2 * r + 1];
+            for (uint32_t k = 0; k < sz; k++) {
+                const float* s = sorted + 4 * (size_t)(st + k);
+                const double x = bearing_coord(g->axis, s[3]);
+                long b0 = 0, b1 = W;
+                if (ascending) {
+                    b0 = std::lower_bound(bc.begin(), bc.end(), x - supp) - bc.begin();
+                    b1 = std::upper_bound(bc.begin(), bc.end(), x + supp) - bc.begin();
+                }
+                for (long b = b0; b < b1; b++) {
+                    double kr, ki;
+                    orc_psf(taps, g->psf_range_taps, g->psf_bearing_taps, g->psf_range_span,
+                            g->psf_bearing_span, (double)s[2] - rr, x - bc[b], &kr, &ki);
+                    acc[2 * b] += (double)s[0] * kr - (double)s[1] * ki;
+                    acc[2 * b + 1] += (double)s[0] * ki + (double)s[1] * kr;
+                }
+            }
+            std::copy(acc.begin(), acc.end(), out + 2 * (size_t)i * W);
+        }
+    });
+}
+
+// --------------------------------------------------------------------------
 // do_sparse_convolve_1d_f (SensorInstance1D.cu:35-72) in FP64 with the
 // intended PSF scaling 1/(c tau) (the reference writes 1.0f/duration*c,
 // SensorInstance1D.cu:30, which is c/tau: a documented deviation).

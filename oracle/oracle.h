@@ -26,6 +26,9 @@
 //                                 linearity, numpy brute force O(S W H)
 //   orc_grid2d                    numpy correlate2d of the deposited grid
 //                                 and equality with orc_conv2d
+//   orc_conv2d_rows               equality with orc_conv2d on the rows it
+//                                 computes (it skips only bearings outside
+//                                 the PSF support, whose terms are exact 0)
 //
 // Not pinnable here (documented deviations): NVIDIA's 8-bit texture filter
 // weights, thrust's tie order, the reference 1D PSF scaling c/tau and the
@@ -89,6 +92,9 @@ void orc_conv2d(const float* sorted, const uint32_t* win, const orc_grid* g,
                 const float* bearings, const float* taps, double* out, int nthreads);
 void orc_conv1d(const float* sorted, const uint32_t* win, const orc_grid* g,
                 const float* taps, double* out, int nthreads);
+void orc_conv2d_rows(const float* sorted, const uint32_t* win, const orc_grid* g,
+                     const float* bearings, const float* taps, const uint32_t* rows,
+                     size_t nrows, double* out, int nthreads);
 void orc_grid2d(const float* samples, size_t n, const orc_key* key,
                 const uint32_t* win, const orc_grid* g, double bearing0,
                 double bearing_pitch, const float* taps, double* out);

@@ -38,6 +38,12 @@ class RtsimError(RuntimeError):
         self.code = code
 
 
+class WaveformSine(C.Structure):
+    _fields_ = [("frequency", C.c_float), ("duration", C.c_float), ("fixed", C.c_int),
+                ("oversampling", C.c_uint32), ("beam", C.POINTER(C.c_float)),
+                ("beam_taps", C.c_uint32)]
+
+
 class SensorDesc(C.Structure):
     _fields_ = [
         ("kind", C.c_int),
@@ -54,6 +60,7 @@ class SensorDesc(C.Structure):
         ("frequency", C.c_float), ("sound_speed", C.c_float),
         ("spreading", C.c_int),
         ("rotation", C.c_float * 9),
+        ("waveform", C.POINTER(WaveformSine)),
     ]
 
 
@@ -669,6 +676,12 @@ def config_from_yaml(path) -> "SensorConfig":
                                                                               d.dir_width)
             cfg.dir_xscale, cfg.dir_yscale = d.dir_xscale, d.dir_yscale
         cfg.rotation = np.array(list(d.rotation), np.float32).reshape(3, 3)
+        if d.waveform:
+            w = d.waveform.contents
+            beam = np.ctypeslib.as_array(w.beam, (w.beam_taps,)).copy() if w.beam else None
+            cfg.waveform = {"frequency": w.frequency, "duration": w.duration,
+                            "fixed": bool(w.fixed), "oversampling": int(w.oversampling),
+                            "beam": beam}
         return cfg
     finally:
         lib().rtsim_info_destroy(info)

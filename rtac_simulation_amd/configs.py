@@ -50,6 +50,9 @@ class SensorConfig:
     spreading: int = 0
     rotation: np.ndarray = field(default_factory=lambda: np.eye(3, dtype=np.float32))
     name: str = ""
+    # sine waveform recipe (rtsim_waveform_sine): {frequency, duration, fixed,
+    # oversampling, beam}; None: the taps in `psf` are fixed
+    waveform: dict | None = None
 
     # keep ctypes views alive while a desc is in use
     _keep: list = field(default_factory=list, repr=False)
@@ -101,6 +104,21 @@ class SensorConfig:
         d.frequency, d.sound_speed, d.spreading = self.frequency, self.sound_speed, self.spreading
         for i, v in enumerate(np.asarray(self.rotation, np.float32).ravel()):
             d.rotation[i] = float(v)
+        if self.waveform is not None:
+            from . import WaveformSine
+            w = WaveformSine()
+            wf = self.waveform
+            w.frequency, w.duration = wf["frequency"], wf["duration"]
+            w.fixed, w.oversampling = int(bool(wf["fixed"])), int(wf["oversampling"])
+            if wf.get("beam") is not None:
+                bm = np.ascontiguousarray(wf["beam"], np.float32)
+                keep.append(bm)
+                w.beam = bm.ctypes.data_as(C.POINTER(C.c_float))
+                w.beam_taps = bm.size
+            else:
+                w.beam_taps = 1
+            keep.append(w)
+            d.waveform = C.pointer(w)
         self._keep = keep
         return d
 

@@ -33,6 +33,8 @@ struct BatchEntry {
     rtsim_hits hits{};
     size_t nhits = 0;
     uint64_t generation = 0;
+    uint64_t pose_version = 0;
+    int stream_index = -1;   // its entry in the streaming stage (-1: no bound hits)
 };
 
 // Host image of one batched launch: the arguments of the pings it covers and
@@ -63,6 +65,11 @@ struct Batch {
     bool dirty = true;
     void* d_desc = nullptr;
     size_t desc_cap = 0;
+    // recorded after every graph launch: the graph, its descriptors and the
+    // pinned pose staging are only rewritten or freed once it has completed
+    hipEvent_t done = nullptr;
+    bool launched = false;
+    StreamArgs* h_pose = nullptr;   // pinned staging of rewritten streaming arguments
 
     Stage<StreamArgs> stream;
     Stage<HistArgs> hist[4][2];   // per pass: small tiles, large tiles
@@ -75,8 +82,19 @@ struct Batch {
     Stage<DepositArgs> deposit;
     std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape, waves per workgroup)
 
+    // waits for the last launch (a no-op when none is in flight)
+    hipError_t wait()
+    {
+        if (!launched) return hipSuccess;
+        launched = false;
+        return hipEventSynchronize(done);
+    }
+
     ~Batch()
     {
+        (void)wait();
+        if (done) (void)hipEventDestroy(done);
+        if (h_pose) (void)hipHostFree(h_pose);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
@@ -137,9 +155,12 @@ static int build(Batch* b)
         PingPlan p;
         int rc = s->plan(en.form, &p, false, true);
         if (rc) return rc;
+        en.stream_index = -1;
         if (en.has_hits && en.nhits) {
             StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
-            b->stream.add(a, stream_blocks(a));
+            const uint32_t blocks = stream_blocks(a);
+            if (blocks) en.stream_index = (int)b->stream.args.size();
+            b->stream.add(a, blocks);
         }
         for (int k = 0; k < p.sort.passes; k++) {
             b->hist[k][p.sort.tile[k] == kSortTile ? 1 : 0].add(p.sort.hist[k], p.sort.hist_blocks[k]);
@@ -171,6 +192,7 @@ static int build(Batch* b)
             s->last_corr = p.corr;
         }
         en.generation = s->generation;
+        en.pose_version = s->pose_version;
     }
     // one device buffer for every descriptor array and prefix table
     size_t off = 0;
@@ -248,6 +270,8 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
 
 static int capture(Batch* b)
 {
+    hipError_t we = b->wait();   // the old graph and descriptors may be in flight
+    if (we != hipSuccess) return hip_fail(we, "waiting for the previous batch launch");
     if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
     if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
     int rc = build(b);
@@ -301,6 +325,12 @@ int rtsim_batch_create(const rtsim_sensor* sensors, const int* forms, uint32_t c
     }
     hipError_t e = hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return hip_fail(e, "hipStreamCreate"); }
+    e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) { delete b; return hip_fail(e, "hipEventCreate"); }
+    if (count) {
+        e = hipHostMalloc((void**)&b->h_pose, sizeof(StreamArgs) * count, hipHostMallocDefault);
+        if (e != hipSuccess) { delete b; return hip_fail(e, "hipHostMalloc"); }
+    }
     *out = reinterpret_cast<rtsim_batch>(b);
     return RTSIM_OK;
 }
@@ -311,6 +341,7 @@ int rtsim_batch_bind_hits(rtsim_batch batch, uint32_t index, const rtsim_hits* h
     if (!b || index >= b->entries.size()) return fail(RTSIM_ERR_INVALID, "bad batch/index");
     BatchEntry& en = b->entries[index];
     if (!hits) {
+        if (en.has_hits) b->dirty = true;
         en.has_hits = false;
     } else {
         if (!hits->range || !hits->dir_x || !hits->dir_y || !hits->dir_z || !hits->amplitude)
@@ -318,11 +349,13 @@ int rtsim_batch_bind_hits(rtsim_batch batch, uint32_t index, const rtsim_hits* h
         if (n != en.sensor->count)
             return fail(RTSIM_ERR_INVALID, "bound hits (%zu) must equal the sample count (%zu)", n,
                         en.sensor->count);
+        // the same arrays again: the captured graph already reads them
+        if (!en.has_hits || en.nhits != n || std::memcmp(&en.hits, hits, sizeof(rtsim_hits)) != 0)
+            b->dirty = true;
         en.has_hits = true;
         en.hits = *hits;
         en.nhits = n;
     }
-    b->dirty = true;
     return RTSIM_OK;
 }
 
@@ -335,15 +368,41 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
     for (auto& en : b->entries)
         if (en.sensor->profiling)
             return fail(RTSIM_ERR_INVALID, "turn sensor profiling off before batching");
+    hipStream_t st = (hipStream_t)stream;
     if (b->dirty) {
         // the capture stream must not run ahead of work already queued by the caller
-        hipError_t e = hipStreamSynchronize((hipStream_t)stream);
+        hipError_t e = hipStreamSynchronize(st);
         if (e != hipSuccess) return hip_fail(e, "hipStreamSynchronize");
         int rc = capture(b);
         if (rc) return rc;
+    } else {
+        // new receiver poses: rewrite those pings' streaming arguments in the
+        // descriptor buffer, ordered before this launch on its stream
+        bool waited = false;
+        for (auto& en : b->entries) {
+            if (en.pose_version == en.sensor->pose_version) continue;
+            en.pose_version = en.sensor->pose_version;
+            if (en.stream_index < 0) continue;
+            if (!waited) {   // the staging may still feed the previous launch's copies
+                hipError_t e = b->wait();
+                if (e != hipSuccess) return hip_fail(e, "waiting for the previous batch launch");
+                waited = true;
+            }
+            StreamArgs& a = b->stream.args[en.stream_index];
+            std::memcpy(a.R, en.sensor->R, sizeof(a.R));
+            b->h_pose[en.stream_index] = a;
+            char* dst = static_cast<char*>(b->d_desc) + b->stream.args_off +
+                        sizeof(StreamArgs) * (size_t)en.stream_index;
+            hipError_t e = hipMemcpyAsync(dst, &b->h_pose[en.stream_index], sizeof(StreamArgs),
+                                          hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(pose)");
+        }
     }
-    hipError_t e = hipGraphLaunch(b->exec, (hipStream_t)stream);
+    hipError_t e = hipGraphLaunch(b->exec, st);
     if (e != hipSuccess) return hip_fail(e, "hipGraphLaunch");
+    e = hipEventRecord(b->done, st);
+    if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    b->launched = true;
     for (auto& en : b->entries) {
         en.sensor->computed = true;
         en.sensor->hist_fused = false;   // the batch's own sort rewrote the counts
@@ -361,6 +420,7 @@ static int time_batch(Batch* b, int iterations, hipStream_t st, hipEvent_t e0, h
     hipError_t e = hipEventRecord(e0, st);
     for (int it = 0; it < iterations && e == hipSuccess; it++) e = hipGraphLaunch(b->exec, st);
     if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventRecord(b->done, st);
     if (e == hipSuccess) e = hipEventSynchronize(e1);
     if (e == hipSuccess) e = hipEventElapsedTime(ms, e0, e1);
     if (e != hipSuccess) return hip_fail(e, "batch autotune");

@@ -81,6 +81,31 @@ int make_overlap(float margin, float lo, float hi, uint32_t N)
 }
 #pragma clang fp contract(on)
 
+// Waveform_Sine (Waveform.h:115-134): sample_rate = 2 f os in FP32, size =
+// (unsigned)(sample_rate * duration), sample idx = exp(i phase) with phase =
+// idx * (2 pi f / sample_rate) evaluated in double and rounded to float; the
+// PSF table is wave[h] * beam[w] in complex FP32 (SensorInstance2D.cu:20-33).
+std::vector<float> sine_psf_table(float frequency, float duration, uint32_t oversampling,
+                                  const float* beam, uint32_t beam_taps, uint32_t* range_taps)
+{
+    const float rate = 2.0f * frequency * (float)oversampling;
+    const float fn = rate * duration;
+    uint32_t n = 0;
+    if (fn >= 1.0f && fn < (float)kMaxWaveformTaps + 1.0f) n = (uint32_t)fn;
+    *range_taps = n;
+    std::vector<float> t(2 * (size_t)n * beam_taps);
+    for (uint32_t h = 0; h < n; h++) {
+        const float phase = (float)((double)h * (2.0 * M_PI * (double)frequency / (double)rate));
+        const float c = std::cos(phase), sn = std::sin(phase);
+        for (uint32_t w = 0; w < beam_taps; w++) {
+            const float b = beam ? beam[w] : 1.0f;
+            t[2 * ((size_t)h * beam_taps + w)] = c * b;
+            t[2 * ((size_t)h * beam_taps + w) + 1] = sn * b;
+        }
+    }
+    return t;
+}
+
 void init_kernel_attributes()
 {
     static std::once_flag once;
@@ -170,6 +195,42 @@ int Sensor::reserve(size_t count)
     if ((rc = dev_alloc(&d_tmax, sort_count_words(count) / 256))) return rc;
     capacity = count;
     return RTSIM_OK;
+}
+
+int Sensor::upload_psf()
+{
+    dev_free(d_psf);
+    int rc = dev_alloc(&d_psf, (size_t)Tr * Tb);
+    if (rc) return rc;
+    HIP_TRY(hipMemcpy(d_psf, psf.data(), sizeof(float) * psf.size(), hipMemcpyHostToDevice));
+    return RTSIM_OK;
+}
+
+// SensorInstance::set_ranges (SensorInstance.cu:23-34) for a sensor with a
+// waveform recipe: an adaptive duration follows the range resolution
+// (Linspace resolution in FP32), then generate_psf_data rebuilds the table
+// and the PSF range span becomes c * tau (SensorInstance2D.cu:35-42).
+int Sensor::apply_waveform()
+{
+    if (!has_wave) return RTSIM_OK;
+    float duration = wave_duration;
+    if (!wave_fixed) {
+        const float resolution = N > 1 ? (range_max - range_min) / (float)(N - 1u) : 0.0f;
+        duration = resolution / sound_speed;
+    }
+    uint32_t T = 0;
+    std::vector<float> table = sine_psf_table(wave_frequency, duration, wave_os, beam.data(),
+                                              (uint32_t)beam.size(), &T);
+    if (T == 0)
+        return fail(RTSIM_ERR_INVALID,
+                    "the sine waveform has no samples (or more than %u) at duration %g s",
+                    kMaxWaveformTaps, (double)duration);
+    wave_duration = duration;
+    psf.swap(table);
+    Tr = T;
+    Tb = (uint32_t)beam.size();
+    psf_range_span = duration * sound_speed;
+    return upload_psf();
 }
 
 // Ranges-dependent state: Binner::reconfigure (Binner.cu:15-22) + ping resize.
@@ -462,11 +523,22 @@ static int validate_desc(const rtsim_sensor_desc* d)
         if (d->bearing_axis != RTSIM_AXIS_ANGLE && d->bearing_axis != RTSIM_AXIS_SINE)
             return fail(RTSIM_ERR_INVALID, "unknown bearing axis %d", d->bearing_axis);
     }
-    if (!d->psf || d->psf_range_taps < 1 || d->psf_bearing_taps < 1)
-        return fail(RTSIM_ERR_INVALID, "PSF taps missing");
-    if (d->kind == RTSIM_SENSOR_1D && d->psf_bearing_taps != 1)
-        return fail(RTSIM_ERR_INVALID, "1D sensor needs psf_bearing_taps == 1");
-    if (!(d->psf_range_span > 0.f)) return fail(RTSIM_ERR_INVALID, "psf_range_span must be > 0");
+    if (const rtsim_waveform_sine* w = d->waveform) {
+        if (!(w->frequency > 0.f) || w->oversampling < 1)
+            return fail(RTSIM_ERR_INVALID, "waveform: frequency and oversampling must be > 0");
+        if (w->fixed && !(w->duration > 0.f))
+            return fail(RTSIM_ERR_INVALID, "waveform: a fixed waveform needs a duration > 0");
+        if (w->beam_taps < 1 || (w->beam_taps > 1 && !w->beam))
+            return fail(RTSIM_ERR_INVALID, "waveform: beam taps missing");
+        if (d->kind == RTSIM_SENSOR_1D && w->beam_taps != 1)
+            return fail(RTSIM_ERR_INVALID, "1D sensor needs beam_taps == 1");
+    } else {
+        if (!d->psf || d->psf_range_taps < 1 || d->psf_bearing_taps < 1)
+            return fail(RTSIM_ERR_INVALID, "PSF taps missing");
+        if (d->kind == RTSIM_SENSOR_1D && d->psf_bearing_taps != 1)
+            return fail(RTSIM_ERR_INVALID, "1D sensor needs psf_bearing_taps == 1");
+        if (!(d->psf_range_span > 0.f)) return fail(RTSIM_ERR_INVALID, "psf_range_span must be > 0");
+    }
     if (d->kind == RTSIM_SENSOR_2D && !(d->psf_bearing_span > 0.f))
         return fail(RTSIM_ERR_INVALID, "psf_bearing_span must be > 0");
     if (d->directivity && (d->dir_width < 1 || d->dir_height < 1))
@@ -494,10 +566,20 @@ int rtsim_sensor_create(const rtsim_sensor_desc* d, rtsim_sensor* out)
     s->axis = d->kind == RTSIM_SENSOR_2D ? d->bearing_axis : RTSIM_AXIS_ANGLE;
     if (d->kind == RTSIM_SENSOR_2D) s->bearings.assign(d->bearings, d->bearings + d->bearing_count);
     else s->bearings.assign(1, 0.f);
-    s->Tr = d->psf_range_taps;
-    s->Tb = d->psf_bearing_taps;
-    s->psf.assign(d->psf, d->psf + 2 * (size_t)s->Tr * s->Tb);
-    s->psf_range_span = d->psf_range_span;
+    if (const rtsim_waveform_sine* w = d->waveform) {
+        s->has_wave = true;
+        s->wave_frequency = w->frequency;
+        s->wave_duration = w->duration;
+        s->wave_fixed = w->fixed != 0;
+        s->wave_os = w->oversampling;
+        if (w->beam) s->beam.assign(w->beam, w->beam + w->beam_taps);
+        else s->beam.assign(1, 1.0f);
+    } else {
+        s->Tr = d->psf_range_taps;
+        s->Tb = d->psf_bearing_taps;
+        s->psf.assign(d->psf, d->psf + 2 * (size_t)s->Tr * s->Tb);
+        s->psf_range_span = d->psf_range_span;
+    }
     s->psf_bearing_span = d->kind == RTSIM_SENSOR_2D ? d->psf_bearing_span : 1.0f;
     s->margin_request = d->margin;
     s->frequency = d->frequency;
@@ -518,16 +600,17 @@ int rtsim_sensor_create(const rtsim_sensor_desc* d, rtsim_sensor* out)
                                                : (double)s->bearings[b];
         bc[b] = split_df(x);
     }
-    if ((rc = dev_alloc(&s->d_bear, s->W)) || (rc = dev_alloc(&s->d_psf, (size_t)s->Tr * s->Tb)) ||
-        (rc = dev_alloc(&s->d_totals, 256))) {
+    if ((rc = dev_alloc(&s->d_bear, s->W)) || (rc = dev_alloc(&s->d_totals, 256))) {
         delete s;
         return rc;
     }
-    if (hipMemcpy(s->d_bear, bc.data(), sizeof(float2) * s->W, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(s->d_psf, s->psf.data(), sizeof(float) * s->psf.size(), hipMemcpyHostToDevice) !=
-            hipSuccess) {
+    if (hipMemcpy(s->d_bear, bc.data(), sizeof(float2) * s->W, hipMemcpyHostToDevice) != hipSuccess) {
         delete s;
         return fail(RTSIM_ERR_HIP, "uploading sensor tables failed");
+    }
+    if ((rc = s->has_wave ? s->apply_waveform() : s->upload_psf())) {
+        delete s;
+        return rc;
     }
     if (!s->dir.empty()) {
         if ((rc = dev_alloc(&s->d_dir, s->dir.size()))) { delete s; return rc; }
@@ -560,12 +643,21 @@ int rtsim_sensor_set_ranges(rtsim_sensor h, float range_min, float range_max,
     GET_SENSOR(h);
     if (range_count < 1) return fail(RTSIM_ERR_INVALID, "range_count must be >= 1");
     if (!(range_max > range_min)) return fail(RTSIM_ERR_INVALID, "range_max must exceed range_min");
+    const float old_min = s->range_min, old_max = s->range_max;
+    const uint32_t old_n = s->N;
     s->range_min = range_min;
     s->range_max = range_max;
     s->N = range_count;
-    s->margin_request = margin;
     s->computed = false;
     s->hist_fused = false;
+    int rc = s->apply_waveform();
+    if (rc) {   // the sensor keeps its previous ranges and table
+        s->range_min = old_min;
+        s->range_max = old_max;
+        s->N = old_n;
+        return rc;
+    }
+    s->margin_request = margin;
     return s->configure_ranges();
 }
 
@@ -576,14 +668,16 @@ int rtsim_sensor_set_pose(rtsim_sensor h, const float* rotation)
     for (int k = 0; k < 9; k++)
         if (!(rotation[k] == rotation[k])) return fail(RTSIM_ERR_INVALID, "rotation has a NaN");
     std::memcpy(s->R, rotation, sizeof(s->R));
-    s->generation++;   // batched stream launches hold the rotation
+    s->pose_version++;   // batched stream launches hold the rotation
     return RTSIM_OK;
 }
 
 int rtsim_sensor_set_sample_count(rtsim_sensor h, size_t count)
 {
     GET_SENSOR(h);
-    if (count > 0x7fffffffu) return fail(RTSIM_ERR_INVALID, "at most 2^31 - 1 samples");
+    if (count > RTSIM_MAX_SAMPLES)
+        return fail(RTSIM_ERR_INVALID, "%zu samples exceed the sort's limit of %u per sensor", count,
+                    RTSIM_MAX_SAMPLES);
     int rc = s->reserve(count);
     if (rc) return rc;
     if (s->count != count) s->generation++;

@@ -13,7 +13,7 @@
 //   directivity.type     fan {size, bearing-width, elevation-sigma, unit}
 //   bearing-axis         angle (default) | sine
 //   binner               {overlap: bins} | {margin: metres}
-//   spreading            none (default) | inverse-square
+//   spreading            inverse-square (default, RayCaster.cu:69-71) | none
 //   frequency            Hz of the round-trip phase (default: globals or waveform)
 #include <cmath>
 #include <complex>
@@ -180,24 +180,12 @@ double sinc(double x) { return x == 0.0 ? 1.0 : std::sin(M_PI * x) / (M_PI * x);
 // Everything a sensor description points to.
 struct Info {
     rtsim_sensor_desc desc{};
+    rtsim_waveform_sine wave{};     // desc.waveform points here for a sine waveform
     std::string name;
-    std::vector<float> bearings, psf, directivity;
+    std::vector<float> bearings, psf, directivity, beam;
 };
 
 namespace {
-
-// Waveform_Sine (Waveform.h:96-138) samples: exp(i idx 2 pi f / (2 f os))
-std::vector<std::complex<float>> sine_waveform(float f, float duration, unsigned os)
-{
-    float sample_rate = 2 * f * os;
-    unsigned n = (unsigned)(sample_rate * duration);
-    std::vector<std::complex<float>> w(n);
-    for (unsigned idx = 0; idx < n; idx++) {
-        float phase = idx * (2.0 * M_PI * f / sample_rate);
-        w[idx] = std::complex<float>(std::cos(phase), std::sin(phase));
-    }
-    return w;
-}
 
 // BeamDirectivity_Sinc (BeamDirectivity.h:94-99) built on rtac_base's
 // signal::SincFunction(span / resolution, oversampling), which is not part of
@@ -312,17 +300,30 @@ void build_info(const NodePtr& root, const std::string& base_dir, Info& I, int d
     if (!wn) cfg_error("invalid 'waveform' node");
     std::string wt = need(wn, "type", "waveform")->as_string("waveform.type");
     unsigned os = (unsigned)num_or(wn, "oversampling", 8);
+    if (os < 1) cfg_error("waveform.oversampling must be >= 1");
     std::vector<std::complex<double>> wave;
     float span_r = 0.f;
     double wave_freq = 0;
+    bool sine = false;
     if (wt == "sine") {
         float f = (float)num(wn, "frequency", "waveform");
+        if (!(f > 0.f)) cfg_error("waveform.frequency must be > 0");
         wave_freq = f;
         std::string mode = str_or(wn, "duration-mode", "adaptive");
         float duration;
         if (mode == "fixed") duration = (float)num(wn, "duration", "waveform");
-        else duration = resolution / c;   // set_ranges (SensorInstance.cu:23-34)
-        for (auto v : sine_waveform(f, duration, os)) wave.emplace_back(v.real(), v.imag());
+        else if (mode == "adaptive") duration = resolution / c;   // set_ranges (SensorInstance.cu:23-34)
+        else cfg_error("waveform.duration-mode must be 'adaptive' or 'fixed'");
+        // the taps come from the recipe the sensor keeps (the table below is
+        // the same one, for the description's readers)
+        sine = true;
+        I.wave.frequency = f;
+        I.wave.duration = duration;
+        I.wave.fixed = mode == "fixed";
+        I.wave.oversampling = os;
+        uint32_t T = 0;
+        std::vector<float> t = sine_psf_table(f, duration, os, nullptr, 1, &T);
+        for (uint32_t h = 0; h < T; h++) wave.emplace_back(t[2 * h], t[2 * h + 1]);
         span_r = duration * c;
     } else if (wt == "hann-baseband" || wt == "lfm") {
         long T = need(wn, "taps", "waveform")->as_long("waveform.taps");
@@ -340,6 +341,7 @@ void build_info(const NodePtr& root, const std::string& base_dir, Info& I, int d
         cfg_error("unsupported waveform type: '" + wt + "'");
     }
     if (wave.empty()) cfg_error("the waveform has no samples (duration too short)");
+    if (wave.size() > kMaxWaveformTaps) cfg_error("the waveform has more than 65536 samples");
 
     // ---- beamsteering (SensorInfoFactory.cpp:202-235) -> bearing taps -----
     std::vector<double> beam(1, 1.0);
@@ -382,7 +384,14 @@ void build_info(const NodePtr& root, const std::string& base_dir, Info& I, int d
             I.psf[2 * ((size_t)h * Tb + w) + 1] = v.imag();
         }
     int psfTr = Tr, psfTb = Tb;
+    if (sine) {
+        I.beam.assign(beam.begin(), beam.end());
+        I.wave.beam = I.beam.data();
+        I.wave.beam_taps = (uint32_t)I.beam.size();
+        d.waveform = &I.wave;
+    }
     if (NodePtr pn = root->get("psf")) {
+        d.waveform = nullptr;   // explicit taps are fixed
         std::string pt = need(pn, "type", "psf")->as_string("psf.type");
         if (pt != "csv") cfg_error("psf.type must be 'csv'");
         psfTr = (int)need(pn, "range-taps", "psf")->as_long("psf.range-taps");
@@ -469,7 +478,8 @@ void build_info(const NodePtr& root, const std::string& base_dir, Info& I, int d
     if (f == 0.0) f = global(dn, "frequency", 0.0);
     if (f == 0.0) f = wave_freq;
     d.frequency = (float)f;
-    std::string sp = str_or(root, "spreading", "none");
+    // RayCaster.cu:69-71 always divides by the squared travel
+    std::string sp = str_or(root, "spreading", "inverse-square");
     if (sp == "none") d.spreading = RTSIM_SPREAD_NONE;
     else if (sp == "inverse-square") d.spreading = RTSIM_SPREAD_INV_R2;
     else cfg_error("spreading must be 'none' or 'inverse-square'");

@@ -20,6 +20,12 @@ int hip_fail(hipError_t e, const char* what);
 const char* fail_message();
 KeyParams make_key(uint32_t N, float lo, float hi, float margin);
 int make_overlap(float margin, float lo, float hi, uint32_t N);
+// Waveform_Sine samples times the beam taps (complex interleaved
+// [range tap][bearing tap]); *range_taps = floor((2 f os) duration), 0 when
+// the duration is too short (or out of the table limit).
+std::vector<float> sine_psf_table(float frequency, float duration, uint32_t oversampling,
+                                  const float* beam, uint32_t beam_taps, uint32_t* range_taps);
+constexpr uint32_t kMaxWaveformTaps = 1u << 16;
 
 // Everything one ping launches, without launching it (a batch gathers the
 // plans of many pings into per-stage launches).
@@ -57,6 +63,12 @@ struct Sensor {
     float frequency = 0.f, sound_speed = 1500.f;
     int spreading = 0;
     float R[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    // sine waveform recipe (rtsim_waveform_sine): the PSF follows the ranges
+    bool has_wave = false;
+    float wave_frequency = 0.f, wave_duration = 0.f;
+    bool wave_fixed = false;
+    uint32_t wave_os = 8;
+    std::vector<float> beam;
     KeyParams key{};
     int overlap = 0;
 
@@ -101,12 +113,17 @@ struct Sensor {
     // bumped whenever buffers or kernel arguments change (captured graphs
     // built from this sensor must then be re-captured)
     uint64_t generation = 0;
+    // bumped by set_pose: a batch rewrites that ping's streaming arguments
+    // in place (the rotation is a kernel argument) instead of re-capturing
+    uint64_t pose_version = 0;
 
     ~Sensor();
     void free_samples();
     void free_grid();
     int reserve(size_t count);
     int configure_ranges();
+    int apply_waveform();           // recipe -> taps, spans (no-op without a recipe)
+    int upload_psf();
     int check_plain_limits() const;
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();

@@ -5,6 +5,7 @@
 #include <cerrno>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <sys/stat.h>
 #include <vector>
@@ -94,6 +95,14 @@ int walk(const char* path, F&& fn)
 {
     std::FILE* f = std::fopen(path, "rb");
     if (!f) return fail(RTSIM_ERR_IO, "cannot open '%s'", path);
+    // the file size bounds every record the headers announce (before anything
+    // is sized from them)
+    long fsize = -1;
+    if (std::fseek(f, 0, SEEK_END) == 0) fsize = std::ftell(f);
+    if (fsize < 0 || std::fseek(f, 0, SEEK_SET) != 0) {
+        std::fclose(f);
+        return fail(RTSIM_ERR_IO, "cannot size '%s'", path);
+    }
     uint64_t index = 0;
     int rc = RTSIM_OK;
     for (;;) {
@@ -115,8 +124,22 @@ int walk(const char* path, F&& fn)
         h.index = get64(hd + 32);
         h.checksum = get64(hd + 40);
         const long payload = std::ftell(f);
-        const size_t nb = h.kind == RTSIM_SENSOR_2D ? h.width : 0;
-        const size_t bytes = sizeof(float) * (nb + 2 * (size_t)h.height * h.width);
+        if ((h.kind != RTSIM_SENSOR_1D && h.kind != RTSIM_SENSOR_2D) ||
+            (h.kind == RTSIM_SENSOR_1D && h.width != 1)) {
+            rc = fail(RTSIM_ERR_IO, "'%s': record %llu has an invalid kind / width", path,
+                      (unsigned long long)index);
+            break;
+        }
+        // record bytes = 4 (bearings) + 8 H W, checked against what is left of
+        // the file without overflow (H, W < 2^32, so H W < 2^64)
+        const uint64_t left = (uint64_t)(fsize - payload);
+        const uint64_t nb = h.kind == RTSIM_SENSOR_2D ? h.width : 0;
+        const uint64_t cells = (uint64_t)h.height * h.width;
+        if (cells > left / 8 || 8 * cells + 4 * nb > left) {
+            rc = fail(RTSIM_ERR_IO, "'%s': truncated record %llu", path, (unsigned long long)index);
+            break;
+        }
+        const size_t bytes = (size_t)(8 * cells + 4 * nb);
         int stop = fn(index, h, f, payload);
         if (stop < 0) {
             rc = RTSIM_ERR_IO;
@@ -169,7 +192,7 @@ int rtsim_sink_write_host(rtsim_sink sink, int kind, uint32_t height, uint32_t w
 }
 
 int rtsim_sink_write(rtsim_sink sink, rtsim_sensor sensor, void* stream)
-{
+try {
     Sink* s = reinterpret_cast<Sink*>(sink);
     Sensor* se = reinterpret_cast<Sensor*>(sensor);
     if (!s || !se) return fail(RTSIM_ERR_INVALID, "sink/sensor is NULL");
@@ -179,6 +202,8 @@ int rtsim_sink_write(rtsim_sink sink, rtsim_sensor sensor, void* stream)
     if (rc) return rc;
     return write_record(s, se->kind, se->N, se->W, se->range_min, se->range_max,
                         se->kind == RTSIM_SENSOR_2D ? se->bearings.data() : nullptr, s->host.data());
+} catch (const std::exception& e) {
+    return fail(RTSIM_ERR_IO, "writing a ping record: %s", e.what());
 }
 
 int rtsim_sink_count(rtsim_sink sink, uint64_t* pings)
@@ -196,7 +221,7 @@ int rtsim_sink_close(rtsim_sink sink)
 }
 
 int rtsim_ping_file_count(const char* path, uint64_t* pings)
-{
+try {
     if (!path || !pings) return fail(RTSIM_ERR_INVALID, "path/pings is NULL");
     uint64_t n = 0;
     int rc = walk(path, [&](uint64_t, const rtsim_ping_header&, std::FILE*, long) {
@@ -206,11 +231,13 @@ int rtsim_ping_file_count(const char* path, uint64_t* pings)
     if (rc) return rc;
     *pings = n;
     return RTSIM_OK;
+} catch (const std::exception& e) {
+    return fail(RTSIM_ERR_IO, "reading '%s': %s", path, e.what());
 }
 
 int rtsim_ping_file_read(const char* path, uint64_t index, rtsim_ping_header* header,
                          float* bearings, float* data)
-{
+try {
     if (!path || !header) return fail(RTSIM_ERR_INVALID, "path/header is NULL");
     bool found = false;
     int inner = RTSIM_OK;
@@ -243,6 +270,8 @@ int rtsim_ping_file_read(const char* path, uint64_t index, rtsim_ping_header* he
     if (!found)
         return fail(RTSIM_ERR_INVALID, "'%s' has no record %llu", path, (unsigned long long)index);
     return RTSIM_OK;
+} catch (const std::exception& e) {
+    return fail(RTSIM_ERR_IO, "reading '%s': %s", path, e.what());
 }
 
 }  // extern "C"

@@ -275,7 +275,8 @@ double Node::as_double(const std::string& what) const
 long Node::as_long(const std::string& what) const
 {
     double v = as_double(what);
-    if (v != (double)(long)v) {
+    // range first: converting a double outside long's range is undefined
+    if (!(v >= -9.2e18 && v <= 9.2e18) || v != (double)(long)v) {
         std::ostringstream o;
         o << what << " (line " << line << "): '" << scalar << "' is not an integer";
         throw std::runtime_error(o.str());

@@ -1,0 +1,296 @@
+// Host-code checks run under AddressSanitizer and UndefinedBehaviorSanitizer
+// (no GPU): every host-only path of the library -- the YAML reader and the
+// sensor factory, the directivity / emitter / scene builders, the ping-file
+// writer and readers, the pose CSV reader of the C++ drop-in -- and the CPU
+// oracle, on well-formed and malformed inputs.  Any report of the sanitizers
+// ends the program with a non-zero status; "host checks ok" is printed last.
+//
+// usage: host_checks <reference config dir> <package config dir> <scratch dir>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <filesystem>
+#include <fstream>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/rtac_simulation_amd/rtac_simulation.h"
+#include "../../oracle/oracle.h"
+
+namespace fs = std::filesystem;
+
+static int failures = 0;
+
+#define EXPECT(cond)                                                        \
+    do {                                                                    \
+        if (!(cond)) {                                                      \
+            std::fprintf(stderr, "FAILED %s:%d: %s (%s)\n", __FILE__,       \
+                         __LINE__, #cond, rtsim_last_error());              \
+            failures++;                                                     \
+        }                                                                   \
+    } while (0)
+
+static void yaml_configs(const fs::path& dir)
+{
+    for (const auto& e : fs::directory_iterator(dir)) {
+        if (e.path().extension() != ".yaml") continue;
+        rtsim_info info = nullptr;
+        const int rc = rtsim_info_from_yaml(e.path().c_str(), &info);
+        EXPECT(rc == RTSIM_OK);
+        if (rc) continue;
+        const rtsim_sensor_desc* d = nullptr;
+        EXPECT(rtsim_info_desc(info, &d) == RTSIM_OK);
+        // touch every table the description points to
+        double acc = 0;
+        for (size_t i = 0; i < 2 * (size_t)d->psf_range_taps * d->psf_bearing_taps; i++) acc += d->psf[i];
+        for (uint32_t i = 0; i < d->bearing_count; i++) acc += d->bearings[i];
+        for (size_t i = 0; i < (size_t)d->dir_width * d->dir_height; i++)
+            acc += std::isfinite(d->directivity[i]) ? d->directivity[i] : 0.0;
+        if (d->waveform)
+            for (uint32_t i = 0; d->waveform->beam && i < d->waveform->beam_taps; i++)
+                acc += d->waveform->beam[i];
+        EXPECT(std::isfinite(acc));
+        EXPECT(rtsim_info_destroy(info) == RTSIM_OK);
+    }
+}
+
+static void malformed_yaml(const fs::path& tmp)
+{
+    const char* texts[] = {
+        "",
+        "\x01\x02\xff\xfe",
+        "type: front-scan\nsampling: [\n",
+        "type: 'unterminated\n",
+        "a: &x [*x, *x]\n",
+        "a: *nowhere\n",
+        "type: single-beam\nsampling:\n  ranges:\n    size: 99999999999999999999\n    range-max: 5\n",
+        "type: single-beam\nsampling:\n  ranges:\n    size: -3\n    range-max: 5\n",
+        "type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: nan\n"
+        "waveform:\n  type: sine\n  frequency: 1e30\n  duration-mode: fixed\n  duration: 1e30\n",
+        "type: single-beam\nsampling:\n  ranges:\n    size: 10\n    range-max: 5\n"
+        "waveform:\n  type: sine\n  frequency: 38000\n  oversampling: 0\n",
+        "type: front-scan\nsampling:\n  bearings:\n    type: linear\n    size: 1\n    aperture: 10\n"
+        "  ranges:\n    size: 10\n    range-max: 5\n",
+        "type: file\npath: loop.yaml\n",
+        "\t- [a, {b: c}]\n  - ? x\n",
+        "type: front-scan\ndirectivity: {type: disk, diameter: 1e-30, wavelength: 1e30}\n",
+    };
+    int k = 0;
+    for (const char* t : texts) {
+        const fs::path p = tmp / ("bad_" + std::to_string(k++) + ".yaml");
+        std::ofstream(p) << t;
+        if (std::string(t).find("loop.yaml") != std::string::npos)
+            std::ofstream(tmp / "loop.yaml") << "type: file\npath: loop.yaml\n";
+        rtsim_info info = nullptr;
+        const int rc = rtsim_info_from_yaml(p.c_str(), &info);
+        EXPECT(rc != RTSIM_OK);
+        if (rc == RTSIM_OK) rtsim_info_destroy(info);
+    }
+}
+
+static void builders()
+{
+    for (float D : {0.01f, 0.15f, 0.5f}) {
+        uint32_t W = 0, H = 0;
+        EXPECT(rtsim_directivity_disk(D, 0.0395f, 1, 8, nullptr, 0, &W, &H) == RTSIM_OK);
+        std::vector<float> t((size_t)W * H);
+        EXPECT(rtsim_directivity_disk(D, 0.0395f, 1, 8, t.data(), t.size(), &W, &H) == RTSIM_OK);
+        EXPECT(rtsim_directivity_disk(D, 0.0395f, 1, 8, t.data(), t.size() - 1, &W, &H) != RTSIM_OK);
+    }
+    uint32_t W = 0, H = 0;
+    EXPECT(rtsim_directivity_rectangle(0.00156f, 0.008f, 0.00125f, 1, 8, nullptr, 0, &W, &H) == RTSIM_OK);
+    std::vector<float> t((size_t)W * H);
+    EXPECT(rtsim_directivity_rectangle(0.00156f, 0.008f, 0.00125f, 0, 8, t.data(), t.size(), &W, &H) ==
+           RTSIM_OK);
+    EXPECT(rtsim_directivity_uniform(4, nullptr, 0, &W, &H) == RTSIM_OK);
+    t.assign((size_t)W * H, 0.f);
+    EXPECT(rtsim_directivity_uniform(4, t.data(), t.size(), &W, &H) == RTSIM_OK);
+
+    size_t n = 0;
+    EXPECT(rtsim_emitter_polar_directions(0.5f, 130.f, 20.f, nullptr, 0, &n) == RTSIM_OK);
+    std::vector<float> dirs(3 * n), vals(2 * n);
+    EXPECT(rtsim_emitter_polar_directions(0.5f, 130.f, 20.f, dirs.data(), dirs.size(), &n) == RTSIM_OK);
+    EXPECT(rtsim_emitter_initial_values(dirs.data(), n, t.data(), W, H, 0.31830988f, 0.31830988f,
+                                        vals.data()) == RTSIM_OK);
+
+    rtsim_scene_desc sc{};
+    sc.kind = RTSIM_SCENE_SEABED;
+    sc.bearing_max = 1.13f;
+    sc.elevation_max = 0.17f;
+    sc.plane_count = 2;
+    const float planes[2][4] = {{-0.34f, 0.f, 0.94f, 5.f}, {-0.05f, 0.07f, 0.99f, 30.f}};
+    std::memcpy(sc.planes, planes, sizeof(planes));
+    sc.height_noise = 0.05f;
+    sc.sphere_count = 50;
+    sc.sphere_radius_min = 0.2f;
+    sc.sphere_radius_max = 2.f;
+    sc.sphere_range_min = 5.f;
+    sc.sphere_range_max = 60.f;
+    sc.sphere_fraction = 0.25f;
+    sc.dynamic_range_db = 60.f;
+    sc.rayleigh_sigma = 1.f;
+    sc.incidence_power = 2;
+    const size_t S = 50000;
+    std::vector<float> r(S), dx(S), dy(S), dz(S), a(2 * S);
+    EXPECT(rtsim_generate_hits(&sc, 7, 0, S, r.data(), dx.data(), dy.data(), dz.data(), a.data(), 4) ==
+           RTSIM_OK);
+    const float rot[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};
+    EXPECT(rtsim_generate_emitter_hits(&sc, 7, dirs.data(), vals.data(), n, rot, 0, S, r.data(),
+                                       dx.data(), dy.data(), dz.data(), a.data(), 4) == RTSIM_OK);
+    rtsim_scene_desc cone{};
+    cone.kind = RTSIM_SCENE_CONE;
+    cone.range_min = 0.5f;
+    cone.range_max = 50.f;
+    cone.off_axis_max = 0.17f;
+    cone.rayleigh_sigma = 1.f;
+    EXPECT(rtsim_generate_hits(&cone, 3, 100, S, r.data(), dx.data(), dy.data(), dz.data(), a.data(), 3) ==
+           RTSIM_OK);
+}
+
+static void ping_files(const fs::path& tmp)
+{
+    const fs::path p = tmp / "sub" / "pings.rtpg";
+    rtsim_sink s = nullptr;
+    EXPECT(rtsim_sink_open(p.c_str(), 1, &s) == RTSIM_OK);
+    std::vector<float> b = {-0.5f, 0.f, 0.5f}, d2(2 * 7 * 3, 0.25f), d1(2 * 9, -1.f);
+    EXPECT(rtsim_sink_write_host(s, RTSIM_SENSOR_2D, 7, 3, 0.1f, 30.f, b.data(), d2.data()) == RTSIM_OK);
+    EXPECT(rtsim_sink_write_host(s, RTSIM_SENSOR_1D, 9, 1, 0.f, 50.f, nullptr, d1.data()) == RTSIM_OK);
+    EXPECT(rtsim_sink_write_host(s, 5, 9, 1, 0.f, 50.f, nullptr, d1.data()) != RTSIM_OK);
+    EXPECT(rtsim_sink_close(s) == RTSIM_OK);
+    uint64_t count = 0;
+    EXPECT(rtsim_ping_file_count(p.c_str(), &count) == RTSIM_OK && count == 2);
+    rtsim_ping_header h{};
+    std::vector<float> rb(3), rd(2 * 7 * 3);
+    EXPECT(rtsim_ping_file_read(p.c_str(), 0, &h, rb.data(), rd.data()) == RTSIM_OK);
+    EXPECT(rtsim_ping_file_read(p.c_str(), 5, &h, nullptr, nullptr) != RTSIM_OK);
+    // every truncation and a few header corruptions
+    std::ifstream in(p, std::ios::binary);
+    std::vector<char> raw((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+    for (size_t cut = 0; cut < raw.size(); cut += 7) {
+        const fs::path q = tmp / "cut.rtpg";
+        std::ofstream(q, std::ios::binary).write(raw.data(), (std::streamsize)cut);
+        (void)rtsim_ping_file_count(q.c_str(), &count);
+        (void)rtsim_ping_file_read(q.c_str(), 1, &h, rb.data(), rd.data());
+    }
+    std::mt19937 gen(5);
+    for (int k = 0; k < 200; k++) {
+        std::vector<char> bad = raw;
+        bad[gen() % 48] = (char)gen();
+        const fs::path q = tmp / "bad.rtpg";
+        std::ofstream(q, std::ios::binary).write(bad.data(), (std::streamsize)bad.size());
+        if (rtsim_ping_file_read(q.c_str(), 0, &h, nullptr, nullptr) == RTSIM_OK &&
+            (size_t)h.height * h.width <= rd.size() / 2 && h.width <= rb.size())
+            (void)rtsim_ping_file_read(q.c_str(), 0, &h, rb.data(), rd.data());
+    }
+}
+
+static void poses(const fs::path& csv, const fs::path& tmp)
+{
+    using namespace rtac::simulation;
+    auto src = PoseSourceList::CreateFromCSV(csv.string());
+    EXPECT(src->size() > 0);
+    while (src->remaining() > 0) {
+        const auto R = src->next_pose().rotation_matrix();
+        EXPECT(std::isfinite(R[0] + R[4] + R[8]));
+    }
+    const fs::path bad = tmp / "bad_poses.csv";
+    std::ofstream(bad) << "# header\nquat,1,2,3,1,0,0\n";
+    bool threw = false;
+    try {
+        PoseSourceList::CreateFromCSV(bad.string());
+    } catch (const std::exception&) {
+        threw = true;
+    }
+    EXPECT(threw);
+}
+
+static void oracle_paths()
+{
+    std::mt19937 gen(3);
+    std::uniform_real_distribution<float> U(0.f, 1.f);
+    const uint32_t H = 40, W = 24;
+    const int Tr = 7, Tb = 5;
+    const size_t n = 3000;
+    std::vector<float> rec(4 * n), bearings(W), taps(2 * Tr * Tb);
+    for (uint32_t b = 0; b < W; b++) bearings[b] = std::asin(-0.9f + 1.8f * b / (W - 1));
+    for (auto& v : taps) v = U(gen) - 0.5f;
+    for (size_t i = 0; i < n; i++) {
+        rec[4 * i] = U(gen) - 0.5f;
+        rec[4 * i + 1] = U(gen) - 0.5f;
+        rec[4 * i + 2] = i % 97 == 0 ? NAN : (i % 89 == 0 ? -1.f : 0.9f + 2.2f * U(gen));
+        rec[4 * i + 3] = std::asin(1.9f * U(gen) - 0.95f);
+    }
+    orc_grid g{1.f, 3.f, H, W, ORC_AXIS_SINE, Tr, Tb, 0.3f, 0.4f};
+    orc_key key;
+    orc_key_make(H, 1.f, 3.f, 0.3f, &key);
+    std::vector<uint32_t> order(n), keys(n), seg(2 * H), win(2 * H), go(n);
+    orc_sort_order(rec.data() + 2, n, 4, order.data());
+    std::vector<float> srt(4 * n);
+    for (size_t i = 0; i < n; i++) std::memcpy(&srt[4 * i], &rec[4 * order[i]], 16);
+    orc_keys(&key, srt.data() + 2, n, 4, keys.data());
+    orc_segments(keys.data(), n, H, seg.data());
+    orc_windows(seg.data(), H, orc_overlap(0.3f, 1.f, 3.f, H), win.data());
+    orc_group_order(rec.data() + 2, n, 4, &key, 1.f, 20.f, 0.f, 3, 64 + Tr - 1, go.data());
+    std::vector<double> out(2 * (size_t)H * W), rows_out(2 * 3 * (size_t)W), one(2 * H);
+    orc_conv2d(srt.data(), win.data(), &g, bearings.data(), taps.data(), out.data(), 3);
+    const uint32_t rows[3] = {0, 17, H - 1};
+    orc_conv2d_rows(srt.data(), win.data(), &g, bearings.data(), taps.data(), rows, 3, rows_out.data(), 2);
+    for (int i = 0; i < 3; i++)
+        for (uint32_t b = 0; b < W; b++)
+            EXPECT(std::fabs(rows_out[2 * (i * W + b)] - out[2 * (rows[i] * W + b)]) <= 1e-12);
+    orc_grid2d(rec.data(), n, &key, win.data(), &g, std::sin((double)bearings[0]),
+               (std::sin((double)bearings[W - 1]) - std::sin((double)bearings[0])) / (W - 1), taps.data(),
+               out.data());
+    orc_grid g1 = g;
+    g1.bearing_count = 1;
+    g1.psf_bearing_taps = 1;
+    orc_conv1d(srt.data(), win.data(), &g1, taps.data(), one.data(), 2);
+    std::vector<float> cs(4 * n), fout(2 * (size_t)H * W);
+    std::vector<uint32_t> cw(2 * H);
+    cpu_binning(rec.data(), n, &g, 0.3f, cs.data(), cw.data(), 2);
+    cpu_gather2d_f32(cs.data(), cw.data(), &g, bearings.data(), taps.data(), fout.data(), 0, H, 2);
+    std::vector<float> hr(n), hx(n), hy(n), hz(n), ha(2 * n), tab(16 * 16, 0.5f);
+    for (size_t i = 0; i < n; i++) {
+        hr[i] = 0.5f + 30.f * U(gen);
+        hx[i] = 1.f;
+        hy[i] = U(gen) - 0.5f;
+        hz[i] = U(gen) - 0.5f;
+        ha[2 * i] = U(gen);
+        ha[2 * i + 1] = U(gen);
+    }
+    orc_stream_params sp{};
+    sp.rotation[0] = sp.rotation[4] = sp.rotation[8] = 1.f;
+    sp.frequency = 1.2e6f;
+    sp.sound_speed = 1500.f;
+    sp.spreading = 1;
+    sp.directivity = tab.data();
+    sp.dir_width = sp.dir_height = 16;
+    sp.dir_xscale = sp.dir_yscale = 0.31830988f;
+    std::vector<double> so(4 * n);
+    orc_stream(n, hr.data(), hx.data(), hy.data(), hz.data(), ha.data(), &sp, so.data());
+}
+
+int main(int argc, char** argv)
+{
+    if (argc < 4) {
+        std::fprintf(stderr, "usage: %s <reference configs> <package configs> <scratch>\n", argv[0]);
+        return 2;
+    }
+    const fs::path ref = argv[1], pkg = argv[2], tmp = argv[3];
+    fs::create_directories(tmp);
+    yaml_configs(ref);
+    yaml_configs(pkg);
+    malformed_yaml(tmp);
+    builders();
+    ping_files(tmp);
+    poses(ref / "pose_source.csv", tmp);
+    oracle_paths();
+    if (failures) {
+        std::fprintf(stderr, "%d host check(s) failed\n", failures);
+        return 1;
+    }
+    std::printf("host checks ok\n");
+    return 0;
+}

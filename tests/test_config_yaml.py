@@ -36,6 +36,8 @@ def test_workload_yaml_matches_python_config(name):
     # the reference disk builder leaves NaN texels on the 90 deg edge (denominator 0)
     assert np.allclose(y.directivity, p.directivity, atol=1e-6, equal_nan=True)
     assert np.float32(y.frequency) == np.float32(p.frequency)
+    assert y.spreading == p.spreading == R.SPREAD_NONE
+    assert y.waveform is None            # explicit taps: fixed
 
 
 def test_reference_oculus_config():
@@ -138,3 +140,39 @@ def test_yaml_reader_features(tmp_path):
     cfg = R.config_from_yaml(tmp_path / "outer.yaml")
     assert (cfg.range_count, cfg.range_min, cfg.range_max) == (64, 2.0, 12.5)
     assert cfg.psf.shape == (7, 1)
+
+
+@pytest.mark.parametrize("fname,fixed", [("oculus_M1200d_1_receiver.yaml", False),
+                                         ("oculus_M750d_1_receiver.yaml", False),
+                                         ("EA400_38KHz_receiver.yaml", True),
+                                         ("EA400_200KHz_receiver.yaml", True)])
+def test_reference_configs_keep_their_waveform_recipe(fname, fixed):
+    """A sine waveform stays a recipe the sensor rebuilds at set_ranges
+    (SensorInstance.cu:23-34); the table the description carries equals the
+    numpy restatement of Waveform_Sine x beam for the configured ranges, and
+    the reference's raygen spreading 1/r^2 (RayCaster.cu:69-71) is the default."""
+    from sine_waveform import after_set_ranges
+    cfg = R.config_from_yaml(GOLDEN / fname)
+    assert cfg.spreading == R.SPREAD_INV_R2
+    wf = cfg.waveform
+    assert wf is not None and wf["fixed"] == fixed and wf["oversampling"] == 8
+    assert (wf["beam"] is None and cfg.Tb == 1) or wf["beam"].size == cfg.Tb
+    want = after_set_ranges(cfg, cfg.range_min, cfg.range_max, cfg.range_count)
+    assert want.psf.shape == cfg.psf.shape
+    assert np.allclose(cfg.psf, want.psf, rtol=0, atol=2e-6)
+    assert np.float32(cfg.psf_range_span) == np.float32(want.psf_range_span)
+
+
+def test_explicit_psf_file_overrides_the_recipe(tmp_path):
+    rng = np.random.default_rng(1)
+    t = rng.standard_normal((5, 2))
+    (tmp_path / "p.csv").write_text("\n".join(f"{a},{b}" for a, b in t))
+    (tmp_path / "s.yaml").write_text(
+        "type: single-beam\nsampling:\n  ranges:\n    size: 64\n    range-max: 10\n"
+        "waveform:\n  type: sine\n  frequency: 5000\n  duration-mode: fixed\n"
+        "  duration: 1.3e-4\n  oversampling: 4\n"
+        "psf:\n  type: csv\n  path: p.csv\n  range-taps: 5\n  bearing-taps: 1\n"
+        "directivity:\n  type: uniform\nspreading: none\n")
+    cfg = R.config_from_yaml(tmp_path / "s.yaml")
+    assert cfg.waveform is None and cfg.spreading == R.SPREAD_NONE
+    assert np.allclose(cfg.psf[:, 0], (t[:, 0] + 1j * t[:, 1]).astype(np.complex64))

@@ -169,3 +169,48 @@ def test_batched_launches_many_uneven_pings(gpu):
                 assert O.rel_l2(s.output(), ping) < 1e-6, (rep, k)
             else:
                 assert np.array_equal(s.output().view(np.uint32), ping.view(np.uint32)), (rep, k)
+
+
+def test_batch_pose_update_is_applied_in_place(gpu):
+    """set_pose between batched computes: the next replay streams the hits
+    with the new rotation (rewritten in the batch's descriptors, not
+    re-captured) and equals an eager compute with that pose bit for bit; the
+    MFMA ping is within its bound of the oracle's ping of the rotated records."""
+    R = gpu
+    c0 = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "p0")
+    c1 = configs._fls(48, 96, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "p1")
+    hits = [R.generate_hits(configs.fls_scene(), 70 + k, n) for k, n in enumerate((30_000, 20_000))]
+    dh = [h.to_device() for h in hits]
+    sensors = [R.make_sensor(c) for c in (c0, c1)]
+    for s, h in zip(sensors, hits):
+        s.set_sample_count(h.n)
+    b = R.Batch(sensors, [2, 0])
+    for k, h in enumerate(dh):
+        b.bind_hits(k, h)
+    stream = R.Stream()
+    b.compute(stream)
+    stream.sync()
+    before = sensors[0].output().copy()
+    c = np.cos(0.2)
+    sn = np.sin(0.2)
+    rot = np.array([[c, -sn, 0], [sn, c, 0], [0, 0, 1]], np.float32)
+    for rep in range(2):
+        sensors[0].set_pose(rot if rep == 0 else np.eye(3, dtype=np.float32))
+        b.bind_hits(0, dh[0])                 # same arrays: the graph is kept
+        b.compute(stream)
+        stream.sync()
+        got = sensors[0].output()
+        cr = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "pr")
+        cr.rotation = rot if rep == 0 else np.eye(3, dtype=np.float32)
+        e = R.make_sensor(cr)
+        e.set_sample_count(hits[0].n)
+        e.write_samples(dh[0])
+        e.compute_output(2)
+        assert np.array_equal(got.view(np.uint32), e.output().view(np.uint32)), rep
+        if rep == 0:
+            assert O.rel_l2(got, before) > 1e-2       # the rotation changed the ping
+            rec = O.stream(cr, hits[0]).astype(np.float32)
+            ref, _ = O.reference_ping(cr, rec)
+            assert O.rel_l2(got, ref) <= 1e-3
+        else:
+            assert np.array_equal(got.view(np.uint32), before.view(np.uint32))

@@ -281,3 +281,70 @@ def test_grid_form_matches_scipy_correlate():
     mask = b["windows"][:, 1] == 0
     want[mask] = 0
     assert O.rel_l2(got, want) < 1e-4
+
+
+def test_stream_inverse_square_spreading_closed_form():
+    """RayCaster.cu:69-71 divides the value by the squared travel: at travels
+    whose round trip holds a whole number of wavelengths (phase 0), a unit
+    amplitude along the axis of a uniform receiver reads exactly 1 / r^2."""
+    n = 6
+    f, c = 768e3, 1500.0
+    lam2 = c / (2 * f)                       # one cycle of 2 r f / c: 2^-10 m, exact in FP32
+    r = (np.array([1, 3, 10, 40, 200, 1000], np.float64) * lam2).astype(np.float32)
+    hits = R.HostHits(n)
+    hits.range[:] = r
+    hits.dir_x[:], hits.dir_y[:], hits.dir_z[:] = 1.0, 0.0, 0.0
+    hits.amplitude[0::2], hits.amplitude[1::2] = 1.0, 0.0
+    cfg = configs._fls(8, 16, 0.1, 2.0, f, configs.psf_31x17(), "spread")
+    cfg.directivity = None
+    cfg.spreading = R.SPREAD_INV_R2
+    out = O.stream(cfg, hits)
+    q = 2.0 * r.astype(np.float64) * f / c
+    ph = 2 * np.pi * (q - np.floor(q))
+    assert not ph.any()                     # the travels sit on whole cycles
+    want = 1.0 / r.astype(np.float64) ** 2
+    assert np.allclose(out[:, 0], want, rtol=1e-12, atol=0)
+    assert np.abs(out[:, 1]).max() < 1e-15 * want.max()
+    cfg.spreading = R.SPREAD_NONE
+    flat = O.stream(cfg, hits)
+    assert np.allclose(np.abs(flat[:, 0] + 1j * flat[:, 1]), 1.0, rtol=1e-12)
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_conv2d_rows_equals_full_gather(axis):
+    """The row-subset gather (each sample visits only the bearings in the PSF
+    support) gives the full gather's rows: the skipped terms are exact 0."""
+    rng = np.random.default_rng(11)
+    taps = (rng.standard_normal((7, 5)) + 1j * rng.standard_normal((7, 5))).astype(np.complex64)
+    cfg = configs._fls(40, 30, 1.0, 3.0, 1.2e6, taps, "rows")
+    if axis == 0:
+        cfg.bearing_axis = 0
+        cfg.bearings = np.sort(rng.uniform(-1.1, 1.1, 40)).astype(np.float32)
+        cfg.psf_bearing_span = float(np.float32(5 * 2.2 / 39))
+    rec = _tiny_records(cfg, 4000, 12)
+    full, b = O.reference_ping(cfg, rec, threads=2)
+    rows = np.array([0, 3, 7, 15, 29])
+    got = O.conv_rows(cfg, b["sorted"], b["windows"], rows, threads=2)
+    assert np.abs(full[rows]).max() > 0
+    assert O.rel_l2(got, full[rows]) < 1e-14
+
+
+def test_cpu_baseline_gather_computes_the_ping():
+    """The CPU baseline (FP32 port of sort -> Binner -> per-cell gather) must
+    compute the ping it is timed on: its binning equals the oracle's bit for
+    bit and its rows are within the plain form's 1e-5 of the FP64 gather."""
+    cfg, scene, _, seed = configs.workload("fls_512x1024", samples=10)
+    cfg = configs._fls(96, 160, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "cpu_base")
+    hits = R.generate_hits(scene, seed, 60_000)
+    rec = O.stream(cfg, hits).astype(np.float32)
+    srt, win = O.cpu_binning(cfg, rec, 4)
+    b = O.binning(cfg, rec)
+    assert np.array_equal(srt.view(np.uint32), b["sorted"].view(np.uint32))
+    assert np.array_equal(win, b["windows"].reshape(-1))
+    rows = np.arange(2, cfg.H, 9)
+    base = np.zeros((rows.size, cfg.W), np.complex64)
+    for i, r in enumerate(rows):
+        base[i] = O.cpu_gather2d(cfg, srt, win, int(r), int(r) + 1, 2)[r]
+    ref = O.conv_rows(cfg, b["sorted"], b["windows"], rows, threads=4)
+    assert np.abs(ref).max() > 0
+    assert O.rel_l2(base, ref) <= 1e-5

@@ -69,6 +69,46 @@ def test_existing_file_and_corruption(tmp_path):
     assert R.ping_file_count(path) == 0
 
 
+def test_truncated_and_oversized_records_are_io_errors(tmp_path):
+    """A header is checked against the file before anything is sized from it:
+    a truncated last record, a header announcing 2^32 x 2^32 cells, an
+    unknown kind and a 1D record wider than 1 are I/O errors, never a crash."""
+    path = tmp_path / "pings.rtpg"
+    s = R.FileSink.Create(path)
+    s.write_host(2, _ping(6, 4, 3), 0.1, 30.0, np.linspace(-1, 1, 4, dtype=np.float32))
+    s.write_host(2, _ping(6, 4, 4), 0.1, 30.0, np.linspace(-1, 1, 4, dtype=np.float32))
+    s.close()
+    raw = path.read_bytes()
+    rec = len(raw) // 2
+    (tmp_path / "cut.rtpg").write_bytes(raw[:-5])
+    with pytest.raises(R.RtsimError) as e:
+        R.ping_file_count(tmp_path / "cut.rtpg")
+    assert e.value.code == R.ERR_IO and "truncated" in str(e.value)
+    assert np.array_equal(R.read_ping(tmp_path / "cut.rtpg", 0)["data"], _ping(6, 4, 3))
+
+    def patched(offset, value):
+        b = bytearray(raw[:rec])
+        b[offset:offset + 4] = np.uint32(value).tobytes()
+        q = tmp_path / f"bad_{offset}_{value}.rtpg"
+        q.write_bytes(bytes(b))
+        return q
+
+    for off, val, msg in [(16, 0xFFFFFFFF, "truncated"), (20, 0xFFFFFFFF, "truncated"),
+                          (8, 7, "kind"), (8, 1, "kind")]:
+        q = patched(off, val)
+        for fn in (lambda: R.ping_file_count(q), lambda: R.read_ping(q, 0)):
+            with pytest.raises(R.RtsimError) as e:
+                fn()
+            assert e.value.code == R.ERR_IO and msg in str(e.value), (off, val, str(e.value))
+    big = bytearray(raw[:rec])
+    big[16:20] = np.uint32(0xFFFFFFFF).tobytes()
+    big[20:24] = np.uint32(0xFFFFFFFF).tobytes()
+    (tmp_path / "huge.rtpg").write_bytes(bytes(big))
+    with pytest.raises(R.RtsimError) as e:
+        R.read_ping(tmp_path / "huge.rtpg", 0)
+    assert e.value.code == R.ERR_IO
+
+
 @pytest.mark.gpu
 def test_sink_writes_computed_pings(gpu, tmp_path):
     R_ = gpu

@@ -69,6 +69,17 @@ def test_drive_equals_per_ping_computes(gpu, tmp_path, in_flight):
         # rotations from float32 quaternion math on both sides agree to ~1 ulp
         from oracle import rel_l2
         assert rel_l2(ping, s.output()) < 1e-5, k
+        # the driver's pose-rotated ping against the oracle's ping of the
+        # oracle's own rotated records (every 8th range row)
+        import oracle as O
+        cfg.rotation = Rk
+        hk = R_.generate_hits(configs.fls_scene(), seed + k, n)
+        rec = O.stream(cfg, hk).astype(np.float32)
+        b = O.binning(cfg, rec)
+        rows = np.arange(k % 8, cfg.H, 8)
+        ref = O.conv_rows(cfg, b["sorted"], b["windows"], rows)
+        assert np.abs(ref).max() > 0, k
+        assert rel_l2(ping[rows], ref) <= 1e-3, k
 
 
 @pytest.mark.gpu
