@@ -261,7 +261,8 @@ def conv_rows(cfg, sorted_records: np.ndarray, win: np.ndarray, rows,
     assert r.size == 0 or int(r.max()) < cfg.H
     out = np.empty(2 * r.size * cfg.W, np.float64)
     b = np.ascontiguousarray(cfg.bearings, np.float32)
-    lib().orc_conv2d_rows(_p(srt), _p(w), C.byref(grid(cfg)), _p(b), _p(_taps(cfg)), _p(r), r.size,
+    taps = _taps(cfg)          # held: the call reads it through a raw pointer
+    lib().orc_conv2d_rows(_p(srt), _p(w), C.byref(grid(cfg)), _p(b), _p(taps), _p(r), r.size,
                           _p(out), threads or nthreads())
     return out.view(np.complex128).reshape(r.size, cfg.W)
 
@@ -302,7 +303,8 @@ def cpu_gather2d(cfg, srt, win, row_begin: int, row_end: int, threads: int | Non
     out = np.zeros(2 * cfg.H * cfg.W, np.float32)
     g = grid(cfg)
     b = np.ascontiguousarray(cfg.bearings, np.float32)
-    lib().cpu_gather2d_f32(_p(srt), _p(win), C.byref(g), _p(b), _p(_taps(cfg)), _p(out),
+    taps = _taps(cfg)          # held: the call reads it through a raw pointer
+    lib().cpu_gather2d_f32(_p(srt), _p(win), C.byref(g), _p(b), _p(taps), _p(out),
                            row_begin, row_end, threads or nthreads())
     return out.view(np.complex64).reshape(cfg.H, cfg.W)
 
