@@ -103,6 +103,12 @@ def test_spreading_and_pose_vs_oracle(gpu, via):
 # ---------------------------------------------------------------------------
 # the reference's example receivers
 # ---------------------------------------------------------------------------
+def _oculus_scene():
+    # ranges uniform over the sensor's span and bearings over +-60 deg: every
+    # range row holds samples, so every window of binOverlap 1 is non-empty
+    return configs.cone_scene(0.2, 9.5, 60.0)
+
+
 def _oculus(R):
     cfg = R.config_from_yaml(GOLDEN / "oculus_M1200d_1_receiver.yaml")
     assert cfg.waveform is not None and not cfg.waveform["fixed"]
@@ -113,7 +119,7 @@ def test_oculus_yaml_ping_vs_oracle(gpu):
     R = gpu
     cfg = _oculus(R)
     assert cfg.bearing_axis == R.AXIS_ANGLE and cfg.W == 256
-    hits = R.generate_hits(configs.seabed_scene([configs.tilted_plane(35.0, 3.0)]), 41, 400_000)
+    hits = R.generate_hits(_oculus_scene(), 41, 400_000)
     s = R.sensor_from_yaml(GOLDEN / "oculus_M1200d_1_receiver.yaml")
     s.set_sample_count(hits.n)
     s.write_samples(hits.to_device())
@@ -132,7 +138,7 @@ def test_oculus_set_ranges_rebuilds_the_waveform(gpu):
     R = gpu
     cfg = _oculus(R)
     s = R.make_sensor(cfg)
-    hits = R.generate_hits(configs.seabed_scene([configs.tilted_plane(35.0, 3.0)]), 42, 300_000)
+    hits = R.generate_hits(_oculus_scene(), 42, 300_000)
     s.set_sample_count(hits.n)
     dh = hits.to_device()
     for lo, hi, n in [(0.1, 4.0, 300), (1.0, 9.0, 720)]:
@@ -145,8 +151,9 @@ def test_oculus_set_ranges_rebuilds_the_waveform(gpu):
         assert b["overlap"] == O.overlap(want.psf_range_span, lo, hi, n)
 
 
-@pytest.mark.parametrize("fname", ["EA400_38KHz_receiver.yaml", "EA400_200KHz_receiver.yaml"])
-def test_echo_sounder_yaml_ping_vs_oracle(gpu, fname):
+@pytest.mark.parametrize("fname,new_count", [("EA400_38KHz_receiver.yaml", 800),
+                                             ("EA400_200KHz_receiver.yaml", 2900)])
+def test_echo_sounder_yaml_ping_vs_oracle(gpu, fname, new_count):
     R = gpu
     cfg = R.config_from_yaml(GOLDEN / fname)
     assert cfg.kind == 1 and cfg.waveform["fixed"]
@@ -158,10 +165,21 @@ def test_echo_sounder_yaml_ping_vs_oracle(gpu, fname):
     s.compute_output(0)
     _check_plain(s, cfg)
     # a fixed waveform keeps its duration: same taps, margin c * tau
-    s.set_ranges(1.0, 30.0, 800)
-    want = after_set_ranges(cfg, 1.0, 30.0, 800)
+    # (new_count keeps binOverlap = (int)(c tau / resolution) >= 1)
+    s.set_ranges(1.0, 30.0, new_count)
+    want = after_set_ranges(cfg, 1.0, 30.0, new_count)
+    assert O.overlap(want.psf_range_span, 1.0, 30.0, new_count) >= 1
     assert want.psf.shape == cfg.psf.shape
     assert np.allclose(want.psf, cfg.psf, rtol=0, atol=2e-6)
     s.write_samples(hits.to_device())
     s.compute_output(0)
     _check_plain(s, want)
+    # ranges too coarse for the fixed pulse: binOverlap 0 empties every
+    # window (Binner.cu:96-97), as in the reference: a zero ping
+    s.set_ranges(1.0, 30.0, 100)
+    zero = after_set_ranges(cfg, 1.0, 30.0, 100)
+    s.write_samples(hits.to_device())
+    s.compute_output(0)
+    b = O.binning(zero, s.samples())
+    assert b["overlap"] == 0 and s.binner_debug()["overlap"] == 0
+    assert not b["windows"].any() and not np.abs(s.output()).any()
