@@ -180,6 +180,30 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
  * sort -> Binner::compute -> PSF convolution, in the chosen form.                 */
 int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
 
+/* No counterpart in the reference: rtsim_sensor_compute for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Computes the ping of `n`
+ * caller-owned device records (16-B records as above, unsorted) into the
+ * caller-owned device ping `ping` ([range][bearing] complex64, fully
+ * written), in `form`, on `stream`, with the sensor's configuration (ranges,
+ * bearings, PSF, pose-independent: the records are already weighted).  Calls
+ * no hipMalloc / hipFree or allocating library routine: every per-ping buffer
+ * is a fixed device array of RTSIM_NOALLOC_SCRATCH_BYTES that each GPU holds
+ * from its first call until the process ends.  Limits: n <=
+ * RTSIM_NOALLOC_MAX_SAMPLES and range_count <= RTSIM_NOALLOC_MAX_BINS
+ * (RTSIM_ERR_INVALID beyond).  The MFMA forms use the sensor's prepared grid
+ * form (an earlier MFMA compute or rtsim_sensor_set_tile_shape prepares it;
+ * RTSIM_ERR_INVALID otherwise).  All calls on one GPU share its array and no
+ * call reads what an earlier one left there, but calls that overlap on one
+ * GPU (other streams, other threads) race; an MFMA-form call also rewrites
+ * the sensor's grid, so it must not overlap that sensor's own computes.  The
+ * sensor's own samples, sorted records, Binner arrays and ping are untouched. */
+#define RTSIM_NOALLOC_MAX_SAMPLES 1048576u
+#define RTSIM_NOALLOC_MAX_BINS 16384u
+#define RTSIM_NOALLOC_SCRATCH_BYTES 60036096u   /* 57.25 MiB per GPU */
+int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void* records, size_t n,
+                                 void* ping, void* stream);
+
 /* Tile-shape autotuner of the MFMA forms (no reference counterpart: the
  * reference has one fixed launch shape, SensorInstance2D.cu:93).  Runs one
  * compute, times every correlation tile shape that fits over `iterations`

@@ -101,6 +101,8 @@ _SIGNATURES = [
     ("rtsim_receiver_write_samples", C.c_int,
      [C.c_void_p, C.POINTER(Hits), C.c_size_t, C.c_size_t, C.c_void_p]),
     ("rtsim_sensor_compute", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtsim_sensor_compute_noalloc", C.c_int,
+     [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     ("rtsim_sensor_autotune", C.c_int,
      [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_float)]),
     ("rtsim_sensor_set_tile_shape", C.c_int, [C.c_void_p, C.c_int]),

@@ -339,9 +339,32 @@ int Sensor::check_plain_limits() const
     return RTSIM_OK;
 }
 
-int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
+PingBuffers Sensor::own_buffers() const
 {
-    const uint32_t n = (uint32_t)count;
+    PingBuffers b;
+    b.samples = d_samples;
+    b.n = (uint32_t)count;
+    b.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
+    b.sorted = d_sorted;
+    b.tmp = d_tmp;
+    b.coord = d_coord;
+    b.kv[0] = d_kv[0];
+    b.kv[1] = d_kv[1];
+    b.order = d_order;
+    b.skeys = d_skeys;
+    b.counts = d_counts;
+    b.totals = d_totals;
+    b.tmax = d_tmax;
+    b.seg = d_seg;
+    b.win = d_win;
+    b.out = d_out;
+    return b;
+}
+
+int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched, const PingBuffers* pbp)
+{
+    const PingBuffers pb = pbp ? *pbp : own_buffers();
+    const uint32_t n = pb.n;
     const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
     int rc = RTSIM_OK;
     if (form == RTSIM_FORM_PLAIN_FP32) {
@@ -359,31 +382,31 @@ int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
     // (Binner.cu:24-49) and bearing coordinates.  The plain form sorts by
     // travel; the grid form by the grouping key (GroupKey).
     SortBuffers sb;
-    sb.kv[0] = d_kv[0];
-    sb.kv[1] = d_kv[1];
-    sb.order = introspect ? d_order : nullptr;
-    sb.counts = d_counts;
-    sb.totals = d_totals;
-    sb.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
-    sb.tmp = d_tmp;
-    sb.seg_init = d_seg;                              // Binner segments start at (~0, 0)
+    sb.kv[0] = pb.kv[0];
+    sb.kv[1] = pb.kv[1];
+    sb.order = introspect ? pb.order : nullptr;
+    sb.counts = pb.counts;
+    sb.totals = pb.totals;
+    sb.tiles_cap = pb.tiles_cap;
+    sb.tmp = pb.tmp;
+    sb.seg_init = pb.seg;                             // Binner segments start at (~0, 0)
     sb.seg_count = N;
     sb.fill = rows.row_start;                         // grid row table starts at n
     sb.fill_value = n;
     sb.fill_count = rows.row_start ? (size_t)rows.GH + 3 : 0;
-    sb.out.sorted = d_sorted;
-    sb.out.skeys = introspect ? d_skeys : nullptr;
-    sb.out.coord = d_coord;
+    sb.out.sorted = pb.sorted;
+    sb.out.skeys = introspect ? pb.skeys : nullptr;
+    sb.out.coord = pb.coord;
     sb.out.xg = nullptr;
     if (grid) {   // the deposit reads grid column coordinates (in the same buffer)
-        sb.out.xg = reinterpret_cast<float*>(d_coord);
+        sb.out.xg = reinterpret_cast<float*>(pb.coord);
         sb.out.xg_b0 = rows.bear0;
         sb.out.xg_inv = rows.inv_db;
         sb.out.xg_off = rows.off_b;
     }
     sb.out.key = key;
     sb.out.axis = axis;
-    sb.out.seg = d_seg;
+    sb.out.seg = pb.seg;
     // the grid form reads the windows' emptiness alone: without introspection
     // its sort marks the keys that hold samples instead of bounding their
     // segments (two thirds of its run-bound atomics)
@@ -396,12 +419,12 @@ int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
         gk.rows = rows;
         gk.key = key;
         gk.vmax = vmax;
-        gk.tmax = d_tmax;
+        gk.tmax = pb.tmax;
         if (group_key_max(gk) >= (1u << 24))
             return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
                         rows.GH, N);
     }
-    hipError_t e = make_sort_plan(d_samples, n, sb, &gk, &p->sort, batched);
+    hipError_t e = make_sort_plan(pb.samples, n, sb, &gk, &p->sort, batched);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
 
     // Binner::compute (Binner.h:96-102): the sort bounded the segments, the
@@ -409,15 +432,15 @@ int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched)
     BinArgs& ba = p->bins;
     ba.N = N;
     ba.ov = overlap;
-    ba.seg = d_seg;
-    ba.win = d_win;
+    ba.seg = pb.seg;
+    ba.win = pb.win;
 
     if (grid) {
-        if ((rc = grid_plan(form, &p->dep, &p->corr))) return rc;
+        if ((rc = grid_plan(form, &p->dep, &p->corr, pb))) return rc;
         p->dep.bins = ba;
         return RTSIM_OK;
     }
-    p->conv = conv_args();
+    p->conv = conv_args(pb);
     return RTSIM_OK;
 }
 
@@ -427,43 +450,52 @@ int Sensor::compute(int form, hipStream_t s)
     PingPlan p;
     int rc = plan(form, &p);
     if (rc) return rc;
-    stage_begin(1, s);
     const bool fused = p.grid && hist_fused;
     hist_fused = false;   // the sort consumes (and overwrites) the counts
-    HIP_TRY(launch_sort_plan(p.sort, s, fused));
-    stage_end(s);
-    if (!p.grid) {
-        stage_begin(2, s);
-        HIP_TRY(launch_windows(p.bins, s));
-        stage_end(s);
-        stage_begin(4, s);
-        if (kind == RTSIM_SENSOR_2D) HIP_TRY(launch_conv2d_plain(p.conv, s));
-        else HIP_TRY(launch_conv1d_plain(p.conv, s));
-        stage_end(s);
-    } else {
-        stage_begin(3, s);
-        HIP_TRY(launch_deposit(p.dep, s));
-        stage_end(s);
-        stage_begin(4, s);
-        HIP_TRY(launch_correlate(p.corr, form == RTSIM_FORM_MFMA_FP16, s));
-        stage_end(s);
-        last_corr = p.corr;
-    }
+    if ((rc = launch_ping(p, form, s, fused, profiling))) return rc;
+    if (p.grid) last_corr = p.corr;
     computed = true;
     computed_form = form;
     return RTSIM_OK;
 }
 
-ConvArgs Sensor::conv_args() const
+int Sensor::launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hist0, bool profile)
+{
+    const bool prof = profiling;
+    profiling = profile && prof;
+    auto stage = [&](int k, auto&& fn) -> hipError_t {
+        stage_begin(k, s);
+        const hipError_t e = fn();
+        stage_end(s);
+        return e;
+    };
+    hipError_t e = stage(1, [&] { return launch_sort_plan(p.sort, s, skip_hist0); });
+    if (e == hipSuccess && !p.grid) {
+        e = stage(2, [&] { return launch_windows(p.bins, s); });
+        if (e == hipSuccess)
+            e = stage(4, [&] {
+                return kind == RTSIM_SENSOR_2D ? launch_conv2d_plain(p.conv, s) : launch_conv1d_plain(p.conv, s);
+            });
+    } else if (e == hipSuccess) {
+        e = stage(3, [&] { return launch_deposit(p.dep, s); });
+        if (e == hipSuccess)
+            e = stage(4, [&] { return launch_correlate(p.corr, form == RTSIM_FORM_MFMA_FP16, s); });
+    }
+    profiling = prof;
+    if (e != hipSuccess) return hip_fail(e, "launching the ping");
+    return RTSIM_OK;
+}
+
+ConvArgs Sensor::conv_args(const PingBuffers& pb) const
 {
     ConvArgs c;
-    c.sorted = d_sorted;
-    c.coord = d_coord;
-    c.win = d_win;
+    c.sorted = pb.sorted;
+    c.coord = pb.coord;
+    c.win = pb.win;
     c.range_c = d_rng;
     c.bear_c = d_bear;
     c.psf = d_psf;
-    c.out = d_out;
+    c.out = pb.out;
     c.H = (int)N;
     c.W = (int)W;
     c.Tr = (int)Tr;

@@ -364,18 +364,18 @@ static bool key_travel_interval(const KeyParams& k, float inside, float* lo, flo
     return true;
 }
 
-int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
+int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp, const PingBuffers& pb)
 {
     int rc = ensure_grid();
     if (rc) return rc;
     GridState* g = static_cast<GridState*>(grid_state);
 
     DepositArgs& d = *dp;
-    d.sorted = d_sorted;
-    d.xg = reinterpret_cast<const float*>(d_coord);   // written by the sort's last pass
+    d.sorted = pb.sorted;
+    d.xg = reinterpret_cast<const float*>(pb.coord);   // written by the sort's last pass
     if (!key_travel_interval(key, 0.5f * (range_min + range_max), &d.t_lo, &d.t_hi))
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: the range midpoint has no Binner key");
-    d.n = (uint32_t)count;
+    d.n = pb.n;
     d.row_start = g->d_row_start;
     d.vmax = g->d_vmax;
     d.grid = g->d_grid;
@@ -390,8 +390,8 @@ int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp)
     CorrArgs& c = *cp;
     c.grid = g->d_grid;
     c.toep = g->d_toep[form == RTSIM_FORM_MFMA_FP16 ? 1 : 0];
-    c.win = d_win;
-    c.out = d_out;
+    c.win = pb.win;
+    c.out = pb.out;
     c.H = (int)N;
     c.W = (int)W;
     c.Tr = (int)Tr;

@@ -39,6 +39,26 @@ struct PingPlan {
     CorrArgs corr{};
 };
 
+// Device buffers one ping runs through: the sensor's own (Sensor::own_buffers)
+// or ones a caller provides (the scratch of rtsim_sensor_compute_noalloc).
+struct PingBuffers {
+    const float4* samples;   // unsorted records (read only)
+    uint32_t n;              // samples
+    uint32_t tiles_cap;      // tile-count columns of `counts`
+    float4* sorted;
+    float4* tmp;             // records between grouping-sort passes
+    float2* coord;           // bearing-axis coordinates (grid form: grid columns)
+    uint2* kv[2];
+    uint32_t* order;         // introspection only (may be null otherwise)
+    uint32_t* skeys;         // introspection only (may be null otherwise)
+    uint32_t* counts;
+    uint32_t* totals;        // 256 words
+    uint32_t* tmax;
+    uint2* seg;              // [N]
+    uint2* win;              // [N]
+    float2* out;             // [N][W]
+};
+
 struct StageEvent {
     int stage;
     hipEvent_t start, stop;
@@ -130,14 +150,19 @@ struct Sensor {
     bool grid_ready() const;         // grid form prepared (no preparation is started)
     int grid_row_map(RowMap* m, uint32_t** vmax = nullptr);   // grid form row table, G-row map
     int compute(int form, hipStream_t s);
-    int grid_plan(int form, DepositArgs* d, CorrArgs* c);
+    // queues a planned ping (profiling events only when `profile`)
+    int launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hist0, bool profile);
+    int grid_plan(int form, DepositArgs* d, CorrArgs* c, const PingBuffers& pb);
     // validates the form; queues nothing.  introspect: the sort also writes
     // the permutation and the Binner keys.  batched: for the batched launches
     // (their sort tiles)
-    int plan(int form, PingPlan* p, bool introspect = false, bool batched = false);
+    int plan(int form, PingPlan* p, bool introspect = false, bool batched = false,
+             const PingBuffers* pb = nullptr);
+    PingBuffers own_buffers() const;
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;
-    ConvArgs conv_args() const;
+    ConvArgs conv_args() const { return conv_args(own_buffers()); }
+    ConvArgs conv_args(const PingBuffers& pb) const;
     hipEvent_t take_event();
     void stage_begin(int stage, hipStream_t s);
     void stage_end(hipStream_t s);
