@@ -71,13 +71,28 @@ def stage_profile(R, sensor, dhits, stream, form, steps):
 def cpu_baseline(cfg, records, budget_s=20.0):
     """The oracle's FP32 restatement of the reference pipeline (sort, keys,
     segments, windows, per-cell gather over the window) on the host cores:
-    the binning on all samples, the gather on a bounded set of range rows,
-    extrapolated to the whole ping (rows are independent)."""
+    the binning on all samples, the gather on every row for a 1D ping, else
+    on a bounded set of range rows spread over the ping and extrapolated to
+    all of them (rows are independent)."""
     import oracle as O
     threads = O.nthreads()
     t0 = time.perf_counter()
     srt, win = O.cpu_binning(cfg, records, threads)
     t_bin = time.perf_counter() - t0
+    n = records.shape[0]
+    if cfg.kind == 1:
+        reps, t_g = 0, 0.0
+        while t_g < min(1.0, budget_s) or reps == 0:
+            t0 = time.perf_counter()
+            O.cpu_gather1d(cfg, srt, win, threads)
+            t_g += time.perf_counter() - t0
+            reps += 1
+        t_ping = t_bin + t_g / reps
+        return {"value": 1.0 / t_ping, "unit": "pings/s", "cores": threads, "kind": "port",
+                "sample": f"FP32 port of the reference pipeline on {cpu_model()}: sort+keys+segments+"
+                          f"windows on all {n} samples ({t_bin:.4f} s) + the gather over all "
+                          f"{cfg.H} bins, mean of {reps} runs", "ms_per_ping": 1e3 * t_ping,
+                "msamples_per_s": n / t_ping / 1e6}
     # rows spread over the range axis; grow the sample until about half the
     # budget has been spent on gathers
     rows_total = 0
@@ -98,10 +113,23 @@ def cpu_baseline(cfg, records, budget_s=20.0):
     t_ping = t_bin + per_row * cfg.H
     return {"value": 1.0 / t_ping, "unit": "pings/s", "cores": threads, "kind": "port",
             "sample": f"FP32 port of the reference pipeline on {cpu_model()}: sort+keys+"
-                      f"segments+windows on all {records.shape[0]} samples ({t_bin:.3f} s) + "
+                      f"segments+windows on all {n} samples ({t_bin:.3f} s) + "
                       f"per-cell gather on {rows_total} of {cfg.H} range rows "
                       f"({t_rows:.2f} s), extrapolated linearly to {cfg.H} rows",
-            "ms_per_ping": 1e3 * t_ping}
+            "ms_per_ping": 1e3 * t_ping, "msamples_per_s": n / t_ping / 1e6}
+
+
+def oracle_rows_error(cfg, records, pings: dict, every: int = 32, start: int = 5):
+    """Relative L2 of each ping vs the FP64 oracle on every `every`-th range
+    row (all rows of a 1D ping)."""
+    import oracle as O
+    b = O.binning(cfg, records)
+    if cfg.kind == 1:
+        ref = O.conv(cfg, b["sorted"], b["windows"])
+        return {k: O.rel_l2(v, ref) for k, v in pings.items()}
+    rows = np.arange(start, cfg.H, every)
+    ref = O.conv_rows(cfg, b["sorted"], b["windows"], rows)
+    return {k: O.rel_l2(v[rows], ref) for k, v in pings.items()}
 
 
 def accuracy(cfg, sensor, plain_ping, form_pings, dbgs):
@@ -195,11 +223,15 @@ def measured_traffic(workload: str, form: str, stage: str):
         return None
 
 
-def bench_batch(R, configs, stream, steps):
+def bench_batch(R, configs, stream, steps, cpu=True):
     """The 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 2D and
     8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch:
     MFMA FP16 form for the 2D sensors, plain form for the 1D ones.  Hits are
-    prefixes of two 4M-hit sets (one per scene)."""
+    prefixes of two 4M-hit sets (one per scene).  Also: the same batch in the
+    plain form (the MFMA pings' error against it), one ping per sensor class
+    against the FP64 oracle, and the CPU port on one ping per class scaled to
+    the whole batch."""
+    import oracle as O
     items = configs.batch_uneven()
     big = {}
     sensors, forms, total = [], [], 0
@@ -225,10 +257,53 @@ def bench_batch(R, configs, stream, steps):
     timer.stop(stream)
     stream.sync()
     ms = timer.elapsed_ms() / steps
-    return {"workload": "64 uneven pings (16x 256x512, 32x 512x1024, 8x 1024x2048, "
-                        "8x 1D 2048), one hipGraph launch", "pings": len(items),
-            "samples": int(total), "ms_per_batch": ms, "pings_per_s": len(items) / (ms * 1e-3),
-            "msamples_per_s": total / (ms * 1e-3) / 1e6, "steps": steps}
+    out = {"workload": "64 uneven pings (16x 256x512, 32x 512x1024, 8x 1024x2048, "
+                       "8x 1D 2048), one hipGraph launch", "pings": len(items),
+           "samples": int(total), "ms_per_batch": ms, "pings_per_s": len(items) / (ms * 1e-3),
+           "msamples_per_s": total / (ms * 1e-3) / 1e6, "steps": steps}
+    mf = [s.output().copy() for s in sensors]
+    # the same batch in the plain FP32 form
+    bp = R.Batch(sensors, [R.FORM_PLAIN_FP32] * len(sensors))
+    for i, (cfg, scene, S, seed) in enumerate(items):
+        bp.bind_hits(i, big[cfg.kind].prefix(S))
+    bp.compute(stream)
+    stream.sync()
+    timer.start(stream)
+    bp.compute(stream)
+    timer.stop(stream)
+    stream.sync()
+    pl = [s.output().copy() for s in sensors]
+    out["plain_fp32_ms_per_batch"] = timer.elapsed_ms()
+    out["mfma_vs_plain_speedup"] = out["plain_fp32_ms_per_batch"] / ms
+    out["mfma_vs_plain_rel_l2_max"] = max(O.rel_l2(a, c) for a, c, f in zip(mf, pl, forms)
+                                          if f != R.FORM_PLAIN_FP32)
+    # one ping per class (first of each) against the oracle, and the CPU port
+    classes, errs, cpu_ms, reps = {}, {}, 0.0, []
+    for i, (cfg, scene, S, seed) in enumerate(items):
+        classes.setdefault((cfg.kind, cfg.W, cfg.H), []).append(i)
+    for key, idx in classes.items():
+        i = idx[0]
+        cfg, _, S, _ = items[i]
+        rec = sensors[i].samples()
+        e = oracle_rows_error(cfg, rec, {"mfma_fp16": mf[i], "plain_fp32": pl[i]}, every=64)
+        if cfg.kind == 1:
+            e.pop("mfma_fp16")
+        errs[f"{cfg.W}x{cfg.H}" if cfg.kind == 2 else f"1D {cfg.H}"] = e
+        if cpu:
+            c = cpu_baseline(cfg, rec, budget_s=4.0)
+            # the gather scales with the samples of each ping of the class
+            per = c["ms_per_ping"] / S
+            cpu_ms += sum(per * items[j][2] for j in idx)
+            reps.append(f"{cfg.W}x{cfg.H} ({S} samples)")
+    out["rel_l2_vs_fp64_oracle_one_ping_per_class"] = errs
+    if cpu:
+        out["cpu_baseline"] = {
+            "value": len(items) / (cpu_ms * 1e-3), "unit": "pings/s", "cores": O.nthreads(),
+            "kind": "port", "ms_per_batch": cpu_ms,
+            "sample": "FP32 port timed on one ping per sensor class (" + ", ".join(reps) +
+                      "; 2D: binning on all samples + the gather on a row subset, extrapolated to "
+                      "all rows), each class scaled to its pings by sample count"}
+    return out
 
 
 def bench_repeat(R, cfg, dhits, S, stream, pings=8, steps=10):
@@ -261,28 +336,87 @@ def bench_repeat(R, cfg, dhits, S, stream, pings=8, steps=10):
             "steps": steps}
 
 
-def bench_large(R, configs, stream, steps):
-    """The 1024 x 4096 ping from 16M samples (63 x 33 PSF): MFMA FP16 after
-    autotuning against one plain-form ping."""
+def bench_large(R, configs, stream, steps, cpu=True):
+    """The 1024 x 4096 ping from 16M samples (63 x 33 PSF): both MFMA forms
+    after autotuning, one plain-form ping, each against the FP64 oracle on
+    every 64th range row, binning checked against the oracle, and the CPU
+    port on a row subset (extrapolated)."""
+    import oracle as O
     cfg, scene, S, seed = configs.workload("fls_1024x4096")
     dh = R.generate_hits(scene, seed, S, nthreads=min(16, os.cpu_count() or 1)).to_device()
     s = R.make_sensor(cfg)
     s.set_sample_count(S)
     s.write_samples(dh, stream=stream)
-    tile = s.autotune(R.FORM_MFMA_FP16, 5, stream)[0]
-    ms_mfma = time_form(R, s, dh, stream, R.FORM_MFMA_FP16, steps, 1)
-    mf = s.output()
+    res, pings, tiles = {}, {}, {}
+    for name, form in (("mfma_bf16", R.FORM_MFMA_BF16), ("mfma_fp16", R.FORM_MFMA_FP16)):
+        s.write_samples(dh, stream=stream)
+        tiles[name] = [s.autotune(form, 5, stream)[0], s.tile_waves()]
+        ms = time_form(R, s, dh, stream, form, steps, 1)
+        pings[name] = s.output()
+        res[name] = {"ms_per_ping": ms, "pings_per_s": 1e3 / ms, "msamples_per_s": S / (ms * 1e-3) / 1e6,
+                     "tile_shape": tiles[name][0], "tile_waves": tiles[name][1]}
     ms_plain = time_form(R, s, dh, stream, R.FORM_PLAIN_FP32, 1, 0)
-    pl = s.output()
-    import oracle as O
-    return {"workload": f"fls_1024x4096: {cfg.W} bearings x {cfg.H} ranges, {S} samples, "
-                        f"{cfg.Tr}x{cfg.Tb} PSF", "samples": S, "tile_shape": tile,
-            "tile_waves": s.tile_waves(),
-            "mfma_fp16": {"ms_per_ping": ms_mfma, "pings_per_s": 1e3 / ms_mfma,
-                          "msamples_per_s": S / (ms_mfma * 1e-3) / 1e6},
-            "plain_fp32": {"ms_per_ping": ms_plain, "pings_per_s": 1e3 / ms_plain},
-            "mfma_vs_plain_speedup": ms_plain / ms_mfma,
-            "mfma_vs_plain_rel_l2": O.rel_l2(mf, pl)}
+    pings["plain_fp32"] = s.output()
+    res["plain_fp32"] = {"ms_per_ping": ms_plain, "pings_per_s": 1e3 / ms_plain,
+                         "msamples_per_s": S / (ms_plain * 1e-3) / 1e6}
+    rec = s.samples()
+    dbg = s.binner_debug()
+    b = O.binning(cfg, rec)
+    exact = bool(all(np.array_equal(dbg[k], b[k]) for k in ("order", "keys", "segments", "windows")))
+    errs = oracle_rows_error(cfg, rec, pings, every=64)
+    for k in res:
+        res[k]["rel_l2_vs_fp64_oracle"] = errs[k]
+    best = min(("mfma_bf16", "mfma_fp16"), key=lambda k: res[k]["ms_per_ping"])
+    out = {"workload": f"fls_1024x4096: {cfg.W} bearings x {cfg.H} ranges, {S} samples, "
+                       f"{cfg.Tr}x{cfg.Tb} PSF", "samples": S, "forms": res,
+           "binning_bit_exact_plain_form": exact,
+           "mfma_vs_plain_speedup": ms_plain / res[best]["ms_per_ping"],
+           "mfma_vs_plain_rel_l2": O.rel_l2(pings[best], pings["plain_fp32"]),
+           "error_rows": "every 64th range row vs the FP64 oracle"}
+    if cpu:
+        out["cpu_baseline"] = cpu_baseline(cfg, rec, budget_s=12.0)
+    return out
+
+
+def bench_small_configs(R, configs, stream, steps, cpu=True):
+    """The echo-sounder ping (1D, 20 000 samples, 2048 bins, 33 taps; plain
+    form: the MFMA forms are 2D only) and the 512 x 1024 ping with the 15 x 9
+    separable PSF (plain and both MFMA forms), timed like the headline ping,
+    each against the FP64 oracle, with the CPU port beside them."""
+    out = {}
+    for name, forms in (("single_beam_38k", {"plain_fp32": R.FORM_PLAIN_FP32}),
+                        ("fls_512x1024_sep", {"plain_fp32": R.FORM_PLAIN_FP32,
+                                              "mfma_bf16": R.FORM_MFMA_BF16,
+                                              "mfma_fp16": R.FORM_MFMA_FP16})):
+        cfg, scene, S, seed = configs.workload(name)
+        dh = R.generate_hits(scene, seed, S, nthreads=min(16, os.cpu_count() or 1)).to_device()
+        s = R.make_sensor(cfg)
+        s.set_sample_count(S)
+        res, pings = {}, {}
+        for fname, form in forms.items():
+            if form != R.FORM_PLAIN_FP32:
+                s.write_samples(dh, stream=stream)
+                s.autotune(form, 10, stream)
+            k = steps if form != R.FORM_PLAIN_FP32 or cfg.kind == 1 else max(3, steps // 10)
+            ms = time_form(R, s, dh, stream, form, k, 2)
+            pings[fname] = s.output()
+            res[fname] = {"ms_per_ping": ms, "pings_per_s": 1e3 / ms,
+                          "msamples_per_s": S / (ms * 1e-3) / 1e6, "steps": k}
+        rec = s.samples()
+        errs = oracle_rows_error(cfg, rec, pings)
+        for fname in res:
+            res[fname]["rel_l2_vs_fp64_oracle"] = errs[fname]
+        o = {"workload": f"{name}: {cfg.W} x {cfg.H}, {S} samples, {cfg.Tr}x{cfg.Tb} PSF",
+             "forms": res}
+        if cfg.kind == 2:
+            best = min((k for k in res if k != "plain_fp32"), key=lambda k: res[k]["ms_per_ping"])
+            o["mfma_vs_plain_speedup"] = res["plain_fp32"]["ms_per_ping"] / res[best]["ms_per_ping"]
+        else:
+            o["note"] = "the MFMA forms restate 2D pings only (RTSIM_ERR_UNSUPPORTED for 1D)"
+        if cpu:
+            o["cpu_baseline"] = cpu_baseline(cfg, rec, budget_s=8.0)
+        out[name] = o
+    return out
 
 
 def main():
@@ -374,9 +508,12 @@ def main():
     if not args.no_extra:
         extra["batch_same"] = bench_repeat(R, cfg, dhits, S, stream)
         print(f"# repeated {extra['batch_same']}", file=sys.stderr, flush=True)
-        extra["batch_uneven"] = bench_batch(R, configs, stream, 3)
+        extra.update(bench_small_configs(R, configs, stream, args.steps, cpu=not args.no_cpu))
+        print(f"# small {extra['single_beam_38k']} {extra['fls_512x1024_sep']}", file=sys.stderr,
+              flush=True)
+        extra["batch_uneven"] = bench_batch(R, configs, stream, 3, cpu=not args.no_cpu)
         print(f"# batch {extra['batch_uneven']}", file=sys.stderr, flush=True)
-        extra["fls_1024x4096"] = bench_large(R, configs, stream, 5)
+        extra["fls_1024x4096"] = bench_large(R, configs, stream, 5, cpu=not args.no_cpu)
         print(f"# large {extra['fls_1024x4096']}", file=sys.stderr, flush=True)
 
     W, H, Tr, Tb = cfg.W, cfg.H, cfg.Tr, cfg.Tb

@@ -59,6 +59,7 @@ _SIGS = {
     "orc_grid2d": (None, [_P, C.c_size_t, C.POINTER(Key), _P, C.POINTER(Grid), C.c_double,
                           C.c_double, _P, _P]),
     "cpu_binning": (None, [_P, C.c_size_t, C.POINTER(Grid), C.c_float, _P, _P, C.c_int]),
+    "cpu_gather1d_f32": (None, [_P, _P, C.POINTER(Grid), _P, _P, C.c_int]),
     "cpu_gather2d_f32": (None, [_P, _P, C.POINTER(Grid), _P, _P, _P, C.c_long, C.c_long,
                                 C.c_int]),
 }
@@ -297,6 +298,16 @@ def cpu_binning(cfg, records: np.ndarray, threads: int | None = None):
     lib().cpu_binning(_p(rec), rec.shape[0], C.byref(g), effective_margin(cfg), _p(srt),
                       _p(win), threads or nthreads())
     return srt, win
+
+
+def cpu_gather1d(cfg, srt, win, threads: int | None = None):
+    out = np.zeros(2 * cfg.H, np.float32)
+    g = grid(cfg)
+    taps = _taps(cfg)          # held: the call reads these through raw pointers
+    s = np.ascontiguousarray(srt, np.float32)
+    w = np.ascontiguousarray(win, np.uint32)
+    lib().cpu_gather1d_f32(_p(s), _p(w), C.byref(g), _p(taps), _p(out), threads or nthreads())
+    return out.view(np.complex64)
 
 
 def cpu_gather2d(cfg, srt, win, row_begin: int, row_end: int, threads: int | None = None):

@@ -556,6 +556,33 @@ void cpu_binning(const float* samples, size_t n, const orc_grid* g, float margin
     orc_windows(seg.data(), g->range_count, ov, win);
 }
 
+// 1D: do_sparse_convolve_1d_f (SensorInstance1D.cu:35-72) in FP32, every
+// window sample through the linearly interpolated range PSF (scaling
+// 1/(c tau), the documented deviation of the 1D kernel), border 0.
+void cpu_gather1d_f32(const float* sorted, const uint32_t* win, const orc_grid* g,
+                      const float* taps, float* out, int nthreads)
+{
+    const long N = g->range_count;
+    const float sy = 1.0f / g->psf_range_span;
+    const float step = N > 1 ? (g->range_hi - g->range_lo) / (float)(N - 1) : 0.f;
+    parallel_rows(nthreads, 0, N, [&](long r0, long r1) {
+        for (long r = r0; r < r1; r++) {
+            const float rr = g->range_lo + (float)r * step;
+            const uint32_t st = win[2 * r], sz = win[2 * r + 1];
+            float ar = 0.f, ai = 0.f;
+            for (uint32_t k = 0; k < sz; k++) {
+                const float* s = sorted + 4 * (size_t)(st + k);
+                float kr, ki;
+                psf_f32(taps, g->psf_range_taps, 1, 0.f, sy, 0.f, s[2] - rr, &kr, &ki);
+                ar += s[0] * kr - s[1] * ki;
+                ai += s[0] * ki + s[1] * kr;
+            }
+            out[2 * r] = ar;
+            out[2 * r + 1] = ai;
+        }
+    });
+}
+
 void cpu_gather2d_f32(const float* sorted, const uint32_t* win, const orc_grid* g,
                       const float* bearings, const float* taps, float* out,
                       long row_begin, long row_end, int nthreads)

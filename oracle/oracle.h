@@ -100,6 +100,8 @@ void orc_grid2d(const float* samples, size_t n, const orc_key* key,
                 double bearing_pitch, const float* taps, double* out);
 void cpu_binning(const float* samples, size_t n, const orc_grid* g, float margin,
                  float* sorted, uint32_t* win, int nthreads);
+void cpu_gather1d_f32(const float* sorted, const uint32_t* win, const orc_grid* g,
+                      const float* taps, float* out, int nthreads);
 void cpu_gather2d_f32(const float* sorted, const uint32_t* win, const orc_grid* g,
                       const float* bearings, const float* taps, float* out,
                       long row_begin, long row_end, int nthreads);

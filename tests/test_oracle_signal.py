@@ -348,3 +348,19 @@ def test_cpu_baseline_gather_computes_the_ping():
     ref = O.conv_rows(cfg, b["sorted"], b["windows"], rows, threads=4)
     assert np.abs(ref).max() > 0
     assert O.rel_l2(base, ref) <= 1e-5
+
+
+def test_cpu_baseline_1d_gather_computes_the_ping():
+    """The 1D CPU baseline (FP32 port of SensorInstance1D.cu:35-72) against the
+    FP64 oracle gather on the echo-sounder workload.  Like the reference it
+    forms t - r_r in FP32: at 50 m one ulp of travel is 1.6e-4 of the 24 mm
+    tap pitch, so the port is held to 1e-4 (the HIP plain form, with
+    double-float range coordinates, meets 1e-5)."""
+    cfg, scene, S, seed = configs.workload("single_beam_38k")
+    hits = R.generate_hits(scene, seed, S)
+    rec = O.stream(cfg, hits).astype(np.float32)
+    srt, win = O.cpu_binning(cfg, rec, 2)
+    base = O.cpu_gather1d(cfg, srt, win, 2)
+    ref, _ = O.reference_ping(cfg, rec, threads=2)
+    assert np.abs(ref).max() > 0
+    assert O.rel_l2(base, ref) <= 1e-4
