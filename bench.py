@@ -93,22 +93,19 @@ def cpu_baseline(cfg, records, budget_s=20.0):
                           f"windows on all {n} samples ({t_bin:.4f} s) + the gather over all "
                           f"{cfg.H} bins, mean of {reps} runs", "ms_per_ping": 1e3 * t_ping,
                 "msamples_per_s": n / t_ping / 1e6}
-    # rows spread over the range axis; grow the sample until about half the
-    # budget has been spent on gathers
+    # rows spread over the range axis (golden-ratio sequence), one at a time,
+    # until about half the budget has been spent on gathers (at least 4 rows)
+    rows = list(dict.fromkeys(int(((0.5 + i * 0.6180339887498949) % 1.0) * cfg.H)
+                              for i in range(4 * cfg.H)))
     rows_total = 0
     t_rows = 0.0
-    step = 64
-    start = 3
-    while t_rows < budget_s / 2 and rows_total < cfg.H:
-        rows = list(range(start, cfg.H, step))
-        t0 = time.perf_counter()
-        for r in rows:
-            O.cpu_gather2d(cfg, srt, win, r, r + 1, threads)
-        t_rows += time.perf_counter() - t0
-        rows_total += len(rows)
-        start += 7
-        if start >= step:
+    for r in rows:
+        if rows_total >= 4 and t_rows >= budget_s / 2:
             break
+        t0 = time.perf_counter()
+        O.cpu_gather2d(cfg, srt, win, r, r + 1, threads)
+        t_rows += time.perf_counter() - t0
+        rows_total += 1
     per_row = t_rows / rows_total
     t_ping = t_bin + per_row * cfg.H
     return {"value": 1.0 / t_ping, "unit": "pings/s", "cores": threads, "kind": "port",

@@ -45,89 +45,86 @@ struct StreamArgs {
     float dir_xs, dir_ys;
 };
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
-struct HistArgs;
-// the streaming kernel fused with the grid form's first sort histogram h
-// (tiles of kSortTileSmall samples; a.n == h.n, a.out == h.rec)
-hipError_t launch_stream_hist(const StreamArgs& a, const HistArgs& h, hipStream_t s);
 uint32_t stream_blocks(const StreamArgs& a);
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 
-// G-row map of the grid (MFMA) form: rho = floor(fma(t - range0, inv_dr, off_r)) + fr.
-// row_start[rho + 1] = first sorted sample whose G row is rho, rho in [-1, GH + 1]
-// (n when the row has none).
-struct RowMap {
+// ---- grid (MFMA) form binning: samples bucketed by blocks of grid rows ------
+// The deposit of the grid form sums every sample exactly (fixed point), so it
+// needs the samples grouped by grid row, in any order inside a group, and
+// from the Binner only which keys hold samples (the windows' emptiness).
+// One counting pass does that: a sample with a Binner key whose range
+// coordinate y = fma(t - range0, inv_dr, off_r) gives G row gy = floor(y) + fr
+// adds weight 1 - frac(y) to grid row gy and frac(y) to row gy + 1; it goes
+// to the bucket of each of those rows inside the grid (buckets of R rows, a
+// sample on a bucket's last row goes to two buckets), as an item
+// {re, im, travel, grid column}.
+struct GridRows {
     float range0, inv_dr, off_r;
     int fr, GH;
-    uint32_t* row_start;  // GH + 3 entries, pre-filled with n; nullptr: no row table
-    // grid column coordinate of a bearing-axis value x: (x - bear0) inv_db + off_b
+};
+struct BucketArgs {
+    const float4* rec;    // records (unsorted)
+    uint32_t n;
+    uint32_t tile;        // samples per tile (a multiple of 1024), one workgroup each
+    uint32_t tiles;
+    KeyParams key;        // Binner key: a sample without one is not binned
+    GridRows rows;
+    int R, NB;            // grid rows per bucket, buckets = ceil(GH / R)
+    uint32_t* counts;     // [NB][tiles]: items per (bucket, tile), then their exclusive scan over tiles
+    uint32_t* totals;     // [NB] items per bucket
+    uint32_t* tmax;       // [tiles] largest finite |re|, |im| of the tile's binned samples (FP32 bits)
+    uint32_t* vmax;       // [1] the largest tile maximum: the deposit's fixed-point scale
+    float4* items;        // {re, im, travel, grid column} of every binned sample, by bucket
+    uint32_t* occ;        // [tiles][occ_words]: bit k set when Binner key k holds a sample of the tile
+    int occ_words;        // ceil(N / 32)
+    // grid column of a sample: (x - bear0) inv_db + off_b, x = sin(bearing)
+    // (axis 1, in FP64) or the bearing
     double bear0;
     float inv_db, off_b;
+    int axis;
 };
-
-// Sort key of the grid (MFMA) form.  K = (G row + 2) + bin slot, where the
-// bin slot is 0 below the Binner's range, key + 1 inside it and n + 1 above
-// it (NaN included).  Both terms are non-decreasing in travel, so samples of
-// equal K share their bin and their G row, and the stable order of K is the
-// travel order up to a permutation inside each run of equal (bin, G row):
-// keys, segments, windows and the G-row table come out exactly as after the
-// travel sort, in ceil(bits(K) / 8) passes instead of 4.
-// With a grouping key the first histogram also writes tmax[tile], the largest
-// finite |re|, |im| of the tile's in-range samples (FP32 bits), and the first
-// scan reduces them into vmax[0]: the deposit's fixed-point scale (no
-// atomics: a few dozen on one address serialise at the end of a kernel).
-struct GroupKey {
-    int on;               // 0: sort by travel image
-    RowMap rows;
-    KeyParams key;
-    uint32_t* vmax;       // 1 word
-    uint32_t* tmax;       // one word per tile
-};
-// largest K of a grouping (the sort runs as many 8-bit passes as it needs)
-inline uint32_t group_key_max(const GroupKey& g) { return (uint32_t)g.rows.GH + 4 + g.key.n + 1; }
+// the streaming kernel fused with the bucket counts of the grid form (tiles
+// of 1024 samples; b.rec == a.out, b.n == a.n)
+hipError_t launch_stream_bucket(const StreamArgs& a, const BucketArgs& b, hipStream_t s);
+// bucket counts and tile maxima (from the records), their scan, the scatter
+// of the items; blocks per launch: tiles, NB, tiles
+hipError_t launch_bucket_hist(const BucketArgs& b, hipStream_t s);
+hipError_t launch_bucket_scan(const BucketArgs& b, hipStream_t s);
+hipError_t launch_bucket_scatter(const BucketArgs& b, hipStream_t s);
+size_t bucket_lds_bytes(const BucketArgs& b);
+hipError_t launch_bucket_hist_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
+                                    hipStream_t s);
+hipError_t launch_bucket_scan_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
+hipError_t launch_bucket_scatter_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
+                                       hipStream_t s);
+constexpr int kMaxBuckets = 8192;        // buckets of one ping (LDS counters)
+constexpr int kMaxOccWords = 2048;       // Binner keys of a grid-form ping: 65536
 
 // ---- sort by travel: stable LSD radix sort of (travel image, index) ---------
-// Items per sort workgroup (256 lanes x 8 or x 4).  The last pass of one
-// ping's sort whose records move uses the large tiles (at 1M samples its
-// tiles fit the chip's workgroup slots either way, and twice the small
-// tiles' run-bound atomics cost more than the large tiles' lower
-// parallelism), every other pass the small ones (more workgroups hide the
-// latency of their loads).  Batched launches use the small tiles in every
-// pass: a batch queues many workgroups per slot, and the small tiles' five
-// resident workgroups per CU (against two) finish them sooner.
-constexpr uint32_t kSortTile = 2048;
+// Four 8-bit passes over tiles of 1024 items (256 lanes x 4).  Used by the
+// plain form (its gather reads the samples of each window in travel order)
+// and by Binner introspection.
 constexpr uint32_t kSortTileSmall = 1024;
-constexpr uint32_t sort_pass_tile(bool last, bool move, bool batched = false)
-{
-    return last && move && !batched ? kSortTile : kSortTileSmall;
-}
+constexpr uint32_t kSortTile = kSortTileSmall;
+constexpr uint32_t sort_pass_tile(bool, bool, bool = false) { return kSortTileSmall; }
 // per-tile digit counts of one pass for `capacity` items
 inline size_t sort_count_words(size_t capacity)
 {
     return 256 * ((capacity + kSortTileSmall - 1) / kSortTileSmall);
 }
 // Written by the last pass for every sorted position i: the record and its
-// bearing-axis coordinate, as (hi, lo) (coord) or, for the grid form, as a
-// grid column coordinate (xg); when skeys is set, also its Binner key
-// (Binner.cu:24-49).
-// The last pass also bounds the runs of equal keys (Binner.cu:51-80): each
-// workgroup lowers seg[k].x to the first and raises seg[k].y past the last
-// position of every key run it writes (atomic min / max; the runs of one key
-// are contiguous, so the result is the segment), and with a grouping key it
-// lowers row_start[rho + 1] to the first position of every G row rho it
-// writes (rows without samples keep the fill value n).
+// bearing-axis coordinate as (hi, lo) (coord); when skeys is set, also its
+// Binner key (Binner.cu:24-49).  The last pass also bounds the runs of equal
+// keys (Binner.cu:51-80): each workgroup lowers seg[k].x to the first and
+// raises seg[k].y past the last position of every key run it writes (atomic
+// min / max; the runs of one key are contiguous, so the result is the segment).
 struct SortOutputs {
     float4* sorted;
     uint32_t* skeys;      // may be null (the keys are not needed downstream)
-    float2* coord;        // used when xg is null
-    float* xg;            // (x - xg_b0) xg_inv + xg_off of x = the bearing-axis value
-    double xg_b0;
-    float xg_inv, xg_off;
+    float2* coord;
     KeyParams key;
     int axis;             // 1: coordinate = sin(bearing) in FP64, else the bearing
     uint2* seg;           // [key.n], initialised to (~0, 0) by the first pass
-    // 1: seg only marks the keys that hold samples, as (0, 1) (a window's
-    // emptiness is all the grid form reads); 0: the Binner segments
-    int seg_marks;
 };
 struct SortBuffers {
     uint2* kv[2];         // (key, sample index) ping-pong between passes
@@ -135,13 +132,8 @@ struct SortBuffers {
     uint32_t* counts;     // [256 digits][tiles_cap]: tile counts, then their exclusive scan
     uint32_t* totals;     // [256] digit totals of the current pass
     uint32_t tiles_cap;
-    float4* tmp;          // records between passes when they move with their keys
-    // prepared by the first sort kernel (before any later kernel reads them)
-    uint2* seg_init;      // seg_init[0..seg_count) = (~0, 0) (Binner segments, min / max)
-    size_t seg_count;
-    uint32_t* fill;       // fill[0..fill_count) = fill_value (grid row table)
-    uint32_t fill_value;
-    size_t fill_count;
+    uint2* seg_init;      // seg_init[0..seg_count) = (~0, 0) (Binner segments, min / max),
+    size_t seg_count;     // prepared by the first histogram
     SortOutputs out;
 };
 // One pass of the sort: per-tile digit counts, their scan, the scatter.
@@ -151,20 +143,19 @@ struct HistArgs {
     uint32_t n;
     int shift;
     uint32_t mask;            // digit = (key >> shift) & mask
-    GroupKey gk;              // pass 0 key: K when gk.on, else the travel image
     uint32_t* counts;         // [256 digits][tiles]
     uint32_t tiles;
-    uint2* seg_init;          // pass 0 only: initialised / filled arrays
+    uint2* seg_init;          // pass 0 only
     size_t seg_count;
-    uint32_t* fill;
-    uint32_t fill_value;
-    size_t fill_count;
 };
+// Exclusive scan of rows of tile counts: one workgroup per row (a digit, or a
+// bucket of the grid form), the row total in totals[row]; with tmax, row 0's
+// workgroup also writes vmax[0] = max of tmax[0..tiles).
 struct ScanArgs {
     uint32_t* counts;
     uint32_t tiles;
     uint32_t* totals;
-    const uint32_t* tmax;     // first pass of a grouping sort: vmax[0] = max of tmax[0..tiles)
+    const uint32_t* tmax;
     uint32_t* vmax;
 };
 struct ScatterArgs {
@@ -174,41 +165,31 @@ struct ScatterArgs {
     uint32_t* order;          // last pass: the permutation (null: not written)
     uint32_t n;
     int shift;
-    uint32_t mask;            // digit = (key >> shift) & mask, mask = 2^dbits - 1
-    int dbits;
+    uint32_t mask;            // digit = (key >> shift) & mask
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;
-    GroupKey gk;              // as HistArgs
-    int move;                 // records move with their keys (grouping key)
-    const float4* rin;        // move: records in the order of kvin (pass 0: the input)
-    float4* rout;             // move, not the last pass: records in the order of kvout
     SortOutputs out;          // last pass only
 };
 // Arguments and workgroup counts of the 3 launches per pass of one sort.
 struct SortPlan {
-    int passes;               // 4 for travel images, fewer for a grouping key
+    int passes;               // 4
     HistArgs hist[4];
     ScanArgs scan[4];
     ScatterArgs scatter[4];
-    uint32_t tile[4];         // items per tile of each pass (sort_pass_tile)
+    uint32_t tile[4];         // items per tile of each pass
     uint32_t hist_blocks[4];  // per pass: tiles (pass 0: >= 1 when it clears arrays)
     uint32_t pass_blocks[4];  // scan: 256 per pass, scatter: tiles (0 when n = 0)
     const uint32_t* order;    // permutation after the sort
 };
-// gk == nullptr or !gk->on: sort by travel.  batched: the plan is for the
-// batched launches (their tiles, sort_pass_tile)
-hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b,
-                          const GroupKey* gk, SortPlan* p, bool batched = false);
-// skip_hist0: the first pass's histogram (and its clears) are already made
-// (launch_stream_hist)
-hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s, bool skip_hist0 = false);
+hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p);
+hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
 // batched sort passes (args are device arrays, one entry per ping)
-hipError_t launch_sort_hist_batch(bool first_pass, bool big_tiles, const HistArgs* args, BatchMap m,
-                                  uint32_t blocks, hipStream_t s);
+hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m, uint32_t blocks,
+                                  hipStream_t s);
 hipError_t launch_sort_scan_batch(const ScanArgs* args, BatchMap m, uint32_t blocks,
                                   hipStream_t s);
-hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, bool move, const ScatterArgs* args,
+hipError_t launch_sort_scatter_batch(bool first_pass, bool last_pass, const ScatterArgs* args,
                                      BatchMap m, uint32_t blocks, hipStream_t s);
 
 // ---- windows after the sort ------------------------------------------------------
