@@ -136,11 +136,9 @@ def accuracy(cfg, sensor, plain_ping, form_pings, dbgs):
     import oracle as O
     rec = sensor.samples()
     b = O.binning(cfg, rec)
-    og = O.group_order(cfg, rec[:, 2])
     exact = True
     for name, dbg in dbgs.items():
-        order = b["order"] if name == "plain_fp32" else og
-        exact = exact and bool(np.array_equal(dbg["order"], order)
+        exact = exact and bool(np.array_equal(dbg["order"], b["order"])
                                and np.array_equal(dbg["keys"], b["keys"])
                                and np.array_equal(dbg["segments"], b["segments"])
                                and np.array_equal(dbg["windows"], b["windows"]))

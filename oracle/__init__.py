@@ -43,8 +43,6 @@ _SIGS = {
     "orc_keys": (None, [C.POINTER(Key), _P, C.c_size_t, C.c_size_t, _P]),
     "orc_travel_image": (C.c_uint32, [C.c_float]),
     "orc_sort_order": (None, [_P, C.c_size_t, C.c_size_t, _P]),
-    "orc_group_order": (None, [_P, C.c_size_t, C.c_size_t, C.POINTER(Key), C.c_float, C.c_float,
-                               C.c_float, C.c_int, C.c_int, _P]),
     "orc_segments": (None, [_P, C.c_size_t, C.c_uint32, _P]),
     "orc_overlap": (C.c_int, [C.c_float, C.c_float, C.c_float, C.c_uint32]),
     "orc_windows": (None, [_P, C.c_uint32, C.c_int, _P]),
@@ -118,27 +116,6 @@ def sort_order(travel: np.ndarray) -> np.ndarray:
     t = np.ascontiguousarray(travel, np.float32)
     out = np.empty(t.size, np.uint32)
     lib().orc_sort_order(_p(t), t.size, 1, _p(out))
-    return out
-
-
-def group_params(cfg) -> tuple:
-    """G-row map of the grid form, (range0, inv_dr, off_r, fr, GH): output
-    rows at the range step dr = (hi - lo) / (N - 1), PSF centre offset, grid
-    height padded to 64-row tiles plus the range taps."""
-    N, Tr = cfg.range_count, cfg.Tr
-    dr = (float(np.float32(cfg.range_max)) - float(np.float32(cfg.range_min))) / (N - 1)
-    fr = (Tr - 1) // 2
-    GH = -(-N // 64) * 64 + Tr - 1
-    return (float(np.float32(cfg.range_min)), float(np.float32(1.0 / dr)),
-            float(np.float32(0.5 * (Tr - 1) - fr)), fr, GH)
-
-
-def group_order(cfg, travel: np.ndarray) -> np.ndarray:
-    """Order of the grid form's grouping sort: stable by (bin slot, G row)."""
-    t = np.ascontiguousarray(travel, np.float32)
-    k = key_make(cfg.range_count, cfg.range_min, cfg.range_max, effective_margin(cfg))
-    out = np.empty(t.size, np.uint32)
-    lib().orc_group_order(_p(t), t.size, 1, C.byref(k), *group_params(cfg), _p(out))
     return out
 
 

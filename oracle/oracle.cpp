@@ -126,34 +126,6 @@ void orc_sort_order(const float* travel, size_t n, size_t stride, uint32_t* orde
                      [&](uint32_t a, uint32_t b) { return img[a] < img[b]; });
 }
 
-// Grouped order of the grid (MFMA) form: samples stably ordered by (bin
-// slot, G row), the slot being 0 below the Binner's range, key + 1 inside,
-// n + 1 above (NaN included), the G row floor(fma(t - range0, inv_dr, off_r))
-// + fr clamped to [-2, GH + 2] (NaN on top).  Lexicographic here; the
-// product encodes the pair as one sum, which this pins.
-void orc_group_order(const float* travel, size_t n, size_t stride, const orc_key* k,
-                     float range0, float inv_dr, float off_r, int fr, int GH, uint32_t* order)
-{
-    std::vector<uint32_t> slot(n);
-    std::vector<int> row(n);
-    for (size_t i = 0; i < n; i++) {
-        const float t = travel[i * stride];
-        const uint32_t key = orc_key_eval(k, t);
-        if (key != ORC_OOR) slot[i] = key + 1;
-        else slot[i] = (!std::isnan(t) && std::fmaf(k->a, t, k->b) < 0.f) ? 0u : k->n + 1;
-        const float d = t - range0;
-        const float r = std::floor(std::fmaf(d, inv_dr, off_r)) + (float)fr;
-        if (std::isnan(r)) row[i] = GH + 2;
-        else if (r < -2.f) row[i] = -2;
-        else if (r > (float)(GH + 2)) row[i] = GH + 2;
-        else row[i] = (int)r;
-    }
-    std::iota(order, order + n, 0u);
-    std::stable_sort(order, order + n, [&](uint32_t a, uint32_t b) {
-        return slot[a] != slot[b] ? slot[a] < slot[b] : row[a] < row[b];
-    });
-}
-
 // --------------------------------------------------------------------------
 // Binner::make_segments (Binner.cu:51-80): memset, then per sorted sample
 // --------------------------------------------------------------------------

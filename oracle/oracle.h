@@ -9,9 +9,6 @@
 //                                 branches, NaN/inf; test_oracle_binner.py
 //   orc_sort_order                std::stable_sort ties; brute-force
 //                                 insertion sort on small inputs
-//   orc_group_order               numpy lexsort of (slot, row) computed
-//                                 independently; equals orc_sort_order up to
-//                                 permutation inside equal-(slot, row) runs
 //   orc_segments / orc_windows    brute force over N <= 64 from a direct
 //                                 reading of the window rule; partition and
 //                                 count invariants
@@ -76,8 +73,6 @@ uint32_t orc_key_eval(const orc_key* k, float t);
 void orc_keys(const orc_key* k, const float* travel, size_t n, size_t stride, uint32_t* keys);
 uint32_t orc_travel_image(float t);
 void orc_sort_order(const float* travel, size_t n, size_t stride, uint32_t* order);
-void orc_group_order(const float* travel, size_t n, size_t stride, const orc_key* k,
-                     float range0, float inv_dr, float off_r, int fr, int GH, uint32_t* order);
 void orc_segments(const uint32_t* keys, size_t n, uint32_t N, uint32_t* seg);
 int orc_overlap(float margin, float lo, float hi, uint32_t N);
 void orc_windows(const uint32_t* seg, uint32_t N, int ov, uint32_t* win);

@@ -72,11 +72,13 @@ struct Batch {
     StreamArgs* h_pose = nullptr;   // pinned staging of rewritten streaming arguments
 
     Stage<StreamArgs> stream;
-    Stage<HistArgs> hist[4][2];   // per pass: small tiles, large tiles
+    // plain pings: the sort by travel, per pass
+    Stage<HistArgs> hist[4];
     Stage<ScanArgs> scan[4];
-    // [pass][last pass of its ping][records move with the keys]: a ping's
-    // sort takes 1-4 passes, in one of two scatter variants
-    Stage<ScatterArgs> scatter[4][2][2];
+    Stage<ScatterArgs> scatter[4];
+    // grid pings: the bucketing (counts, their scan, the scatter)
+    Stage<BucketArgs> bucket_hist, bucket_scatter;
+    Stage<ScanArgs> bucket_scan;
     Stage<BinArgs> windows;                // plain pings (the grid form's are in its deposit)
     Stage<ConvArgs> plain2d, plain1d;
     Stage<DepositArgs> deposit;
@@ -138,12 +140,13 @@ static int build(Batch* b)
 {
     b->stream = {};
     for (int k = 0; k < 4; k++) {
-        b->hist[k][0] = {};
-        b->hist[k][1] = {};
+        b->hist[k] = {};
         b->scan[k] = {};
-        for (auto& sl : b->scatter[k])
-            for (auto& sv : sl) sv = {};
+        b->scatter[k] = {};
     }
+    b->bucket_hist = {};
+    b->bucket_scan = {};
+    b->bucket_scatter = {};
     b->windows = {};
     b->plain2d = {};
     b->plain1d = {};
@@ -153,7 +156,7 @@ static int build(Batch* b)
     for (auto& en : b->entries) {
         Sensor* s = en.sensor;
         PingPlan p;
-        int rc = s->plan(en.form, &p, false, true);
+        int rc = s->plan(en.form, &p);
         if (rc) return rc;
         en.stream_index = -1;
         if (en.has_hits && en.nhits) {
@@ -162,13 +165,12 @@ static int build(Batch* b)
             if (blocks) en.stream_index = (int)b->stream.args.size();
             b->stream.add(a, blocks);
         }
-        for (int k = 0; k < p.sort.passes; k++) {
-            b->hist[k][p.sort.tile[k] == kSortTile ? 1 : 0].add(p.sort.hist[k], p.sort.hist_blocks[k]);
-            b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks[k] ? p.sort.hist[k].mask + 1 : 0u);
-            b->scatter[k][k == p.sort.passes - 1][p.sort.scatter[k].move ? 1 : 0]
-                .add(p.sort.scatter[k], p.sort.pass_blocks[k]);
-        }
         if (!p.grid) {
+            for (int k = 0; k < p.sort.passes; k++) {
+                b->hist[k].add(p.sort.hist[k], p.sort.hist_blocks[k]);
+                b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks[k] ? p.sort.hist[k].mask + 1 : 0u);
+                b->scatter[k].add(p.sort.scatter[k], p.sort.pass_blocks[k]);
+            }
             b->windows.add(p.bins, windows_blocks(p.bins));
             if (s->kind == RTSIM_SENSOR_2D) {
                 b->plain2d.add(p.conv, conv2d_plain_blocks(p.conv));
@@ -177,6 +179,12 @@ static int build(Batch* b)
                 b->plain1d.add(p.conv, conv1d_plain_blocks(p.conv));
             }
         } else {
+            const BucketArgs& bk = p.bucket;
+            b->bucket_hist.add(bk, bk.tiles);
+            b->bucket_hist.shm = std::max(b->bucket_hist.shm, sizeof(uint32_t) * (size_t)bk.NB);
+            b->bucket_scan.add(ScanArgs{bk.counts, bk.tiles, bk.totals, bk.tmax, bk.vmax}, (uint32_t)bk.NB);
+            b->bucket_scatter.add(bk, bk.tiles);
+            b->bucket_scatter.shm = std::max(b->bucket_scatter.shm, bucket_lds_bytes(bk));
             b->deposit.add(p.dep, deposit_blocks(p.dep));
             b->deposit.shm = std::max(b->deposit.shm, deposit_lds_bytes(p.dep));
             const std::tuple<int, int, int> key{en.form, p.corr.shape, p.corr.waves8};
@@ -198,12 +206,13 @@ static int build(Batch* b)
     size_t off = 0;
     layout(b->stream, off);
     for (int k = 0; k < 4; k++) {
-        layout(b->hist[k][0], off);
-        layout(b->hist[k][1], off);
+        layout(b->hist[k], off);
         layout(b->scan[k], off);
-        for (auto& sl : b->scatter[k])
-            for (auto& sv : sl) layout(sv, off);
+        layout(b->scatter[k], off);
     }
+    layout(b->bucket_hist, off);
+    layout(b->bucket_scan, off);
+    layout(b->bucket_scatter, off);
     layout(b->windows, off);
     layout(b->plain2d, off);
     layout(b->plain1d, off);
@@ -220,12 +229,13 @@ static int build(Batch* b)
     std::vector<char> host(off, 0);
     pack(b->stream, host);
     for (int k = 0; k < 4; k++) {
-        pack(b->hist[k][0], host);
-        pack(b->hist[k][1], host);
+        pack(b->hist[k], host);
         pack(b->scan[k], host);
-        for (const auto& sl : b->scatter[k])
-            for (const auto& sv : sl) pack(sv, host);
+        pack(b->scatter[k], host);
     }
+    pack(b->bucket_hist, host);
+    pack(b->bucket_scan, host);
+    pack(b->bucket_scatter, host);
     pack(b->windows, host);
     pack(b->plain2d, host);
     pack(b->plain1d, host);
@@ -243,17 +253,17 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
 #define RT_TRY(x) if ((e = (x)) != hipSuccess) return e
     RT_TRY(launch_stream_batch(dargs(b, b->stream), dmap(b, b->stream), b->stream.blocks(), s));
     for (int k = 0; k < 4; k++) {
-        for (int big = 0; big < 2; big++)
-            RT_TRY(launch_sort_hist_batch(k == 0, big == 1, dargs(b, b->hist[k][big]),
-                                          dmap(b, b->hist[k][big]), b->hist[k][big].blocks(), s));
+        RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]), b->hist[k].blocks(),
+                                      s));
         RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
-        for (int last = 0; last < 2; last++)
-            for (int mv = 0; mv < 2; mv++) {
-                const Stage<ScatterArgs>& st = b->scatter[k][last][mv];
-                RT_TRY(launch_sort_scatter_batch(k == 0, last == 1, mv == 1, dargs(b, st), dmap(b, st),
-                                                 st.blocks(), s));
-            }
+        RT_TRY(launch_sort_scatter_batch(k == 0, k == 3, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
+                                         b->scatter[k].blocks(), s));
     }
+    RT_TRY(launch_bucket_hist_batch(dargs(b, b->bucket_hist), dmap(b, b->bucket_hist), b->bucket_hist.blocks(),
+                                    b->bucket_hist.shm, s));
+    RT_TRY(launch_sort_scan_batch(dargs(b, b->bucket_scan), dmap(b, b->bucket_scan), b->bucket_scan.blocks(), s));
+    RT_TRY(launch_bucket_scatter_batch(dargs(b, b->bucket_scatter), dmap(b, b->bucket_scatter),
+                                       b->bucket_scatter.blocks(), b->bucket_scatter.shm, s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
                                      b->plain2d.shm, s));

@@ -165,7 +165,7 @@ void Sensor::free_samples()
 {
     dev_free(d_samples);
     dev_free(d_sorted);
-    dev_free(d_tmp);
+    dev_free(d_items);
     dev_free(d_coord);
     dev_free(d_kv[0]);
     dev_free(d_kv[1]);
@@ -178,14 +178,14 @@ void Sensor::free_samples()
 
 int Sensor::reserve(size_t count)
 {
-    if (count <= capacity) return RTSIM_OK;
+    if (count <= capacity && d_samples) return RTSIM_OK;   // (a sensor of 0 samples gets 1-item buffers)
     if (count > 0xFFFFFFFFu)   // sample indices are 32-bit
         return fail(RTSIM_ERR_INVALID, "sample count %zu exceeds 2^32 - 1", count);
     free_samples();
     int rc;
     if ((rc = dev_alloc(&d_samples, count))) return rc;
     if ((rc = dev_alloc(&d_sorted, count))) return rc;
-    if ((rc = dev_alloc(&d_tmp, count))) return rc;
+    if ((rc = dev_alloc(&d_items, 2 * count))) return rc;   // a sample is binned at most twice
     if ((rc = dev_alloc(&d_coord, count))) return rc;
     if ((rc = dev_alloc(&d_kv[0], count))) return rc;
     if ((rc = dev_alloc(&d_kv[1], count))) return rc;
@@ -194,7 +194,7 @@ int Sensor::reserve(size_t count)
     if ((rc = dev_alloc(&d_counts, sort_count_words(count)))) return rc;
     if ((rc = dev_alloc(&d_tmax, sort_count_words(count) / 256))) return rc;
     capacity = count;
-    return RTSIM_OK;
+    return grid_reserve();   // the grid form's count and occupancy buffers, when prepared
 }
 
 int Sensor::upload_psf()
@@ -341,13 +341,14 @@ int Sensor::check_plain_limits() const
 
 PingBuffers Sensor::own_buffers() const
 {
-    PingBuffers b;
+    PingBuffers b{};
     b.samples = d_samples;
     b.n = (uint32_t)count;
     b.tiles_cap = (uint32_t)(sort_count_words(capacity) / 256);
     b.sorted = d_sorted;
-    b.tmp = d_tmp;
     b.coord = d_coord;
+    b.items = d_items;
+    grid_buffers(&b);   // bucket counts and key occupancy (when the grid form is prepared)
     b.kv[0] = d_kv[0];
     b.kv[1] = d_kv[1];
     b.order = d_order;
@@ -361,87 +362,67 @@ PingBuffers Sensor::own_buffers() const
     return b;
 }
 
-int Sensor::plan(int form, PingPlan* p, bool introspect, bool batched, const PingBuffers* pbp)
+// Sort by travel (Receiver.cu:28-34) whose last pass gathers the records,
+// their bearing coordinates (and on request the Binner keys, Binner.cu:24-49,
+// and the permutation) and bounds the segments (Binner.cu:51-80), then the
+// windows (Binner::compute, Binner.h:96-102).
+static int plan_sort(Sensor* s, const PingBuffers& pb, bool introspect, PingPlan* p)
 {
-    const PingBuffers pb = pbp ? *pbp : own_buffers();
-    const uint32_t n = pb.n;
-    const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
-    int rc = RTSIM_OK;
-    if (form == RTSIM_FORM_PLAIN_FP32) {
-        if ((rc = check_plain_limits())) return rc;
-    } else if (!grid) {
-        return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
-    }
-    p->form = form;
-    p->grid = grid;
-    RowMap rows{};
-    uint32_t* vmax = nullptr;
-    if (grid && (rc = grid_row_map(&rows, &vmax))) return rc;   // also checks the grid form applies
-
-    // sort (Receiver.cu:28-34); the last pass gathers the records, their keys
-    // (Binner.cu:24-49) and bearing coordinates.  The plain form sorts by
-    // travel; the grid form by the grouping key (GroupKey).
-    SortBuffers sb;
+    SortBuffers sb{};
     sb.kv[0] = pb.kv[0];
     sb.kv[1] = pb.kv[1];
     sb.order = introspect ? pb.order : nullptr;
     sb.counts = pb.counts;
     sb.totals = pb.totals;
     sb.tiles_cap = pb.tiles_cap;
-    sb.tmp = pb.tmp;
     sb.seg_init = pb.seg;                             // Binner segments start at (~0, 0)
-    sb.seg_count = N;
-    sb.fill = rows.row_start;                         // grid row table starts at n
-    sb.fill_value = n;
-    sb.fill_count = rows.row_start ? (size_t)rows.GH + 3 : 0;
+    sb.seg_count = s->N;
     sb.out.sorted = pb.sorted;
     sb.out.skeys = introspect ? pb.skeys : nullptr;
     sb.out.coord = pb.coord;
-    sb.out.xg = nullptr;
-    if (grid) {   // the deposit reads grid column coordinates (in the same buffer)
-        sb.out.xg = reinterpret_cast<float*>(pb.coord);
-        sb.out.xg_b0 = rows.bear0;
-        sb.out.xg_inv = rows.inv_db;
-        sb.out.xg_off = rows.off_b;
-    }
-    sb.out.key = key;
-    sb.out.axis = axis;
+    sb.out.key = s->key;
+    sb.out.axis = s->axis;
     sb.out.seg = pb.seg;
-    // the grid form reads the windows' emptiness alone: without introspection
-    // its sort marks the keys that hold samples instead of bounding their
-    // segments (two thirds of its run-bound atomics)
-    sb.out.seg_marks = grid && !introspect ? 1 : 0;
-    // the grid form only needs samples grouped by bin and G row (its deposit
-    // sums in exact fixed point, independent of the order inside a group)
-    GroupKey gk{};
-    if (grid) {
-        gk.on = 1;
-        gk.rows = rows;
-        gk.key = key;
-        gk.vmax = vmax;
-        gk.tmax = pb.tmax;
-        if (group_key_max(gk) >= (1u << 24))
-            return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows + %u bins exceed the grouping key",
-                        rows.GH, N);
-    }
-    hipError_t e = make_sort_plan(pb.samples, n, sb, &gk, &p->sort, batched);
+    hipError_t e = make_sort_plan(pb.samples, pb.n, sb, &p->sort);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
-
-    // Binner::compute (Binner.h:96-102): the sort bounded the segments, the
-    // windows follow (the grid form computes them inside its deposit launch)
     BinArgs& ba = p->bins;
-    ba.N = N;
-    ba.ov = overlap;
+    ba.N = s->N;
+    ba.ov = s->overlap;
     ba.seg = pb.seg;
     ba.win = pb.win;
+    return RTSIM_OK;
+}
 
-    if (grid) {
-        if ((rc = grid_plan(form, &p->dep, &p->corr, pb))) return rc;
-        p->dep.bins = ba;
-        return RTSIM_OK;
+int Sensor::plan(int form, PingPlan* p, const PingBuffers* pbp)
+{
+    const bool grid = form == RTSIM_FORM_MFMA_BF16 || form == RTSIM_FORM_MFMA_FP16;
+    int rc = RTSIM_OK;
+    if (form == RTSIM_FORM_PLAIN_FP32) {
+        if ((rc = check_plain_limits())) return rc;
+    } else if (!grid) {
+        return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
+    } else if ((rc = ensure_grid())) {   // also checks the grid form applies
+        return rc;
     }
+    const PingBuffers pb = pbp ? *pbp : own_buffers();
+    p->form = form;
+    p->grid = grid;
+    if (grid) {
+        // the deposit sums every sample exactly, so it needs them grouped by
+        // grid row only (one bucketing pass) and the windows' emptiness
+        p->sort = SortPlan{};
+        return grid_plan(form, &p->bucket, &p->dep, &p->corr, pb);
+    }
+    if ((rc = plan_sort(this, pb, false, p))) return rc;
     p->conv = conv_args(pb);
     return RTSIM_OK;
+}
+
+int Sensor::plan_binning(PingPlan* p)
+{
+    p->form = RTSIM_FORM_PLAIN_FP32;
+    p->grid = false;
+    return plan_sort(this, own_buffers(), true, p);
 }
 
 int Sensor::compute(int form, hipStream_t s)
@@ -469,15 +450,22 @@ int Sensor::launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hi
         stage_end(s);
         return e;
     };
-    hipError_t e = stage(1, [&] { return launch_sort_plan(p.sort, s, skip_hist0); });
-    if (e == hipSuccess && !p.grid) {
-        e = stage(2, [&] { return launch_windows(p.bins, s); });
+    hipError_t e;
+    if (!p.grid) {
+        e = stage(1, [&] { return launch_sort_plan(p.sort, s); });
+        if (e == hipSuccess) e = stage(2, [&] { return launch_windows(p.bins, s); });
         if (e == hipSuccess)
             e = stage(4, [&] {
                 return kind == RTSIM_SENSOR_2D ? launch_conv2d_plain(p.conv, s) : launch_conv1d_plain(p.conv, s);
             });
-    } else if (e == hipSuccess) {
-        e = stage(3, [&] { return launch_deposit(p.dep, s); });
+    } else {
+        e = stage(1, [&] {
+            hipError_t r = skip_hist0 ? hipSuccess : launch_bucket_hist(p.bucket, s);
+            if (r == hipSuccess) r = launch_bucket_scan(p.bucket, s);
+            if (r == hipSuccess) r = launch_bucket_scatter(p.bucket, s);
+            return r;
+        });
+        if (e == hipSuccess) e = stage(3, [&] { return launch_deposit(p.dep, s); });
         if (e == hipSuccess)
             e = stage(4, [&] { return launch_correlate(p.corr, form == RTSIM_FORM_MFMA_FP16, s); });
     }
@@ -749,13 +737,14 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
     StreamArgs a = s->stream_args(*hits, n, offset);
     (void)hipGetLastError();
     // a write of every sample into a sensor whose grid form is prepared also
-    // makes that form's first sort histogram (one launch fewer per ping)
+    // makes that form's bucket counts (one launch fewer per ping), when its
+    // tiles are the streaming kernel's (1024 samples)
     PingPlan p;
     const bool fuse = offset == 0 && n == s->count && n > 0 && s->grid_ready() &&
-                      s->plan(RTSIM_FORM_MFMA_FP16, &p) == RTSIM_OK && p.sort.passes > 1 &&
-                      p.sort.tile[0] == kSortTileSmall;
+                      s->plan(RTSIM_FORM_MFMA_FP16, &p) == RTSIM_OK && p.bucket.tile == 1024 &&
+                      stream_blocks(a) == p.bucket.tiles;
     s->stage_begin(0, st);
-    if (fuse) HIP_TRY(launch_stream_hist(a, p.sort.hist[0], st));
+    if (fuse) HIP_TRY(launch_stream_bucket(a, p.bucket, st));
     else HIP_TRY(launch_stream(a, st));
     s->stage_end(st);
     s->hist_fused = fuse;
@@ -795,16 +784,17 @@ int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments
     if (!s->computed) return fail(RTSIM_ERR_INVALID, "compute has not run since the last change");
     size_t n = s->count;
     const bool grid = s->computed_form == RTSIM_FORM_MFMA_BF16 || s->computed_form == RTSIM_FORM_MFMA_FP16;
-    if ((keys || order || (grid && (segments || windows))) && n) {
+    if (((keys || order) && n) || (grid && (segments || windows))) {
         // compute writes neither the keys nor the permutation (nothing
-        // downstream reads them), and the grid form only marks the keys that
-        // hold samples: sort the same samples again with all of them kept,
-        // then the windows (which also write empty segments back as (0, 0)).
-        // Both are deterministic, so every array they rewrite (sorted
-        // records, coordinates, segments, windows, row table) gets the same
-        // contents.
+        // downstream reads them), and the grid form does not sort at all (it
+        // bins by grid row and keeps only the windows' emptiness): sort the
+        // same samples by travel with all of them kept, then the windows
+        // (which also write empty segments back as (0, 0)).  Both are
+        // deterministic, so every array they rewrite (sorted records,
+        // coordinates, segments, windows) gets the contents a plain-form
+        // compute leaves.
         PingPlan p;
-        int rc = s->plan(s->computed_form, &p, true);
+        int rc = s->plan_binning(&p);
         if (rc) return rc;
         (void)hipGetLastError();
         HIP_TRY(launch_sort_plan(p.sort, st));

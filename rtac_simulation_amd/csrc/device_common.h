@@ -58,42 +58,68 @@ __device__ __forceinline__ double sin_bearing(double x)
 }
 
 // Grid (MFMA) form: range coordinate of a travel in grid rows, one explicit
-// fma so that the row table and the deposit round identically.
+// fma so that the bucketing and the deposit round identically.
 __device__ __forceinline__ float grid_y_of(float t, float range0, float inv_dr, float off_r)
 {
     return __builtin_fmaf(t - range0, inv_dr, off_r);
 }
 
-// G row (grid row + fr) of a travel, clamped to [-2, GH + 2]; NaN maps to the
-// top (the sort puts NaN travel last).
-__device__ __forceinline__ int g_row_of(const RowMap& m, float t)
+// G row (grid row of the first range weight) of a travel, clamped to
+// [-2, GH + 1] (beyond, both of its rows are outside the grid).
+__device__ __forceinline__ int g_row_of(const GridRows& m, float t)
 {
-    const float r = floorf(grid_y_of(t, m.range0, m.inv_dr, m.off_r)) + (float)m.fr;
-    if (!(r >= -2.0f)) return r != r ? m.GH + 2 : -2;
-    return r > (float)(m.GH + 2) ? m.GH + 2 : (int)r;
+    float r = floorf(grid_y_of(t, m.range0, m.inv_dr, m.off_r)) + (float)m.fr;
+    r = fminf(fmaxf(r, -2.0f), (float)m.GH + 1.0f);
+    return (int)r;
 }
 
-// Bin slot of a travel: 0 below the Binner's range, bin_key + 1 inside it,
-// n + 1 above it or NaN.  Non-decreasing in travel (the truncated fma is, and
-// each clamp compares it with one fixed bound).
-__device__ __forceinline__ uint32_t bin_slot(const KeyParams& k, float t)
+// The buckets of a sample with G row gy: b0 = the bucket of grid row gy, b1
+// = the bucket of row gy + 1 when it is another one; -1 where a row is
+// outside the grid [0, GH) or the bucket repeats.
+__device__ __forceinline__ void bucket_pair(const BucketArgs& a, int gy, int& b0, int& b1)
 {
-    if (t != t) return k.n + 1;
-    float v = __builtin_fmaf(k.a, t, k.b);
-    if (v != v) return k.n + 1;
-    int res;
-    if (v >= 2147483648.0f)       res = INT_MAX;
-    else if (v <= -2147483648.0f) res = INT_MIN;
-    else                          res = (int)v;
-    if (res < 0) return ((float)res >= k.lower) ? 1u : 0u;
-    if ((uint32_t)res >= k.n) return ((float)res <= k.upper) ? k.n : k.n + 1;
-    return (uint32_t)res + 1;
+    b0 = gy >= 0 && gy < a.rows.GH ? gy / a.R : -1;
+    const int c1 = gy + 1 >= 0 && gy + 1 < a.rows.GH ? (gy + 1) / a.R : -1;
+    b1 = c1 != b0 ? c1 : -1;
 }
 
-// Grouping sort key K = (G row + 2) + bin slot of a travel.
-__device__ __forceinline__ uint32_t group_key(const GroupKey& g, float t)
+// Grid column coordinate of a bearing: sin(bearing) in FP64 (axis 1) or the
+// bearing, mapped to (x - bear0) inv_db + off_b.
+__device__ __forceinline__ float grid_x_of(const BucketArgs& a, float bearing)
 {
-    return (uint32_t)(g_row_of(g.rows, t) + 2) + bin_slot(g.key, t);
+    const double x = a.axis == 1 ? sin_bearing((double)bearing) : (double)bearing;
+    return (float)((x - a.bear0) * (double)a.inv_db) + a.off_b;
+}
+
+// Inclusive wave scan (wave64) of a 32-bit value.
+__device__ __forceinline__ uint32_t sort_wave_scan(uint32_t x, uint32_t lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o, 64);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+// Exclusive block scan (NT threads, a multiple of 64) of one value per
+// thread; `ws` holds NT / 64 words.  Ends with the last wave's partial in ws:
+// a second call needs its own ws (or a barrier first).
+template <int NT = 256>
+__device__ __forceinline__ uint32_t sort_block_scan(uint32_t x, uint32_t* ws, uint32_t* total)
+{
+    constexpr int NW = NT / 64;
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t incl = sort_wave_scan(x, lane);
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    uint32_t off = 0, all = 0;
+#pragma unroll
+    for (int k = 0; k < NW; k++) {
+        off += (uint32_t)k < w ? ws[k] : 0u;
+        all += ws[k];
+    }
+    if (total) *total = all;
+    return off + incl - x;
 }
 
 // do_make_bins (Binner.cu:82-106) for output bin blk * 256 + threadIdx.x.

@@ -91,21 +91,47 @@ struct GridState {
     double bear0 = 0;
     float tap_scale = 1.f;
     int max_lds = 0;                     // device: LDS per workgroup
+    // bucketing: NB buckets of R grid rows, key occupancy in occ_words words
+    int R = 1, NB = 0, occ_words = 0;
     float2* d_grid = nullptr;
-    uint32_t* d_row_start = nullptr;
-    uint32_t* d_vmax = nullptr;          // deposit scale (set by the sort)
+    uint32_t* d_vmax = nullptr;          // deposit scale (set by the bucket scan)
+    uint32_t* d_totals = nullptr;        // [NB] items per bucket
+    // sized for `cap` samples: bucket counts [NB][tiles], key occupancy [tiles][occ_words]
+    size_t cap = 0;
+    uint32_t* d_counts = nullptr;
+    size_t counts_words = 0;
+    uint32_t* d_occ = nullptr;
+    size_t occ_cap = 0;
     uint16_t* d_toep[2] = {nullptr, nullptr};   // tap columns, [0] bf16, [1] fp16
     size_t lds = 0;
 
+    void free_sized()
+    {
+        if (d_counts) (void)hipFree(d_counts);
+        if (d_occ) (void)hipFree(d_occ);
+        d_counts = nullptr;
+        d_occ = nullptr;
+        cap = counts_words = occ_cap = 0;
+    }
     ~GridState()
     {
+        free_sized();
         if (d_grid) (void)hipFree(d_grid);
-        if (d_row_start) (void)hipFree(d_row_start);
         if (d_vmax) (void)hipFree(d_vmax);
+        if (d_totals) (void)hipFree(d_totals);
         for (auto p : d_toep)
             if (p) (void)hipFree(p);
     }
 };
+
+// Bucket-count words the bucketing may use for `samples` samples: NB rows of
+// one count per tile of 1024 samples, at most kBucketCountBudget (larger pings
+// take larger tiles), plus one tile column of slack.
+size_t bucket_count_words(int NB, size_t samples)
+{
+    const size_t t1 = (samples + 1023) / 1024;
+    return std::min((size_t)NB * std::max<size_t>(t1, 1), kBucketCountBudget) + (size_t)NB;
+}
 
 }  // namespace
 
@@ -211,9 +237,21 @@ static int prepare_grid(Sensor* s, GridState* g)
         g->shape = cached & 3;
         g->waves8 = cached >> 2;
     }
-    const size_t dep_lds = sizeof(long long) * 2 * g->GW;
-    if (dep_lds > (size_t)max_lds)
+    // bucketing: rows per bucket so that about 256 buckets cover the grid
+    // (each is one deposit workgroup), within the deposit's LDS budget and
+    // the counters' limit; a sample on a bucket's last row is binned twice
+    const size_t row_lds = sizeof(long long) * 2 * (size_t)g->GW;
+    const int r_lds = (int)std::min<size_t>(kDepositLdsBudget / row_lds, kMaxBucketRows);
+    if (r_lds < 1 || row_lds > (size_t)max_lds)
         return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: deposit row of %d cells too wide", g->GW);
+    g->R = std::max(1, std::min(r_lds, g->GH / 256));
+    g->NB = (g->GH + g->R - 1) / g->R;
+    if (g->NB > kMaxBuckets)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: %d grid rows need more than %d buckets", g->GH,
+                    kMaxBuckets);
+    g->occ_words = (int)((s->N + 31) / 32);
+    if (g->occ_words > kMaxOccWords)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: at most %d range bins", 32 * kMaxOccWords);
 
     // taps: power-of-two scale so max |T| in [0.5, 1), split hi/lo, laid out
     // as padded tap columns [shift c][plane p][bearing tap j][x] =
@@ -254,10 +292,10 @@ static int prepare_grid(Sensor* s, GridState* g)
     // rows [GH, GHa) are read by the correlation and never written
     if ((err = hipMemset(g->d_grid, 0, sizeof(float2) * (size_t)g->GHa * g->GW)) != hipSuccess)
         return hip_fail(err, "hipMemset(grid)");
-    if ((err = hipMalloc((void**)&g->d_row_start, sizeof(uint32_t) * (g->GH + 3))) != hipSuccess)
-        return hip_fail(err, "hipMalloc(row_start)");
     if ((err = hipMalloc((void**)&g->d_vmax, sizeof(uint32_t))) != hipSuccess)
         return hip_fail(err, "hipMalloc(vmax)");
+    if ((err = hipMalloc((void**)&g->d_totals, sizeof(uint32_t) * (size_t)g->NB)) != hipSuccess)
+        return hip_fail(err, "hipMalloc(bucket totals)");
     for (int d = 0; d < 2; d++) {
         if ((err = hipMalloc((void**)&g->d_toep[d], sizeof(uint16_t) * tsize)) != hipSuccess)
             return hip_fail(err, "hipMalloc(toeplitz)");
@@ -279,7 +317,7 @@ int Sensor::ensure_grid()
         if (g->status) g->why = fail_message();
     }
     if (!g->ready) return fail(g->status, "%s", g->why.c_str());
-    return RTSIM_OK;
+    return grid_reserve();
 }
 
 bool Sensor::grid_ready() const
@@ -287,105 +325,102 @@ bool Sensor::grid_ready() const
     return grid_state && static_cast<const GridState*>(grid_state)->ready;
 }
 
-int Sensor::grid_row_map(RowMap* m, uint32_t** vmax)
+// The bucketing's per-sample buffers (bucket counts, key occupancy) for the
+// sensor's sample capacity.
+int Sensor::grid_reserve()
 {
-    int rc = ensure_grid();
-    if (rc) return rc;
     GridState* g = static_cast<GridState*>(grid_state);
-    m->range0 = range_min;
-    m->inv_dr = g->inv_dr;
-    m->off_r = g->off_r;
-    m->fr = g->fr;
-    m->GH = g->GH;
-    m->row_start = g->d_row_start;
-    m->bear0 = g->bear0;
-    m->inv_db = g->inv_db;
-    m->off_b = g->off_b;
-    if (vmax) *vmax = g->d_vmax;
+    if (!g || !g->ready || (g->d_counts && g->cap >= capacity)) return RTSIM_OK;
+    g->free_sized();
+    const size_t cw = bucket_count_words(g->NB, capacity);
+    const size_t ow = std::max<size_t>((capacity + 1023) / 1024, 1) * (size_t)g->occ_words;
+    hipError_t e;
+    if ((e = hipMalloc((void**)&g->d_counts, sizeof(uint32_t) * cw)) != hipSuccess)
+        return hip_fail(e, "hipMalloc(bucket counts)");
+    if ((e = hipMalloc((void**)&g->d_occ, sizeof(uint32_t) * ow)) != hipSuccess) {
+        g->free_sized();
+        return hip_fail(e, "hipMalloc(key occupancy)");
+    }
+    g->counts_words = cw;
+    g->occ_cap = ow;
+    g->cap = capacity;
+    generation++;
     return RTSIM_OK;
 }
 
-// Binner::Key::operator() (Binner.h:47-53) on the host: the same FP32 fused
-// multiply-add, truncation and clamps as bin_key on the device.
-static uint32_t host_bin_key(const KeyParams& k, float t)
+void Sensor::grid_buffers(PingBuffers* pb) const
 {
-    if (t != t) return kOutOfRange;
-    const float v = std::fmaf(k.a, t, k.b);
-    if (v != v) return kOutOfRange;
-    int res;
-    if (v >= 2147483648.0f)       res = INT_MAX;
-    else if (v <= -2147483648.0f) res = INT_MIN;
-    else                          res = (int)v;
-    if (res < 0) return ((float)res >= k.lower) ? 0u : kOutOfRange;
-    if ((uint32_t)res >= k.n) return ((float)res <= k.upper) ? k.n - 1u : kOutOfRange;
-    return (uint32_t)res;
+    const GridState* g = static_cast<const GridState*>(grid_state);
+    if (!g || !g->ready) return;
+    pb->bcounts = g->d_counts;
+    pb->bcounts_cap = g->counts_words;
+    pb->occ = g->d_occ;
+    pb->occ_cap = g->occ_cap;
 }
 
-// float of an order-preserving 32-bit image (the image is bits | 2^31 for
-// x >= 0, ~bits for x < 0)
-static float float_of_image(uint32_t u)
-{
-    const uint32_t bits = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
-    float f;
-    std::memcpy(&f, &bits, 4);
-    return f;
-}
-static uint32_t image_of_float(float f)
-{
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-// The travels with a Binner key form one interval of floats: the key's
-// integer is non-decreasing in travel and each clamp compares it with one
-// bound, and -inf, +inf and NaN are out of range.  Bisection over the ordered
-// float images from a travel inside the range finds its ends exactly, so the
-// deposit tests `lo <= t && t <= hi` instead of evaluating the key.
-static bool key_travel_interval(const KeyParams& k, float inside, float* lo, float* hi)
-{
-    if (host_bin_key(k, inside) == kOutOfRange) return false;
-    const uint32_t um = image_of_float(inside);
-    uint32_t out = image_of_float(-INFINITY), in = um;   // key(out) is OOR, key(in) is not
-    while (in - out > 1) {
-        const uint32_t mid = out + (in - out) / 2;
-        if (host_bin_key(k, float_of_image(mid)) == kOutOfRange) out = mid;
-        else in = mid;
-    }
-    *lo = float_of_image(in);
-    in = um;
-    out = image_of_float(INFINITY);
-    while (out - in > 1) {
-        const uint32_t mid = in + (out - in) / 2;
-        if (host_bin_key(k, float_of_image(mid)) == kOutOfRange) out = mid;
-        else in = mid;
-    }
-    *hi = float_of_image(in);
-    return true;
-}
-
-int Sensor::grid_plan(int form, DepositArgs* dp, CorrArgs* cp, const PingBuffers& pb)
+// Plans the grid form of one ping over the buffers pb: the bucketing (counts,
+// scan, scatter), the deposit with the windows, the correlation.
+int Sensor::grid_plan(int form, BucketArgs* bp, DepositArgs* dp, CorrArgs* cp, const PingBuffers& pb)
 {
     int rc = ensure_grid();
     if (rc) return rc;
     GridState* g = static_cast<GridState*>(grid_state);
+    const uint32_t n = pb.n;
+    if (!pb.items || !pb.bcounts || !pb.occ || !pb.tmax)
+        return fail(RTSIM_ERR_INVALID, "MFMA form: bucketing buffers missing");
+
+    // tiles of 1024 samples, or larger ones when the count matrix would not
+    // fit its budget (the streaming kernel makes the counts of 1024-sample tiles)
+    const size_t t1 = (n + 1023) / 1024;
+    const size_t room = pb.bcounts_cap > (size_t)g->NB ? pb.bcounts_cap - (size_t)g->NB : 0;
+    size_t m = 1;
+    if ((size_t)g->NB * t1 > room) m = room ? ((size_t)g->NB * t1 + room - 1) / room : 0;
+    if (!m) return fail(RTSIM_ERR_INVALID, "MFMA form: no room for the bucket counts");
+    const uint32_t tile = (uint32_t)(1024 * m);
+    const uint32_t tiles = (uint32_t)(((size_t)n + tile - 1) / tile);
+    if ((size_t)g->NB * tiles > pb.bcounts_cap || (size_t)tiles * g->occ_words > pb.occ_cap ||
+        tiles > 256u * 64u)
+        return fail(RTSIM_ERR_INVALID, "MFMA form: %u samples exceed the bucketing buffers", n);
+
+    BucketArgs& b = *bp;
+    b = BucketArgs{};
+    b.rec = pb.samples;
+    b.n = n;
+    b.tile = tile;
+    b.tiles = tiles;
+    b.key = key;
+    b.rows = GridRows{range_min, g->inv_dr, g->off_r, g->fr, g->GH};
+    b.R = g->R;
+    b.NB = g->NB;
+    b.counts = pb.bcounts;
+    b.totals = g->d_totals;
+    b.tmax = pb.tmax;
+    b.vmax = g->d_vmax;
+    b.items = pb.items;
+    b.occ = pb.occ;
+    b.occ_words = g->occ_words;
+    b.bear0 = g->bear0;
+    b.inv_db = g->inv_db;
+    b.off_b = g->off_b;
+    b.axis = axis;
 
     DepositArgs& d = *dp;
-    d.sorted = pb.sorted;
-    d.xg = reinterpret_cast<const float*>(pb.coord);   // written by the sort's last pass
-    if (!key_travel_interval(key, 0.5f * (range_min + range_max), &d.t_lo, &d.t_hi))
-        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: the range midpoint has no Binner key");
-    d.n = pb.n;
-    d.row_start = g->d_row_start;
+    d.items = pb.items;
+    d.totals = g->d_totals;
+    d.NB = g->NB;
+    d.R = g->R;
     d.vmax = g->d_vmax;
     d.grid = g->d_grid;
     d.GH = g->GH;
     d.GW = g->GW;
-    d.fr = g->fr;
     d.fb = g->fb;
-    d.range0 = range_min;
-    d.inv_dr = g->inv_dr;
-    d.off_r = g->off_r;
+    d.rows = b.rows;
+    d.occ = pb.occ;
+    d.occ_words = g->occ_words;
+    d.tiles = tiles;
+    d.N = N;
+    d.ov = overlap;
+    d.win = pb.win;
 
     CorrArgs& c = *cp;
     c.grid = g->d_grid;

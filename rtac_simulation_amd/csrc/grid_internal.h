@@ -23,18 +23,27 @@ int corr_copy_stride_host(int Tb, int L);
 
 // Extended grid G: rho = grid row + fr, kappa = grid column + fb, so that
 // out(r, b) = sum T[i][j] G[r + i][b + j].
+constexpr int kMaxBucketRows = 16;   // grid rows per bucket of the deposit
+// bucket-count words of one ping (pings whose 1024-sample tiles would need
+// more take larger tiles) and the deposit's LDS per workgroup (its rows)
+constexpr size_t kBucketCountBudget = (size_t)2 << 20;
+constexpr size_t kDepositLdsBudget = 64 * 1024;
 struct DepositArgs {
-    const float4* sorted;
-    const float* xg;           // grid column coordinate per sorted sample (from the sort)
-    float t_lo, t_hi;          // travels with a Binner key: [t_lo, t_hi] (others are skipped)
-    uint32_t n;
-    const uint32_t* row_start; // GH + 3 entries: first sample of each G row present, else n
-    const uint32_t* vmax;      // 1 word: the largest finite in-range |re|, |im| (FP32 bits)
-    BinArgs bins;              // the launch's last workgroups compute the windows
+    const float4* items;       // {re, im, travel, grid column}, grouped by bucket
+    const uint32_t* totals;    // [NB] items per bucket
+    int NB, R;                 // buckets of R grid rows
+    const uint32_t* vmax;      // 1 word: the largest finite binned |re|, |im| (FP32 bits)
     float2* grid;              // [GH][GW]
     int GH, GW;
-    int fr, fb;                // floor((Tr - 1) / 2), floor((Tb - 1) / 2)
-    float range0, inv_dr, off_r;        // y = (t - range0) / dr + off_r
+    int fb;                    // floor((Tb - 1) / 2)
+    GridRows rows;             // G row of a travel (the bucketing's map)
+    // windows (the launch's last workgroups): key occupancy per bucketing tile
+    const uint32_t* occ;       // [tiles][occ_words]
+    int occ_words;
+    uint32_t tiles;
+    uint32_t N;                // range bins
+    int ov;                    // binOverlap
+    uint2* win;                // [N]: (0, 1) window with samples, (0, 0) empty
 };
 
 struct CorrArgs {

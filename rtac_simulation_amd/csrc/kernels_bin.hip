@@ -84,78 +84,70 @@ __device__ __forceinline__ float4 stream_sample(const StreamArgs& p, float r, fl
     return o;
 }
 
-// One workgroup per tile of kSortTileSmall hits, one hit per thread (more per
-// thread measured 5-10 % slower: fewer resident waves hide the sample's
-// dependent table lookups less well).  With h (the grid form's first sort
-// histogram) the workgroup also makes that tile's digit counts of the
-// grouping key, its largest finite in-range |re|, |im| and the first
-// histogram's clears, as k_sort_hist's first pass makes them from the same
-// records (a histogram and a maximum: the thread mapping is irrelevant).
-// Every streaming launch (one ping, fused or not, and batched) runs this one
-// body with the histogram switched at run time (a compile-time switch lets the
+// One workgroup per tile of 1024 hits, one hit per thread (more per thread
+// measured 5-10 % slower: fewer resident waves hide the sample's dependent
+// table lookups less well).  With b (the grid form's bucketing) the
+// workgroup also makes that tile's bucket counts and its largest finite
+// |re|, |im| of the binned samples, as k_bucket_hist makes them from the same
+// records (counts and a maximum: the thread mapping is irrelevant).  Every
+// streaming launch (one ping, fused or not, and batched) runs this one body
+// with the counts switched at run time (a compile-time switch lets the
 // compiler fuse multiply-adds differently, and the records would differ in
 // their last bits between launches).
-constexpr uint32_t kStreamThreads = kSortTileSmall;
+constexpr uint32_t kStreamThreads = 1024;
 constexpr int kStreamWaves = (int)kStreamThreads / 64;
 
-__device__ __forceinline__ void stream_body(const StreamArgs& p, const HistArgs& hist, int with_hist,
-                                            uint32_t blk, uint32_t nblk)
+__device__ __forceinline__ void stream_body(const StreamArgs& p, const BucketArgs& bk, int with_hist,
+                                            uint32_t blk)
 {
-    const HistArgs* h = with_hist ? &hist : nullptr;
-    __shared__ uint32_t cnt[256];
+    const BucketArgs* b = with_hist ? &bk : nullptr;
+    extern __shared__ uint32_t hist[];   // [NB] when counting
     __shared__ float wmax[kStreamWaves];
     const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
-    if (h) {
-        if (t < 256) cnt[t] = 0u;
-        for (size_t i = (size_t)blk * kStreamThreads + t; i < h->seg_count;
-             i += (size_t)nblk * kStreamThreads)
-            h->seg_init[i] = make_uint2(0xffffffffu, 0u);
-        for (size_t i = (size_t)blk * kStreamThreads + t; i < h->fill_count;
-             i += (size_t)nblk * kStreamThreads)
-            h->fill[i] = h->fill_value;
-        if (blk == 0 && t == 0) h->gk.vmax[0] = 0u;
-    }
+    if (b)
+        for (int k = (int)t; k < b->NB; k += kStreamThreads) hist[k] = 0u;
     const uint32_t i = blk * kStreamThreads + t;
     const uint32_t ic = min(i, p.n - 1);
     const float r = p.range[ic], d0 = p.dx[ic], d1 = p.dy[ic], d2 = p.dz[ic];
     const float2 A = p.amp[ic];
-    if (h) __syncthreads();   // counts cleared
+    if (b) __syncthreads();   // counts cleared
     float m = 0.f;
     if (i < p.n) {
         const float4 o = stream_sample(p, r, d0, d1, d2, A);
         p.out[i] = o;
-        if (h) {
-            const uint32_t slot = bin_slot(h->gk.key, o.z);
-            const uint32_t K = (uint32_t)(g_row_of(h->gk.rows, o.z) + 2) + slot;
-            atomicAdd(&cnt[(K >> h->shift) & h->mask], 1u);
+        if (b && bin_key(b->key, o.z) != kOutOfRange) {
+            int b0, b1;
+            bucket_pair(*b, g_row_of(b->rows, o.z), b0, b1);
+            if (b0 >= 0) atomicAdd(&hist[b0], 1u);
+            if (b1 >= 0) atomicAdd(&hist[b1], 1u);
             const float c = fmaxf(fabsf(o.x), fabsf(o.y));
-            if (slot - 1u < h->gk.key.n && c <= 3.402823466e38f) m = fmaxf(m, c);
+            if (c <= 3.402823466e38f) m = fmaxf(m, c);
         }
     }
-    if (!h) return;
+    if (!b) return;
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o, 64));
     if (lane == 0) wmax[w] = m;
     __syncthreads();
-    if (t <= h->mask) h->counts[(size_t)t * h->tiles + blk] = cnt[t];
+    for (int k = (int)t; k < b->NB; k += kStreamThreads) b->counts[(size_t)k * b->tiles + blk] = hist[k];
     if (t == 0) {
         float mm = wmax[0];
         for (int k = 1; k < kStreamWaves; k++) mm = fmaxf(mm, wmax[k]);
-        h->gk.tmax[blk] = __float_as_uint(mm);
+        b->tmax[blk] = __float_as_uint(mm);
     }
 }
 
-__global__ __launch_bounds__(kStreamThreads) void k_stream(StreamArgs p, HistArgs h, int with_hist)
+__global__ __launch_bounds__(kStreamThreads) void k_stream(StreamArgs p, BucketArgs b, int with_hist)
 {
-    stream_body(p, h, with_hist, blockIdx.x, gridDim.x);
+    stream_body(p, b, with_hist, blockIdx.x);
 }
 
 __global__ __launch_bounds__(kStreamThreads) void k_stream_batch(const StreamArgs* __restrict__ args,
-                                                                 BatchMap m, HistArgs h, int with_hist)
+                                                                 BatchMap m, BucketArgs b, int with_hist)
 {
     uint32_t blk, nblk;
     const uint32_t q = batch_locate(m, blk, nblk);
     const StreamArgs a = args[q];   // a private copy: the body's stores cannot alias it
-    stream_body(a, h, with_hist, blk, nblk);
+    stream_body(a, b, with_hist, blk);
 }
 
 uint32_t stream_blocks(const StreamArgs& a) { return (a.n + kStreamThreads - 1) / kStreamThreads; }
@@ -163,22 +155,22 @@ uint32_t stream_blocks(const StreamArgs& a) { return (a.n + kStreamThreads - 1) 
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s)
 {
     if (a.n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stream, dim3(stream_blocks(a)), dim3(kStreamThreads), 0, s, a, HistArgs{}, 0);
+    hipLaunchKernelGGL(k_stream, dim3(stream_blocks(a)), dim3(kStreamThreads), 0, s, a, BucketArgs{}, 0);
     return hipGetLastError();
 }
 
 hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(kStreamThreads), 0, s, args, m, HistArgs{}, 0);
+    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(kStreamThreads), 0, s, args, m, BucketArgs{}, 0);
     return hipGetLastError();
 }
 
-hipError_t launch_stream_hist(const StreamArgs& a, const HistArgs& h, hipStream_t s)
+hipError_t launch_stream_bucket(const StreamArgs& a, const BucketArgs& b, hipStream_t s)
 {
-    if (a.n == 0 || h.tiles == 0) return hipErrorInvalidValue;
-    if (stream_blocks(a) != h.tiles) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_stream, dim3(h.tiles), dim3(kStreamThreads), 0, s, a, h, 1);
+    if (a.n == 0 || b.tiles == 0 || b.tile != kStreamThreads || stream_blocks(a) != b.tiles)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_stream, dim3(b.tiles), dim3(kStreamThreads), sizeof(uint32_t) * b.NB, s, a, b, 1);
     return hipGetLastError();
 }
 

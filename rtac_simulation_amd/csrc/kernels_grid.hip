@@ -1,13 +1,16 @@
 // Grid (MFMA) form of the PSF convolution.
 //
 // Restates do_sparse_convolve_2d_f (SensorInstance2D.cu:44-83) as
-//   1. a bilinear (cloud-in-cell) deposit of every in-range sample onto the
-//      output grid extended by the PSF half-widths (k_deposit), and
+//   1. a bilinear (cloud-in-cell) deposit of every sample with a Binner key
+//      onto the output grid extended by the PSF half-widths (k_deposit, from
+//      the items the bucketing pass grouped by blocks of grid rows,
+//      kernels_bucket.hip), and
 //   2. a "valid" 2D correlation of that grid with the PSF taps,
 //        out(r, b) = sum_{i < Tr, j < Tb} T[i][j] G[r + i][b + j],
 //      computed as complex Toeplitz implicit GEMMs on MFMA (k_correlate),
 //   3. with the empty-window rule of do_make_bins (Binner.cu:96-97) applied
-//      as a row mask in the epilogue.
+//      as a row mask in the epilogue; the deposit launch's last workgroups
+//      find which windows are empty from the bucketing's key occupancy.
 // In exact arithmetic this equals the gather whenever the output axes are
 // uniform at the tap pitch and the Binner overlap covers the PSF support.
 // This file holds the deposit; the correlation is in kernels_corr.hip.
@@ -16,133 +19,141 @@
 
 namespace rtsim {
 
-// ---------------------------------------------------------------------------
-// deposit
-// ---------------------------------------------------------------------------
-// grid range coordinate of a travel (the row table uses the same rounding)
-__device__ __forceinline__ float grid_y(const DepositArgs& a, float t)
-{
-    return grid_y_of(t, a.range0, a.inv_dr, a.off_r);
-}
-
-// First entry below n at index >= j0 of the G-row table (rows without samples
-// hold n), and the same from j0 + 2 (n when there is none): the first samples
-// of G rows >= j0 - 1 and >= j0 + 1.  Each wave scans 64 entries per step.
-__device__ __forceinline__ void run_bounds(const uint32_t* __restrict__ marks, int count,
-                                           uint32_t n, int j0, uint32_t& s0, uint32_t& s1)
-{
-    const uint32_t lane = lane_id();
-    s0 = n;
-    s1 = n;
-    bool need0 = true;
-    for (int base = j0; base < count; base += 64) {
-        const int j = base + (int)lane;
-        const uint32_t m = j < count ? marks[j] : n;
-        const uint64_t hit = __ballot(m < n);
-        if (need0 && hit) {
-            s0 = __shfl(m, __ffsll((long long)hit) - 1, 64);
-            need0 = false;
-        }
-        const uint64_t hit1 = base == j0 ? hit & ~3ull : hit;
-        if (hit1) {
-            s1 = __shfl(m, __ffsll((long long)hit1) - 1, 64);
-            break;
-        }
-    }
-}
-
-// One workgroup per G row rho.  Its samples are the sorted run whose G row
-// is rho - 1 or rho (each sample's range weights go to rows gy and gy + 1);
-// each adds w_y w_x v to two columns of the row.  The row is accumulated in
-// LDS as 64-bit fixed point: integer adds are exact and independent of their
-// order, so the grid is bit-reproducible (and integer LDS atomics run at the
-// rate of plain LDS traffic, unlike FP32 ones).  With M the largest finite
-// |re|, |im| over the ping's in-range samples (found by the sort), products
-// are scaled by 2^k with (run length) M 2^k < 2^62, so no partial sum can
+// One workgroup per bucket b: grid rows [b R, b R + R).  Its items are every
+// binned sample whose grid row gy or gy + 1 lies in those rows (the bucketing
+// put a sample on a bucket's last row into both buckets); each adds
+// w_y w_x v to two columns of the rows of the bucket it reaches.  The rows are
+// accumulated in LDS as 64-bit fixed point: integer adds are exact and
+// independent of their order, so the grid is bit-reproducible whatever order
+// the bucketing left the items in (and integer LDS atomics run at the rate of
+// plain LDS traffic, unlike FP32 ones).  With M the largest finite |re|, |im|
+// over the ping's binned samples (from the bucket scan), products are scaled
+// by 2^k with (items in the bucket) M 2^k < 2^62, so no partial sum can
 // overflow; each product is rounded once, in FP32, then summed exactly.  A
-// run holding a non-finite in-range value yields a NaN row.
-constexpr int kDepositU = 4;   // samples per thread whose loads are in flight together
-constexpr int kDepositThreads = 256;   // 512 and 1024 measured 0.6 and 2.5 us slower
+// row reached by a non-finite value is NaN.
+constexpr int kDepositU = 4;          // items per thread whose loads are in flight together
+// 16 waves per bucket: with one workgroup per bucket (a few hundred on a
+// 1024-row grid) fewer waves leave the CUs idle while the LDS atomics wait
+constexpr int kDepositThreads = 1024;
 
-__device__ __forceinline__ void deposit_body(const DepositArgs& a, int rho)
+__device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
 {
-    extern __shared__ long long sacc[];               // [2][GW]: re plane, im plane
+    extern __shared__ long long sacc[];               // [R][2][GW]: re plane, im plane per row
+    __shared__ uint32_t ws[kDepositThreads / 64];
+    __shared__ uint32_t bad[kMaxBucketRows];
     const int t = threadIdx.x;
     const int GW = a.GW;
-    unsigned long long* sre = reinterpret_cast<unsigned long long*>(sacc);
-    unsigned long long* sim = sre + GW;
-    for (int k = t; k < 2 * GW; k += kDepositThreads) sacc[k] = 0;
-    uint32_t s0, s1;   // first samples with G row >= rho - 1 and >= rho + 1
-    run_bounds(a.row_start, a.GH + 3, a.n, rho, s0, s1);
+    const int r0 = b * a.R;
+    const int rows = min(a.R, a.GH - r0);
+    for (int k = t; k < 2 * GW * rows; k += kDepositThreads) sacc[k] = 0;
+    if (t < kMaxBucketRows) bad[t] = 0u;
+    // the bucket's items start after those of the buckets before it
+    uint32_t s = 0;
+    for (int i = t; i < b; i += kDepositThreads) s += a.totals[i];
+    uint32_t part;
+    (void)sort_block_scan<kDepositThreads>(s, ws, &part);
+    const uint32_t s0 = part, cnt = a.totals[b], s1 = s0 + cnt;
     const float M = __uint_as_float(a.vmax[0]);
     int em = 0;
     frexpf(M, &em);                                   // M < 2^em (em = 0 for M = 0)
-    const int ec = 32 - __clz(max(s1 - s0, 1u));     // run length < 2^ec
+    const int ec = 32 - __clz(max(cnt, 1u));          // items < 2^ec
     const int k = 62 - em - ec;
-    constexpr int U = kDepositU;
     __syncthreads();                                  // LDS cleared
-    int bad = 0;
-    float2* row = a.grid + (size_t)rho * GW;
+    unsigned long long* acc = reinterpret_cast<unsigned long long*>(sacc);
 
-    for (uint32_t sb = s0; sb < s1; sb += U * kDepositThreads) {
-        float4 rec[U];
-        float xg[U];
+    for (uint32_t sb = s0; sb < s1; sb += kDepositU * kDepositThreads) {
+        float4 it[kDepositU];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const uint32_t s = min(sb + u * kDepositThreads + t, s1 - 1);
-            rec[u] = a.sorted[s];
-            xg[u] = a.xg[s];
-        }
-        // weights for every slot first (no branch between the loads and their
-        // use), then the atomics; a slot past the run or out of range gets
-        // column -2 and adds nothing
-        int col[U];
-        float wx[U], vr[U], vi[U];
+        for (int u = 0; u < kDepositU; u++) it[u] = a.items[min(sb + u * kDepositThreads + t, s1 - 1)];
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const float y = grid_y(a, rec[u].z);
-            const float x = xg[u];
-            const float fy = floorf(y), fx = floorf(x);
-            const float wy1 = y - fy;
-            const int gy = (int)fy + a.fr;
-            // this row receives weight 1 - wy1 from gy == rho, wy1 from gy == rho - 1
-            const float wy = gy == rho ? 1.0f - wy1 : (gy == rho - 1 ? wy1 : 0.0f);
-            const bool ok = sb + u * kDepositThreads + t < s1 && rec[u].z >= a.t_lo && rec[u].z <= a.t_hi;
-            bad |= ok && !(fmaxf(fabsf(rec[u].x), fabsf(rec[u].y)) <= 3.402823466e38f &&
-                           rec[u].x == rec[u].x && rec[u].y == rec[u].y);
-            wx[u] = x - fx;
-            col[u] = ok ? (int)fx + a.fb : -2;
-            vr[u] = wy * rec[u].x;
-            vi[u] = wy * rec[u].y;
-        }
+        for (int u = 0; u < kDepositU; u++) {
+            if (sb + u * kDepositThreads + t >= s1) continue;
+            const float y = grid_y_of(it[u].z, a.rows.range0, a.rows.inv_dr, a.rows.off_r);
+            const float wy1 = y - floorf(y);
+            const int l0 = g_row_of(a.rows, it[u].z) - r0;   // rows l0 (1 - wy1) and l0 + 1 (wy1)
+            const float x = it[u].w;
+            const float fx = floorf(x);
+            const float wx = x - fx;
+            const int gx = (int)fx + a.fb;
+            const bool nonfinite = !(fmaxf(fabsf(it[u].x), fabsf(it[u].y)) <= 3.402823466e38f &&
+                                     it[u].x == it[u].x && it[u].y == it[u].y);
 #pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int gx = col[u];
-            if (gx >= 0 && gx < GW) {
-                const float w = 1.0f - wx[u];
-                atomicAdd(&sre[gx], (unsigned long long)(long long)ldexpf(w * vr[u], k));
-                atomicAdd(&sim[gx], (unsigned long long)(long long)ldexpf(w * vi[u], k));
-            }
-            if (gx + 1 >= 0 && gx + 1 < GW) {
-                atomicAdd(&sre[gx + 1], (unsigned long long)(long long)ldexpf(wx[u] * vr[u], k));
-                atomicAdd(&sim[gx + 1], (unsigned long long)(long long)ldexpf(wx[u] * vi[u], k));
+            for (int h = 0; h < 2; h++) {
+                const int l = l0 + h;
+                if (l < 0 || l >= rows) continue;
+                if (nonfinite) bad[l] = 1u;
+                const float wy = h ? wy1 : 1.0f - wy1;
+                const float vr = wy * it[u].x, vi = wy * it[u].y;
+                unsigned long long* re = acc + (size_t)l * 2 * GW;
+                unsigned long long* im = re + GW;
+                if (gx >= 0 && gx < GW) {
+                    const float w = 1.0f - wx;
+                    atomicAdd(&re[gx], (unsigned long long)(long long)ldexpf(w * vr, k));
+                    atomicAdd(&im[gx], (unsigned long long)(long long)ldexpf(w * vi, k));
+                }
+                if (gx + 1 >= 0 && gx + 1 < GW) {
+                    atomicAdd(&re[gx + 1], (unsigned long long)(long long)ldexpf(wx * vr, k));
+                    atomicAdd(&im[gx + 1], (unsigned long long)(long long)ldexpf(wx * vi, k));
+                }
             }
         }
     }
-    if (__syncthreads_or(bad)) {
-        for (int c = t; c < GW; c += kDepositThreads) row[c] = make_float2(NAN, NAN);
-        return;
+    __syncthreads();
+    for (int l = 0; l < rows; l++) {
+        float2* row = a.grid + (size_t)(r0 + l) * GW;
+        const long long* re = sacc + (size_t)l * 2 * GW;
+        if (bad[l]) {
+            for (int c = t; c < GW; c += kDepositThreads) row[c] = make_float2(NAN, NAN);
+        } else {
+            for (int c = t; c < GW; c += kDepositThreads)
+                row[c] = make_float2(ldexpf((float)re[c], -k), ldexpf((float)re[GW + c], -k));
+        }
     }
-    for (int c = t; c < GW; c += kDepositThreads)
-        row[c] = make_float2(ldexpf((float)sacc[c], -k), ldexpf((float)sacc[GW + c], -k));
 }
 
-// workgroups [0, GH) deposit G rows, the rest compute the Binner windows
-// (do_make_bins, Binner.cu:82-106) for the correlation's empty-row mask
+// Windows of output bins [wb 256, wb 256 + 256) (do_make_bins, Binner.cu:
+// 82-106): a window holds samples when at least two of the bins
+// [r - ov, r + ov] (inside [0, N)) hold samples; the occupancy is the OR of
+// the bucketing's per-tile key bits.  Only the emptiness is kept:
+// win[r] = (0, 1) for a window with samples, (0, 0) for an empty one (the
+// correlation masks rows whose window size is 0).
+__device__ __forceinline__ void occupancy_windows(const DepositArgs& a, int wb)
+{
+    extern __shared__ long long sacc[];
+    uint32_t* lw = reinterpret_cast<uint32_t*>(sacc);
+    const int t = threadIdx.x;
+    const int N = (int)a.N, ov = a.ov;
+    const int first = wb * kDepositThreads;
+    const int lo_bin = max(0, first - ov), hi_bin = min(N - 1, first + kDepositThreads - 1 + ov);
+    const int wlo = lo_bin >> 5, nw = (hi_bin >> 5) - wlo + 1;
+    for (int w = t; w < nw; w += kDepositThreads) lw[w] = 0u;
+    __syncthreads();
+    const uint32_t total = (uint32_t)nw * a.tiles;
+    for (uint32_t idx = t; idx < total; idx += kDepositThreads) {
+        const uint32_t w = idx % (uint32_t)nw, tile = idx / (uint32_t)nw;
+        const uint32_t v = a.occ[(size_t)tile * a.occ_words + wlo + w];
+        if (v) atomicOr(&lw[w], v);
+    }
+    __syncthreads();
+    const int r = first + t;
+    if (r >= N) return;
+    const int lo = max(0, r - ov), hi = min(N - 1, r + ov);
+    int c = 0;
+    for (int w = lo >> 5; w <= (hi >> 5) && c < 2; w++) {
+        uint32_t v = lw[w - wlo];
+        const int wb0 = w << 5;
+        if (wb0 < lo) v &= ~0u << (lo - wb0);
+        if (wb0 + 31 > hi) v &= ~0u >> (wb0 + 31 - hi);
+        c += __popc(v);
+    }
+    a.win[r] = make_uint2(0u, c >= 2 ? 1u : 0u);
+}
+
+// workgroups [0, NB) deposit buckets, the rest compute the windows
 __device__ __forceinline__ void deposit_or_windows(const DepositArgs& a, uint32_t blk)
 {
-    if (blk < (uint32_t)a.GH) deposit_body(a, (int)blk);
-    else windows_body(a.bins, blk - (uint32_t)a.GH);
+    if (blk < (uint32_t)a.NB) deposit_body(a, (int)blk);
+    else occupancy_windows(a, (int)(blk - (uint32_t)a.NB));
 }
 
 __global__ __launch_bounds__(kDepositThreads) void k_deposit(DepositArgs a) { deposit_or_windows(a, blockIdx.x); }
@@ -157,9 +168,15 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_batch(const Deposit
 
 uint32_t deposit_blocks(const DepositArgs& a)
 {
-    return (uint32_t)a.GH + (a.bins.N + kDepositThreads - 1) / kDepositThreads;   // + windows
+    return (uint32_t)a.NB + (a.N + kDepositThreads - 1) / kDepositThreads;   // + windows
 }
-size_t deposit_lds_bytes(const DepositArgs& a) { return sizeof(long long) * 2 * a.GW; }
+
+size_t deposit_lds_bytes(const DepositArgs& a)
+{
+    const size_t dep = sizeof(long long) * 2 * (size_t)a.GW * a.R;
+    const size_t win = sizeof(uint32_t) * ((size_t)a.occ_words + 2);
+    return dep > win ? dep : win;
+}
 
 hipError_t launch_deposit(const DepositArgs& a, hipStream_t s)
 {

@@ -2,14 +2,16 @@
 // allocate device memory (code under HIP graph capture, tight serving loops,
 // harnesses that forbid hipMalloc).  The records come from the caller and the
 // ping goes to the caller; every per-ping buffer the kernels need beyond them
-// (the sort's key / index pairs, tile counts and digit totals, the sorted
-// records and their coordinates, the records between grouping-sort passes,
-// the Binner segments and windows) lives in one file-scope device array of
-// fixed size.  Each GPU holds its own copy of that array until the process
-// ends; its address on a device is looked up once, on that device's first
-// call, and every later call on the device reuses it.  The sensor's own
-// configuration tables (PSF, coordinates) and, for the MFMA forms, its
-// prepared grid state are read as set up beforehand; nothing here allocates.
+// lives in one file-scope device array of fixed size: for the plain form the
+// sort's key / index pairs, tile counts and digit totals, the sorted records
+// and their coordinates; for the grid form the bucketed items, the bucket
+// counts, the key occupancy and the tile maxima (the two sets overlap: a call
+// uses one of them); for both the Binner segments and windows.  Each GPU holds
+// its own copy of that array until the process ends; its address on a device
+// is looked up once, on that device's first call, and every later call on the
+// device reuses it.  The sensor's own configuration tables (PSF, coordinates)
+// and, for the MFMA forms, its prepared grid state are read as set up
+// beforehand; nothing here allocates.
 #include <hip/hip_runtime.h>
 
 #include <mutex>
@@ -24,19 +26,28 @@ namespace {
 
 constexpr size_t kMaxS = RTSIM_NOALLOC_MAX_SAMPLES;
 constexpr size_t kMaxN = RTSIM_NOALLOC_MAX_BINS;
+constexpr size_t kTilesCap = (kMaxS + kSortTileSmall - 1) / kSortTileSmall;
 constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
-// scratch layout (bytes), every buffer 256-byte aligned
+// scratch layout (bytes), every buffer 256-byte aligned; the plain-form and
+// grid-form sets share the space from offset 0
 constexpr size_t kOffSorted = 0;
-constexpr size_t kOffTmp = kOffSorted + align256(sizeof(float4) * kMaxS);
-constexpr size_t kOffCoord = kOffTmp + align256(sizeof(float4) * kMaxS);
+constexpr size_t kOffCoord = kOffSorted + align256(sizeof(float4) * kMaxS);
 constexpr size_t kOffKv0 = kOffCoord + align256(sizeof(float2) * kMaxS);
 constexpr size_t kOffKv1 = kOffKv0 + align256(sizeof(uint2) * kMaxS);
-constexpr size_t kTilesCap = (kMaxS + kSortTileSmall - 1) / kSortTileSmall;
 constexpr size_t kOffCounts = kOffKv1 + align256(sizeof(uint2) * kMaxS);
 constexpr size_t kOffTotals = kOffCounts + align256(sizeof(uint32_t) * 256 * kTilesCap);
-constexpr size_t kOffTmax = kOffTotals + align256(sizeof(uint32_t) * 256);
-constexpr size_t kOffSeg = kOffTmax + align256(sizeof(uint32_t) * kTilesCap);
+constexpr size_t kEndPlain = kOffTotals + align256(sizeof(uint32_t) * 256);
+
+constexpr size_t kOffItems = 0;
+constexpr size_t kBcountWords = kBucketCountBudget + (size_t)kMaxBuckets;
+constexpr size_t kOffBcounts = kOffItems + align256(sizeof(float4) * 2 * kMaxS);
+constexpr size_t kOccWords = kTilesCap * (kMaxN / 32);
+constexpr size_t kOffOcc = kOffBcounts + align256(sizeof(uint32_t) * kBcountWords);
+constexpr size_t kOffTmax = kOffOcc + align256(sizeof(uint32_t) * kOccWords);
+constexpr size_t kEndGrid = kOffTmax + align256(sizeof(uint32_t) * kTilesCap);
+
+constexpr size_t kOffSeg = kEndPlain > kEndGrid ? kEndPlain : kEndGrid;
 constexpr size_t kOffWin = kOffSeg + align256(sizeof(uint2) * kMaxN);
 constexpr size_t kScratchBytes = kOffWin + align256(sizeof(uint2) * kMaxN);
 static_assert(kScratchBytes == RTSIM_NOALLOC_SCRATCH_BYTES, "scratch size in rtsim.h");
@@ -99,23 +110,26 @@ extern "C" int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void
     pb.n = (uint32_t)n;
     pb.tiles_cap = (uint32_t)kTilesCap;
     pb.sorted = reinterpret_cast<float4*>(base + kOffSorted);
-    pb.tmp = reinterpret_cast<float4*>(base + kOffTmp);
     pb.coord = reinterpret_cast<float2*>(base + kOffCoord);
     pb.kv[0] = reinterpret_cast<uint2*>(base + kOffKv0);
     pb.kv[1] = reinterpret_cast<uint2*>(base + kOffKv1);
-    pb.order = nullptr;
-    pb.skeys = nullptr;
     pb.counts = reinterpret_cast<uint32_t*>(base + kOffCounts);
     pb.totals = reinterpret_cast<uint32_t*>(base + kOffTotals);
+    pb.items = reinterpret_cast<float4*>(base + kOffItems);
+    pb.bcounts = reinterpret_cast<uint32_t*>(base + kOffBcounts);
+    pb.bcounts_cap = kBcountWords;
+    pb.occ = reinterpret_cast<uint32_t*>(base + kOffOcc);
+    pb.occ_cap = kOccWords;
     pb.tmax = reinterpret_cast<uint32_t*>(base + kOffTmax);
     pb.seg = reinterpret_cast<uint2*>(base + kOffSeg);
     pb.win = reinterpret_cast<uint2*>(base + kOffWin);
     pb.out = static_cast<float2*>(ping);
     (void)hipGetLastError();
     PingPlan p;
-    if ((rc = s->plan(form, &p, false, false, &pb))) return rc;
+    if ((rc = s->plan(form, &p, &pb))) return rc;
     // every array the ping reads is written first by its own kernels (the
-    // sort's first histogram clears the segments and the row table): nothing
-    // a previous call left in the scratch is read
+    // sort's first histogram clears the segments; the bucketing writes every
+    // count, occupancy word and item it reads): nothing a previous call left
+    // in the scratch is read
     return s->launch_ping(p, form, (hipStream_t)stream, false, false);
 }

@@ -91,10 +91,10 @@ hipError_t launch_stream_bucket(const StreamArgs& a, const BucketArgs& b, hipStr
 hipError_t launch_bucket_hist(const BucketArgs& b, hipStream_t s);
 hipError_t launch_bucket_scan(const BucketArgs& b, hipStream_t s);
 hipError_t launch_bucket_scatter(const BucketArgs& b, hipStream_t s);
-size_t bucket_lds_bytes(const BucketArgs& b);
+size_t bucket_lds_bytes(const BucketArgs& b);   // the scatter's dynamic LDS
+// (batched: the bucket scan runs k_sort_scan's batched launch over ScanArgs)
 hipError_t launch_bucket_hist_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
                                     hipStream_t s);
-hipError_t launch_bucket_scan_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_bucket_scatter_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
                                        hipStream_t s);
 constexpr int kMaxBuckets = 8192;        // buckets of one ping (LDS counters)
@@ -184,6 +184,8 @@ struct SortPlan {
 };
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p);
 hipError_t launch_sort_plan(const SortPlan& p, hipStream_t s);
+// k_sort_scan over `rows` rows of a count matrix (also the grid form's buckets)
+hipError_t launch_scan_rows(const ScanArgs& a, uint32_t rows, hipStream_t s);
 // batched sort passes (args are device arrays, one entry per ping)
 hipError_t launch_sort_hist_batch(bool first_pass, const HistArgs* args, BatchMap m, uint32_t blocks,
                                   hipStream_t s);

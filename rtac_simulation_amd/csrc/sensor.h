@@ -32,9 +32,10 @@ constexpr uint32_t kMaxWaveformTaps = 1u << 16;
 struct PingPlan {
     int form = 0;
     bool grid = false;
-    SortPlan sort{};
+    SortPlan sort{};          // plain form (and Binner introspection)
     BinArgs bins{};
     ConvArgs conv{};
+    BucketArgs bucket{};      // grid form
     DepositArgs dep{};
     CorrArgs corr{};
 };
@@ -46,8 +47,12 @@ struct PingBuffers {
     uint32_t n;              // samples
     uint32_t tiles_cap;      // tile-count columns of `counts`
     float4* sorted;
-    float4* tmp;             // records between grouping-sort passes
-    float2* coord;           // bearing-axis coordinates (grid form: grid columns)
+    float2* coord;           // bearing-axis coordinates (hi, lo)
+    float4* items;           // grid form: 2 x n bucketed items
+    uint32_t* bcounts;       // grid form: bucket counts
+    size_t bcounts_cap;      // words
+    uint32_t* occ;           // grid form: key occupancy per tile
+    size_t occ_cap;          // words
     uint2* kv[2];
     uint32_t* order;         // introspection only (may be null otherwise)
     uint32_t* skeys;         // introspection only (may be null otherwise)
@@ -101,15 +106,14 @@ struct Sensor {
     int computed_form = -1;       // form of the last compute (batched or not)
     float4* d_samples = nullptr;  // receivedSamples_ (unsorted)
     float4* d_sorted = nullptr;   // receivedSamples_ after sort()
-    float4* d_tmp = nullptr;      // records between sort passes (grid form)
-    float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample;
-                                  // grid form: a float grid column coordinate
+    float4* d_items = nullptr;    // grid form: bucketed items (2 per sample at most)
+    float2* d_coord = nullptr;    // bearing-axis coordinate (hi, lo) per sorted sample
     uint2* d_kv[2] = {nullptr, nullptr};  // sort: (key, index) between passes
     uint32_t* d_order = nullptr;  // sort permutation (written on request)
     uint32_t* d_skeys = nullptr;  // Binner::keys_ of the sorted samples (written on request)
     uint32_t* d_totals = nullptr; // sort: digit totals of a pass
     uint32_t* d_counts = nullptr; // sort: per-tile digit counts (capacity-sized)
-    uint32_t* d_tmax = nullptr;   // grouping sort: per-tile largest |re|, |im|
+    uint32_t* d_tmax = nullptr;   // grid form: per-tile largest |re|, |im|
 
     // ---- per-configuration device tables -----------------------------------
     uint2* d_seg = nullptr;       // Binner::binIndexes_
@@ -148,16 +152,18 @@ struct Sensor {
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();
     bool grid_ready() const;         // grid form prepared (no preparation is started)
-    int grid_row_map(RowMap* m, uint32_t** vmax = nullptr);   // grid form row table, G-row map
+    int grid_reserve();              // grid form buffers sized for `capacity`
+    void grid_buffers(PingBuffers* pb) const;   // the grid form's own buffers
     int compute(int form, hipStream_t s);
-    // queues a planned ping (profiling events only when `profile`)
+    // queues a planned ping (profiling events only when `profile`); skip_hist0:
+    // the grid form's bucket counts are already made (the streaming kernel)
     int launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hist0, bool profile);
-    int grid_plan(int form, DepositArgs* d, CorrArgs* c, const PingBuffers& pb);
-    // validates the form; queues nothing.  introspect: the sort also writes
-    // the permutation and the Binner keys.  batched: for the batched launches
-    // (their sort tiles)
-    int plan(int form, PingPlan* p, bool introspect = false, bool batched = false,
-             const PingBuffers* pb = nullptr);
+    int grid_plan(int form, BucketArgs* b, DepositArgs* d, CorrArgs* c, const PingBuffers& pb);
+    // validates the form; queues nothing.  pb: the buffers (default: own)
+    int plan(int form, PingPlan* p, const PingBuffers* pb = nullptr);
+    // Binner introspection: the sort by travel with the keys and the
+    // permutation kept, then the windows (any form)
+    int plan_binning(PingPlan* p);
     PingBuffers own_buffers() const;
     int autotune(int form, int iterations, hipStream_t s, int* best_shape, float* best_ms);
     StreamArgs stream_args(const rtsim_hits& hits, size_t n, size_t offset) const;

@@ -225,14 +225,13 @@ static void oracle_paths()
     orc_grid g{1.f, 3.f, H, W, ORC_AXIS_SINE, Tr, Tb, 0.3f, 0.4f};
     orc_key key;
     orc_key_make(H, 1.f, 3.f, 0.3f, &key);
-    std::vector<uint32_t> order(n), keys(n), seg(2 * H), win(2 * H), go(n);
+    std::vector<uint32_t> order(n), keys(n), seg(2 * H), win(2 * H);
     orc_sort_order(rec.data() + 2, n, 4, order.data());
     std::vector<float> srt(4 * n);
     for (size_t i = 0; i < n; i++) std::memcpy(&srt[4 * i], &rec[4 * order[i]], 16);
     orc_keys(&key, srt.data() + 2, n, 4, keys.data());
     orc_segments(keys.data(), n, H, seg.data());
     orc_windows(seg.data(), H, orc_overlap(0.3f, 1.f, 3.f, H), win.data());
-    orc_group_order(rec.data() + 2, n, 4, &key, 1.f, 20.f, 0.f, 3, 64 + Tr - 1, go.data());
     std::vector<double> out(2 * (size_t)H * W), rows_out(2 * 3 * (size_t)W), one(2 * H);
     orc_conv2d(srt.data(), win.data(), &g, bearings.data(), taps.data(), out.data(), 3);
     const uint32_t rows[3] = {0, 17, H - 1};

@@ -173,31 +173,35 @@ def test_mfma_bitwise_reproducible(gpu, form):
 
 
 @pytest.mark.parametrize("form", FORMS)
-def test_mfma_grouped_binning_full_size(gpu, form):
-    """The grid form sorts by the grouping key (bin slot + G row): keys,
-    segments and windows bit-exact against the oracle's travel-sorted Binner,
-    the order equal to the oracle's grouped order (stable by bin slot, G row),
-    the sorted records their permutation."""
+def test_mfma_binning_full_size(gpu, form):
+    """The grid form bins by grid row without sorting; its window mask comes
+    from the Binner-key occupancy of the bucketing: every row whose oracle
+    window is empty is exactly zero, the other rows hold the ping.  Binner
+    introspection after it (a sort by travel) is bit-exact against the
+    oracle: order, keys, segments, windows and the sorted records."""
     R = gpu
     cfg, scene, S, seed = configs.workload("fls_512x1024")
     hits = R.generate_hits(scene, seed, S)
     s = _sensor(R, cfg, hits)
     s.compute_output(form)
+    ping = s.output()
     rec = s.samples()
     b = O.binning(cfg, rec)
+    empty = b["windows"][:, 1] == 0
+    assert empty.any() and (~empty).any()
+    assert not np.abs(ping[empty]).any()
+    assert (np.abs(ping[~empty]).max(axis=1) > 0).mean() > 0.99
     d = s.binner_debug()
-    og = O.group_order(cfg, rec[:, 2])
-    assert np.array_equal(d["order"], og), "grouped order"
-    for key in ("keys", "segments", "windows"):
+    for key in ("order", "keys", "segments", "windows"):
         assert np.array_equal(d[key], b[key]), key
-    assert np.array_equal(s.samples(sorted=True).view(np.uint32), rec[og].view(np.uint32))
+    assert np.array_equal(s.samples(sorted=True).view(np.uint32), b["sorted"].view(np.uint32))
 
 
 @pytest.mark.parametrize("form", FORMS)
 def test_mfma_ping_independent_of_sample_order(gpu, form):
-    """Permuting the input records changes the order inside every (bin, G row)
-    run but not the ping: the deposit sums in exact fixed point, so the two
-    pings agree bit for bit."""
+    """Permuting the input records changes the order of the items inside
+    every bucket but not the ping: the deposit sums in exact fixed point, so
+    the two pings agree bit for bit."""
     R = gpu
     cfg = configs._fls(128, 256, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "perm")
     hits = R.generate_hits(configs.fls_scene(), 27, 150_000)
@@ -208,14 +212,12 @@ def test_mfma_ping_independent_of_sample_order(gpu, form):
     perm = np.random.default_rng(3).permutation(rec.shape[0])
     s.upload_samples(np.ascontiguousarray(rec[perm]))
     s.compute_output(form)
-    assert not np.array_equal(s.binner_debug()["order"], O.group_order(cfg, rec[:, 2]))
     assert np.array_equal(s.output().view(np.uint32), a)
 
 
-def test_mfma_grouping_sort_three_passes(gpu):
-    """33000 range bins make the grouping key exceed 16 bits: the sort takes
-    three passes (the middle one moving records too); binning stays exact and
-    the ping matches the oracle."""
+def test_mfma_tall_grid(gpu):
+    """33000 range bins: 2066 buckets of 16 grid rows and 1032 words of key
+    occupancy per tile; binning stays exact and the ping matches the oracle."""
     R = gpu
     cfg = configs._fls(64, 33000, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "tall")
     hits = R.generate_hits(configs.fls_scene(), 31, 200_000)
@@ -223,15 +225,12 @@ def test_mfma_grouping_sort_three_passes(gpu):
     s.compute_output(2)
     rec = s.samples()
     b = O.binning(cfg, rec)
-    d = s.binner_debug()
-    assert np.array_equal(d["order"], O.group_order(cfg, rec[:, 2]))
-    for key in ("keys", "segments", "windows"):
-        assert np.array_equal(d[key], b[key]), key
     rows = np.arange(3, cfg.H, 97)
-    win = np.zeros_like(b["windows"])
-    win[rows] = b["windows"][rows]
-    ref = O.conv(cfg, b["sorted"], win)
-    assert O.rel_l2(s.output()[rows], ref[rows]) <= MFMA_TOL
+    ref = O.conv_rows(cfg, b["sorted"], b["windows"], rows)
+    assert O.rel_l2(s.output()[rows], ref) <= MFMA_TOL
+    d = s.binner_debug()
+    for key in ("order", "keys", "segments", "windows"):
+        assert np.array_equal(d[key], b[key]), key
 
 
 def _key_interval(k, inside):
@@ -323,9 +322,9 @@ def test_mfma_eight_wave_correlation_large_psf(gpu):
 @pytest.mark.gpu
 def test_mfma_first_histogram_fused_into_write_samples(gpu):
     """Once the grid form is prepared, a write of every sample also makes the
-    grouping sort's first histogram; the computes that use it give the same
-    ping (bit for bit) and binning as computes that run the histogram, and
-    writes by other means, partial writes and other forms discard it."""
+    bucket counts; the computes that use them give the same ping (bit for
+    bit) and binning as computes that count themselves, and writes by other
+    means, partial writes and other forms discard them."""
     R = gpu
     cfg = configs._fls(128, 256, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "fused")
     hits = R.generate_hits(configs.fls_scene(), 29, 120_000)
@@ -336,7 +335,7 @@ def test_mfma_first_histogram_fused_into_write_samples(gpu):
     s.compute_output(2)
     ref = s.output().view(np.uint32).copy()
     b = O.binning(cfg, s.samples())
-    s.compute_output(2)                      # histogram run by the sort itself
+    s.compute_output(2)                      # bucket counts made by the compute itself
     assert np.array_equal(s.output().view(np.uint32), ref)
     for form in (2, 1, 2):
         s.write_samples(dh)                  # fused

@@ -220,9 +220,8 @@ def test_repeated_compute_is_stable(gpu):
 
 @pytest.mark.parametrize("case", ["ties", "all_oor", "all_null", "nan_inf_zero", "single_bin", "empty"])
 def test_mfma_binner_edge_cases(gpu, case):
-    """The grid form's grouping sort on the edge cases: keys, segments and
-    windows as the oracle's travel sort, the order as the oracle's grouping
-    order, the ping within the MFMA tolerance (or zero)."""
+    """The grid form's bucketing on the edge cases: the ping within the MFMA
+    tolerance (or zero), Binner introspection after it bit-exact."""
     R = gpu
     cfg = configs._fls(32, 64, 0.1, 30.0, 1.2e6, configs.psf_31x17(), case)
     rec = _records_case(case) if case != "empty" else np.zeros((0, 4), np.float32)
@@ -234,9 +233,8 @@ def test_mfma_binner_edge_cases(gpu, case):
         s.compute_output(form)
         b = O.binning(cfg, rec)
         d = s.binner_debug()
-        for key in ("keys", "segments", "windows"):
+        for key in ("order", "keys", "segments", "windows"):
             assert np.array_equal(d[key], b[key]), (form, key)
-        assert np.array_equal(d["order"], O.group_order(cfg, rec[:, 2])), form
         ref = O.conv(cfg, b["sorted"], b["windows"])
         out = s.output()
         if np.abs(ref).max() == 0:

@@ -234,70 +234,25 @@ def test_single_nonempty_bin_window_is_empty():
     assert tuple(w[4]) == (0, 12) and tuple(w[3]) == (0, 12) and tuple(w[1]) == (0, 0)
 
 
-def _group_ref(cfg, t):
-    """Grouped order of the grid form restated with exact rational fmas: bin
-    slot from the (pinned) key, out-of-range samples split at the range
-    midpoint (every OOR travel lies beyond one end), G row from the rounded
-    fma, then a stable lexicographic sort."""
-    range0, inv_dr, off_r, fr, GH = O.group_params(cfg)
-    k = O.key_make(cfg.range_count, cfg.range_min, cfg.range_max, O.effective_margin(cfg))
-    keys = O.keys(k, t)
-    mid = 0.5 * (cfg.range_min + cfg.range_max)
-    slot, row = [], []
-    for x, key in zip(t, keys):
-        x = np.float32(x)
-        if key != OOR:
-            slot.append(int(key) + 1)
-        else:
-            slot.append(0 if (not math.isnan(x) and x < mid) else cfg.range_count + 1)
-        if math.isnan(x):
-            row.append(GH + 2)
-            continue
-        d = np.float32(x - np.float32(range0))
-        y = d if math.isinf(d) else np.float32(_fmaf(d, np.float32(inv_dr), np.float32(off_r)))
-        if math.isnan(y):
-            row.append(GH + 2)
-        elif math.isinf(y):
-            row.append(GH + 2 if y > 0 else -2)
-        else:
-            row.append(int(min(max(math.floor(y) + fr, -2), GH + 2)))
-    return np.lexsort((np.arange(t.size), np.array(row), np.array(slot))).astype(np.uint32)
-
-
-def test_group_order_matches_independent_restatement():
-    from rtac_simulation_amd import configs
-    cfg = configs._fls(48, 96, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "g")
-    rng = np.random.default_rng(11)
-    t = rng.uniform(-1.0, 31.0, 3000).astype(np.float32)
-    t[::7] = t[::7][::-1]                        # repeated travels (ties)
-    t[::97] = -1.0                               # null samples
-    t[1::101] = np.nan
-    t[2::103] = -0.0
-    t[3::107] = np.inf
-    r = O.group_params(cfg)
-    edges = ((np.arange(-3, 40) - r[2] - r[3]) / r[1] + r[0]).astype(np.float32)
-    t[4::11][:edges.size] = edges[:t[4::11].size]   # travels on G-row boundaries
-    got = O.group_order(cfg, t)
-    assert np.array_equal(got, _group_ref(cfg, t))
-
-
-def test_group_order_is_travel_order_within_runs():
-    """Sorted by the grouped order, the keys and hence segments and windows
-    equal those of the travel order; inside each equal-key run the travels
-    are a permutation of the travel-sorted ones."""
-    from rtac_simulation_amd import configs
-    cfg = configs._fls(64, 256, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "g")
-    rng = np.random.default_rng(12)
-    t = np.round(rng.uniform(-0.5, 30.5, 50_000), 3).astype(np.float32)
-    t[::500] = -1.0
-    t[1::700] = np.nan
-    og, ot = O.group_order(cfg, t), O.sort_order(t)
-    assert np.array_equal(np.sort(og), np.arange(t.size))
-    k = O.key_make(cfg.range_count, cfg.range_min, cfg.range_max, O.effective_margin(cfg))
-    kg, kt = O.keys(k, t[og]), O.keys(k, t[ot])
-    assert np.array_equal(kg, kt)
-    seg = O.segments(kg, cfg.range_count)
-    for b in range(cfg.range_count):
-        s0, s1 = seg[b]
-        assert np.array_equal(np.sort(t[og][s0:s1]), t[ot][s0:s1])
-    assert not np.array_equal(og, ot)            # the orders do differ inside runs
+def test_window_emptiness_is_two_occupied_bins():
+    """The grid form keeps only which windows hold samples and finds them
+    from key occupancy: a window is non-empty exactly when at least two bins
+    of [r - ov, r + ov] (clipped to [0, N)) hold samples -- the reading of
+    Binner.cu:90-106 (skip empty bins inward, istart >= iend is empty) this
+    checks against the oracle's windows on random occupancies."""
+    rng = np.random.default_rng(21)
+    for trial in range(200):
+        N = int(rng.integers(1, 80))
+        ov = int(rng.integers(0, 12))
+        occ = rng.random(N) < rng.uniform(0.02, 0.6)
+        seg = np.zeros((N, 2), np.uint32)
+        pos = 0
+        for k in range(N):
+            if occ[k]:
+                n = int(rng.integers(1, 5))
+                seg[k] = (pos, pos + n)
+                pos += n
+        win = O.windows(seg, ov)
+        for r in range(N):
+            lo, hi = max(0, r - ov), min(N - 1, r + ov)
+            assert (win[r, 1] > 0) == (occ[lo:hi + 1].sum() >= 2), (trial, r)
