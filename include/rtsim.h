@@ -200,7 +200,7 @@ int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
  * sensor's own samples, sorted records, Binner arrays and ping are untouched. */
 #define RTSIM_NOALLOC_MAX_SAMPLES 1048576u
 #define RTSIM_NOALLOC_MAX_BINS 16384u
-#define RTSIM_NOALLOC_SCRATCH_BYTES 44339200u   /* 42.3 MiB per GPU */
+#define RTSIM_NOALLOC_SCRATCH_BYTES 44347392u   /* 42.3 MiB per GPU */
 int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void* records, size_t n,
                                  void* ping, void* stream);
 

@@ -80,7 +80,7 @@ struct Batch {
     Stage<BucketArgs> bucket_hist, bucket_scatter;
     Stage<ScanArgs> bucket_scan;
     Stage<BinArgs> windows;                // plain pings (the grid form's are in its deposit)
-    Stage<ConvArgs> plain2d, plain1d;
+    Stage<ConvArgs> plain_rank, plain2d, plain1d;
     Stage<DepositArgs> deposit;
     std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape, waves per workgroup)
 
@@ -148,7 +148,9 @@ static int build(Batch* b)
     b->bucket_scan = {};
     b->bucket_scatter = {};
     b->windows = {};
+    b->plain_rank = {};
     b->plain2d = {};
+    b->plain2d.variant = 1;   // taps in LDS unless a ping's do not fit there
     b->plain1d = {};
     b->deposit = {};
     b->corr.clear();
@@ -173,8 +175,11 @@ static int build(Batch* b)
             }
             b->windows.add(p.bins, windows_blocks(p.bins));
             if (s->kind == RTSIM_SENSOR_2D) {
-                b->plain2d.add(p.conv, conv2d_plain_blocks(p.conv));
+                const uint32_t blocks = conv2d_plain_blocks(p.conv);
+                b->plain_rank.add(p.conv, blocks ? 1u : 0u);
+                b->plain2d.add(p.conv, blocks);
                 b->plain2d.shm = std::max(b->plain2d.shm, plain2d_lds_bytes(p.conv));
+                if (blocks && !p.conv.psf_in_lds) b->plain2d.variant = 0;
             } else {
                 b->plain1d.add(p.conv, conv1d_plain_blocks(p.conv));
             }
@@ -214,6 +219,7 @@ static int build(Batch* b)
     layout(b->bucket_scan, off);
     layout(b->bucket_scatter, off);
     layout(b->windows, off);
+    layout(b->plain_rank, off);
     layout(b->plain2d, off);
     layout(b->plain1d, off);
     layout(b->deposit, off);
@@ -237,6 +243,7 @@ static int build(Batch* b)
     pack(b->bucket_scan, host);
     pack(b->bucket_scatter, host);
     pack(b->windows, host);
+    pack(b->plain_rank, host);
     pack(b->plain2d, host);
     pack(b->plain1d, host);
     pack(b->deposit, host);
@@ -265,8 +272,9 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
     RT_TRY(launch_bucket_scatter_batch(dargs(b, b->bucket_scatter), dmap(b, b->bucket_scatter),
                                        b->bucket_scatter.blocks(), b->bucket_scatter.shm, s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
+    RT_TRY(launch_plain_rank_batch(dargs(b, b->plain_rank), dmap(b, b->plain_rank), b->plain_rank.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
-                                     b->plain2d.shm, s));
+                                     b->plain2d.shm, b->plain2d.variant != 0, s));
     RT_TRY(launch_conv1d_plain_batch(dargs(b, b->plain1d), dmap(b, b->plain1d), b->plain1d.blocks(), s));
     RT_TRY(launch_deposit_batch(dargs(b, b->deposit), dmap(b, b->deposit), b->deposit.blocks(),
                                 b->deposit.shm, s));

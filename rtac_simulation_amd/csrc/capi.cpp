@@ -147,11 +147,13 @@ Sensor::~Sensor()
     free_samples();
     free_grid();
     dev_free(d_psf);
+    dev_free(d_psf_cells);
     dev_free(d_bear);
     dev_free(d_rng);
     dev_free(d_dir);
     dev_free(d_seg);
     dev_free(d_win);
+    dev_free(d_work);
     dev_free(d_out);
     dev_free(d_totals);
     for (auto& p : events) {
@@ -200,9 +202,32 @@ int Sensor::reserve(size_t count)
 int Sensor::upload_psf()
 {
     dev_free(d_psf);
+    dev_free(d_psf_cells);
     int rc = dev_alloc(&d_psf, (size_t)Tr * Tb);
     if (rc) return rc;
     HIP_TRY(hipMemcpy(d_psf, psf.data(), sizeof(float) * psf.size(), hipMemcpyHostToDevice));
+    // bilinear cells for j in [-1, Tr], i in [-1, Tb] (FP64, rounded once):
+    // A = T[j][i], B = T[j][i+1] - A, C = T[j+1][i] - A,
+    // D = T[j+1][i+1] - T[j+1][i] - T[j][i+1] + A
+    const long R = (long)Tr, B = (long)Tb, stride = B + 2;
+    auto tap = [&](long j, long i, int c) {
+        return j >= 0 && j < R && i >= 0 && i < B ? (double)psf[2 * (j * B + i) + c] : 0.0;
+    };
+    std::vector<float> cells(8 * (size_t)stride * (size_t)(R + 2));
+    for (long j = -1; j <= R; j++)
+        for (long i = -1; i <= B; i++) {
+            float* q = &cells[8 * (size_t)((j + 1) * stride + (i + 1))];
+            for (int c = 0; c < 2; c++) {
+                const double a = tap(j, i, c), b = tap(j, i + 1, c), cc = tap(j + 1, i, c),
+                             d = tap(j + 1, i + 1, c);
+                q[c] = (float)a;
+                q[2 + c] = (float)(b - a);
+                q[4 + c] = (float)(cc - a);
+                q[6 + c] = (float)(d - cc - b + a);
+            }
+        }
+    if ((rc = dev_alloc(&d_psf_cells, cells.size() / 4))) return rc;
+    HIP_TRY(hipMemcpy(d_psf_cells, cells.data(), sizeof(float) * cells.size(), hipMemcpyHostToDevice));
     return RTSIM_OK;
 }
 
@@ -249,11 +274,13 @@ int Sensor::configure_ranges()
     dev_free(d_rng);
     dev_free(d_seg);
     dev_free(d_win);
+    dev_free(d_work);
     dev_free(d_out);
     int rc_;
     if ((rc_ = dev_alloc(&d_rng, N))) return rc_;
     if ((rc_ = dev_alloc(&d_seg, N))) return rc_;
     if ((rc_ = dev_alloc(&d_win, N))) return rc_;
+    if ((rc_ = dev_alloc(&d_work, plain2d_work_words(N)))) return rc_;
     if ((rc_ = dev_alloc(&d_out, (size_t)N * W))) return rc_;
     HIP_TRY(hipMemcpy(d_rng, rc.data(), sizeof(float2) * N, hipMemcpyHostToDevice));
     HIP_TRY(hipMemset(d_out, 0, sizeof(float2) * (size_t)N * W));
@@ -333,8 +360,8 @@ int Sensor::check_plain_limits() const
             max_lds = 64 * 1024;
         ConvArgs c = conv_args();
         if (plain2d_lds_bytes(c) > (size_t)max_lds)
-            return fail(RTSIM_ERR_UNSUPPORTED,
-                        "plain form: a row of %u bearings exceeds the %d B LDS budget", W, max_lds);
+            return fail(RTSIM_ERR_UNSUPPORTED, "plain form: %zu B of LDS exceed the device's %d B",
+                        plain2d_lds_bytes(c), max_lds);
     }
     return RTSIM_OK;
 }
@@ -358,6 +385,7 @@ PingBuffers Sensor::own_buffers() const
     b.tmax = d_tmax;
     b.seg = d_seg;
     b.win = d_win;
+    b.work = d_work;
     b.out = d_out;
     return b;
 }
@@ -483,6 +511,8 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     c.range_c = d_rng;
     c.bear_c = d_bear;
     c.psf = d_psf;
+    c.psf_cells = d_psf_cells;
+    c.work = pb.work;
     c.out = pb.out;
     c.H = (int)N;
     c.W = (int)W;
@@ -493,8 +523,8 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     c.inv_pb = (float)((double)Tb / (double)psf_bearing_span);
     c.half_b = 0.5f * (float)(Tb - 1);
     c.supp_b = (float)(0.5 * (double)(Tb + 1) * (double)psf_bearing_span / (double)Tb);
-    // tables up to 32 KiB go to LDS; larger ones are read through L1/L2
-    c.psf_in_lds = sizeof(float2) * (size_t)Tr * Tb <= 32 * 1024;
+    // cell tables up to 32 KiB go to LDS; larger ones are read through L1/L2
+    c.psf_in_lds = 2 * sizeof(float4) * ((size_t)Tr + 2) * ((size_t)Tb + 2) <= 32 * 1024;
     return c;
 }
 

@@ -7,11 +7,11 @@
 //
 // K is the bilinear (texture linear filter, border 0) interpolant of the PSF
 // taps (ReductionKernel.h:44-73) with full-FP32 weights.  The reference
-// gathers every window sample into every bearing column; here each sample
-// of view(r) scatters only into the bearing columns where K can be non-zero
-// (|x_s - x_b| < (Tb + 1)/2 tap pitches), which skips exactly the zero terms.
-// Output coordinates are double-float (hi, lo) pairs so that t_s - r_r and
-// x_s - x_b keep full FP32 relative accuracy.
+// gathers every window sample into every bearing column; here a cell sums
+// only the window samples inside the PSF support (|x_s - x_b| < (Tb + 1)/2
+// tap pitches and the range tap coordinate inside (-1, Tr)), which skips
+// exactly the zero terms.  Output coordinates are double-float (hi, lo)
+// pairs so that t_s - r_r and x_s - x_b keep full FP32 relative accuracy.
 #include "device_common.h"
 
 namespace rtsim {
@@ -43,96 +43,379 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
     return acc;
 }
 
+// ---- 2D: register-accumulating gather over output tiles ---------------------
+// A workgroup owns a tile of 16 x 16 output cells made of four 8 x 8 blocks;
+// four waves share a block, and each lane of a wave owns one cell of it and
+// sums its terms in two registers.  The samples that can reach the tile (the
+// union of its rows' windows, those whose bearing coordinate lies within the
+// PSF's bearing support of its columns) are staged in LDS one chunk at a
+// time, in window order; each block's four waves then take, in turn, the
+// staged samples that can reach the block, and every lane adds
+// v_s K(x_s - x_b, t_s - r_r) when sample s is in its row's window and inside
+// the PSF support (the reference's border mode makes every other term 0).
+// At the end the four partial sums of a cell are added in a fixed order, so
+// the ping is bit-reproducible.  The taps are read as bilinear cells with a
+// zero border (A, B, C, D: K = A + ax B + ay (C + ax D) inside the cell): a
+// lookup is two 16-byte LDS reads at one address and three FMAs per
+// component, and a term outside the support or the window reads only zeros,
+// so the lanes need no branch.  (Terms of a non-finite sample value
+// therefore reach every lane of the blocks it is staged for: 0 x inf is NaN,
+// as in the reference, whose gather multiplies every window sample.)
+//
+// Work per tile follows the samples, which crowd into some ranges: a first
+// kernel orders the bands of 16 rows by the length of their union window,
+// heaviest first, and deals them to the 8 XCDs in turn; each XCD runs its
+// bands in that order (every tile of a band on the same XCD, which reads the
+// band's samples through its own L2), so the longest tiles start first.
+constexpr int kCellRows = 8, kCellCols = 8;        // a block
+constexpr int kTileRows = (int)kPlainBandRows, kTileCols = 16;   // 2 x 2 blocks
+static_assert(kTileRows == 2 * kCellRows, "a band is two blocks tall");
+constexpr int kTileBlocks = 4;
+constexpr int kWavesPerBlock = 4;
+constexpr int kTileWaves = kTileBlocks * kWavesPerBlock;
+constexpr int kTileThreads = 64 * kTileWaves;      // 1024
+constexpr int kChunkPer = 2;                       // samples per thread per staging pass
+constexpr int kChunk = kTileThreads * kChunkPer;
+constexpr int kRankThreads = 1024;
+constexpr int kRankLds = 4096;                     // band lengths staged per pass
+
+static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float2)) * kChunk,
+              "partial sums reuse the staging space");
+static_assert(kChunkPer * kTileWaves <= 64, "one wave's count per lane");
+
 size_t plain2d_lds_bytes(const ConvArgs& a)
 {
-    return sizeof(float) * 2 * a.W + sizeof(float2) * a.W +
-           (a.psf_in_lds ? sizeof(float2) * a.Tr * a.Tb : 0);
+    const size_t taps = a.psf_in_lds ? 2 * sizeof(float4) * (size_t)(a.Tr + 2) * (a.Tb + 2) : 0;
+    return (sizeof(float4) + sizeof(float2)) * kChunk + taps;
 }
 
-// One workgroup per output range row.  LDS holds the row accumulator, the
-// bearing coordinates and (when it fits) the PSF table.
-__device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, int r)
+__host__ __device__ inline uint32_t plain2d_bands(const ConvArgs& a) { return (uint32_t)(a.H + kTileRows - 1) / kTileRows; }
+
+uint32_t conv2d_plain_blocks(const ConvArgs& a)
 {
-    extern __shared__ float smem[];
-    const int W = a.W;
-    float* acc = smem;                                  // 2 W
-    float2* bear = reinterpret_cast<float2*>(smem + 2 * W); // W
-    // PSF table in LDS when it fits, else read through the caches
-    const float2* psf = a.psf_in_lds ? bear + W : a.psf;
-    const int t = threadIdx.x;
-    for (int b = t; b < W; b += blockDim.x) {
-        acc[2 * b] = 0.f;
-        acc[2 * b + 1] = 0.f;
-        bear[b] = a.bear_c[b];
-    }
-    if (a.psf_in_lds)
-        for (int k = t; k < a.Tr * a.Tb; k += blockDim.x) (bear + W)[k] = a.psf[k];
-    __syncthreads();
-
-    const uint2 win = a.win[r];
-    const float2 rc = a.range_c[r];
-    for (uint32_t k = t; k < win.y; k += blockDim.x) {
-        const uint32_t s = win.x + k;
-        const float4 rec = a.sorted[s];
-        const float dr = (rec.z - rc.x) - rc.y;
-        const float yB = dr * a.inv_pr + a.half_r;
-        if (!(yB > -1.0f && yB < (float)a.Tr)) continue;
-        const float2 xc = a.coord[s];
-        // first bearing whose coordinate exceeds x_s - supp (binary search)
-        const float lo_x = xc.x - a.supp_b * 1.0001f - 1e-30f;
-        int l = 0, h = W;
-        while (l < h) {
-            int m = (l + h) >> 1;
-            if (bear[m].x < lo_x) l = m + 1; else h = m;
-        }
-        const float hi_x = xc.x + a.supp_b * 1.0001f + 1e-30f;
-        for (int b = l; b < W && bear[b].x <= hi_x; b++) {
-            const float dx = (xc.x - bear[b].x) + (xc.y - bear[b].y);
-            const float xB = dx * a.inv_pb + a.half_b;
-            if (!(xB > -1.0f && xB < (float)a.Tb)) continue;
-            const float2 K = psf_bilinear(psf, a.Tr, a.Tb, xB, yB);
-            atomicAdd(&acc[2 * b], rec.x * K.x - rec.y * K.y);
-            atomicAdd(&acc[2 * b + 1], rec.x * K.y + rec.y * K.x);
-        }
-    }
-    __syncthreads();
-    float2* out = a.out + (size_t)r * W;
-    for (int b = t; b < W; b += blockDim.x) out[b] = make_float2(acc[2 * b], acc[2 * b + 1]);
+    if (a.H <= 0 || a.W <= 0) return 0u;
+    const uint32_t per_xcd = (plain2d_bands(a) + 7) / 8;
+    return 8u * per_xcd * (uint32_t)((a.W + kTileCols - 1) / kTileCols);
 }
 
-__global__ __launch_bounds__(256) void k_conv2d_plain(ConvArgs a) { conv2d_plain_body(a, blockIdx.x); }
+// Union of the windows of rows [r0, r0 + n) as [u0, u1) (u0 >= u1: empty).
+__device__ __forceinline__ void band_union(const ConvArgs& a, int r0, int n, uint32_t& u0, uint32_t& u1)
+{
+    u0 = 0xFFFFFFFFu;
+    u1 = 0u;
+    for (int r = r0; r < r0 + n && r < a.H; r++) {
+        const uint2 w = a.win[r];
+        if (w.y) {
+            u0 = min(u0, w.x);
+            u1 = max(u1, w.x + w.y);
+        }
+    }
+}
 
-__global__ __launch_bounds__(256) void k_conv2d_plain_batch(const ConvArgs* __restrict__ args,
-                                                            BatchMap m)
+// work[0, B): band lengths; work[B, 2B): band of each rank (longest first,
+// ties by band index).  One workgroup per ping.
+__device__ __forceinline__ void plain_rank_body(const ConvArgs& a, uint32_t* work)
+{
+    __shared__ uint32_t lens[kRankLds];
+    const int B = (int)plain2d_bands(a);
+    const int t = threadIdx.x;
+    for (int b = t; b < B; b += kRankThreads) {
+        uint32_t u0, u1;
+        band_union(a, b * kTileRows, kTileRows, u0, u1);
+        work[b] = u1 > u0 ? u1 - u0 : 0u;
+    }
+    __threadfence_block();
+    __syncthreads();
+    for (int b0 = 0; b0 < B; b0 += kRankThreads) {
+        const int b = b0 + t;
+        const uint32_t mine = b < B ? work[b] : 0u;
+        uint32_t rank = 0;
+        for (int j0 = 0; j0 < B; j0 += kRankLds) {
+            const int nj = min(kRankLds, B - j0);
+            __syncthreads();
+            for (int j = t; j < nj; j += kRankThreads) lens[j] = work[j0 + j];
+            __syncthreads();
+            for (int j = 0; j < nj; j++) {
+                const uint32_t o = lens[j];
+                rank += (o > mine || (o == mine && j0 + j < b)) ? 1u : 0u;
+            }
+        }
+        if (b < B) work[B + rank] = (uint32_t)b;
+    }
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_plain_rank(ConvArgs a, uint32_t* work)
+{
+    plain_rank_body(a, work);
+}
+
+__global__ __launch_bounds__(kRankThreads) void k_plain_rank_batch(const ConvArgs* __restrict__ args,
+                                                                   BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    const ConvArgs a = args[p];
+    plain_rank_body(a, a.work);
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ float wave_max(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_umin(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = min(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ uint32_t wave_umax(uint32_t v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o, 64));
+    return v;
+}
+
+// VGPR copy of a wave-uniform value: an FMA may read one scalar operand only,
+// so constants that meet in one FMA are kept in vector registers
+__device__ __forceinline__ float in_vgpr(float x)
+{
+    float v;
+    asm volatile("v_mov_b32 %0, %1" : "=v"(v) : "s"(x));
+    return v;
+}
+
+template <bool kTapsLds>
+__device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t blk)
+{
+    extern __shared__ float4 lds4[];
+    float4* taps_lds = lds4;                                    // [Tr + 2][Tb + 2] cells when staged
+    const int ntaps = kTapsLds ? 2 * (a.Tr + 2) * (a.Tb + 2) : 0;
+    float4* cs_rec = lds4 + ntaps;                              // [kChunk] (re, im, t, index)
+    float2* cs_x = reinterpret_cast<float2*>(cs_rec + kChunk);  // [kChunk] bearing coordinate
+    __shared__ uint32_t counts[kChunkPer * kTileWaves];
+    __shared__ float4 wave_box[kTileWaves];                     // (index lo, index hi, x lo, x hi)
+
+    // workgroup -> (band, tile column): XCD x = blk mod 8 runs the bands of
+    // rank x, x + 8, ... in that order
+    const int B = (int)plain2d_bands(a);
+    const int tiles_c = (a.W + kTileCols - 1) / kTileCols;
+    const uint32_t k = blk >> 3;
+    const int rank = (int)(blk & 7u) + 8 * (int)(k / (uint32_t)tiles_c);
+    if (rank >= B) return;                                      // padding of the last round
+    const int band = (int)a.work[B + rank];
+    const int tc = (int)(k % (uint32_t)tiles_c);
+
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int blkw = w / kWavesPerBlock, sub = w % kWavesPerBlock;
+    const int r = band * kTileRows + (blkw >> 1) * kCellRows + (lane >> 3);
+    const int c = tc * kTileCols + (blkw & 1) * kCellCols + (lane & 7);
+    const int stride = a.Tb + 2;                                // cells per row
+    const float4* taps = a.psf_cells;
+    if constexpr (kTapsLds) {
+        for (int j = t; j < ntaps; j += kTileThreads) taps_lds[j] = a.psf_cells[j];
+        taps = taps_lds;
+    }
+    taps += 2 * (stride + 1);                                   // cell of tap (0, 0)
+
+    // this lane's cell: its row's window and range, its column's coordinate
+    uint2 win = make_uint2(0u, 0u);
+    float2 rc = make_float2(0.f, 0.f), bx = make_float2(0.f, 0.f);
+    if (r < a.H) {
+        win = a.win[r];
+        rc = a.range_c[r];
+    }
+    const bool col_ok = c < a.W;
+    if (col_ok) bx = a.bear_c[c];
+    // the block: index range of its rows' windows, bearing range of its
+    // columns widened by the PSF's bearing support
+    const float supp = a.supp_b * 1.0001f + 1e-30f;
+    const uint32_t wlo = wave_umin(win.y ? win.x : 0xFFFFFFFFu);
+    const uint32_t whi = wave_umax(win.y ? win.x + win.y : 0u);
+    const float xlo = wave_min(col_ok ? bx.x - supp : INFINITY);
+    const float xhi = wave_max(col_ok ? bx.x + supp : -INFINITY);
+    if (lane == 0) wave_box[w] = make_float4(__uint_as_float(wlo), __uint_as_float(whi), xlo, xhi);
+    __syncthreads();
+    uint32_t u0 = 0xFFFFFFFFu, u1 = 0u;
+    float gxlo = INFINITY, gxhi = -INFINITY;
+#pragma unroll
+    for (int j = 0; j < kTileWaves; j += kWavesPerBlock) {
+        const float4 b = wave_box[j];
+        u0 = min(u0, __float_as_uint(b.x));
+        u1 = max(u1, __float_as_uint(b.y));
+        gxlo = fminf(gxlo, b.z);
+        gxhi = fmaxf(gxhi, b.w);
+    }
+
+    const float inv_pb = in_vgpr(a.inv_pb), half_b = in_vgpr(a.half_b);
+    const float inv_pr = in_vgpr(a.inv_pr), half_r = in_vgpr(a.half_r);
+    const float Tbf = (float)a.Tb, Trf = (float)a.Tr, strf = (float)stride;
+    float acc_re = 0.f, acc_im = 0.f;
+
+    // one term: v K(x_s - x_b, t_s - r_r) when s is in the row's window.
+    // Tap coordinates are clamped to [-1, Tb] x [-1, Tr]: a clamped or
+    // out-of-window term then lands on a cell of the zero border at a corner
+    // (ax or ay = 0) where K is 0, so it adds exactly 0 (for finite v)
+    auto term = [&](const float4& e, const float2& ex, float scale) {
+        const uint32_t s = __float_as_uint(e.w);
+        const float dx = (ex.x - bx.x) + (ex.y - bx.y);
+        const float xB = __builtin_amdgcn_fmed3f(__builtin_fmaf(dx, inv_pb, half_b), -1.0f, Tbf);
+        const float dr = (e.z - rc.x) - rc.y;
+        const float y0 = __builtin_fmaf(dr, inv_pr, half_r);
+        const float yB = __builtin_amdgcn_fmed3f(s - win.x < win.y ? y0 : -1.0f, -1.0f, Trf);
+        const float fi = floorf(xB), fj = floorf(yB);
+        const float ax = xB - fi, ay = yB - fj;
+        // cell index fj stride + fi: small integers, exact in FP32
+        const float4* p = taps + 2 * (int)__builtin_fmaf(fj, strf, fi);
+        const float4 ab = p[0], cd = p[1];                      // (A, B), (C, D)
+        const float kr = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.z, cd.x), __builtin_fmaf(ax, ab.z, ab.x));
+        const float ki = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.w, cd.y), __builtin_fmaf(ax, ab.w, ab.y));
+        const float vr = e.x * scale, vi = e.y * scale;
+        acc_re = __builtin_fmaf(vr, kr, acc_re);
+        acc_re = __builtin_fmaf(-vi, ki, acc_re);
+        acc_im = __builtin_fmaf(vr, ki, acc_im);
+        acc_im = __builtin_fmaf(vi, kr, acc_im);
+    };
+
+    uint32_t turn = (uint32_t)sub;                              // the block's waves take groups in turn
+    for (uint32_t base = u0; base < u1; base += kChunk) {
+        // stage the chunk's samples that can reach the tile, in window order
+        float4 rec[kChunkPer];
+        float2 xc[kChunkPer];
+        uint64_t keep[kChunkPer];
+#pragma unroll
+        for (int u = 0; u < kChunkPer; u++) {
+            const uint32_t s = base + (uint32_t)(u * kTileThreads + t);
+            const uint32_t sl = min(s, u1 - 1u);                // in bounds; lanes past u1 keep nothing
+            const float4 v = a.sorted[sl];
+            xc[u] = a.coord[sl];
+            rec[u] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
+            keep[u] = __ballot(s < u1 && xc[u].x >= gxlo && xc[u].x <= gxhi);
+            if (lane == 0) counts[u * kTileWaves + w] = (uint32_t)__popcll(keep[u]);
+        }
+        __syncthreads();
+        // staged position of each (u, wave) run: a scan of the counts in
+        // (u, wave) order across the lanes of every wave (every lane takes
+        // part in the shuffles: a lane reads nothing from an inactive one)
+        const uint32_t cv = lane < kChunkPer * kTileWaves ? counts[lane] : 0u;
+        const uint32_t incl = sort_wave_scan(cv, (uint32_t)lane);
+        const uint32_t n = (uint32_t)__shfl((int)incl, kChunkPer * kTileWaves - 1, 64);
+#pragma unroll
+        for (int u = 0; u < kChunkPer; u++) {
+            const uint32_t at = (uint32_t)__shfl((int)(incl - cv), u * kTileWaves + w, 64);
+            if ((keep[u] >> lane) & 1u) {
+                const uint32_t pos = at + popc_below(keep[u]);
+                cs_rec[pos] = rec[u];
+                cs_x[pos] = xc[u];
+            }
+        }
+        __syncthreads();
+
+        // this wave's share of its block: every staged group of 64 in turn,
+        // two samples at a time
+        const uint32_t groups = (n + 63) / 64;
+        for (; turn < groups; turn += kWavesPerBlock) {
+            const uint32_t g = turn * 64;
+            bool rel = false;
+            if (g + lane < n) {
+                const uint32_t si = __float_as_uint(cs_rec[g + lane].w);
+                const float x = cs_x[g + lane].x;
+                rel = si >= wlo && si < whi && x >= xlo && x <= xhi;
+            }
+            uint64_t m = __ballot(rel);
+            while (m) {
+                const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
+                const bool two = m != 0;
+                const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(m) : j1;
+                if (two) m &= m - 1;
+                const float4 e1 = cs_rec[g + j1], e2 = cs_rec[g + j2];   // the same address in every lane
+                const float2 x1 = cs_x[g + j1], x2 = cs_x[g + j2];
+                term(e1, x1, 1.0f);
+                term(e2, x2, two ? 1.0f : 0.0f);
+            }
+        }
+        turn -= groups;   // the turn carries over into the next chunk's groups
+        // the next chunk's staging overwrites counts only after every wave
+        // has read them (before the barrier above), and the staged samples
+        // only after its first barrier, which every wave reaches after this loop
+    }
+    // the block's four partial sums, added in wave order
+    __syncthreads();
+    float2* part = reinterpret_cast<float2*>(cs_rec);
+    part[t] = make_float2(acc_re, acc_im);
+    __syncthreads();
+    if (sub == 0 && r < a.H && col_ok) {
+        float2 v = part[t];
+#pragma unroll
+        for (int j = 1; j < kWavesPerBlock; j++) {
+            const float2 o = part[t + 64 * j];
+            v.x += o.x;
+            v.y += o.y;
+        }
+        a.out[(size_t)r * a.W + c] = v;
+    }
+}
+
+// 8 waves per SIMD (two workgroups per CU): at most 64 VGPRs
+template <bool kTapsLds>
+__global__ __launch_bounds__(kTileThreads, 8) void k_conv2d_plain(ConvArgs a)
+{
+    conv2d_plain_body<kTapsLds>(a, blockIdx.x);
+}
+
+template <bool kTapsLds>
+__global__ __launch_bounds__(kTileThreads, 8) void k_conv2d_plain_batch(const ConvArgs* __restrict__ args,
+                                                                     BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const ConvArgs a = args[p];   // a private copy: the body's stores cannot alias it
-    conv2d_plain_body(a, (int)blk);
+    conv2d_plain_body<kTapsLds>(a, blk);
 }
-
-uint32_t conv2d_plain_blocks(const ConvArgs& a) { return a.W ? (uint32_t)a.H : 0u; }
 
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
 {
-    if (a.H == 0 || a.W == 0) return hipSuccess;
-    size_t shm = plain2d_lds_bytes(a);
-    hipLaunchKernelGGL(k_conv2d_plain, dim3(a.H), dim3(256), shm, s, a);
+    const uint32_t blocks = conv2d_plain_blocks(a);
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_plain_rank, dim3(1), dim3(kRankThreads), 0, s, a, a.work);
+    if (a.psf_in_lds)
+        hipLaunchKernelGGL(k_conv2d_plain<true>, dim3(blocks), dim3(kTileThreads), plain2d_lds_bytes(a), s, a);
+    else
+        hipLaunchKernelGGL(k_conv2d_plain<false>, dim3(blocks), dim3(kTileThreads), plain2d_lds_bytes(a), s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_plain_rank_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_plain_rank_batch, dim3(blocks), dim3(kRankThreads), 0, s, args, m);
     return hipGetLastError();
 }
 
 hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, size_t shm,
-                                     hipStream_t s)
+                                     bool taps_lds, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_conv2d_plain_batch, dim3(blocks), dim3(256), shm, s, args, m);
+    if (taps_lds)
+        hipLaunchKernelGGL(k_conv2d_plain_batch<true>, dim3(blocks), dim3(kTileThreads), shm, s, args, m);
+    else
+        hipLaunchKernelGGL(k_conv2d_plain_batch<false>, dim3(blocks), dim3(kTileThreads), shm, s, args, m);
     return hipGetLastError();
 }
 
 void init_plain_attributes(int max_lds)
 {
-    set_dynamic_lds_limit((const void*)k_conv2d_plain, max_lds);
-    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<false>, max_lds);
 }
 
 // 1D: one wave per output range bin, FP32 partial sums reduced across the

@@ -6,7 +6,8 @@
 // sort's key / index pairs, tile counts and digit totals, the sorted records
 // and their coordinates; for the grid form the bucketed items, the bucket
 // counts, the key occupancy and the tile maxima (the two sets overlap: a call
-// uses one of them); for both the Binner segments and windows.  Each GPU holds
+// uses one of them); for both the Binner segments and windows, and the plain
+// 2D gather's band order.  Each GPU holds
 // its own copy of that array until the process ends; its address on a device
 // is looked up once, on that device's first call, and every later call on the
 // device reuses it.  The sensor's own configuration tables (PSF, coordinates)
@@ -49,7 +50,8 @@ constexpr size_t kEndGrid = kOffTmax + align256(sizeof(uint32_t) * kTilesCap);
 
 constexpr size_t kOffSeg = kEndPlain > kEndGrid ? kEndPlain : kEndGrid;
 constexpr size_t kOffWin = kOffSeg + align256(sizeof(uint2) * kMaxN);
-constexpr size_t kScratchBytes = kOffWin + align256(sizeof(uint2) * kMaxN);
+constexpr size_t kOffWork = kOffWin + align256(sizeof(uint2) * kMaxN);
+constexpr size_t kScratchBytes = kOffWork + align256(sizeof(uint32_t) * plain2d_work_words(kMaxN));
 static_assert(kScratchBytes == RTSIM_NOALLOC_SCRATCH_BYTES, "scratch size in rtsim.h");
 
 constexpr int kMaxDevices = 64;
@@ -123,6 +125,7 @@ extern "C" int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void
     pb.tmax = reinterpret_cast<uint32_t*>(base + kOffTmax);
     pb.seg = reinterpret_cast<uint2*>(base + kOffSeg);
     pb.win = reinterpret_cast<uint2*>(base + kOffWin);
+    pb.work = reinterpret_cast<uint32_t*>(base + kOffWork);
     pb.out = static_cast<float2*>(ping);
     (void)hipGetLastError();
     PingPlan p;

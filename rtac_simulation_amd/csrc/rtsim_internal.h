@@ -218,14 +218,21 @@ struct ConvArgs {
     const float2* range_c;   // per output row: (hi, lo) of the range in metres
     const float2* bear_c;    // per output bearing: (hi, lo) of the axis coordinate
     const float2* psf;       // [Tr][Tb] complex
+    const float4* psf_cells; // 2D: bilinear cells [Tr + 2][Tb + 2] x 2 float4, j, i from -1:
+                             // (A, B), (C, D) with K(j + ay, i + ax) = A + ax B + ay (C + ax D),
+                             // taps T zero outside [0, Tr) x [0, Tb)
     float2* out;             // [H][W]
     int H, W, Tr, Tb;
     float inv_pr, half_r;    // range tap coordinate yB = dr * inv_pr + half_r
     float inv_pb, half_b;    // bearing tap coordinate xB = dx * inv_pb + half_b
     float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
-    int psf_in_lds;          // stage the PSF table in LDS (else read it from global)
+    int psf_in_lds;          // 2D: stage the cells in LDS (else read them from global)
+    uint32_t* work;          // 2D: band lengths and band order (plain2d_work_words)
 };
 size_t plain2d_lds_bytes(const ConvArgs& a);
+// the 2D gather's bands of output rows, and its work words for `rows` rows
+constexpr uint32_t kPlainBandRows = 16;
+constexpr size_t plain2d_work_words(uint32_t rows) { return 2 * (((size_t)rows + kPlainBandRows - 1) / kPlainBandRows); }
 void init_plain_attributes(int max_lds);
 void init_grid_attributes(int max_lds);
 
@@ -245,7 +252,9 @@ hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
 uint32_t conv2d_plain_blocks(const ConvArgs& a);
 uint32_t conv1d_plain_blocks(const ConvArgs& a);
 hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
-                                     size_t shm, hipStream_t s);
+                                     size_t shm, bool taps_lds, hipStream_t s);
+// the band ordering that precedes the 2D gather: one workgroup per ping
+hipError_t launch_plain_rank_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_conv1d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
                                      hipStream_t s);
 

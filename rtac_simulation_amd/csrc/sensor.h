@@ -61,6 +61,7 @@ struct PingBuffers {
     uint32_t* tmax;
     uint2* seg;              // [N]
     uint2* win;              // [N]
+    uint32_t* work;          // plain 2D gather: band lengths and order
     float2* out;             // [N][W]
 };
 
@@ -118,8 +119,10 @@ struct Sensor {
     // ---- per-configuration device tables -----------------------------------
     uint2* d_seg = nullptr;       // Binner::binIndexes_
     uint2* d_win = nullptr;       // bins_ as (start, size)
+    uint32_t* d_work = nullptr;   // plain 2D gather: band lengths and order
     float2* d_out = nullptr;      // ping [N][W]
     float2* d_psf = nullptr;
+    float4* d_psf_cells = nullptr;  // plain 2D gather: bilinear cells of the taps (ConvArgs::psf_cells)
     float2* d_bear = nullptr;     // bearing coordinate (hi, lo)
     float2* d_rng = nullptr;      // range coordinate (hi, lo)
     float* d_dir = nullptr;
