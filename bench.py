@@ -220,8 +220,8 @@ def measured_traffic(workload: str, form: str, stage: str):
 
 def bench_batch(R, configs, stream, steps, cpu=True):
     """The 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 2D and
-    8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch:
-    MFMA FP16 form for the 2D sensors, plain form for the 1D ones.  Hits are
+    8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch in
+    the MFMA FP16 form (the 1D pings as one-column grids).  Hits are
     prefixes of two 4M-hit sets (one per scene).  Also: the same batch in the
     plain form (the MFMA pings' error against it), one ping per sensor class
     against the FP64 oracle, and the CPU port on one ping per class scaled to
@@ -238,7 +238,7 @@ def bench_batch(R, configs, stream, steps, cpu=True):
         s = R.make_sensor(cfg)
         s.set_sample_count(S)
         sensors.append(s)
-        forms.append(R.FORM_MFMA_FP16 if cfg.kind == 2 else R.FORM_PLAIN_FP32)
+        forms.append(R.FORM_MFMA_FP16)
         total += S
     b = R.Batch(sensors, forms)
     for i, (cfg, scene, S, seed) in enumerate(items):
@@ -281,8 +281,6 @@ def bench_batch(R, configs, stream, steps, cpu=True):
         cfg, _, S, _ = items[i]
         rec = sensors[i].samples()
         e = oracle_rows_error(cfg, rec, {"mfma_fp16": mf[i], "plain_fp32": pl[i]}, every=64)
-        if cfg.kind == 1:
-            e.pop("mfma_fp16")
         errs[f"{cfg.W}x{cfg.H}" if cfg.kind == 2 else f"1D {cfg.H}"] = e
         if cpu:
             c = cpu_baseline(cfg, rec, budget_s=4.0)
@@ -374,15 +372,14 @@ def bench_large(R, configs, stream, steps, cpu=True):
 
 
 def bench_small_configs(R, configs, stream, steps, cpu=True):
-    """The echo-sounder ping (1D, 20 000 samples, 2048 bins, 33 taps; plain
-    form: the MFMA forms are 2D only) and the 512 x 1024 ping with the 15 x 9
-    separable PSF (plain and both MFMA forms), timed like the headline ping,
-    each against the FP64 oracle, with the CPU port beside them."""
+    """The echo-sounder ping (1D, 20 000 samples, 2048 bins, 33 taps) and the
+    512 x 1024 ping with the 15 x 9 separable PSF, each in the plain and both
+    MFMA forms (the 1D ping as a one-column grid), timed like the headline
+    ping, each against the FP64 oracle, with the CPU port beside them."""
     out = {}
-    for name, forms in (("single_beam_38k", {"plain_fp32": R.FORM_PLAIN_FP32}),
-                        ("fls_512x1024_sep", {"plain_fp32": R.FORM_PLAIN_FP32,
-                                              "mfma_bf16": R.FORM_MFMA_BF16,
-                                              "mfma_fp16": R.FORM_MFMA_FP16})):
+    all_forms = {"plain_fp32": R.FORM_PLAIN_FP32, "mfma_bf16": R.FORM_MFMA_BF16,
+                 "mfma_fp16": R.FORM_MFMA_FP16}
+    for name, forms in (("single_beam_38k", all_forms), ("fls_512x1024_sep", all_forms)):
         cfg, scene, S, seed = configs.workload(name)
         dh = R.generate_hits(scene, seed, S, nthreads=min(16, os.cpu_count() or 1)).to_device()
         s = R.make_sensor(cfg)
@@ -403,11 +400,8 @@ def bench_small_configs(R, configs, stream, steps, cpu=True):
             res[fname]["rel_l2_vs_fp64_oracle"] = errs[fname]
         o = {"workload": f"{name}: {cfg.W} x {cfg.H}, {S} samples, {cfg.Tr}x{cfg.Tb} PSF",
              "forms": res}
-        if cfg.kind == 2:
-            best = min((k for k in res if k != "plain_fp32"), key=lambda k: res[k]["ms_per_ping"])
-            o["mfma_vs_plain_speedup"] = res["plain_fp32"]["ms_per_ping"] / res[best]["ms_per_ping"]
-        else:
-            o["note"] = "the MFMA forms restate 2D pings only (RTSIM_ERR_UNSUPPORTED for 1D)"
+        best = min((k for k in res if k != "plain_fp32"), key=lambda k: res[k]["ms_per_ping"])
+        o["mfma_vs_plain_speedup"] = res["plain_fp32"]["ms_per_ping"] / res[best]["ms_per_ping"]
         if cpu:
             o["cpu_baseline"] = cpu_baseline(cfg, rec, budget_s=8.0)
         out[name] = o

@@ -163,26 +163,33 @@ static int default_waves8(const Sensor* s, const GridState* g, int sh)
 
 static int prepare_grid(Sensor* s, GridState* g)
 {
-    if (s->kind != RTSIM_SENSOR_2D)
-        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: 1D sensors use the plain form");
-    if (s->W < 2 || s->N < 2)
-        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form needs >= 2 bearings and >= 2 ranges");
-    // uniform bearing axis at the PSF bearing pitch
-    std::vector<double> x(s->W);
-    for (uint32_t b = 0; b < s->W; b++)
-        x[b] = s->axis == RTSIM_AXIS_SINE ? std::sin((double)s->bearings[b]) : (double)s->bearings[b];
-    const double pb = (x[s->W - 1] - x[0]) / (double)(s->W - 1);
-    double dev = 0;
-    for (uint32_t b = 0; b < s->W; b++) dev = std::fmax(dev, std::fabs(x[b] - (x[0] + b * pb)));
-    if (dev > 5e-5 * pb)
-        return fail(RTSIM_ERR_UNSUPPORTED,
-                    "MFMA form needs bearings uniform in the PSF axis (deviation %.3g pitch)",
-                    dev / pb);
-    const double psf_pb = (double)s->psf_bearing_span / s->Tb;
-    if (std::fabs(psf_pb - pb) > 5e-5 * pb)
-        return fail(RTSIM_ERR_UNSUPPORTED,
-                    "MFMA form needs the PSF bearing pitch (%.6g) equal to the bearing pitch (%.6g)",
-                    psf_pb, pb);
+    // a 1D ping is the grid form with one column: every sample (bearing 0)
+    // lands on column 0 with weight 1 and the taps are one column wide
+    const bool one_d = s->kind == RTSIM_SENSOR_1D;
+    if (!one_d && s->kind != RTSIM_SENSOR_2D)
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form: unknown sensor kind %d", s->kind);
+    if (s->N < 2 || (!one_d && s->W < 2))
+        return fail(RTSIM_ERR_UNSUPPORTED, "MFMA form needs >= 2 ranges (and >= 2 bearings in 2D)");
+    double x0 = 0.0, psf_pb = 1.0;
+    if (!one_d) {
+        // uniform bearing axis at the PSF bearing pitch
+        std::vector<double> x(s->W);
+        for (uint32_t b = 0; b < s->W; b++)
+            x[b] = s->axis == RTSIM_AXIS_SINE ? std::sin((double)s->bearings[b]) : (double)s->bearings[b];
+        const double pb = (x[s->W - 1] - x[0]) / (double)(s->W - 1);
+        double dev = 0;
+        for (uint32_t b = 0; b < s->W; b++) dev = std::fmax(dev, std::fabs(x[b] - (x[0] + b * pb)));
+        if (dev > 5e-5 * pb)
+            return fail(RTSIM_ERR_UNSUPPORTED,
+                        "MFMA form needs bearings uniform in the PSF axis (deviation %.3g pitch)",
+                        dev / pb);
+        psf_pb = (double)s->psf_bearing_span / s->Tb;
+        if (std::fabs(psf_pb - pb) > 5e-5 * pb)
+            return fail(RTSIM_ERR_UNSUPPORTED,
+                        "MFMA form needs the PSF bearing pitch (%.6g) equal to the bearing pitch (%.6g)",
+                        psf_pb, pb);
+        x0 = x[0];
+    }
     const double dr = ((double)s->range_max - (double)s->range_min) / (double)(s->N - 1);
     const double psf_pr = (double)s->psf_range_span / s->Tr;
     if (std::fabs(psf_pr - dr) > 5e-5 * dr)
@@ -211,7 +218,7 @@ static int prepare_grid(Sensor* s, GridState* g)
     g->off_b = (float)(0.5 * (s->Tb - 1) - g->fb);
     g->inv_dr = (float)(1.0 / dr);
     g->inv_db = (float)(1.0 / psf_pb);
-    g->bear0 = x[0];
+    g->bear0 = x0;
 
     int dev_id = 0, max_lds = 0;
     (void)hipGetDevice(&dev_id);

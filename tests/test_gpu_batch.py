@@ -109,6 +109,7 @@ def test_batch_recaptures_after_resize(gpu):
 def test_batch_rejects_unsupported_form(gpu):
     R = gpu
     c1, sc1, _, _ = configs.workload("single_beam_38k")
+    c1.psf_range_span *= 2.0        # PSF pitch of two range steps: no grid form
     s = R.make_sensor(c1)
     s.set_sample_count(100)
     with pytest.raises(R.RtsimError) as e:

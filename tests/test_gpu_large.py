@@ -7,8 +7,9 @@ C ABI, against the FP64 oracle:
     which bin without sorting, Binner introspection sorts by travel), every
     64th range row within the form's bound of the oracle;
   * the 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 and
-    8 x 1D 2048, 1e5..4e6 samples each) computed by one batched graph launch,
-    each ping checked the same way.
+    8 x 1D 2048, 1e5..4e6 samples each; plain and both MFMA forms, 1D pings
+    included) computed by one batched graph launch, each ping checked the
+    same way.
 
 At these sizes the sort's tile-count scan runs its LDS-staged branch (more
 than 2048 tiles per pass) and pings of up to 16M samples go through every
@@ -83,15 +84,9 @@ def test_large_ping_vs_oracle(large, form):
 
 
 def _batch_forms(items):
-    """Plain FP32 for the 1D pings and every fourth 2D ping, the MFMA forms
-    (BF16 and FP16 in turn) for the other 2D pings."""
-    forms = []
-    for i, (cfg, _, _, _) in enumerate(items):
-        if cfg.kind == 1 or i % 4 == 3:
-            forms.append(0)
-        else:
-            forms.append(1 if i % 2 else 2)
-    return forms
+    """Plain FP32 for every fourth ping, the MFMA forms (BF16 and FP16 in
+    turn) for the others, 1D pings included."""
+    return [0 if i % 4 == 3 else (1 if i % 2 else 2) for i in range(len(items))]
 
 
 def test_uneven_batch_vs_oracle(gpu):

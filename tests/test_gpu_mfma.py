@@ -71,10 +71,33 @@ def test_mfma_full_size_vs_plain_and_oracle(gpu, form):
     assert O.rel_l2(mf[rows], ref[rows]) <= MFMA_TOL
 
 
+@pytest.mark.parametrize("form", FORMS)
+@pytest.mark.parametrize("S", [20_000, 3_000_000])
+def test_mfma_1d_vs_oracle(gpu, form, S):
+    """A 1D ping in the grid form (one column, one tap column): the echo
+    sounder's 33-tap ping (SensorInstance1D.cu:35-72) against the FP64 oracle,
+    its masked bins exactly zero, its binning bit-exact."""
+    R = gpu
+    cfg, scene, _, seed = configs.workload("single_beam_38k")
+    hits = R.generate_hits(scene, seed, S)
+    s = _sensor(R, cfg, hits)
+    s.compute_output(form)
+    got = s.output()
+    rec = s.samples()
+    ref, b = O.reference_ping(cfg, rec)
+    err = O.rel_l2(got, ref)
+    assert err <= MFMA_TOL, err
+    assert not np.abs(got[b["windows"][:, 1] == 0]).any()
+    d = s.binner_debug()
+    for key in ("order", "keys", "segments", "windows"):
+        assert np.array_equal(d[key], b[key]), key
+
+
 def test_mfma_rejects_what_it_cannot_restate(gpu):
     R = gpu
-    # 1D sensor
+    # 1D sensor whose PSF pitch (two range steps) differs from the range step
     cfg1, scene1, S1, seed1 = configs.workload("single_beam_38k")
+    cfg1.psf_range_span *= 2.0
     s = _sensor(R, cfg1, R.generate_hits(scene1, seed1, 1000))
     with pytest.raises(R.RtsimError) as e:
         s.compute_output(1)
