@@ -72,6 +72,7 @@ struct Batch {
     StreamArgs* h_pose = nullptr;   // pinned staging of rewritten streaming arguments
 
     Stage<StreamArgs> stream;
+    Stage<BucketArgs> stream_bucket;     // parallel to stream: the bucket counts made on the way
     // plain pings: the sort by travel, per pass
     Stage<HistArgs> hist[4];
     Stage<ScanArgs> scan[4];
@@ -139,6 +140,7 @@ static BatchMap dmap(const Batch* b, const Stage<A>& st)
 static int build(Batch* b)
 {
     b->stream = {};
+    b->stream_bucket = {};
     for (int k = 0; k < 4; k++) {
         b->hist[k] = {};
         b->scan[k] = {};
@@ -161,11 +163,19 @@ static int build(Batch* b)
         int rc = s->plan(en.form, &p);
         if (rc) return rc;
         en.stream_index = -1;
+        bool counted = false;   // the grid form's bucket counts made by the streaming kernel
         if (en.has_hits && en.nhits) {
             StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
             const uint32_t blocks = stream_blocks(a);
             if (blocks) en.stream_index = (int)b->stream.args.size();
             b->stream.add(a, blocks);
+            BucketArgs fused{};
+            if (p.grid && stream_bucket_fusable(a, p.bucket)) {
+                fused = p.bucket;
+                counted = true;
+                b->stream_bucket.shm = std::max(b->stream_bucket.shm, sizeof(uint32_t) * (size_t)fused.NB);
+            }
+            b->stream_bucket.add(fused, blocks);
         }
         if (!p.grid) {
             for (int k = 0; k < p.sort.passes; k++) {
@@ -185,8 +195,10 @@ static int build(Batch* b)
             }
         } else {
             const BucketArgs& bk = p.bucket;
-            b->bucket_hist.add(bk, bk.tiles);
-            b->bucket_hist.shm = std::max(b->bucket_hist.shm, sizeof(uint32_t) * (size_t)bk.NB);
+            if (!counted) {
+                b->bucket_hist.add(bk, bk.tiles);
+                b->bucket_hist.shm = std::max(b->bucket_hist.shm, sizeof(uint32_t) * (size_t)bk.NB);
+            }
             b->bucket_scan.add(ScanArgs{bk.counts, bk.tiles, bk.totals, bk.tmax, bk.vmax}, (uint32_t)bk.NB);
             b->bucket_scatter.add(bk, bk.tiles);
             b->bucket_scatter.shm = std::max(b->bucket_scatter.shm, bucket_lds_bytes(bk));
@@ -210,6 +222,7 @@ static int build(Batch* b)
     // one device buffer for every descriptor array and prefix table
     size_t off = 0;
     layout(b->stream, off);
+    layout(b->stream_bucket, off);
     for (int k = 0; k < 4; k++) {
         layout(b->hist[k], off);
         layout(b->scan[k], off);
@@ -234,6 +247,7 @@ static int build(Batch* b)
     }
     std::vector<char> host(off, 0);
     pack(b->stream, host);
+    pack(b->stream_bucket, host);
     for (int k = 0; k < 4; k++) {
         pack(b->hist[k], host);
         pack(b->scan[k], host);
@@ -258,7 +272,8 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
 {
     hipError_t e;
 #define RT_TRY(x) if ((e = (x)) != hipSuccess) return e
-    RT_TRY(launch_stream_batch(dargs(b, b->stream), dmap(b, b->stream), b->stream.blocks(), s));
+    RT_TRY(launch_stream_batch(dargs(b, b->stream), dmap(b, b->stream), b->stream.blocks(),
+                               dargs(b, b->stream_bucket), b->stream_bucket.shm, s));
     for (int k = 0; k < 4; k++) {
         RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]), b->hist[k].blocks(),
                                       s));

@@ -159,6 +159,17 @@ __device__ __forceinline__ uint32_t lane_id()
     return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// Tile of workgroup `blk` of `nblk` such that each XCD runs a contiguous
+// range of tiles.  Workgroups are dealt to the 8 XCDs round robin (XCD
+// blk mod 8); XCD x gets the tiles [x q + min(x, r), ...) with q = nblk / 8,
+// r = nblk mod 8.  Scatters whose neighbouring tiles write neighbouring
+// addresses then fill their cache lines in one L2 instead of several.
+__device__ __forceinline__ uint32_t xcd_contiguous_tile(uint32_t blk, uint32_t nblk)
+{
+    const uint32_t q = nblk >> 3, r = nblk & 7u, x = blk & 7u, k = blk >> 3;
+    return x * q + min(x, r) + k;
+}
+
 // Batched launch: the ping of this workgroup (largest p with first[p] <=
 // blockIdx.x, so pings without workgroups are skipped), its local workgroup
 // index and its workgroup count.

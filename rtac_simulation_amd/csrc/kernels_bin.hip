@@ -141,13 +141,16 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream(StreamArgs p, BucketA
     stream_body(p, b, with_hist, blockIdx.x);
 }
 
+// bargs[q]: ping q's bucket counts to make on the way (tiles == 0: none)
 __global__ __launch_bounds__(kStreamThreads) void k_stream_batch(const StreamArgs* __restrict__ args,
-                                                                 BatchMap m, BucketArgs b, int with_hist)
+                                                                 BatchMap m,
+                                                                 const BucketArgs* __restrict__ bargs)
 {
     uint32_t blk, nblk;
     const uint32_t q = batch_locate(m, blk, nblk);
-    const StreamArgs a = args[q];   // a private copy: the body's stores cannot alias it
-    stream_body(a, b, with_hist, blk);
+    const StreamArgs a = args[q];   // private copies: the body's stores cannot alias them
+    const BucketArgs b = bargs[q];
+    stream_body(a, b, b.tiles != 0, blk);
 }
 
 uint32_t stream_blocks(const StreamArgs& a) { return (a.n + kStreamThreads - 1) / kStreamThreads; }
@@ -159,17 +162,23 @@ hipError_t launch_stream(const StreamArgs& a, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks,
+                               const BucketArgs* bargs, size_t shm, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(kStreamThreads), 0, s, args, m, BucketArgs{}, 0);
+    hipLaunchKernelGGL(k_stream_batch, dim3(blocks), dim3(kStreamThreads), shm, s, args, m, bargs);
     return hipGetLastError();
+}
+
+bool stream_bucket_fusable(const StreamArgs& a, const BucketArgs& b)
+{
+    return a.n != 0 && b.tiles != 0 && b.tile == kStreamThreads && stream_blocks(a) == b.tiles &&
+           b.rec == a.out && b.n == a.n;
 }
 
 hipError_t launch_stream_bucket(const StreamArgs& a, const BucketArgs& b, hipStream_t s)
 {
-    if (a.n == 0 || b.tiles == 0 || b.tile != kStreamThreads || stream_blocks(a) != b.tiles)
-        return hipErrorInvalidValue;
+    if (!stream_bucket_fusable(a, b)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(k_stream, dim3(b.tiles), dim3(kStreamThreads), sizeof(uint32_t) * b.NB, s, a, b, 1);
     return hipGetLastError();
 }

@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(BucketArgs a) { 
 
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(BucketArgs a)
 {
-    bucket_scatter_body(a, blockIdx.x);
+    bucket_scatter_body(a, xcd_contiguous_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist_batch(const BucketArgs* __restrict__ args,
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter_batch(
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const BucketArgs a = args[p];
-    bucket_scatter_body(a, blk);
+    bucket_scatter_body(a, xcd_contiguous_tile(blk, nblk));
 }
 
 size_t bucket_lds_bytes(const BucketArgs& b) { return sizeof(uint32_t) * ((size_t)b.NB + b.occ_words); }

@@ -332,7 +332,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 template <bool FIRST, bool LAST>
 __global__ __launch_bounds__(256) void k_sort_scatter(ScatterArgs a)
 {
-    sort_scatter_body<FIRST, LAST>(a, blockIdx.x);
+    sort_scatter_body<FIRST, LAST>(a, xcd_contiguous_tile(blockIdx.x, gridDim.x));
 }
 
 template <bool FIRST, bool LAST>
@@ -341,7 +341,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* _
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
-    sort_scatter_body<FIRST, LAST>(args[p], blk);
+    sort_scatter_body<FIRST, LAST>(args[p], xcd_contiguous_tile(blk, nblk));
 }
 
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p)

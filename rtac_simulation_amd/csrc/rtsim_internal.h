@@ -27,6 +27,8 @@ struct BatchMap {
     uint32_t nping;
 };
 
+struct BucketArgs;
+
 // ---- streaming kernel: ReceiverView::set_sample + raygen phase -------------
 struct StreamArgs {
     const float* range;
@@ -46,7 +48,10 @@ struct StreamArgs {
 };
 hipError_t launch_stream(const StreamArgs& a, hipStream_t s);
 uint32_t stream_blocks(const StreamArgs& a);
-hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
+// batched: bargs[q] (parallel to args) holds ping q's bucket counts to make
+// on the way, or tiles == 0; shm = 4 x the largest NB among them
+hipError_t launch_stream_batch(const StreamArgs* args, BatchMap m, uint32_t blocks,
+                               const BucketArgs* bargs, size_t shm, hipStream_t s);
 
 // ---- grid (MFMA) form binning: samples bucketed by blocks of grid rows ------
 // The deposit of the grid form sums every sample exactly (fixed point), so it
@@ -86,6 +91,8 @@ struct BucketArgs {
 // the streaming kernel fused with the bucket counts of the grid form (tiles
 // of 1024 samples; b.rec == a.out, b.n == a.n)
 hipError_t launch_stream_bucket(const StreamArgs& a, const BucketArgs& b, hipStream_t s);
+// whether launch_stream_bucket accepts (a, b): one tile per streaming workgroup
+bool stream_bucket_fusable(const StreamArgs& a, const BucketArgs& b);
 // bucket counts and tile maxima (from the records), their scan, the scatter
 // of the items; blocks per launch: tiles, NB, tiles
 hipError_t launch_bucket_hist(const BucketArgs& b, hipStream_t s);
