@@ -149,6 +149,7 @@ static int build(Batch* b)
     b->bucket_hist = {};
     b->bucket_scan = {};
     b->bucket_scatter = {};
+    b->bucket_scatter.variant = 1;   // staged unless a ping has too many buckets for it
     b->windows = {};
     b->plain_rank = {};
     b->plain2d = {};
@@ -202,6 +203,7 @@ static int build(Batch* b)
             b->bucket_scan.add(ScanArgs{bk.counts, bk.tiles, bk.totals, bk.tmax, bk.vmax}, (uint32_t)bk.NB);
             b->bucket_scatter.add(bk, bk.tiles);
             b->bucket_scatter.shm = std::max(b->bucket_scatter.shm, bucket_lds_bytes(bk));
+            if (bk.tiles && !bucket_scatter_staged(bk)) b->bucket_scatter.variant = 0;
             b->deposit.add(p.dep, deposit_blocks(p.dep));
             b->deposit.shm = std::max(b->deposit.shm, deposit_lds_bytes(p.dep));
             const std::tuple<int, int, int> key{en.form, p.corr.shape, p.corr.waves8};
@@ -285,7 +287,8 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
                                     b->bucket_hist.shm, s));
     RT_TRY(launch_sort_scan_batch(dargs(b, b->bucket_scan), dmap(b, b->bucket_scan), b->bucket_scan.blocks(), s));
     RT_TRY(launch_bucket_scatter_batch(dargs(b, b->bucket_scatter), dmap(b, b->bucket_scatter),
-                                       b->bucket_scatter.blocks(), b->bucket_scatter.shm, s));
+                                       b->bucket_scatter.blocks(), b->bucket_scatter.shm,
+                                       b->bucket_scatter.variant != 0, s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
     RT_TRY(launch_plain_rank_batch(dargs(b, b->plain_rank), dmap(b, b->plain_rank), b->plain_rank.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),

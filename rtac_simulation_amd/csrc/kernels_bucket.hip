@@ -70,15 +70,22 @@ __device__ __forceinline__ void bucket_hist_body(const BucketArgs& a, uint32_t t
 }
 
 // k_bucket_scatter: tile `tile`'s items into their buckets, its occupancy bits.
-__device__ __forceinline__ void bucket_scatter_body(const BucketArgs& a, uint32_t tile)
+// The cursor of bucket b starts at the sum of the totals of the buckets
+// before it plus this tile's place among the tiles (the scanned counts).
+//
+// Staged form (kStaged): each pass of kScatterPass samples first ranks its
+// items per bucket in LDS, stages them in bucket order with their global
+// positions, then writes them out in that order: the items of one bucket
+// run go to consecutive addresses from consecutive lanes, instead of every
+// lane of a store hitting another bucket (partial cache lines, written back
+// half empty).  Unstaged form: every item is stored at its bucket's cursor
+// directly (pings with too many buckets for the staging LDS).
+constexpr int kScatterPass = kBucketU * kBucketThreads;   // samples per pass
+constexpr int kStageItems = 2 * kScatterPass;              // a sample makes at most two items
+
+__device__ __forceinline__ void bucket_cursors(const BucketArgs& a, uint32_t tile, uint32_t* cur, uint32_t* ws)
 {
-    extern __shared__ uint32_t lds[];    // cursor[NB], then occupancy[occ_words]
-    uint32_t* cur = lds;
-    uint32_t* locc = lds + a.NB;
-    __shared__ uint32_t ws[4];
     const uint32_t t = threadIdx.x;
-    // bucket b starts at the sum of the totals of the buckets before it; this
-    // tile's part of it after the earlier tiles' (the scanned counts)
     const int per = (a.NB + kBucketThreads - 1) / kBucketThreads;
     const int b0 = (int)t * per, b1 = min(b0 + per, a.NB);
     uint32_t s = 0;
@@ -88,24 +95,96 @@ __device__ __forceinline__ void bucket_scatter_body(const BucketArgs& a, uint32_
         cur[b] = run + a.counts[(size_t)b * a.tiles + tile];
         run += a.totals[b];
     }
+}
+
+// the item of record r and its buckets (c0 = c1 = -1: not binned)
+__device__ __forceinline__ void bucket_item(const BucketArgs& a, const float4& r, bool valid, int& c0, int& c1,
+                                            float4& item, uint32_t& key)
+{
+    c0 = c1 = -1;
+    key = valid ? bin_key(a.key, r.z) : kOutOfRange;
+    if (key == kOutOfRange) return;
+    bucket_pair(a, g_row_of(a.rows, r.z), c0, c1);
+    item = make_float4(r.x, r.y, r.z, grid_x_of(a, r.w));
+}
+
+template <bool kStaged>
+__device__ __forceinline__ void bucket_scatter_body(const BucketArgs& a, uint32_t tile)
+{
+    extern __shared__ uint32_t lds[];
+    uint32_t* cur = lds;                                  // [NB] global cursor
+    uint32_t* locc = lds + a.NB;                          // [occ_words]
+    uint32_t* lcnt = locc + a.occ_words;                  // staged: [NB] items of the pass
+    uint32_t* lofs = lcnt + a.NB;                         // staged: [NB] their offset in the stage
+    float4* sitem = reinterpret_cast<float4*>(lds + ((2 * a.NB + a.occ_words + a.NB + 3) & ~3));
+    uint32_t* spos = reinterpret_cast<uint32_t*>(sitem + kStageItems);
+    __shared__ uint32_t ws[4], ws2[4];
+    const uint32_t t = threadIdx.x;
+    bucket_cursors(a, tile, cur, ws);
     for (int w = (int)t; w < a.occ_words; w += kBucketThreads) locc[w] = 0u;
+    if (kStaged)
+        for (int b = (int)t; b < a.NB; b += kBucketThreads) lcnt[b] = 0u;
     __syncthreads();
+    const int per = (a.NB + kBucketThreads - 1) / kBucketThreads;
+    const int rb0 = (int)t * per, rb1 = min(rb0 + per, a.NB);   // this thread's buckets
     const uint32_t i0 = tile * a.tile, i1 = min(i0 + a.tile, a.n);
-    for (uint32_t ib = i0; ib < i1; ib += kBucketU * kBucketThreads) {
+    for (uint32_t ib = i0; ib < i1; ib += kScatterPass) {
         float4 r[kBucketU];
 #pragma unroll
         for (int u = 0; u < kBucketU; u++) r[u] = a.rec[min(ib + u * kBucketThreads + t, i1 - 1)];
+        if constexpr (!kStaged) {
 #pragma unroll
-        for (int u = 0; u < kBucketU; u++) {
-            if (ib + u * kBucketThreads + t >= i1) continue;
-            const uint32_t k = bin_key(a.key, r[u].z);
-            if (k == kOutOfRange) continue;
-            int c0, c1;
-            bucket_pair(a, g_row_of(a.rows, r[u].z), c0, c1);
-            const float4 item = make_float4(r[u].x, r[u].y, r[u].z, grid_x_of(a, r[u].w));
-            if (c0 >= 0) a.items[atomicAdd(&cur[c0], 1u)] = item;
-            if (c1 >= 0) a.items[atomicAdd(&cur[c1], 1u)] = item;
-            atomicOr(&locc[k >> 5], 1u << (k & 31));
+            for (int u = 0; u < kBucketU; u++) {
+                int c0, c1;
+                float4 item;
+                uint32_t k;
+                bucket_item(a, r[u], ib + u * kBucketThreads + t < i1, c0, c1, item, k);
+                if (k == kOutOfRange) continue;
+                if (c0 >= 0) a.items[atomicAdd(&cur[c0], 1u)] = item;
+                if (c1 >= 0) a.items[atomicAdd(&cur[c1], 1u)] = item;
+                atomicOr(&locc[k >> 5], 1u << (k & 31));
+            }
+        } else {
+            int c0[kBucketU], c1[kBucketU];
+            uint32_t k0[kBucketU], k1[kBucketU];
+            float4 item[kBucketU];
+#pragma unroll
+            for (int u = 0; u < kBucketU; u++) {
+                uint32_t k;
+                bucket_item(a, r[u], ib + u * kBucketThreads + t < i1, c0[u], c1[u], item[u], k);
+                k0[u] = c0[u] >= 0 ? atomicAdd(&lcnt[c0[u]], 1u) : 0u;
+                k1[u] = c1[u] >= 0 ? atomicAdd(&lcnt[c1[u]], 1u) : 0u;
+                if (k != kOutOfRange) atomicOr(&locc[k >> 5], 1u << (k & 31));
+            }
+            __syncthreads();                              // the pass's counts
+            uint32_t s = 0, total;
+            for (int b = rb0; b < rb1; b++) s += lcnt[b];
+            uint32_t run = sort_block_scan(s, ws2, &total);
+            for (int b = rb0; b < rb1; b++) {
+                lofs[b] = run;
+                run += lcnt[b];
+            }
+            __syncthreads();                              // stage offsets
+#pragma unroll
+            for (int u = 0; u < kBucketU; u++) {
+                if (c0[u] >= 0) {
+                    const uint32_t q = lofs[c0[u]] + k0[u];
+                    sitem[q] = item[u];
+                    spos[q] = cur[c0[u]] + k0[u];
+                }
+                if (c1[u] >= 0) {
+                    const uint32_t q = lofs[c1[u]] + k1[u];
+                    sitem[q] = item[u];
+                    spos[q] = cur[c1[u]] + k1[u];
+                }
+            }
+            __syncthreads();                              // staged
+            for (int b = rb0; b < rb1; b++) {
+                cur[b] += lcnt[b];
+                lcnt[b] = 0u;
+            }
+            for (uint32_t q = t; q < total; q += kBucketThreads) a.items[spos[q]] = sitem[q];
+            __syncthreads();                              // counts cleared, stage free
         }
     }
     __syncthreads();
@@ -115,9 +194,10 @@ __device__ __forceinline__ void bucket_scatter_body(const BucketArgs& a, uint32_
 
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist(BucketArgs a) { bucket_hist_body(a, blockIdx.x); }
 
+template <bool kStaged>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter(BucketArgs a)
 {
-    bucket_scatter_body(a, xcd_contiguous_tile(blockIdx.x, gridDim.x));
+    bucket_scatter_body<kStaged>(a, xcd_contiguous_tile(blockIdx.x, gridDim.x));
 }
 
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist_batch(const BucketArgs* __restrict__ args,
@@ -129,16 +209,49 @@ __global__ __launch_bounds__(kBucketThreads) void k_bucket_hist_batch(const Buck
     bucket_hist_body(a, blk);
 }
 
+template <bool kStaged>
 __global__ __launch_bounds__(kBucketThreads) void k_bucket_scatter_batch(
     const BucketArgs* __restrict__ args, BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const BucketArgs a = args[p];
-    bucket_scatter_body(a, xcd_contiguous_tile(blk, nblk));
+    bucket_scatter_body<kStaged>(a, xcd_contiguous_tile(blk, nblk));
 }
 
-size_t bucket_lds_bytes(const BucketArgs& b) { return sizeof(uint32_t) * ((size_t)b.NB + b.occ_words); }
+namespace {
+int g_scatter_lds_limit = 64 * 1024;   // dynamic LDS a scatter workgroup may take (set at init)
+}
+
+static size_t staged_bytes(const BucketArgs& b)
+{
+    const size_t words = ((3 * (size_t)b.NB + b.occ_words + 3) & ~(size_t)3);
+    return sizeof(uint32_t) * words + (sizeof(float4) + sizeof(uint32_t)) * kStageItems;
+}
+
+// Staging pays when a pass spreads its items thinly over many buckets (runs
+// of a few items); with a few hundred buckets the direct stores already
+// fill their lines in L2 (measured on one ping: 1M samples, 264 buckets:
+// 12.6 us direct, ~16 us staged; 16M samples, 1386 buckets: 309 us direct,
+// ~255 us staged; the 64-ping batch with up to 724 buckets was faster direct).
+constexpr int kStagedMinBuckets = 1024;
+
+bool bucket_scatter_staged(const BucketArgs& b)
+{
+    return b.NB >= kStagedMinBuckets && staged_bytes(b) <= (size_t)g_scatter_lds_limit;
+}
+
+size_t bucket_lds_bytes(const BucketArgs& b)
+{
+    return bucket_scatter_staged(b) ? staged_bytes(b) : sizeof(uint32_t) * ((size_t)b.NB + b.occ_words);
+}
+
+void init_bucket_attributes(int max_lds)
+{
+    g_scatter_lds_limit = max_lds - 64;   // the kernels' static LDS
+    set_dynamic_lds_limit((const void*)k_bucket_scatter<true>, max_lds);
+    set_dynamic_lds_limit((const void*)k_bucket_scatter_batch<true>, max_lds);
+}
 
 hipError_t launch_bucket_hist(const BucketArgs& b, hipStream_t s)
 {
@@ -154,8 +267,11 @@ hipError_t launch_bucket_scan(const BucketArgs& b, hipStream_t s)
 
 hipError_t launch_bucket_scatter(const BucketArgs& b, hipStream_t s)
 {
-    if (b.tiles)
-        hipLaunchKernelGGL(k_bucket_scatter, dim3(b.tiles), dim3(kBucketThreads), bucket_lds_bytes(b), s, b);
+    if (!b.tiles) return hipGetLastError();
+    if (bucket_scatter_staged(b))
+        hipLaunchKernelGGL(k_bucket_scatter<true>, dim3(b.tiles), dim3(kBucketThreads), bucket_lds_bytes(b), s, b);
+    else
+        hipLaunchKernelGGL(k_bucket_scatter<false>, dim3(b.tiles), dim3(kBucketThreads), bucket_lds_bytes(b), s, b);
     return hipGetLastError();
 }
 
@@ -168,10 +284,13 @@ hipError_t launch_bucket_hist_batch(const BucketArgs* args, BatchMap m, uint32_t
 }
 
 hipError_t launch_bucket_scatter_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
-                                       hipStream_t s)
+                                       bool staged, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_bucket_scatter_batch, dim3(blocks), dim3(kBucketThreads), shm, s, args, m);
+    if (staged)
+        hipLaunchKernelGGL(k_bucket_scatter_batch<true>, dim3(blocks), dim3(kBucketThreads), shm, s, args, m);
+    else
+        hipLaunchKernelGGL(k_bucket_scatter_batch<false>, dim3(blocks), dim3(kBucketThreads), shm, s, args, m);
     return hipGetLastError();
 }
 

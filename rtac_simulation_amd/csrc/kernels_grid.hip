@@ -111,8 +111,8 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
     }
 }
 
-// Windows of output bins [wb 256, wb 256 + 256) (do_make_bins, Binner.cu:
-// 82-106): a window holds samples when at least two of the bins
+// Windows of output bins [wb T, wb T + T), T = kDepositThreads (do_make_bins,
+// Binner.cu:82-106): a window holds samples when at least two of the bins
 // [r - ov, r + ov] (inside [0, N)) hold samples; the occupancy is the OR of
 // the bucketing's per-tile key bits.  Only the emptiness is kept:
 // win[r] = (0, 1) for a window with samples, (0, 0) for an empty one (the
@@ -196,6 +196,7 @@ void init_grid_attributes(int max_lds)
 {
     set_dynamic_lds_limit((const void*)k_deposit, max_lds);
     set_dynamic_lds_limit((const void*)k_deposit_batch, max_lds);
+    init_bucket_attributes(max_lds);
     init_corr_attributes(max_lds);
 }
 

@@ -99,11 +99,15 @@ hipError_t launch_bucket_hist(const BucketArgs& b, hipStream_t s);
 hipError_t launch_bucket_scan(const BucketArgs& b, hipStream_t s);
 hipError_t launch_bucket_scatter(const BucketArgs& b, hipStream_t s);
 size_t bucket_lds_bytes(const BucketArgs& b);   // the scatter's dynamic LDS
+// whether the scatter stages its items in LDS (false: too many buckets for it)
+bool bucket_scatter_staged(const BucketArgs& b);
+void init_bucket_attributes(int max_lds);
 // (batched: the bucket scan runs k_sort_scan's batched launch over ScanArgs)
 hipError_t launch_bucket_hist_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
                                     hipStream_t s);
+// staged: every ping of the launch has bucket_scatter_staged
 hipError_t launch_bucket_scatter_batch(const BucketArgs* args, BatchMap m, uint32_t blocks, size_t shm,
-                                       hipStream_t s);
+                                       bool staged, hipStream_t s);
 constexpr int kMaxBuckets = 8192;        // buckets of one ping (LDS counters)
 constexpr int kMaxOccWords = 2048;       // Binner keys of a grid-form ping: 65536
 
