@@ -3,12 +3,16 @@
 Workload: the 512 x 1024 ping (512 bearings uniform in sin over +-65 deg x
 1024 range bins over [0.1, 30] m) from 1 000 000 synthetic seabed samples,
 non-separable 31 x 17 complex PSF.  One step = one ping from ray hits already
-resident in HBM: streaming kernel (directivity x exp(i 2kr)) -> sort (by
-travel for the plain form, by the grouping key of bin and grid row for the
-MFMA forms) with the Binner's keys and segments -> windows -> PSF
-convolution (plain gather, or deposit + MFMA correlation).  Every form is
-timed over the same K steps with HIP events on the compute stream; the
-headline value is the MFMA form's pings/s.
+resident in HBM: streaming kernel (directivity x exp(i 2kr)) -> binning
+(plain form: the sort by travel with the Binner's keys and segments, then
+the windows; MFMA forms: the bucketing of samples by blocks of grid rows
+with the Binner keys' occupancy) -> PSF convolution (plain register gather,
+or deposit + MFMA correlation).  Every form is timed over the same K steps
+with HIP events on the compute stream; the headline value is the MFMA
+form's pings/s.
+
+--summarize-profiles DIR turns the rocprofv3 CSV outputs of the commands
+quoted in profiles/*.md into those summaries.
 
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
 runs on one GPU.
@@ -158,6 +162,7 @@ def accuracy(cfg, sensor, plain_ping, form_pings, dbgs):
 STAGE_BYTES = {
     "stream": lambda S, cfg, g: 40.0 * S,                    # r 4 + dir 12 + A 8 in, 16 out
     "sort": lambda S, cfg, g: 32.0 * S,                      # one read + one write of records
+                                                             # (grid forms: the bucketing)
     "bins": lambda S, cfg, g: 16.0 * cfg.H,                  # segments read, windows written
     "deposit": lambda S, cfg, g: 16.0 * S + 8.0 * g,         # samples in, grid out
 }
@@ -408,6 +413,310 @@ def bench_small_configs(R, configs, stream, steps, cpu=True):
     return out
 
 
+# ---------------------------------------------------------------------------
+# rocprofv3 summaries (the text files under profiles/)
+# ---------------------------------------------------------------------------
+CU_NUM, SIMD_NUM, XCC_NUM = 256, 1024, 8
+
+# stage of each kernel of a ping (k_stream and k_sort_scan serve both forms:
+# they take the form of the next form-specific dispatch)
+_STAGE_OF = [("k_stream", "stream"), ("k_sort_hist", "sort"), ("k_sort_scan", "sort"),
+             ("k_sort_scatter", "sort"), ("k_bucket_hist", "sort"), ("k_bucket_scatter", "sort"),
+             ("k_windows", "bins"), ("k_deposit", "deposit"), ("k_plain_rank", "convolution"),
+             ("k_conv2d_plain", "convolution"), ("k_conv1d_plain", "convolution"),
+             ("k_correlate", "convolution")]
+
+
+def _stage(name):
+    for key, st in _STAGE_OF:
+        if key + "(" in name or key + "<" in name:
+            return st
+    return None
+
+
+def _form_of(name):
+    if "k_correlate<true" in name:
+        return "mfma_fp16"
+    if "k_correlate<false" in name:
+        return "mfma_bf16"
+    if any(k in name for k in ("k_sort_hist", "k_sort_scatter", "k_windows", "k_plain_rank", "k_conv2d_plain",
+                               "k_conv1d_plain")):
+        return "plain_fp32"
+    if any(k in name for k in ("k_bucket_", "k_deposit")):
+        return "mfma"
+    return None
+
+
+def _read_csv(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _one(d: Path, suffix: str) -> Path:
+    hits = sorted(d.rglob(f"*{suffix}"))
+    if not hits:
+        raise FileNotFoundError(f"no *{suffix} under {d}")
+    return hits[0]
+
+
+def _counters(d: Path):
+    """[(dispatch, kernel, {counter: value}, duration ns)] in dispatch order."""
+    by = {}
+    for r in _read_csv(_one(d, "counter_collection.csv")):
+        k = int(r["Dispatch_Id"])
+        e = by.setdefault(k, [r["Kernel_Name"], {}, int(r["End_Timestamp"]) - int(r["Start_Timestamp"])])
+        e[1][r["Counter_Name"]] = e[1].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+    return [(k, v[0], v[1], v[2]) for k, v in sorted(by.items())]
+
+
+def _assign_forms(disp):
+    """Form of every dispatch: form-specific kernels name it; shared ones
+    take the form of the next form-specific dispatch (a grid-form ping's
+    bucketing kernels take the dtype of its correlation)."""
+    forms = [None] * len(disp)
+    nxt = None
+    for i in range(len(disp) - 1, -1, -1):
+        f = _form_of(disp[i][1])
+        if f == "mfma":
+            f = nxt if nxt in ("mfma_fp16", "mfma_bf16") else "mfma_fp16"
+        if f is not None:
+            nxt = f
+        forms[i] = f or nxt
+    return forms
+
+
+def _calibration(fetch_dir: Path, write_dir: Path):
+    nbytes_kib = 256 * 1024
+    f = [c["FETCH_SIZE"] for _, n, c, _ in _counters(fetch_dir) if "copyBuffer" in n]
+    w = [c["WRITE_SIZE"] for _, n, c, _ in _counters(write_dir) if "copyBuffer" in n]
+    cf, cw = sum(f) / len(f), sum(w) / len(w)
+    return nbytes_kib / cf, nbytes_kib / cw, cf, cw
+
+
+def _traffic(workload, fetch_dir, write_dir, cal, cmds):
+    ff, wf, cf, cw = cal
+    fd, wd = _counters(fetch_dir), _counters(write_dir)
+    per_kernel = {}
+    out = {"workload": workload, "calibration": {
+        "fetch_factor": ff, "write_factor": wf, "copy_fetch_kib": cf, "copy_write_kib": cw,
+        "copy": "hipMemcpy device-to-device of 256 MiB (one __amd_rocclr_copyBuffer dispatch), "
+                "bench.py --copy-calibration"}, "commands": cmds, "forms": {}}
+    stage_sum = {}
+    pings = {}
+    for disp, counter, factor in ((fd, "FETCH_SIZE", ff), (wd, "WRITE_SIZE", wf)):
+        forms = _assign_forms(disp)
+        for (k, name, c, _), form in zip(disp, forms):
+            b = c[counter] * 1024.0 * factor
+            pk = per_kernel.setdefault(name, {"FETCH_SIZE": [], "WRITE_SIZE": []})
+            pk[counter].append(b)
+            st = _stage(name)
+            if st is None or form is None:
+                continue
+            stage_sum.setdefault((form, st, counter), []).append((name, b))
+        if counter == "FETCH_SIZE":
+            for (k, name, c, _), form in zip(disp, forms):
+                # one ping per deposit (grid forms) or per gather (plain form)
+                if "k_deposit(" in name or "k_conv2d_plain<" in name or "k_conv1d_plain(" in name:
+                    pings[form] = pings.get(form, 0) + 1
+    for (form, st, counter), items in stage_sum.items():
+        n = pings.get(form, 0)
+        if not n:
+            continue
+        if st == "convolution" and form != "plain_fp32":
+            # the autotuner adds correlations of every shape: bytes per launch
+            val = sum(b for _, b in items) / len(items)
+        else:
+            val = sum(b for _, b in items) / n
+        s = out["forms"].setdefault(form, {"pings": n, "stages": {}})["stages"].setdefault(st, {"bytes": 0.0})
+        s["bytes"] += val
+        s[counter.lower()] = val
+    out["kernels"] = {n: {"fetch_mb": (sum(v["FETCH_SIZE"]) / len(v["FETCH_SIZE"]) / 1e6) if v["FETCH_SIZE"] else None,
+                          "write_mb": (sum(v["WRITE_SIZE"]) / len(v["WRITE_SIZE"]) / 1e6) if v["WRITE_SIZE"] else None}
+                      for n, v in sorted(per_kernel.items())}
+    return out
+
+
+def _traffic_md(t, title, algorithmic):
+    lines = [f"# HBM traffic per ping: {title}", "", "Counter passes (one counter each) over the same run, and the "
+             "same two over the copy calibration, from the repository root on one MI355X:", "", "```"]
+    lines += t["commands"] + ["```", ""]
+    c = t["calibration"]
+    lines += [f"Calibration: a 256 MiB device-to-device `hipMemcpy` reads FETCH_SIZE = {c['copy_fetch_kib']:.0f} KiB "
+              f"and WRITE_SIZE = {c['copy_write_kib']:.0f} KiB: FETCH_SIZE is scaled by {c['fetch_factor']:.2f} and "
+              f"WRITE_SIZE by {c['write_factor']:.2f} below.", "",
+              "Per stage and ping (MB = 1e6 B; read + written).  Kernels used by both forms (`k_stream`, "
+              "`k_sort_scan`) count for the form of the next dispatch; the grid forms' correlation row is per "
+              "launch (the autotuner adds launches of every tile shape).  Write-backs of one kernel's dirty "
+              "lines can land in the next kernel's WRITE_SIZE.  The plain form's sort row also carries the "
+              "Binner-introspection sorts the run makes (they use the same kernels).", ""]
+    forms = [f for f in ("mfma_fp16", "mfma_bf16", "plain_fp32") if f in t["forms"]]
+    lines.append("| stage | " + " | ".join(forms) + " | algorithmic |")
+    lines.append("|---|" + "---:|" * (len(forms) + 1))
+    for st in ("stream", "sort", "bins", "deposit", "convolution"):
+        row = []
+        for f in forms:
+            v = t["forms"][f]["stages"].get(st)
+            row.append(f"{v['bytes'] / 1e6:.2f}" if v else "")
+        lines.append(f"| {st} | " + " | ".join(row) + f" | {algorithmic.get(st, '')} |")
+    lines += ["", "Per kernel (bytes per dispatch, scaled):", "", "| kernel | fetch MB | write MB |", "|---|---:|---:|"]
+    for n, v in t["kernels"].items():
+        fm = f"{v['fetch_mb']:.2f}" if v["fetch_mb"] is not None else ""
+        wm = f"{v['write_mb']:.2f}" if v["write_mb"] is not None else ""
+        lines.append(f"| `{n}` | {fm} | {wm} |")
+    return "\n".join(lines) + "\n"
+
+
+def _kernel_stats_md(d: Path, title, cmd, gaps=False):
+    rows = _read_csv(_one(d, "kernel_stats.csv"))
+    lines = [f"# Kernel durations: {title}", "", "```", cmd, "```", "",
+             "| kernel | calls | average us | min us | max us | share of GPU time |", "|---|---:|---:|---:|---:|---:|"]
+    for r in rows:
+        lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['AverageNs']) / 1e3:.2f} | "
+                     f"{float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | {float(r['Percentage']):.1f}% |")
+    if gaps:
+        tr = _read_csv(_one(d, "kernel_trace.csv"))
+        ks = sorted((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), r["Kernel_Name"]) for r in tr
+                    if not r["Kernel_Name"].startswith("__amd"))
+        # graph launches: consecutive kernels of the batch's stage sequence,
+        # split where the stream kernel starts again
+        launches, cur = [], []
+        for k in ks:
+            if "k_stream_batch" in k[2] and cur:
+                launches.append(cur)
+                cur = []
+            cur.append(k)
+        if cur:
+            launches.append(cur)
+        sigs = {}
+        for g in launches:
+            sigs.setdefault(tuple(n.split("(")[0] for _, _, n in g), []).append(g)
+        for sig, gs in sigs.items():
+            if len(gs) < 2:
+                continue   # one-off launches (autotuning, warm-up)
+            kinds = sorted({n.split("::")[-1].split("<")[0] for n in sig})
+            lines += ["", f"Inside one graph launch of {len(sig)} kernels ({', '.join(kinds)}); the last "
+                      f"{min(5, len(gs))} of {len(gs)} launches:", "",
+                      "| launch | span us | kernel-busy us | idle between kernels |", "|---:|---:|---:|---:|"]
+            for i, g in enumerate(gs[-5:]):
+                span = g[-1][1] - g[0][0]
+                busy, end = 0, None
+                for s0, e, _ in g:
+                    if end is None or s0 >= end:
+                        busy += e - s0
+                        end = e
+                    elif e > end:
+                        busy += e - end
+                        end = e
+                lines.append(f"| {i} | {span / 1e3:.1f} | {busy / 1e3:.1f} | {100.0 * (span - busy) / span:.2f}% |")
+    return "\n".join(lines) + "\n"
+
+
+def _mfma_md(runs):
+    lines = ["# MFMA counters of the correlation (k_correlate)", "",
+             "MfmaFlops = (SQ_INSTS_VALU_MFMA_MOPS_F16 + _BF16) x 512 and MfmaUtil = SQ_VALU_MFMA_BUSY_CYCLES / "
+             "(GRBM_GUI_ACTIVE x SIMD_NUM), the definitions rocprofv3 lists for gfx950 (GRBM_GUI_ACTIVE is "
+             f"reported summed over the {XCC_NUM} XCDs: its per-XCD value is the sum / {XCC_NUM}; "
+             f"SIMD_NUM = {SIMD_NUM}).  Algorithmic flops: 8 W H Tr Tb (complex MACs of the gather's support).  "
+             "One row per kernel instantiation (dtype, tile shape, waves); the autotuner runs each.", ""]
+    for title, cmd, d, W, H, Tr, Tb in runs:
+        alg = 8.0 * W * H * Tr * Tb
+        lines += [f"## {title}", "", "```", cmd, "```", "",
+                  "| kernel | launches | issued GFLOP | issued / algorithmic | MFMA busy | duration us (counter run) "
+                  "| issued TFLOP/s | algorithmic TFLOP/s |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
+        agg = {}
+        for _, n, c, dur in _counters(d):
+            if "k_correlate" not in n:
+                continue
+            a = agg.setdefault(n, [0, 0.0, 0.0, 0.0, 0.0])
+            a[0] += 1
+            a[1] += (c.get("SQ_INSTS_VALU_MFMA_MOPS_F16", 0.0) + c.get("SQ_INSTS_VALU_MFMA_MOPS_BF16", 0.0)) * 512
+            a[2] += c.get("SQ_VALU_MFMA_BUSY_CYCLES", 0.0)
+            a[3] += c.get("GRBM_GUI_ACTIVE", 0.0) / XCC_NUM
+            a[4] += dur
+        for n, (k, fl, busy, act, dur) in sorted(agg.items()):
+            fl, busy, act, dur = fl / k, busy / k, act / k, dur / k
+            lines.append(f"| `{n}` | {k} | {fl / 1e9:.2f} | {fl / alg:.2f} | {100.0 * busy / (act * SIMD_NUM):.1f}% | "
+                         f"{dur / 1e3:.1f} | {fl / dur / 1e3:.1f} | {alg / dur / 1e3:.1f} |")
+        lines.append("")
+    return "\n".join(lines)
+
+
+def summarize_profiles(src: Path):
+    """Writes the profiles/ summaries from the rocprofv3 outputs under `src`
+    (one folder per run, named as in the commands the summaries quote)."""
+    prof = ROOT / "profiles"
+    B = "python3 bench.py"
+
+    def cmd(name, args, pmc=None):
+        what = f"--pmc {pmc}" if pmc else "--kernel-trace --stats"
+        return (f"cd /root/repo && TMPDIR=/tmp rocprofv3 {what} --output-format csv -d gpurun_out/prof/{name} "
+                f"-o {'kt' if pmc is None else 'p'} -- {B} {args}")
+
+    c3 = "--steps 20 --warmup 2 --no-cpu --no-check --no-extra"
+    c3p = "--steps 2 --warmup 1 --no-cpu --no-check --no-extra"
+    stats = [("kt_c3", "fls_512x1024 (every form; MFMA autotuning included)", c3, False, "fls_512x1024"),
+             ("kt_c5", "fls_1024x4096 (both MFMA forms after autotuning, one plain ping)",
+              "--only-extra fls_1024x4096", False, "fls_1024x4096"),
+             ("kt_c4", "the 64 uneven pings in one graph (MFMA FP16), then in the plain form",
+              "--only-extra batch_uneven", True, "batch_uneven"),
+             ("kt_c3x8", "8 pings of fls_512x1024 in one graph (MFMA FP16)", "--only-extra batch_same", True,
+              "batch_same_8x512x1024"),
+             ("kt_small", "single_beam_38k (1D) and fls_512x1024_sep, every form", "--only-extra small --steps 20",
+              False, "small_configs")]
+    for name, title, args, gaps, out in stats:
+        d = src / name
+        if d.exists():
+            (prof / f"kernel_stats_{out}.md").write_text(_kernel_stats_md(d, title, cmd(name, args), gaps))
+    cal = _calibration(src / "fetch_cal", src / "write_cal")
+    calcmds = [cmd("fetch_cal", "--copy-calibration", "FETCH_SIZE"), cmd("write_cal", "--copy-calibration", "WRITE_SIZE")]
+    from rtac_simulation_amd import configs
+    for wl, sub, args in (("fls_512x1024", "c3", c3p), ("fls_1024x4096", "c5", "--only-extra fls_1024x4096")):
+        fdir, wdir = src / f"fetch_{sub}", src / f"write_{sub}"
+        if not (fdir.exists() and wdir.exists()):
+            continue
+        cmds = [cmd(f"fetch_{sub}", args, "FETCH_SIZE"), cmd(f"write_{sub}", args, "WRITE_SIZE")] + calcmds
+        t = _traffic(wl, fdir, wdir, cal, cmds)
+        cfg, _, S, _ = configs.workload(wl)
+        g = grid_cells(cfg)
+        alg = {k: f"{STAGE_BYTES[k](S, cfg, g) / 1e6:.1f}" for k in STAGE_BYTES}
+        alg["convolution"] = f"({8.0 * cfg.W * cfg.H * cfg.Tr * cfg.Tb / 1e9:.2f} GFLOP)"
+        (prof / f"pmc_traffic_{wl}.json").write_text(json.dumps(t, indent=1) + "\n")
+        (prof / f"pmc_traffic_{wl}.md").write_text(
+            _traffic_md(t, f"{wl}: {cfg.W} bearings x {cfg.H} ranges, {S} samples, {cfg.Tr}x{cfg.Tb} PSF", alg))
+    pmc = "SQ_INSTS_VALU_MFMA_MOPS_F16 SQ_INSTS_VALU_MFMA_MOPS_BF16 SQ_VALU_MFMA_BUSY_CYCLES SQ_BUSY_CU_CYCLES " \
+          "SQ_INSTS_VALU SQ_WAVES GRBM_GUI_ACTIVE"
+    runs = []
+    for sub, wl, args in (("mfma_c3", "fls_512x1024", c3p), ("mfma_c5", "fls_1024x4096", "--only-extra fls_1024x4096")):
+        if (src / sub).exists():
+            cfg, _, S, _ = configs.workload(wl)
+            runs.append((f"{wl}: {cfg.W} x {cfg.H}, {cfg.Tr}x{cfg.Tb} PSF", cmd(sub, args, pmc), src / sub,
+                         cfg.W, cfg.H, cfg.Tr, cfg.Tb))
+    if runs:
+        (prof / "pmc_mfma_correlate.md").write_text(_mfma_md(runs))
+
+
+def run_only_extra(args):
+    """One extra workload alone (no CPU baseline), its JSON on stdout."""
+    import rtac_simulation_amd as R
+    from rtac_simulation_amd import configs
+    if R.device_count() < 1:
+        print("no HIP device visible", file=sys.stderr)
+        sys.exit(1)
+    stream = R.Stream()
+    if args.only_extra == "batch_same":
+        cfg, scene, S, seed = configs.workload("fls_512x1024")
+        dhits = R.generate_hits(scene, seed, S, nthreads=min(16, os.cpu_count() or 1)).to_device()
+        out = bench_repeat(R, cfg, dhits, S, stream)
+    elif args.only_extra == "small":
+        out = bench_small_configs(R, configs, stream, args.steps, cpu=False)
+    elif args.only_extra == "batch_uneven":
+        out = bench_batch(R, configs, stream, 3, cpu=False)
+    else:
+        out = bench_large(R, configs, stream, 5, cpu=False)
+    print(json.dumps({args.only_extra: out}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -422,9 +731,20 @@ def main():
                          "WRITE_SIZE counters to bytes")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the batched and 1024x4096 workloads")
+    ap.add_argument("--only-extra", choices=["batch_same", "small", "batch_uneven", "fls_1024x4096"],
+                    help="run just this extra workload and print its JSON (for profiling it alone)")
+    ap.add_argument("--summarize-profiles", metavar="DIR",
+                    help="write the profiles/ summaries from rocprofv3 CSV outputs under DIR "
+                         "(no GPU needed)")
     args = ap.parse_args()
+    if args.summarize_profiles:
+        summarize_profiles(Path(args.summarize_profiles))
+        return
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.only_extra:
+        run_only_extra(args)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU "
@@ -480,10 +800,10 @@ def main():
     roof = dict(per_stage[dominant])
     traffic = measured_traffic(args.workload, best, dominant)
     roof["traffic"] = traffic
-    roof["kernel"] = {"stream": "k_stream",
-                      "sort": "2 grouping-key passes: k_sort_scan + k_sort_scatter each, k_sort_hist "
-                              "for the second (k_stream made the first histogram)",
-                      "bins": "k_windows", "deposit": "k_deposit",
+    roof["kernel"] = {"stream": "k_stream (with the bucket counts)",
+                      "sort": "bucketing: k_sort_scan over the bucket counts + k_bucket_scatter "
+                              "(k_stream made the counts)",
+                      "bins": "k_windows", "deposit": "k_deposit (+ the windows' emptiness)",
                       "convolution": "k_correlate"}[dominant]
     roof["measured"] = "HIP events around the stage on the compute stream, mean over " \
                        f"{max(5, args.steps // 5)} pings"
