@@ -135,8 +135,9 @@ def oracle_rows_error(cfg, records, pings: dict, every: int = 32, start: int = 5
 
 def accuracy(cfg, sensor, plain_ping, form_pings, dbgs):
     """Relative L2 vs the FP64 oracle on every 32nd range row; binning check
-    of every form (the plain form sorts by travel, the grid forms by the
-    grouping key: same keys, segments and windows)."""
+    after every form (the grid forms do not sort: Binner introspection sorts
+    by travel for them, and the keys, segments and windows must be the
+    oracle's whatever the form)."""
     import oracle as O
     rec = sensor.samples()
     b = O.binning(cfg, rec)

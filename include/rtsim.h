@@ -169,15 +169,23 @@ int rtsim_sensor_samples(rtsim_sensor h, void** dev_ptr);
  * n hits with the phase/spreading of RayCaster.cu:62-74: writes records
  * [offset, offset + n).  A write of every sample (offset 0, n = sample count)
  * into a sensor whose MFMA form is prepared (by an earlier MFMA compute or a
- * tile-shape call) also makes that form's first sort histogram, which the
- * next MFMA compute then skips; any other write, upload, count or range change
- * and any compute discard it.                                                     */
+ * tile-shape call) also makes that form's bucket counts, which the next MFMA
+ * compute then skips; any other write, upload, count or range change and any
+ * compute discard them.                                                           */
 int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t n,
                                  size_t offset, void* stream);
 
 /* SensorInstance::compute_output (SensorInstance.h:67; 2D: fill_ping
- * SensorInstance2D.h:84-93; 1D: reduce_samples SensorInstance1D.h:63-69):
- * sort -> Binner::compute -> PSF convolution, in the chosen form.                 */
+ * SensorInstance2D.h:84-93; 1D: reduce_samples SensorInstance1D.h:63-69).
+ * Plain form: sort by travel -> Binner::compute -> the PSF gather in FP32.
+ * MFMA forms (1D and 2D): the samples with a Binner key are bucketed by grid
+ * row, deposited on the output grid (bilinear, exact fixed point) and
+ * correlated with the taps on MFMA; rows whose Binner window is empty are 0.
+ * They equal the gather in exact arithmetic and are refused
+ * (RTSIM_ERR_UNSUPPORTED) unless the bearing coordinates are uniform in the
+ * PSF axis (within 5e-5 of a pitch), the PSF pitch equals the grid pitch in
+ * range and bearing (within 5e-5), and the Binner overlap covers the PSF,
+ * >= (range taps + 3) / 2 bins.                                                   */
 int rtsim_sensor_compute(rtsim_sensor h, int form, void* stream);
 
 /* No counterpart in the reference: rtsim_sensor_compute for callers that
@@ -229,19 +237,21 @@ int rtsim_sensor_tile_waves(rtsim_sensor h, int* waves);
 int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height,
                         uint32_t* width);
 
-/* Sorted samples after compute (receivedSamples_ after sort(), Receiver.cu:28).   */
+/* Sorted samples (receivedSamples_ after sort(), Receiver.cu:28) as the last
+ * sort by travel left them: a plain-form compute or rtsim_sensor_binner_debug.
+ * The MFMA forms do not sort the samples.                                         */
 int rtsim_sensor_sorted_samples(rtsim_sensor h, const void** dev_ptr);
 
 /* Binner internals (Binner.h:56-77) copied to host for bit-exact checks.  Any
  * pointer may be NULL.  keys: sorted keys (n); segments: (start, end) per bin
  * (2N); windows: (start, size) per bin (2N); order: sort permutation (n);
- * overlap: binOverlap.  Synchronises the stream.  Compute keeps neither the
- * keys nor the permutation (nothing downstream reads them), and an MFMA-form
- * compute only marks the bins that hold samples (the windows' emptiness is all
- * it reads): asking for the keys or the permutation, or for the segments or
- * windows after an MFMA-form compute, sorts the resident samples again on the
- * stream with all of them kept (the sort is deterministic, so every array it
- * rewrites keeps its contents).  The samples must be those of the last
+ * overlap: binOverlap.  Synchronises the stream.  A plain-form compute keeps
+ * neither the keys nor the permutation (nothing downstream reads them), and
+ * an MFMA-form compute does not sort at all (it keeps only which bins hold
+ * samples): asking for the keys or the permutation, or for the segments or
+ * windows after an MFMA-form compute, sorts the resident samples by travel on
+ * the stream with all of them kept (the sort is deterministic, so every array
+ * it rewrites keeps its contents).  The samples must be those of the last
  * compute (RTSIM_ERR_INVALID after a change).                                  */
 int rtsim_sensor_binner_debug(rtsim_sensor h, uint32_t* keys, uint32_t* segments,
                               uint32_t* windows, uint32_t* order, int* overlap,
@@ -256,10 +266,11 @@ int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream);
 /* Per-stage timing with HIP events recorded on the compute stream.  When on,
  * each compute records events around its stages; stage_times returns the
  * mean milliseconds per stage over the computes since the last reset.
- * Stages: 0 stream, 1 sort (its last pass also gathers the records, keys and
- * bearing coordinates), 2 bins (segments + windows; MFMA forms: + grid row
- * table), 3 grid deposit (MFMA forms only), 4 convolution kernel (plain
- * gather or MFMA correlation).                                                */
+ * Stages: 0 stream, 1 binning (plain form: the sort by travel, whose last
+ * pass also gathers the records and bearing coordinates and bounds the
+ * segments; MFMA forms: the bucket counts' scan and the scatter), 2 windows
+ * (plain form), 3 grid deposit with the windows' emptiness (MFMA forms),
+ * 4 convolution (plain: band ordering + gather; MFMA: correlation).           */
 int rtsim_sensor_set_profiling(rtsim_sensor h, int on);
 int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes);
 

@@ -132,7 +132,8 @@ extern "C" int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void
     if ((rc = s->plan(form, &p, &pb))) return rc;
     // every array the ping reads is written first by its own kernels (the
     // sort's first histogram clears the segments; the bucketing writes every
-    // count, occupancy word and item it reads): nothing a previous call left
-    // in the scratch is read
+    // count, occupancy word and item it reads; the band ordering writes the
+    // plain gather's work words): nothing a previous call left in the
+    // scratch is read
     return s->launch_ping(p, form, (hipStream_t)stream, false, false);
 }

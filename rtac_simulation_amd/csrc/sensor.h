@@ -101,8 +101,8 @@ struct Sensor {
     // ---- samples -----------------------------------------------------------
     size_t count = 0, capacity = 0;
     bool computed = false;
-    // the last write_samples also made the grid form's first sort histogram
-    // (and its clears) for exactly the resident samples: compute skips it
+    // the last write_samples also made the grid form's bucket counts (and
+    // tile maxima) for exactly the resident samples: compute skips them
     bool hist_fused = false;
     int computed_form = -1;       // form of the last compute (batched or not)
     float4* d_samples = nullptr;  // receivedSamples_ (unsorted)

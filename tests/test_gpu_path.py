@@ -244,9 +244,9 @@ def test_mfma_binner_edge_cases(gpu, case):
 
 
 def test_fused_stream_kernel_writes_identical_records(gpu):
-    """The streaming kernel fused with the first sort histogram (used once the
-    grid form is prepared) writes the same records, bit for bit, as the
-    plain streaming kernel."""
+    """The streaming kernel fused with the grid form's bucket counts (used
+    once the grid form is prepared) writes the same records, bit for bit, as
+    the plain streaming kernel."""
     R = gpu
     cfg = configs._fls(64, 128, 0.1, 30.0, 1.2e6, configs.psf_31x17(), "fusedrec")
     h = R.generate_hits(configs.fls_scene(), 60, 30_000).to_device()
