@@ -343,7 +343,7 @@ def test_mfma_eight_wave_correlation_large_psf(gpu):
 
 
 @pytest.mark.gpu
-def test_mfma_first_histogram_fused_into_write_samples(gpu):
+def test_mfma_bucket_counts_fused_into_write_samples(gpu):
     """Once the grid form is prepared, a write of every sample also makes the
     bucket counts; the computes that use them give the same ping (bit for
     bit) and binning as computes that count themselves, and writes by other
