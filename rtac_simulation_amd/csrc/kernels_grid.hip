@@ -29,9 +29,22 @@ namespace rtsim {
 // plain LDS traffic, unlike FP32 ones).  With M the largest finite |re|, |im|
 // over the ping's binned samples (from the bucket scan), products are scaled
 // by 2^k with (items in the bucket) M 2^k < 2^62, so no partial sum can
-// overflow; each product is rounded once, in FP32, then summed exactly.  A
-// row reached by a non-finite value is NaN.
+// overflow, and M 2^k < 2^51 (the exact conversion below); each product is rounded once, in FP32, then converted to fixed
+// point (rounded to the nearest integer) and summed exactly.  A row reached
+// by a non-finite value is NaN.
 constexpr int kDepositU = 4;          // items per thread whose loads are in flight together
+
+// v 2^k rounded to the nearest integer, as a 64-bit two's complement word,
+// for |v 2^k| < 2^51: the FMA lands v 2^k + 1.5 2^52 in the binade where the
+// double's unit in the last place is 1, so its mantissa bits minus the
+// constant's are the integer (four instructions instead of the generic
+// float -> int64 conversion's dozen)
+__device__ __forceinline__ unsigned long long fixed_of(float v, double scale)
+{
+    constexpr double kMagic = 6755399441055744.0;   // 1.5 * 2^52
+    const double d = __builtin_fma((double)v, scale, kMagic);
+    return (unsigned long long)(__double_as_longlong(d) - __double_as_longlong(kMagic));
+}
 // 16 waves per bucket: with one workgroup per bucket (a few hundred on a
 // 1024-row grid) fewer waves leave the CUs idle while the LDS atomics wait
 constexpr int kDepositThreads = 1024;
@@ -57,7 +70,8 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
     int em = 0;
     frexpf(M, &em);                                   // M < 2^em (em = 0 for M = 0)
     const int ec = 32 - __clz(max(cnt, 1u));          // items < 2^ec
-    const int k = 62 - em - ec;
+    const int k = min(62 - em - ec, 51 - em);         // sums < 2^62, products < 2^51
+    const double scale = ldexp(1.0, k);
     __syncthreads();                                  // LDS cleared
     unsigned long long* acc = reinterpret_cast<unsigned long long*>(sacc);
 
@@ -88,12 +102,12 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
                 unsigned long long* im = re + GW;
                 if (gx >= 0 && gx < GW) {
                     const float w = 1.0f - wx;
-                    atomicAdd(&re[gx], (unsigned long long)(long long)ldexpf(w * vr, k));
-                    atomicAdd(&im[gx], (unsigned long long)(long long)ldexpf(w * vi, k));
+                    atomicAdd(&re[gx], fixed_of(w * vr, scale));
+                    atomicAdd(&im[gx], fixed_of(w * vi, scale));
                 }
                 if (gx + 1 >= 0 && gx + 1 < GW) {
-                    atomicAdd(&re[gx + 1], (unsigned long long)(long long)ldexpf(wx * vr, k));
-                    atomicAdd(&im[gx + 1], (unsigned long long)(long long)ldexpf(wx * vi, k));
+                    atomicAdd(&re[gx + 1], fixed_of(wx * vr, scale));
+                    atomicAdd(&im[gx + 1], fixed_of(wx * vi, scale));
                 }
             }
         }
@@ -128,11 +142,18 @@ __device__ __forceinline__ void occupancy_windows(const DepositArgs& a, int wb)
     const int wlo = lo_bin >> 5, nw = (hi_bin >> 5) - wlo + 1;
     for (int w = t; w < nw; w += kDepositThreads) lw[w] = 0u;
     __syncthreads();
-    const uint32_t total = (uint32_t)nw * a.tiles;
-    for (uint32_t idx = t; idx < total; idx += kDepositThreads) {
-        const uint32_t w = idx % (uint32_t)nw, tile = idx / (uint32_t)nw;
-        const uint32_t v = a.occ[(size_t)tile * a.occ_words + wlo + w];
-        if (v) atomicOr(&lw[w], v);
+    // thread t ORs word t mod nw_e of every phases-th tile in a register (its
+    // loads are independent, so many are in flight), then once into LDS
+    const int nw_e = min(nw, kDepositThreads), phases = kDepositThreads / nw_e;
+    const int slot = t % nw_e, phase = t / nw_e;
+    if (phase < phases) {
+        for (int w = slot; w < nw; w += nw_e) {
+            const uint32_t* col = a.occ + wlo + w;
+            uint32_t v = 0u;
+#pragma unroll 8
+            for (int tile = phase; tile < (int)a.tiles; tile += phases) v |= col[(size_t)tile * a.occ_words];
+            if (v) atomicOr(&lw[w], v);
+        }
     }
     __syncthreads();
     const int r = first + t;
