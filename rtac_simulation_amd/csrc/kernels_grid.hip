@@ -91,23 +91,24 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
             const int gx = (int)fx + a.fb;
             const bool nonfinite = !(fmaxf(fabsf(it[u].x), fabsf(it[u].y)) <= 3.402823466e38f &&
                                      it[u].x == it[u].x && it[u].y == it[u].y);
+            const bool c0 = (unsigned)gx < (unsigned)GW, c1 = (unsigned)(gx + 1) < (unsigned)GW;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int l = l0 + h;
-                if (l < 0 || l >= rows) continue;
+                if ((unsigned)l >= (unsigned)rows) continue;
                 if (nonfinite) bad[l] = 1u;
                 const float wy = h ? wy1 : 1.0f - wy1;
                 const float vr = wy * it[u].x, vi = wy * it[u].y;
-                unsigned long long* re = acc + (size_t)l * 2 * GW;
-                unsigned long long* im = re + GW;
-                if (gx >= 0 && gx < GW) {
+                // 32-bit LDS offsets: row l's re plane, its im plane GW further
+                unsigned long long* re = acc + (__mul24(l, 2 * GW) + gx);
+                if (c0) {
                     const float w = 1.0f - wx;
-                    atomicAdd(&re[gx], fixed_of(w * vr, scale));
-                    atomicAdd(&im[gx], fixed_of(w * vi, scale));
+                    atomicAdd(re, fixed_of(w * vr, scale));
+                    atomicAdd(re + GW, fixed_of(w * vi, scale));
                 }
-                if (gx + 1 >= 0 && gx + 1 < GW) {
-                    atomicAdd(&re[gx + 1], fixed_of(wx * vr, scale));
-                    atomicAdd(&im[gx + 1], fixed_of(wx * vi, scale));
+                if (c1) {
+                    atomicAdd(re + 1, fixed_of(wx * vr, scale));
+                    atomicAdd(re + GW + 1, fixed_of(wx * vi, scale));
                 }
             }
         }
