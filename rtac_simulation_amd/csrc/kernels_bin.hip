@@ -138,7 +138,7 @@ __device__ __forceinline__ void stream_body(const StreamArgs& p, const BucketArg
 
 __global__ __launch_bounds__(kStreamThreads) void k_stream(StreamArgs p, BucketArgs b, int with_hist)
 {
-    stream_body(p, b, with_hist, blockIdx.x);
+    stream_body(p, b, with_hist, xcd_contiguous_tile(blockIdx.x, gridDim.x));
 }
 
 // bargs[q]: ping q's bucket counts to make on the way (tiles == 0: none)
@@ -150,7 +150,7 @@ __global__ __launch_bounds__(kStreamThreads) void k_stream_batch(const StreamArg
     const uint32_t q = batch_locate(m, blk, nblk);
     const StreamArgs a = args[q];   // private copies: the body's stores cannot alias them
     const BucketArgs b = bargs[q];
-    stream_body(a, b, b.tiles != 0, blk);
+    stream_body(a, b, b.tiles != 0, xcd_contiguous_tile(blk, nblk));
 }
 
 uint32_t stream_blocks(const StreamArgs& a) { return (a.n + kStreamThreads - 1) / kStreamThreads; }

@@ -126,20 +126,24 @@ __device__ __forceinline__ void deposit_body(const DepositArgs& a, int b)
     }
 }
 
-// Windows of output bins [wb T, wb T + T), T = kDepositThreads (do_make_bins,
+// Windows of output bins [wb B, wb B + B), B = kWindowBins (do_make_bins,
 // Binner.cu:82-106): a window holds samples when at least two of the bins
 // [r - ov, r + ov] (inside [0, N)) hold samples; the occupancy is the OR of
 // the bucketing's per-tile key bits.  Only the emptiness is kept:
 // win[r] = (0, 1) for a window with samples, (0, 0) for an empty one (the
-// correlation masks rows whose window size is 0).
+// correlation masks rows whose window size is 0).  A workgroup covers few bins
+// so that its OR over every tile is a single round of loads per thread.
+constexpr int kWindowBins = 64;
+static_assert(kWindowBins <= kDepositThreads, "one thread per bin of the group");
+
 __device__ __forceinline__ void occupancy_windows(const DepositArgs& a, int wb)
 {
     extern __shared__ long long sacc[];
     uint32_t* lw = reinterpret_cast<uint32_t*>(sacc);
     const int t = threadIdx.x;
     const int N = (int)a.N, ov = a.ov;
-    const int first = wb * kDepositThreads;
-    const int lo_bin = max(0, first - ov), hi_bin = min(N - 1, first + kDepositThreads - 1 + ov);
+    const int first = wb * kWindowBins;
+    const int lo_bin = max(0, first - ov), hi_bin = min(N - 1, first + kWindowBins - 1 + ov);
     const int wlo = lo_bin >> 5, nw = (hi_bin >> 5) - wlo + 1;
     for (int w = t; w < nw; w += kDepositThreads) lw[w] = 0u;
     __syncthreads();
@@ -158,7 +162,7 @@ __device__ __forceinline__ void occupancy_windows(const DepositArgs& a, int wb)
     }
     __syncthreads();
     const int r = first + t;
-    if (r >= N) return;
+    if (t >= kWindowBins || r >= N) return;
     const int lo = max(0, r - ov), hi = min(N - 1, r + ov);
     int c = 0;
     for (int w = lo >> 5; w <= (hi >> 5) && c < 2; w++) {
@@ -190,7 +194,7 @@ __global__ __launch_bounds__(kDepositThreads) void k_deposit_batch(const Deposit
 
 uint32_t deposit_blocks(const DepositArgs& a)
 {
-    return (uint32_t)a.NB + (a.N + kDepositThreads - 1) / kDepositThreads;   // + windows
+    return (uint32_t)a.NB + (a.N + kWindowBins - 1) / kWindowBins;   // + windows
 }
 
 size_t deposit_lds_bytes(const DepositArgs& a)
