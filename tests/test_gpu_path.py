@@ -215,7 +215,8 @@ def test_repeated_compute_is_stable(gpu):
     for _ in range(3):
         s.compute_output(0)
     assert np.array_equal(o1, s.binner_debug()["order"])
-    assert O.rel_l2(s.output(), first) < 1e-6
+    # the gather sums every cell in a fixed order: bit-reproducible
+    assert np.array_equal(s.output().view(np.uint32), first.view(np.uint32))
 
 
 @pytest.mark.parametrize("case", ["ties", "all_oor", "all_null", "nan_inf_zero", "single_bin", "empty"])
