@@ -224,6 +224,20 @@ def measured_traffic(workload: str, form: str, stage: str):
         return None
 
 
+def measured_mfma(workload: str, fp16: bool, tile_shape, tile_waves):
+    """MFMA counters of the correlation instantiation the autotuner picked,
+    from the committed rocprofv3 PMC summary, or None when none covers it."""
+    if tile_shape is None:
+        return None
+    tr, tc = {0: (1, 1), 1: (1, 2), 2: (2, 1)}.get(tile_shape, (2, 2))
+    key = f"k_correlate<{'true' if fp16 else 'false'}, {tr}, {tc}, {'true' if tile_waves == 8 else 'false'}>"
+    try:
+        d = json.loads((ROOT / "profiles" / "pmc_mfma_correlate.json").read_text())
+        return dict(d[workload][key], kernel=key)
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def bench_batch(R, configs, stream, steps, cpu=True):
     """The 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 2D and
     8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch in
@@ -620,8 +634,10 @@ def _mfma_md(runs):
              f"reported summed over the {XCC_NUM} XCDs: its per-XCD value is the sum / {XCC_NUM}; "
              f"SIMD_NUM = {SIMD_NUM}).  Algorithmic flops: 8 W H Tr Tb (complex MACs of the gather's support).  "
              "One row per kernel instantiation (dtype, tile shape, waves); the autotuner runs each.", ""]
-    for title, cmd, d, W, H, Tr, Tb in runs:
+    js = {}
+    for title, cmd, d, W, H, Tr, Tb, wl in runs:
         alg = 8.0 * W * H * Tr * Tb
+        jw = js.setdefault(wl, {})
         lines += [f"## {title}", "", "```", cmd, "```", "",
                   "| kernel | launches | issued GFLOP | issued / algorithmic | MFMA busy | duration us (counter run) "
                   "| issued TFLOP/s | algorithmic TFLOP/s |", "|---|---:|---:|---:|---:|---:|---:|---:|"]
@@ -639,8 +655,11 @@ def _mfma_md(runs):
             fl, busy, act, dur = fl / k, busy / k, act / k, dur / k
             lines.append(f"| `{n}` | {k} | {fl / 1e9:.2f} | {fl / alg:.2f} | {100.0 * busy / (act * SIMD_NUM):.1f}% | "
                          f"{dur / 1e3:.1f} | {fl / dur / 1e3:.1f} | {alg / dur / 1e3:.1f} |")
+            key = n[n.index("k_correlate"):n.index(">") + 1]
+            jw[key] = {"issued_gflop": fl / 1e9, "issued_over_algorithmic": fl / alg,
+                       "mfma_busy": busy / (act * SIMD_NUM), "duration_us_counter_run": dur / 1e3}
         lines.append("")
-    return "\n".join(lines)
+    return "\n".join(lines), js
 
 
 def summarize_profiles(src: Path):
@@ -692,9 +711,11 @@ def summarize_profiles(src: Path):
         if (src / sub).exists():
             cfg, _, S, _ = configs.workload(wl)
             runs.append((f"{wl}: {cfg.W} x {cfg.H}, {cfg.Tr}x{cfg.Tb} PSF", cmd(sub, args, pmc), src / sub,
-                         cfg.W, cfg.H, cfg.Tr, cfg.Tb))
+                         cfg.W, cfg.H, cfg.Tr, cfg.Tb, wl))
     if runs:
-        (prof / "pmc_mfma_correlate.md").write_text(_mfma_md(runs))
+        md, js = _mfma_md(runs)
+        (prof / "pmc_mfma_correlate.md").write_text(md)
+        (prof / "pmc_mfma_correlate.json").write_text(json.dumps(js, indent=1) + "\n")
 
 
 def run_only_extra(args):
@@ -764,6 +785,9 @@ def main():
     sensor = R.make_sensor(cfg)
     sensor.set_sample_count(S)
     stream = R.Stream()
+    # this GPU's own copy bandwidth and MFMA rates, beside the published peaks
+    peaks = R.measure_peaks(stream)
+    print(f"# measured peaks {peaks}", file=sys.stderr, flush=True)
 
     forms = {"plain_fp32": R.FORM_PLAIN_FP32, "mfma_bf16": R.FORM_MFMA_BF16,
              "mfma_fp16": R.FORM_MFMA_FP16}
@@ -797,6 +821,13 @@ def main():
     # rooflines of every stage of the headline form; the dominant one leads
     st = stages[best]
     per_stage = {k: stage_roofline(k, v, S, cfg, dtype) for k, v in st.items() if v > 0}
+    for k, r in per_stage.items():
+        mp = peaks["hbm_copy_gbs"] if r["bound"] == "hbm" else peaks[f"mfma_{dtype}_tflops"]
+        r["frac_of_measured_peak"] = r["achieved"] / mp if mp > 0 else None
+    if "convolution" in per_stage:
+        per_stage["convolution"]["mfma_counters"] = measured_mfma(
+            args.workload, dtype == "fp16", tiles[best][0] if best in tiles else None,
+            tiles[best][1] if best in tiles else None)
     dominant = max(per_stage, key=lambda k: st[k])
     roof = dict(per_stage[dominant])
     traffic = measured_traffic(args.workload, best, dominant)
@@ -854,6 +885,9 @@ def main():
         "binning_bit_exact": exact,
         "roofline": roof,
         "stage_rooflines": per_stage,
+        "measured_peaks": dict(peaks, how="rtsim_measure_peaks: best of 3 launches of a 512 MiB "
+                                          "float4 copy (read + write bytes) and of dense "
+                                          "v_mfma_f32_32x32x16 chains"),
         "cpu_baseline": cpu,
         "extra": extra or None,
     }

@@ -274,6 +274,14 @@ int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream);
 int rtsim_sensor_set_profiling(rtsim_sensor h, int on);
 int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes);
 
+/* No reference counterpart: this GPU's own peaks for roofline fractions (a
+ * float4 copy between two 512 MiB buffers, reads + writes per second; four
+ * independent chains of v_mfma_f32_32x32x16_{f16,bf16} per wave, two waves per
+ * SIMD), best of three launches each, on `stream` (synchronised).  Allocates
+ * and frees 1 GiB.  Any output pointer may be NULL.                            */
+int rtsim_measure_peaks(void* stream, double* hbm_gbs, double* mfma_fp16_tflops,
+                        double* mfma_bf16_tflops);
+
 /* Streams and event timers for callers without their own HIP binding
  * (hipStreamCreate at the default priority, hipEvent pairs).                     */
 int rtsim_stream_create(void** stream);

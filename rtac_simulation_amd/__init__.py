@@ -101,6 +101,8 @@ _SIGNATURES = [
     ("rtsim_receiver_write_samples", C.c_int,
      [C.c_void_p, C.POINTER(Hits), C.c_size_t, C.c_size_t, C.c_void_p]),
     ("rtsim_sensor_compute", C.c_int, [C.c_void_p, C.c_int, C.c_void_p]),
+    ("rtsim_measure_peaks", C.c_int,
+     [C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     ("rtsim_sensor_compute_noalloc", C.c_int,
      [C.c_void_p, C.c_int, C.c_void_p, C.c_size_t, C.c_void_p, C.c_void_p]),
     ("rtsim_sensor_autotune", C.c_int,
@@ -210,6 +212,15 @@ def device_count() -> int:
     c = C.c_int(0)
     check(lib().rtsim_device_count(C.byref(c)))
     return c.value
+
+
+def measure_peaks(stream: "Stream | None" = None) -> dict:
+    """This GPU's own HBM copy bandwidth (GB/s, read + write) and dense MFMA
+    rates (TFLOP/s, FP16 and BF16) from rtsim_measure_peaks."""
+    h, f16, b16 = C.c_double(0), C.c_double(0), C.c_double(0)
+    check(lib().rtsim_measure_peaks(stream.handle if stream else None, C.byref(h), C.byref(f16),
+                                    C.byref(b16)))
+    return {"hbm_copy_gbs": h.value, "mfma_fp16_tflops": f16.value, "mfma_bf16_tflops": b16.value}
 
 
 def _ptr(a: np.ndarray) -> C.c_void_p:

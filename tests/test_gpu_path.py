@@ -259,3 +259,13 @@ def test_fused_stream_kernel_writes_identical_records(gpu):
     b.set_tile_shape(0)                 # prepares the grid form: the next write is fused
     b.write_samples(h)
     assert np.array_equal(a.samples().view(np.uint32), b.samples().view(np.uint32))
+
+
+def test_measure_peaks_in_hardware_range(gpu):
+    """rtsim_measure_peaks reports figures an MI355X can reach: copy
+    bandwidth above 2 TB/s and at most the 8 TB/s HBM3E peak; dense MFMA
+    rates above 250 TFLOP/s and at most the 2.5 PFLOP/s FP16/BF16 peak."""
+    p = gpu.measure_peaks()
+    assert 2000.0 < p["hbm_copy_gbs"] <= 8000.0, p
+    for k in ("mfma_fp16_tflops", "mfma_bf16_tflops"):
+        assert 250.0 < p[k] <= 2500.0, p
