@@ -238,6 +238,44 @@ def measured_mfma(workload: str, fp16: bool, tile_shape, tile_waves):
         return None
 
 
+def time_other_mode(b, sensors, stream, steps):
+    """The batch's mean ms in the launch mode the autotuner did not keep
+    (graph of per-stage launches or one persistent launch), with the tiles the
+    mode needs; the kept mode and tiles are restored afterwards."""
+    import rtac_simulation_amd as R
+    saved = [(s.tile_shape()[0], s.tile_waves()) for s in sensors]
+    was = b.persistent
+    b.persistent = not was
+    b.compute(stream)
+    stream.sync()
+    b.check()
+    timer = R.Timer()
+    timer.start(stream)
+    for _ in range(steps):
+        b.compute(stream)
+    timer.stop(stream)
+    stream.sync()
+    b.check()
+    ms = timer.elapsed_ms() / steps
+    b.persistent = was
+    for s, (shape, waves) in zip(sensors, saved):
+        s.set_tile_shape(shape)
+        s.set_tile_waves(waves)
+    b.compute(stream)
+    stream.sync()
+    return ms
+
+
+def launch_modes(b, sensors, stream, steps, ms):
+    """Both launch modes of a batch: the kept one (timed as `ms`) and the other."""
+    other = time_other_mode(b, sensors, stream, steps)
+    kept = "persistent" if b.persistent else "graph"
+    return {"kept": kept, "graph_ms": other if kept == "persistent" else ms,
+            "persistent_ms": ms if kept == "persistent" else other,
+            "how": "rtsim_batch_autotune times both modes (per-stage launches in one hipGraph, or one "
+                   "persistent launch of every stage of every MFMA ping) and keeps the faster"}
+
+
 def bench_batch(R, configs, stream, steps, cpu=True):
     """The 64 uneven pings (16 x 256x512, 32 x 512x1024, 8 x 1024x2048 2D and
     8 x 2048 1D; samples log-uniform in [1e5, 4e6]) from one graph launch in
@@ -277,6 +315,7 @@ def bench_batch(R, configs, stream, steps, cpu=True):
            "samples": int(total), "ms_per_batch": ms, "pings_per_s": len(items) / (ms * 1e-3),
            "msamples_per_s": total / (ms * 1e-3) / 1e6, "steps": steps}
     mf = [s.output().copy() for s in sensors]
+    out["launch_modes"] = launch_modes(b, sensors, stream, steps, ms)
     # the same batch in the plain FP32 form
     bp = R.Batch(sensors, [R.FORM_PLAIN_FP32] * len(sensors))
     for i, (cfg, scene, S, seed) in enumerate(items):
@@ -342,11 +381,35 @@ def bench_repeat(R, cfg, dhits, S, stream, pings=8, steps=10):
     stream.sync()
     ms = timer.elapsed_ms() / steps
     shape = sensors[0].tile_shape()[0]
+    waves = sensors[0].tile_waves()
+    modes = launch_modes(b, sensors, stream, steps, ms)
     return {"workload": f"{pings} pings of {cfg.W}x{cfg.H}, {S} samples each, one hipGraph launch "
                         "(MFMA FP16)", "pings": pings, "tile_shape": shape,
-            "tile_waves": sensors[0].tile_waves(), "ms_per_batch": ms,
+            "tile_waves": waves, "ms_per_batch": ms,
             "pings_per_s": pings / (ms * 1e-3), "msamples_per_s": pings * S / (ms * 1e-3) / 1e6,
-            "steps": steps}
+            "steps": steps, "launch_modes": modes}
+
+
+def bench_single_modes(R, cfg, dhits, S, stream, steps=20):
+    """The headline ping as a batch of one (MFMA FP16, tiles from the batch
+    autotuner) in both launch modes: the graph of per-stage launches and the
+    persistent launch, beside the eager write + compute the headline times."""
+    s = R.make_sensor(cfg)
+    s.set_sample_count(S)
+    b = R.Batch([s], [R.FORM_MFMA_FP16])
+    b.bind_hits(0, dhits)
+    b.autotune(5, stream)
+    stream.sync()
+    timer = R.Timer()
+    timer.start(stream)
+    for _ in range(steps):
+        b.compute(stream)
+    timer.stop(stream)
+    stream.sync()
+    b.check()
+    ms = timer.elapsed_ms() / steps
+    return {"workload": f"one ping of {cfg.W}x{cfg.H}, {S} samples, as a batch (MFMA FP16)",
+            "ms_per_ping": ms, "steps": steps, "launch_modes": launch_modes(b, [s], stream, steps, ms)}
 
 
 def bench_large(R, configs, stream, steps, cpu=True):
@@ -849,6 +912,8 @@ def main():
     if not args.no_extra:
         extra["batch_same"] = bench_repeat(R, cfg, dhits, S, stream)
         print(f"# repeated {extra['batch_same']}", file=sys.stderr, flush=True)
+        extra["batch_single"] = bench_single_modes(R, cfg, dhits, S, stream)
+        print(f"# single as a batch {extra['batch_single']}", file=sys.stderr, flush=True)
         extra.update(bench_small_configs(R, configs, stream, args.steps, cpu=not args.no_cpu))
         print(f"# small {extra['single_beam_38k']} {extra['fls_512x1024_sep']}", file=sys.stderr,
               flush=True)

@@ -168,6 +168,9 @@ _SIGNATURES = [
     ("rtsim_batch_bind_hits", C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Hits), C.c_size_t]),
     ("rtsim_batch_compute", C.c_int, [C.c_void_p, C.c_void_p]),
     ("rtsim_batch_autotune", C.c_int, [C.c_void_p, C.c_int, C.c_void_p, C.POINTER(C.c_float)]),
+    ("rtsim_batch_set_persistent", C.c_int, [C.c_void_p, C.c_int]),
+    ("rtsim_batch_persistent", C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    ("rtsim_batch_check", C.c_int, [C.c_void_p]),
     ("rtsim_batch_destroy", C.c_int, [C.c_void_p]),
     # ping files (rtsim_sink.h)
     ("rtsim_sink_open", C.c_int, [C.c_char_p, C.c_int, C.POINTER(C.c_void_p)]),
@@ -644,6 +647,21 @@ class Batch:
 
     def compute(self, stream: "Stream | None" = None):
         check(lib().rtsim_batch_compute(self.handle, stream.handle if stream else None))
+
+    @property
+    def persistent(self) -> bool:
+        """Grid-form pings in one persistent launch (rtsim_batch_set_persistent)."""
+        on = C.c_int()
+        check(lib().rtsim_batch_persistent(self.handle, C.byref(on)))
+        return bool(on.value)
+
+    @persistent.setter
+    def persistent(self, on: bool):
+        check(lib().rtsim_batch_set_persistent(self.handle, 1 if on else 0))
+
+    def check(self):
+        """Waits for the last launch; raises if its persistent launch was abandoned."""
+        check(lib().rtsim_batch_check(self.handle))
 
     def autotune(self, iterations: int = 5, stream: "Stream | None" = None) -> float:
         """Per class of MFMA-form sensors, the fastest tile shape and wave

@@ -9,6 +9,11 @@
 // captured once into a hipGraph and replayed; the per-stage descriptor
 // arrays and the graph are rebuilt only when a sensor's configuration,
 // sample count or tile shape changes.
+//
+// Persistent mode (rtsim_batch_set_persistent): the grid-form pings run in
+// ONE persistent launch instead (persist.h): every stage of every grid-form
+// ping from one ticket-ordered task list, a ping's stages overlapping the
+// other pings' stages; plain-form pings keep their per-stage launches.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -21,6 +26,7 @@
 #include "../../include/rtsim.h"
 #include "../../include/rtsim_batch.h"
 #include "grid_internal.h"
+#include "persist.h"
 #include "rtsim_internal.h"
 #include "sensor.h"
 
@@ -35,6 +41,7 @@ struct BatchEntry {
     uint64_t generation = 0;
     uint64_t pose_version = 0;
     int stream_index = -1;   // its entry in the streaming stage (-1: no bound hits)
+    int persist_index = -1;  // its entry among the persistent launch's pings (-1: none)
 };
 
 // Host image of one batched launch: the arguments of the pings it covers and
@@ -85,6 +92,17 @@ struct Batch {
     Stage<DepositArgs> deposit;
     std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape, waves per workgroup)
 
+    // persistent mode: the grid-form pings, their task segments, control words
+    bool persistent = false;
+    std::vector<PersistPing> pp;
+    std::vector<uint32_t> seg_first, seg_ping;
+    size_t pp_off = 0, segf_off = 0, segp_off = 0;
+    uint32_t ntasks = 0, persist_grid = 0;
+    size_t persist_shm = 0;
+    uint64_t spin_limit = 0;
+    uint32_t* d_ctl = nullptr;
+    size_t ctl_cap = 0;   // words
+
     // waits for the last launch (a no-op when none is in flight)
     hipError_t wait()
     {
@@ -102,6 +120,7 @@ struct Batch {
         if (graph) (void)hipGraphDestroy(graph);
         if (capture_stream) (void)hipStreamDestroy(capture_stream);
         if (d_desc) (void)hipFree(d_desc);
+        if (d_ctl) (void)hipFree(d_ctl);
     }
 };
 
@@ -136,6 +155,83 @@ static BatchMap dmap(const Batch* b, const Stage<A>& st)
                     st.nping()};
 }
 
+// A grid-form ping of the persistent launch: its stage arguments and the
+// task count of every level (persist.h); *nflags advances past its
+// deposit-task flags.
+static int persist_ping(Batch* b, BatchEntry& en, const PingPlan& p, uint32_t* nflags)
+{
+    Sensor* s = en.sensor;
+    if (!p.corr.waves8)
+        return fail(RTSIM_ERR_UNSUPPORTED,
+                    "the persistent launch runs 8-wave correlation tiles (rtsim_sensor_set_tile_waves(8))");
+    PersistPing q{};
+    q.bk = p.bucket;
+    q.dep = p.dep;
+    // tiles of kPersistTile samples (fewer count columns; the deposit sums
+    // exactly, so the grid does not depend on the tiling) unless the ping's
+    // bucket counts already need larger ones
+    if (q.bk.tile < kPersistTile) {
+        q.bk.tile = kPersistTile;
+        q.bk.tiles = (uint32_t)(((size_t)q.bk.n + kPersistTile - 1) / kPersistTile);
+        q.dep.tiles = q.bk.tiles;
+    }
+    if (en.has_hits && en.nhits) {
+        q.st = s->stream_args(en.hits, en.nhits, 0);
+        q.ntask[kLvStream] = (uint32_t)((en.nhits + kPersistTile - 1) / kPersistTile);
+        q.fuse = q.bk.tile == kPersistTile && q.bk.rec == q.st.out && q.bk.n == q.st.n ? 1 : 0;
+    }
+    q.ntask[kLvHist] = q.fuse ? 0u : q.bk.tiles;
+    q.ntask[kLvScan] = (uint32_t)q.bk.NB;
+    q.ntask[kLvScatter] = q.bk.tiles;
+    q.staged = q.bk.tiles && bucket_scatter_staged(q.bk) ? 1 : 0;
+    q.ntask[kLvDeposit] = deposit_blocks(q.dep);
+    q.corr = p.corr;
+    q.ntask[kLvCorr] = correlate_blocks(q.corr);
+    q.variant = (en.form == RTSIM_FORM_MFMA_FP16 ? 4 : 0) + q.corr.shape;
+    // a task waits for its ping's nearest earlier level that has tasks
+    int last = -1;
+    for (int lv = 0; lv < kLevels; lv++) {
+        q.wait[lv] = last;
+        if (q.ntask[lv]) last = lv;
+    }
+    q.flags = *nflags;
+    *nflags += q.ntask[kLvDeposit];
+    b->persist_shm = std::max(b->persist_shm, persist_lds_bytes(q));
+    en.persist_index = (int)b->pp.size();
+    if (en.has_hits && en.nhits) en.stream_index = en.persist_index;
+    b->pp.push_back(q);
+    s->last_corr = p.corr;
+    return RTSIM_OK;
+}
+
+// The persistent launch's task list (level-major segments) and its place in
+// the descriptor buffer.
+static void persist_layout(Batch* b, size_t& off)
+{
+    b->seg_first.clear();
+    b->seg_ping.clear();
+    uint32_t total = 0;
+    for (int lv = 0; lv < kLevels; lv++)
+        for (size_t q = 0; q < b->pp.size(); q++) {
+            const uint32_t n = b->pp[q].ntask[lv];
+            if (!n) continue;
+            b->seg_first.push_back(total);
+            b->seg_ping.push_back((uint32_t)lv << 24 | (uint32_t)q);
+            total += n;
+        }
+    b->seg_first.push_back(total);
+    b->ntasks = total;
+    off = (off + 255) & ~(size_t)255;
+    b->pp_off = off;
+    off += sizeof(PersistPing) * b->pp.size();
+    off = (off + 15) & ~(size_t)15;
+    b->segf_off = off;
+    off += sizeof(uint32_t) * b->seg_first.size();
+    off = (off + 15) & ~(size_t)15;
+    b->segp_off = off;
+    off += sizeof(uint32_t) * b->seg_ping.size();
+}
+
 // Per-stage descriptors of every ping (no device work is queued).
 static int build(Batch* b)
 {
@@ -157,13 +253,25 @@ static int build(Batch* b)
     b->plain1d = {};
     b->deposit = {};
     b->corr.clear();
+    b->pp.clear();
+    b->seg_first.clear();
+    b->seg_ping.clear();
+    b->ntasks = 0;
+    b->persist_shm = 0;
     std::map<std::tuple<int, int, int>, size_t> corr_index;
+    uint32_t nflags = 0;
     for (auto& en : b->entries) {
         Sensor* s = en.sensor;
         PingPlan p;
         int rc = s->plan(en.form, &p);
         if (rc) return rc;
         en.stream_index = -1;
+        en.persist_index = -1;
+        if (p.grid && b->persistent) {
+            int prc = persist_ping(b, en, p, &nflags);
+            if (prc) return prc;
+            continue;
+        }
         bool counted = false;   // the grid form's bucket counts made by the streaming kernel
         if (en.has_hits && en.nhits) {
             StreamArgs a = s->stream_args(en.hits, en.nhits, 0);
@@ -239,6 +347,7 @@ static int build(Batch* b)
     layout(b->plain1d, off);
     layout(b->deposit, off);
     for (auto& c : b->corr) layout(c, off);
+    if (!b->pp.empty()) persist_layout(b, off);
     if (off > b->desc_cap) {
         if (b->d_desc) (void)hipFree(b->d_desc);
         b->d_desc = nullptr;
@@ -264,8 +373,30 @@ static int build(Batch* b)
     pack(b->plain1d, host);
     pack(b->deposit, host);
     for (auto& c : b->corr) pack(c, host);
+    if (!b->pp.empty()) {
+        std::memcpy(host.data() + b->pp_off, b->pp.data(), sizeof(PersistPing) * b->pp.size());
+        std::memcpy(host.data() + b->segf_off, b->seg_first.data(), sizeof(uint32_t) * b->seg_first.size());
+        std::memcpy(host.data() + b->segp_off, b->seg_ping.data(), sizeof(uint32_t) * b->seg_ping.size());
+    }
     hipError_t e = hipMemcpy(b->d_desc, host.data(), off, hipMemcpyHostToDevice);
     if (e != hipSuccess) return hip_fail(e, "hipMemcpy(batch descriptors)");
+    if (!b->pp.empty()) {
+        // control words: zero, sized for this build's counters and flags
+        const size_t words = (size_t)kCtlWords + b->pp.size() * kLevels * kCtrWords + nflags;
+        if (words > b->ctl_cap) {
+            if (b->d_ctl) (void)hipFree(b->d_ctl);
+            b->d_ctl = nullptr;
+            b->ctl_cap = 0;
+            if ((e = hipMalloc((void**)&b->d_ctl, sizeof(uint32_t) * words)) != hipSuccess)
+                return hip_fail(e, "hipMalloc(persistent control words)");
+            b->ctl_cap = words;
+        }
+        if ((e = hipMemset(b->d_ctl, 0, sizeof(uint32_t) * b->ctl_cap)) != hipSuccess)
+            return hip_fail(e, "hipMemset(persistent control words)");
+        b->persist_grid = std::min(persist_blocks(b->persist_shm), b->ntasks);
+        if (!b->persist_grid) return fail(RTSIM_ERR_HIP, "persistent launch: no workgroup fits");
+        b->spin_limit = 2 * persist_clock_hz();   // two seconds
+    }
     return RTSIM_OK;
 }
 
@@ -300,6 +431,19 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
         RT_TRY(launch_correlate_batch(c.variant == RTSIM_FORM_MFMA_FP16, c.args.empty() ? 0 : c.args[0].shape,
                                       !c.args.empty() && c.args[0].waves8, dargs(b, c), dmap(b, c),
                                       c.blocks(), c.shm, s));
+    if (b->ntasks) {
+        const char* base = static_cast<const char*>(b->d_desc);
+        PersistArgs a{};
+        a.pings = reinterpret_cast<const PersistPing*>(base + b->pp_off);
+        a.seg_first = reinterpret_cast<const uint32_t*>(base + b->segf_off);
+        a.seg_ping = reinterpret_cast<const uint32_t*>(base + b->segp_off);
+        a.nseg = (uint32_t)b->seg_ping.size();
+        a.ntasks = b->ntasks;
+        a.nping = (uint32_t)b->pp.size();
+        a.ctl = b->d_ctl;
+        a.spin_limit = b->spin_limit;
+        RT_TRY(launch_persist(a, b->persist_grid, b->persist_shm, s));
+    }
 #undef RT_TRY
     return hipSuccess;
 }
@@ -424,12 +568,17 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
                 if (e != hipSuccess) return hip_fail(e, "waiting for the previous batch launch");
                 waited = true;
             }
-            StreamArgs& a = b->stream.args[en.stream_index];
+            const bool pers = en.persist_index >= 0;
+            StreamArgs& a = pers ? b->pp[en.persist_index].st : b->stream.args[en.stream_index];
             std::memcpy(a.R, en.sensor->R, sizeof(a.R));
-            b->h_pose[en.stream_index] = a;
-            char* dst = static_cast<char*>(b->d_desc) + b->stream.args_off +
-                        sizeof(StreamArgs) * (size_t)en.stream_index;
-            hipError_t e = hipMemcpyAsync(dst, &b->h_pose[en.stream_index], sizeof(StreamArgs),
+            // staging slot: one per entry (stream indices and persistent
+            // indices can coincide, entry indices cannot)
+            const size_t slot = (size_t)(&en - b->entries.data());
+            b->h_pose[slot] = a;
+            char* dst = static_cast<char*>(b->d_desc) +
+                        (pers ? b->pp_off + sizeof(PersistPing) * (size_t)en.persist_index + offsetof(PersistPing, st)
+                              : b->stream.args_off + sizeof(StreamArgs) * (size_t)en.stream_index);
+            hipError_t e = hipMemcpyAsync(dst, &b->h_pose[slot], sizeof(StreamArgs),
                                           hipMemcpyHostToDevice, st);
             if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(pose)");
         }
@@ -449,6 +598,7 @@ int rtsim_batch_compute(rtsim_batch batch, void* stream)
 
 // Mean milliseconds of one replay of the batch's graph (captured first when
 // out of date).
+int rtsim_batch_set_persistent(rtsim_batch batch, int on);
 static int time_batch(Batch* b, int iterations, hipStream_t st, hipEvent_t e0, hipEvent_t e1, float* ms)
 {
     int rc = rtsim_batch_compute(reinterpret_cast<rtsim_batch>(b), st);   // capture + warm
@@ -464,29 +614,17 @@ static int time_batch(Batch* b, int iterations, hipStream_t st, hipEvent_t e0, h
     return RTSIM_OK;
 }
 
-int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float* best_ms)
+// MFMA-form sensors of the batch by class (grid and PSF size)
+using SensorClasses = std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::vector<Sensor*>>;
+
+// Per class in turn (the others held at their current choice), times every
+// tile shape and wave count the launch mode allows and keeps the fastest;
+// *bms holds the batch's time with the current choice on entry and the best
+// one on return.
+static int tune_tiles(Batch* b, SensorClasses& classes, int iterations, hipStream_t st, hipEvent_t e0,
+                      hipEvent_t e1, float* bms)
 {
-    Batch* b = reinterpret_cast<Batch*>(batch);
-    if (!b) return fail(RTSIM_ERR_INVALID, "batch is NULL");
-    if (iterations < 1) iterations = 1;
-    hipStream_t st = (hipStream_t)stream;
-    // MFMA-form sensors by class (grid and PSF size); one class at a time,
-    // the others held at their current choice
-    std::map<std::tuple<uint32_t, uint32_t, uint32_t, uint32_t>, std::vector<Sensor*>> classes;
-    for (auto& en : b->entries)
-        if (en.form == RTSIM_FORM_MFMA_BF16 || en.form == RTSIM_FORM_MFMA_FP16) {
-            Sensor* s = en.sensor;
-            classes[std::make_tuple((uint32_t)s->N, (uint32_t)s->W, (uint32_t)s->Tr, (uint32_t)s->Tb)]
-                .push_back(s);
-        }
-    hipEvent_t e0, e1;
-    if (hipEventCreate(&e0) != hipSuccess) return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
-    if (hipEventCreate(&e1) != hipSuccess) {
-        (void)hipEventDestroy(e0);
-        return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
-    }
-    float bms = 0.f;
-    int rc = time_batch(b, iterations, st, e0, e1, &bms);
+    int rc = RTSIM_OK;
     for (auto it = classes.begin(); rc == RTSIM_OK && it != classes.end(); ++it) {
         std::vector<Sensor*>& ss = it->second;
         int best_shape = 0, best_waves = 4;
@@ -504,11 +642,12 @@ int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float*
         for (int v = 0; v < 2 * kCorrShapes && rc == RTSIM_OK; v++) {
             const int shape = v % kCorrShapes, waves = v < kCorrShapes ? 4 : 8;
             if (shape == best_shape && waves == best_waves) continue;
+            if (b->persistent && waves != 8) continue;   // the persistent launch runs 8-wave tiles
             float ms = 0.f;
             if (!apply(shape, waves)) continue;   // does not fit some sensor's LDS budget
             if ((rc = time_batch(b, iterations, st, e0, e1, &ms))) break;
-            if (ms < bms) {
-                bms = ms;
+            if (ms < *bms) {
+                *bms = ms;
                 best_shape = shape;
                 best_waves = waves;
             }
@@ -516,12 +655,128 @@ int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float*
         if (!apply(best_shape, best_waves) && rc == RTSIM_OK)
             rc = fail(RTSIM_ERR_HIP, "batch autotune could not restore a tile shape");
     }
+    return rc;
+}
+
+int rtsim_batch_autotune(rtsim_batch batch, int iterations, void* stream, float* best_ms)
+{
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return fail(RTSIM_ERR_INVALID, "batch is NULL");
+    if (iterations < 1) iterations = 1;
+    hipStream_t st = (hipStream_t)stream;
+    SensorClasses classes;
+    std::vector<std::pair<int, int>> tiles;   // every MFMA-form entry's (shape, waves), in entry order
+    for (auto& en : b->entries)
+        if (en.form == RTSIM_FORM_MFMA_BF16 || en.form == RTSIM_FORM_MFMA_FP16) {
+            Sensor* s = en.sensor;
+            classes[std::make_tuple((uint32_t)s->N, (uint32_t)s->W, (uint32_t)s->Tr, (uint32_t)s->Tb)]
+                .push_back(s);
+        }
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return fail(RTSIM_ERR_HIP, "hipEventCreate failed");
+    }
+    float bms = 0.f;
+    int rc = time_batch(b, iterations, st, e0, e1, &bms);
+    if (rc == RTSIM_OK) rc = tune_tiles(b, classes, iterations, st, e0, e1, &bms);
+    // the other launch mode of the MFMA-form pings, tuned the same way; the
+    // faster mode and its tiles are kept
+    if (rc == RTSIM_OK && !classes.empty()) {
+        const bool was = b->persistent;
+        auto save = [&]() {
+            tiles.clear();
+            for (auto& en : b->entries) {
+                if (en.form != RTSIM_FORM_MFMA_BF16 && en.form != RTSIM_FORM_MFMA_FP16) continue;
+                const rtsim_sensor h = reinterpret_cast<rtsim_sensor>(en.sensor);
+                int shape = 0, waves = 4, r = 0, c = 0;
+                (void)rtsim_sensor_tile_shape(h, &shape, &r, &c);
+                (void)rtsim_sensor_tile_waves(h, &waves);
+                tiles.emplace_back(shape, waves);
+            }
+        };
+        auto restore = [&]() {
+            size_t k = 0;
+            for (auto& en : b->entries) {
+                if (en.form != RTSIM_FORM_MFMA_BF16 && en.form != RTSIM_FORM_MFMA_FP16) continue;
+                const rtsim_sensor h = reinterpret_cast<rtsim_sensor>(en.sensor);
+                if (rtsim_sensor_set_tile_shape(h, tiles[k].first) || rtsim_sensor_set_tile_waves(h, tiles[k].second))
+                    return fail(RTSIM_ERR_HIP, "batch autotune could not restore a tile shape");
+                k++;
+            }
+            return (int)RTSIM_OK;
+        };
+        save();
+        float oms = 0.f;
+        if (rtsim_batch_set_persistent(batch, was ? 0 : 1) == RTSIM_OK) {
+            rc = time_batch(b, iterations, st, e0, e1, &oms);
+            if (rc == RTSIM_OK) rc = tune_tiles(b, classes, iterations, st, e0, e1, &oms);
+            if (rc == RTSIM_OK) {
+                if (oms < bms) {
+                    bms = oms;
+                } else {
+                    rc = rtsim_batch_set_persistent(batch, was ? 1 : 0);
+                    if (rc == RTSIM_OK) rc = restore();
+                }
+            }
+        }
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc) return rc;
     if (best_ms) *best_ms = bms;
-    // leave every ping consistent with the chosen shapes
+    // leave every ping consistent with the chosen mode and shapes
     return rtsim_batch_compute(batch, stream);
+}
+
+int rtsim_batch_set_persistent(rtsim_batch batch, int on)
+{
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return fail(RTSIM_ERR_INVALID, "batch is NULL");
+    if (on) {
+        // 8-wave correlation tiles for every grid-form ping: the current
+        // shape when it allows 8 waves, else the first shape that does
+        for (auto& en : b->entries) {
+            if (en.form != RTSIM_FORM_MFMA_BF16 && en.form != RTSIM_FORM_MFMA_FP16) continue;
+            const rtsim_sensor h = reinterpret_cast<rtsim_sensor>(en.sensor);
+            if (rtsim_sensor_set_tile_waves(h, 8) == RTSIM_OK) continue;
+            bool ok = false;
+            for (int shape = 0; shape < kCorrShapes && !ok; shape++)
+                ok = rtsim_sensor_set_tile_shape(h, shape) == RTSIM_OK && rtsim_sensor_set_tile_waves(h, 8) == RTSIM_OK;
+            if (!ok)
+                return fail(RTSIM_ERR_UNSUPPORTED, "no 8-wave correlation tile fits sensor %zu's LDS budget",
+                            (size_t)(&en - b->entries.data()));
+        }
+    }
+    if (b->persistent != (on != 0)) b->dirty = true;
+    b->persistent = on != 0;
+    return RTSIM_OK;
+}
+
+int rtsim_batch_persistent(rtsim_batch batch, int* on)
+{
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b || !on) return fail(RTSIM_ERR_INVALID, "batch/on is NULL");
+    *on = b->persistent ? 1 : 0;
+    return RTSIM_OK;
+}
+
+int rtsim_batch_check(rtsim_batch batch)
+{
+    Batch* b = reinterpret_cast<Batch*>(batch);
+    if (!b) return fail(RTSIM_ERR_INVALID, "batch is NULL");
+    hipError_t e = b->wait();
+    if (e != hipSuccess) return hip_fail(e, "waiting for the batch");
+    if (!b->d_ctl) return RTSIM_OK;
+    uint32_t status = 0;
+    if ((e = hipMemcpy(&status, b->d_ctl + kCtlStatus, sizeof(status), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "reading the persistent launch's status");
+    if (!status) return RTSIM_OK;
+    // a timed-out launch left its counters as they were: start the next one clean
+    if ((e = hipMemset(b->d_ctl, 0, sizeof(uint32_t) * b->ctl_cap)) != hipSuccess)
+        return hip_fail(e, "clearing the persistent control words");
+    return fail(RTSIM_ERR_HIP, "persistent launch: a task waited longer than its limit and the launch was abandoned");
 }
 
 int rtsim_batch_destroy(rtsim_batch batch)

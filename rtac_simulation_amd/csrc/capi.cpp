@@ -18,6 +18,7 @@
 
 #include "../../include/rtsim.h"
 #include "rtsim_internal.h"
+#include "persist.h"
 #include "sensor.h"
 
 namespace rtsim {
@@ -117,6 +118,7 @@ void init_kernel_attributes()
             max_lds = 64 * 1024;
         init_plain_attributes(max_lds);
         init_grid_attributes(max_lds);
+        init_persist_attributes(max_lds);
     });
 }
 
@@ -375,6 +377,7 @@ PingBuffers Sensor::own_buffers() const
     b.sorted = d_sorted;
     b.coord = d_coord;
     b.items = d_items;
+    b.items_cap = d_items ? 2 * capacity : 0;
     grid_buffers(&b);   // bucket counts and key occupancy (when the grid form is prepared)
     b.kv[0] = d_kv[0];
     b.kv[1] = d_kv[1];

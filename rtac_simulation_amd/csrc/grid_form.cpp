@@ -404,6 +404,7 @@ int Sensor::grid_plan(int form, BucketArgs* bp, DepositArgs* dp, CorrArgs* cp, c
     b.tmax = pb.tmax;
     b.vmax = g->d_vmax;
     b.items = pb.items;
+    b.items_cap = (uint32_t)std::min<size_t>(pb.items_cap, 0xFFFFFFFFu);
     b.occ = pb.occ;
     b.occ_words = g->occ_words;
     b.bear0 = g->bear0;
@@ -413,6 +414,7 @@ int Sensor::grid_plan(int form, BucketArgs* bp, DepositArgs* dp, CorrArgs* cp, c
 
     DepositArgs& d = *dp;
     d.items = pb.items;
+    d.items_cap = b.items_cap;
     d.totals = g->d_totals;
     d.NB = g->NB;
     d.R = g->R;

@@ -30,6 +30,7 @@ constexpr size_t kBucketCountBudget = (size_t)2 << 20;
 constexpr size_t kDepositLdsBudget = 64 * 1024;
 struct DepositArgs {
     const float4* items;       // {re, im, travel, grid column}, grouped by bucket
+    uint32_t items_cap;        // items the buffer holds (reads stop there)
     const uint32_t* totals;    // [NB] items per bucket
     int NB, R;                 // buckets of R grid rows
     const uint32_t* vmax;      // 1 word: the largest finite binned |re|, |im| (FP32 bits)

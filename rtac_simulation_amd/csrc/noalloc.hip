@@ -118,6 +118,7 @@ extern "C" int rtsim_sensor_compute_noalloc(rtsim_sensor h, int form, const void
     pb.counts = reinterpret_cast<uint32_t*>(base + kOffCounts);
     pb.totals = reinterpret_cast<uint32_t*>(base + kOffTotals);
     pb.items = reinterpret_cast<float4*>(base + kOffItems);
+    pb.items_cap = 2 * kMaxS;
     pb.bcounts = reinterpret_cast<uint32_t*>(base + kOffBcounts);
     pb.bcounts_cap = kBcountWords;
     pb.occ = reinterpret_cast<uint32_t*>(base + kOffOcc);

@@ -80,6 +80,7 @@ struct BucketArgs {
     uint32_t* tmax;       // [tiles] largest finite |re|, |im| of the tile's binned samples (FP32 bits)
     uint32_t* vmax;       // [1] the largest tile maximum: the deposit's fixed-point scale
     float4* items;        // {re, im, travel, grid column} of every binned sample, by bucket
+    uint32_t items_cap;   // items the buffer holds (stores beyond it are dropped)
     uint32_t* occ;        // [tiles][occ_words]: bit k set when Binner key k holds a sample of the tile
     int occ_words;        // ceil(N / 32)
     // grid column of a sample: (x - bear0) inv_db + off_b, x = sin(bearing)

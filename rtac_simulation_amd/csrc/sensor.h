@@ -49,6 +49,7 @@ struct PingBuffers {
     float4* sorted;
     float2* coord;           // bearing-axis coordinates (hi, lo)
     float4* items;           // grid form: 2 x n bucketed items
+    size_t items_cap;        // grid form: items the buffer holds
     uint32_t* bcounts;       // grid form: bucket counts
     size_t bcounts_cap;      // words
     uint32_t* occ;           // grid form: key occupancy per tile
