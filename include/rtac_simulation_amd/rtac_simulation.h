@@ -394,14 +394,24 @@ class Batch {
         check(rtsim_batch_bind_hits(h_, index, hits, n));
     }
     void compute(void* stream = nullptr) { check(rtsim_batch_compute(h_, stream)); }
-    // tile shape and wave count per class of MFMA sensors, tuned on the whole
-    // batch; returns its mean replay time in ms
+    // tile shape and wave count per class of MFMA sensors and the launch
+    // mode, tuned on the whole batch; returns its mean replay time in ms
     float autotune(int iterations = 5, void* stream = nullptr)
     {
         float ms = 0.f;
         check(rtsim_batch_autotune(h_, iterations, stream, &ms));
         return ms;
     }
+    // MFMA-form pings in one persistent launch (rtsim_batch_set_persistent)
+    void set_persistent(bool on) { check(rtsim_batch_set_persistent(h_, on ? 1 : 0)); }
+    bool persistent() const
+    {
+        int on = 0;
+        check(rtsim_batch_persistent(h_, &on));
+        return on != 0;
+    }
+    // waits for the last launch; throws if its persistent launch was abandoned
+    void check_launch() { check(rtsim_batch_check(h_)); }
 
    private:
     std::vector<SensorInstance::Ptr> sensors_;
