@@ -660,7 +660,9 @@ def _kernel_stats_md(d: Path, title, cmd, gaps=False):
         # split where the stream kernel starts again
         launches, cur = [], []
         for k in ks:
-            if "k_stream_batch" in k[2] and cur:
+            # a graph launch starts with its streaming kernel; a persistent
+            # launch is one kernel
+            if ("k_stream_batch" in k[2] or "k_persist" in k[2] or (cur and "k_persist" in cur[-1][2])) and cur:
                 launches.append(cur)
                 cur = []
             cur.append(k)
