@@ -118,6 +118,7 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int tt, int gx
     constexpr int NT = CorrGeom<TR, TC, W8>::NT;
     extern __shared__ __align__(16) uint16_t lds[];
     __shared__ float red[NW];
+    __shared__ uint32_t s_win[32 * TR];          // window sizes of the tile's output rows
     // the wave index in a scalar register: everything derived from it (tile,
     // tap subset, step range, LDS offsets) stays scalar
     const int t = threadIdx.x, lane = t & 63, wave = __builtin_amdgcn_readfirstlane(t >> 6);
@@ -161,6 +162,8 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int tt, int gx
         }
     };
     if (in_regs) load_regs();   // before the table copy
+    // the output rows' window sizes (the epilogue's mask), read under the staging
+    if (t < 32 * TR) s_win[t] = a.win[min(r0 + t, a.H - 1)].y;
     {   // tap table: a contiguous copy, uint4 at a time
         const uint4* src = reinterpret_cast<const uint4*>(a.toep);
         uint4* dst = reinterpret_cast<uint4*>(Tt);
@@ -292,15 +295,6 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int tt, int gx
         }
         f32x16 re = acc[0] + acc[1] + acc[2];
         f32x16 im = acc[3] + acc[4] + acc[5];
-        // window sizes of this lane's output rows (D rows 8 (q / 4) + 4 hh + q % 4),
-        // for the epilogue mask (loaded here: held across the MFMA loop, their
-        // registers would be missed there)
-        uint32_t wsz[16];
-#pragma unroll
-        for (int q = 0; q < 16; q++) {
-            const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
-            wsz[q] = a.win[min(rr, a.H - 1)].y;
-        }
 
         // ---- partial sums of the JS waves of a tile, through LDS (over Gt) ------
         if (JS > 1) {
@@ -336,7 +330,8 @@ __device__ __forceinline__ void correlate_body(const CorrArgs& a, int tt, int gx
             const int rr = r0 + 32 * tr + 8 * (q / 4) + 4 * hh + (q % 4);
             if (rr < a.H && cc < a.W) {
                 float2 v2 = make_float2(re[q] * inv, im[q] * inv);
-                if (wsz[q] == 0) v2 = make_float2(0.f, 0.f);
+                // window size of D row 8 (q / 4) + 4 hh + q % 4 (staged above)
+                if (s_win[32 * tr + 8 * (q / 4) + 4 * hh + (q % 4)] == 0) v2 = make_float2(0.f, 0.f);
                 a.out[(size_t)rr * a.W + cc] = v2;
             }
         }
