@@ -49,8 +49,8 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
 // sums its terms in two registers.  The samples that can reach the tile (the
 // union of its rows' windows, those whose bearing coordinate lies within the
 // PSF's bearing support of its columns) are staged in LDS one chunk at a
-// time, in window order; each block's four waves then take, in turn, the
-// staged samples that can reach the block, and every lane adds
+// time, in window order; each block's four waves then share the staged
+// samples that can reach the block (every fourth one), and every lane adds
 // v_s K(x_s - x_b, t_s - r_r) when sample s is in its row's window and inside
 // the PSF support (the reference's border mode makes every other term 0).
 // At the end the four partial sums of a cell are added in a fixed order, so
@@ -79,14 +79,14 @@ constexpr int kChunk = kTileThreads * kChunkPer;
 constexpr int kRankThreads = 1024;
 constexpr int kRankLds = 4096;                     // band lengths staged per pass
 
-static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float2)) * kChunk,
+static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float)) * kChunk,
               "partial sums reuse the staging space");
 static_assert(kChunkPer * kTileWaves <= 64, "one wave's count per lane");
 
 size_t plain2d_lds_bytes(const ConvArgs& a)
 {
     const size_t taps = a.psf_in_lds ? 2 * sizeof(float4) * (size_t)(a.Tr + 2) * (a.Tb + 2) : 0;
-    return (sizeof(float4) + sizeof(float2)) * kChunk + taps;
+    return (sizeof(float4) + sizeof(float)) * kChunk + taps;
 }
 
 __host__ __device__ inline uint32_t plain2d_bands(const ConvArgs& a) { return (uint32_t)(a.H + kTileRows - 1) / kTileRows; }
@@ -198,9 +198,9 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     float4* taps_lds = lds4;                                    // [Tr + 2][Tb + 2] cells when staged
     const int ntaps = kTapsLds ? 2 * (a.Tr + 2) * (a.Tb + 2) : 0;
     float4* cs_rec = lds4 + ntaps;                              // [kChunk] (re, im, t, index)
-    float2* cs_x = reinterpret_cast<float2*>(cs_rec + kChunk);  // [kChunk] bearing coordinate
+    float* cs_x = reinterpret_cast<float*>(cs_rec + kChunk);    // [kChunk] xs
     __shared__ uint32_t counts[kChunkPer * kTileWaves];
-    __shared__ float4 wave_box[kTileWaves];                     // (index lo, index hi, x lo, x hi)
+    __shared__ float4 wave_box[kTileWaves];                     // (index lo, index hi, xs lo, xs hi)
 
     // workgroup -> (band, tile column): XCD x = blk mod 8 runs the bands of
     // rank x, x + 8, ... in that order
@@ -224,7 +224,23 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     }
     taps += 2 * (stride + 1);                                   // cell of tap (0, 0)
 
-    // this lane's cell: its row's window and range, its column's coordinate
+    // The bearing tap coordinate of a term split into the sample's part and
+    // the lane's: with the tile's origin o_b = the coordinate of its middle
+    // column (double-float),
+    //   xB = (x_s - x_b) / pb + (Tb - 1) / 2 = xs + xc,
+    //   xs = (x_s - o_b) / pb,   xc = (Tb - 1) / 2 - (x_b - o_b) / pb.
+    // The staging computes xs once per sample and tile, each lane xc once, so
+    // a term's bearing coordinate costs one add.  |xs| and |xc| stay within
+    // the PSF's bearing span plus 8 grid columns, so their FP32 rounding is
+    // of the order of xB's own.  (The range coordinate keeps the exact
+    // difference t_s - r_r per term: a grid row can span many range taps, so
+    // a tile origin would put ys far from 0.)
+    const float2 o_b = a.bear_c[min(tc * kTileCols + kTileCols / 2, a.W - 1)];
+    const float inv_pb = a.inv_pb;
+    const float inv_pr = in_vgpr(a.inv_pr), half_r = in_vgpr(a.half_r);
+    const float Tbf = (float)a.Tb, Trf = (float)a.Tr, strf = (float)stride;
+
+    // this lane's cell: its row's window and range, its column's xc
     uint2 win = make_uint2(0u, 0u);
     float2 rc = make_float2(0.f, 0.f), bx = make_float2(0.f, 0.f);
     if (r < a.H) {
@@ -233,13 +249,15 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     }
     const bool col_ok = c < a.W;
     if (col_ok) bx = a.bear_c[c];
-    // the block: index range of its rows' windows, bearing range of its
-    // columns widened by the PSF's bearing support
-    const float supp = a.supp_b * 1.0001f + 1e-30f;
+    const float xc = __builtin_fmaf(-((bx.x - o_b.x) + (bx.y - o_b.y)), inv_pb, a.half_b);
+    // the block: index range of its rows' windows, and the xs for which a
+    // column's xB lies in (-1, Tb) (widened by 1/64 tap: a term just outside
+    // is clamped to the zero border and adds exactly 0)
+    constexpr float kBoxMargin = 1.0f / 64.0f;
     const uint32_t wlo = wave_umin(win.y ? win.x : 0xFFFFFFFFu);
     const uint32_t whi = wave_umax(win.y ? win.x + win.y : 0u);
-    const float xlo = wave_min(col_ok ? bx.x - supp : INFINITY);
-    const float xhi = wave_max(col_ok ? bx.x + supp : -INFINITY);
+    const float xlo = wave_min(col_ok ? -1.0f - xc - kBoxMargin : INFINITY);
+    const float xhi = wave_max(col_ok ? Tbf - xc + kBoxMargin : -INFINITY);
     if (lane == 0) wave_box[w] = make_float4(__uint_as_float(wlo), __uint_as_float(whi), xlo, xhi);
     __syncthreads();
     uint32_t u0 = 0xFFFFFFFFu, u1 = 0u;
@@ -253,19 +271,15 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         gxhi = fmaxf(gxhi, b.w);
     }
 
-    const float inv_pb = in_vgpr(a.inv_pb), half_b = in_vgpr(a.half_b);
-    const float inv_pr = in_vgpr(a.inv_pr), half_r = in_vgpr(a.half_r);
-    const float Tbf = (float)a.Tb, Trf = (float)a.Tr, strf = (float)stride;
     float acc_re = 0.f, acc_im = 0.f;
 
     // one term: v K(x_s - x_b, t_s - r_r) when s is in the row's window.
     // Tap coordinates are clamped to [-1, Tb] x [-1, Tr]: a clamped or
     // out-of-window term then lands on a cell of the zero border at a corner
     // (ax or ay = 0) where K is 0, so it adds exactly 0 (for finite v)
-    auto term = [&](const float4& e, const float2& ex, float scale) {
+    auto term = [&](const float4& e, float xs) {
         const uint32_t s = __float_as_uint(e.w);
-        const float dx = (ex.x - bx.x) + (ex.y - bx.y);
-        const float xB = __builtin_amdgcn_fmed3f(__builtin_fmaf(dx, inv_pb, half_b), -1.0f, Tbf);
+        const float xB = __builtin_amdgcn_fmed3f(xs + xc, -1.0f, Tbf);
         const float dr = (e.z - rc.x) - rc.y;
         const float y0 = __builtin_fmaf(dr, inv_pr, half_r);
         const float yB = __builtin_amdgcn_fmed3f(s - win.x < win.y ? y0 : -1.0f, -1.0f, Trf);
@@ -276,27 +290,29 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         const float4 ab = p[0], cd = p[1];                      // (A, B), (C, D)
         const float kr = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.z, cd.x), __builtin_fmaf(ax, ab.z, ab.x));
         const float ki = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.w, cd.y), __builtin_fmaf(ax, ab.w, ab.y));
-        const float vr = e.x * scale, vi = e.y * scale;
-        acc_re = __builtin_fmaf(vr, kr, acc_re);
-        acc_re = __builtin_fmaf(-vi, ki, acc_re);
-        acc_im = __builtin_fmaf(vr, ki, acc_im);
-        acc_im = __builtin_fmaf(vi, kr, acc_im);
+        acc_re = __builtin_fmaf(e.x, kr, acc_re);
+        acc_re = __builtin_fmaf(-e.y, ki, acc_re);
+        acc_im = __builtin_fmaf(e.x, ki, acc_im);
+        acc_im = __builtin_fmaf(e.y, kr, acc_im);
     };
 
-    uint32_t turn = (uint32_t)sub;                              // the block's waves take groups in turn
     for (uint32_t base = u0; base < u1; base += kChunk) {
         // stage the chunk's samples that can reach the tile, in window order
         float4 rec[kChunkPer];
-        float2 xc[kChunkPer];
+        float xsu[kChunkPer];
         uint64_t keep[kChunkPer];
 #pragma unroll
         for (int u = 0; u < kChunkPer; u++) {
             const uint32_t s = base + (uint32_t)(u * kTileThreads + t);
             const uint32_t sl = min(s, u1 - 1u);                // in bounds; lanes past u1 keep nothing
-            const float4 v = a.sorted[sl];
-            xc[u] = a.coord[sl];
+            const float2 x = a.coord[sl];
+            xsu[u] = ((x.x - o_b.x) + (x.y - o_b.y)) * inv_pb;
+            const bool kp = s < u1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
+            keep[u] = __ballot(kp);
+            // few samples of the window reach the tile: only theirs are loaded
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (kp) v = a.sorted[sl];
             rec[u] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
-            keep[u] = __ballot(s < u1 && xc[u].x >= gxlo && xc[u].x <= gxhi);
             if (lane == 0) counts[u * kTileWaves + w] = (uint32_t)__popcll(keep[u]);
         }
         __syncthreads();
@@ -312,36 +328,40 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
             if ((keep[u] >> lane) & 1u) {
                 const uint32_t pos = at + popc_below(keep[u]);
                 cs_rec[pos] = rec[u];
-                cs_x[pos] = xc[u];
+                cs_x[pos] = xsu[u];
             }
         }
         __syncthreads();
 
-        // this wave's share of its block: every staged group of 64 in turn,
+        // this wave's share of its block: a quarter of the relevant samples of
+        // every staged group of 64 (few samples of a chunk reach a tile, so a
+        // chunk often stages a single group: every wave takes part in it),
         // two samples at a time
-        const uint32_t groups = (n + 63) / 64;
-        for (; turn < groups; turn += kWavesPerBlock) {
-            const uint32_t g = turn * 64;
+        for (uint32_t g = 0; g < n; g += 64) {
             bool rel = false;
             if (g + lane < n) {
                 const uint32_t si = __float_as_uint(cs_rec[g + lane].w);
-                const float x = cs_x[g + lane].x;
+                const float x = cs_x[g + lane];
                 rel = si >= wlo && si < whi && x >= xlo && x <= xhi;
             }
-            uint64_t m = __ballot(rel);
-            while (m) {
+            // this wave's share: the relevant samples of rank sub, sub + 4, ...
+            const uint64_t all = __ballot(rel);
+            uint64_t m = __ballot(rel && (popc_below(all) & (kWavesPerBlock - 1)) == (uint32_t)sub);
+            while (m & (m - 1)) {                               // two or more left
                 const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1;
-                const bool two = m != 0;
-                const uint32_t j2 = two ? (uint32_t)__builtin_ctzll(m) : j1;
-                if (two) m &= m - 1;
+                const uint32_t j2 = (uint32_t)__builtin_ctzll(m);
+                m &= m - 1;
                 const float4 e1 = cs_rec[g + j1], e2 = cs_rec[g + j2];   // the same address in every lane
-                const float2 x1 = cs_x[g + j1], x2 = cs_x[g + j2];
-                term(e1, x1, 1.0f);
-                term(e2, x2, two ? 1.0f : 0.0f);
+                const float x1 = cs_x[g + j1], x2 = cs_x[g + j2];
+                term(e1, x1);
+                term(e2, x2);
+            }
+            if (m) {
+                const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
+                term(cs_rec[g + j1], cs_x[g + j1]);
             }
         }
-        turn -= groups;   // the turn carries over into the next chunk's groups
         // the next chunk's staging overwrites counts only after every wave
         // has read them (before the barrier above), and the staged samples
         // only after its first barrier, which every wave reaches after this loop
