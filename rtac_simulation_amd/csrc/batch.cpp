@@ -88,6 +88,7 @@ struct Batch {
     Stage<BucketArgs> bucket_hist, bucket_scatter;
     Stage<ScanArgs> bucket_scan;
     Stage<BinArgs> windows;                // plain pings (the grid form's are in its deposit)
+    Stage<ConvArgs> strip_count, strip_scan, strip_scatter;   // plain 2D pings' strip lists
     Stage<ConvArgs> plain_rank, plain2d, plain1d;
     Stage<DepositArgs> deposit;
     std::vector<Stage<CorrArgs>> corr;   // one per (form, tile shape, waves per workgroup)
@@ -247,6 +248,9 @@ static int build(Batch* b)
     b->bucket_scatter = {};
     b->bucket_scatter.variant = 1;   // staged unless a ping has too many buckets for it
     b->windows = {};
+    b->strip_count = {};
+    b->strip_scan = {};
+    b->strip_scatter = {};
     b->plain_rank = {};
     b->plain2d = {};
     b->plain2d.variant = 1;   // taps in LDS unless a ping's do not fit there
@@ -294,6 +298,10 @@ static int build(Batch* b)
             }
             b->windows.add(p.bins, windows_blocks(p.bins));
             if (s->kind == RTSIM_SENSOR_2D) {
+                const bool lists = p.conv.strip_ent && p.conv.n;
+                b->strip_count.add(p.conv, lists ? strip_blocks(p.conv.n) : 0u);
+                b->strip_scan.add(p.conv, lists ? plain_strips((uint32_t)p.conv.W) : 0u);
+                b->strip_scatter.add(p.conv, lists ? strip_blocks(p.conv.n) : 0u);
                 const uint32_t blocks = conv2d_plain_blocks(p.conv);
                 b->plain_rank.add(p.conv, blocks ? 1u : 0u);
                 b->plain2d.add(p.conv, blocks);
@@ -342,6 +350,9 @@ static int build(Batch* b)
     layout(b->bucket_scan, off);
     layout(b->bucket_scatter, off);
     layout(b->windows, off);
+    layout(b->strip_count, off);
+    layout(b->strip_scan, off);
+    layout(b->strip_scatter, off);
     layout(b->plain_rank, off);
     layout(b->plain2d, off);
     layout(b->plain1d, off);
@@ -368,6 +379,9 @@ static int build(Batch* b)
     pack(b->bucket_scan, host);
     pack(b->bucket_scatter, host);
     pack(b->windows, host);
+    pack(b->strip_count, host);
+    pack(b->strip_scan, host);
+    pack(b->strip_scatter, host);
     pack(b->plain_rank, host);
     pack(b->plain2d, host);
     pack(b->plain1d, host);
@@ -421,6 +435,10 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
                                        b->bucket_scatter.blocks(), b->bucket_scatter.shm,
                                        b->bucket_scatter.variant != 0, s));
     RT_TRY(launch_windows_batch(dargs(b, b->windows), dmap(b, b->windows), b->windows.blocks(), s));
+    RT_TRY(launch_strip_count_batch(dargs(b, b->strip_count), dmap(b, b->strip_count), b->strip_count.blocks(), s));
+    RT_TRY(launch_strip_scan_batch(dargs(b, b->strip_scan), dmap(b, b->strip_scan), b->strip_scan.blocks(), s));
+    RT_TRY(launch_strip_scatter_batch(dargs(b, b->strip_scatter), dmap(b, b->strip_scatter),
+                                      b->strip_scatter.blocks(), s));
     RT_TRY(launch_plain_rank_batch(dargs(b, b->plain_rank), dmap(b, b->plain_rank), b->plain_rank.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
                                      b->plain2d.shm, b->plain2d.variant != 0, s));

@@ -158,6 +158,7 @@ Sensor::~Sensor()
     dev_free(d_work);
     dev_free(d_out);
     dev_free(d_totals);
+    free_strips();
     for (auto& p : events) {
         (void)hipEventDestroy(p.start);
         (void)hipEventDestroy(p.stop);
@@ -177,7 +178,36 @@ void Sensor::free_samples()
     dev_free(d_skeys);
     dev_free(d_tmax);
     dev_free(d_counts);
+    free_strips();
     capacity = 0;
+}
+
+void Sensor::free_strips()
+{
+    dev_free(d_strip_ent);
+    dev_free(d_strip_cnt);
+    strip_cap = strip_cnt_cap = 0;
+}
+
+// The plain 2D gather's strip lists: capacity x strip_mult entries and the
+// counts of every strip and block.  An allocation failure only leaves the
+// gather scanning the windows.
+int Sensor::ensure_strips()
+{
+    if (kind != RTSIM_SENSOR_2D || strip_mult == 0 || capacity == 0) return RTSIM_OK;
+    const size_t need = capacity * (size_t)strip_mult;
+    const size_t words = strip_cnt_words(W, (uint32_t)capacity);
+    if (d_strip_ent && strip_cap >= need && strip_cnt_cap >= words) return RTSIM_OK;
+    free_strips();
+    if (dev_alloc(&d_strip_ent, need) || dev_alloc(&d_strip_cnt, words)) {
+        free_strips();
+        (void)hipGetLastError();
+    } else {
+        strip_cap = need;
+        strip_cnt_cap = words;
+    }
+    generation++;   // the gather's arguments change
+    return RTSIM_OK;
 }
 
 int Sensor::reserve(size_t count)
@@ -215,21 +245,33 @@ int Sensor::upload_psf()
     auto tap = [&](long j, long i, int c) {
         return j >= 0 && j < R && i >= 0 && i < B ? (double)psf[2 * (j * B + i) + c] : 0.0;
     };
-    std::vector<float> cells(8 * (size_t)stride * (size_t)(R + 2));
+    // two planes of float4 cells: (A, B) of every cell, then (C, D) of every cell
+    const size_t ncells = (size_t)stride * (size_t)(R + 2);
+    std::vector<float> cells(8 * ncells);
     for (long j = -1; j <= R; j++)
         for (long i = -1; i <= B; i++) {
-            float* q = &cells[8 * (size_t)((j + 1) * stride + (i + 1))];
+            float* ab = &cells[4 * (size_t)((j + 1) * stride + (i + 1))];
+            float* cd = ab + 4 * ncells;
             for (int c = 0; c < 2; c++) {
                 const double a = tap(j, i, c), b = tap(j, i + 1, c), cc = tap(j + 1, i, c),
                              d = tap(j + 1, i + 1, c);
-                q[c] = (float)a;
-                q[2 + c] = (float)(b - a);
-                q[4 + c] = (float)(cc - a);
-                q[6 + c] = (float)(d - cc - b + a);
+                ab[c] = (float)a;
+                ab[2 + c] = (float)(b - a);
+                cd[c] = (float)(cc - a);
+                cd[2 + c] = (float)(d - cc - b + a);
             }
         }
     if ((rc = dev_alloc(&d_psf_cells, cells.size() / 4))) return rc;
     HIP_TRY(hipMemcpy(d_psf_cells, cells.data(), sizeof(float) * cells.size(), hipMemcpyHostToDevice));
+    // how many strips of the plain 2D gather a sample can reach (0: no lists)
+    strip_mult = 0;
+    if (kind == RTSIM_SENSOR_2D && W > 0 && bearings.size() >= W) {
+        std::vector<double> x(W);
+        for (uint32_t b = 0; b < W; b++)
+            x[b] = axis == RTSIM_AXIS_SINE ? std::sin((double)bearings[b]) : (double)bearings[b];
+        const double pitch = (double)psf_bearing_span / (double)Tb;
+        strip_mult = strip_multiplicity(x.data(), W, 0.5 * (double)(Tb + 1) * pitch, pitch);
+    }
     return RTSIM_OK;
 }
 
@@ -389,6 +431,11 @@ PingBuffers Sensor::own_buffers() const
     b.seg = d_seg;
     b.win = d_win;
     b.work = d_work;
+    if (d_strip_ent && strip_mult) {
+        b.strip_ent = d_strip_ent;
+        b.strip_cnt = d_strip_cnt;
+        b.strip_cap = strip_cap;
+    }
     b.out = d_out;
     return b;
 }
@@ -430,6 +477,7 @@ int Sensor::plan(int form, PingPlan* p, const PingBuffers* pbp)
     int rc = RTSIM_OK;
     if (form == RTSIM_FORM_PLAIN_FP32) {
         if ((rc = check_plain_limits())) return rc;
+        if (!pbp && (rc = ensure_strips())) return rc;
     } else if (!grid) {
         return fail(RTSIM_ERR_INVALID, "unknown form %d", form);
     } else if ((rc = ensure_grid())) {   // also checks the grid form applies
@@ -484,7 +532,12 @@ int Sensor::launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hi
     hipError_t e;
     if (!p.grid) {
         e = stage(1, [&] { return launch_sort_plan(p.sort, s); });
-        if (e == hipSuccess) e = stage(2, [&] { return launch_windows(p.bins, s); });
+        if (e == hipSuccess)
+            e = stage(2, [&] {
+                hipError_t r = launch_windows(p.bins, s);
+                if (r == hipSuccess && kind == RTSIM_SENSOR_2D) r = launch_plain_strips(p.conv, s);
+                return r;
+            });
         if (e == hipSuccess)
             e = stage(4, [&] {
                 return kind == RTSIM_SENSOR_2D ? launch_conv2d_plain(p.conv, s) : launch_conv1d_plain(p.conv, s);
@@ -516,6 +569,12 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     c.psf = d_psf;
     c.psf_cells = d_psf_cells;
     c.work = pb.work;
+    c.n = pb.n;
+    const bool lists = kind == RTSIM_SENSOR_2D && pb.strip_ent && pb.strip_cap >= (size_t)pb.n * strip_mult &&
+                       strip_mult > 0;
+    c.strip_ent = lists ? pb.strip_ent : nullptr;
+    c.strip_cnt = lists ? pb.strip_cnt : nullptr;
+    c.strip_cap = lists ? pb.strip_cap : 0;
     c.out = pb.out;
     c.H = (int)N;
     c.W = (int)W;
@@ -526,8 +585,9 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     c.inv_pb = (float)((double)Tb / (double)psf_bearing_span);
     c.half_b = 0.5f * (float)(Tb - 1);
     c.supp_b = (float)(0.5 * (double)(Tb + 1) * (double)psf_bearing_span / (double)Tb);
-    // cell tables up to 32 KiB go to LDS; larger ones are read through L1/L2
-    c.psf_in_lds = 2 * sizeof(float4) * ((size_t)Tr + 2) * ((size_t)Tb + 2) <= 32 * 1024;
+    // cell planes of up to 1024 cells (16 KiB each) go to LDS; larger ones are
+    // read through L1/L2
+    c.psf_in_lds = ((size_t)Tr + 2) * ((size_t)Tb + 2) <= 1024;
     return c;
 }
 

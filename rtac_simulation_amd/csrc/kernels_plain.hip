@@ -14,6 +14,8 @@
 // pairs so that t_s - r_r and x_s - x_b keep full FP32 relative accuracy.
 #include "device_common.h"
 
+#include <algorithm>
+
 namespace rtsim {
 
 // PSF value at tap coordinates (xB along bearing taps, yB along range taps).
@@ -56,7 +58,9 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
 // At the end the four partial sums of a cell are added in a fixed order, so
 // the ping is bit-reproducible.  The taps are read as bilinear cells with a
 // zero border (A, B, C, D: K = A + ax B + ay (C + ax D) inside the cell): a
-// lookup is two 16-byte LDS reads at one address and three FMAs per
+// lookup is two 16-byte LDS reads, one from each plane of cells ((A, B) and
+// (C, D): the 8 lanes of a block row read 8 consecutive cells, 128
+// contiguous bytes, so a read has no bank conflict), and three FMAs per
 // component, and a term outside the support or the window reads only zeros,
 // so the lanes need no branch.  (Terms of a non-finite sample value
 // therefore reach every lane of the blocks it is staged for: 0 x inf is NaN,
@@ -69,6 +73,7 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
 // band's samples through its own L2), so the longest tiles start first.
 constexpr int kCellRows = 8, kCellCols = 8;        // a block
 constexpr int kTileRows = (int)kPlainBandRows, kTileCols = 16;   // 2 x 2 blocks
+static_assert(kTileCols == (int)kStripCols, "a strip is one tile column");
 static_assert(kTileRows == 2 * kCellRows, "a band is two blocks tall");
 constexpr int kTileBlocks = 4;
 constexpr int kWavesPerBlock = 4;
@@ -83,9 +88,14 @@ static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float)) 
               "partial sums reuse the staging space");
 static_assert(kChunkPer * kTileWaves <= 64, "one wave's count per lane");
 
+// Staged taps: the (A, B) plane, then the (C, D) plane kPlaneCells cells
+// further, so a lookup's two reads share one address (a fixed offset).
+// ConvArgs::psf_in_lds holds for at most kPlaneCells cells a plane.
+constexpr int kPlaneCells = 1024;
+
 size_t plain2d_lds_bytes(const ConvArgs& a)
 {
-    const size_t taps = a.psf_in_lds ? 2 * sizeof(float4) * (size_t)(a.Tr + 2) * (a.Tb + 2) : 0;
+    const size_t taps = a.psf_in_lds ? 2 * sizeof(float4) * (size_t)kPlaneCells : 0;
     return (sizeof(float4) + sizeof(float)) * kChunk + taps;
 }
 
@@ -98,20 +108,6 @@ uint32_t conv2d_plain_blocks(const ConvArgs& a)
     return 8u * per_xcd * (uint32_t)((a.W + kTileCols - 1) / kTileCols);
 }
 
-// Union of the windows of rows [r0, r0 + n) as [u0, u1) (u0 >= u1: empty).
-__device__ __forceinline__ void band_union(const ConvArgs& a, int r0, int n, uint32_t& u0, uint32_t& u1)
-{
-    u0 = 0xFFFFFFFFu;
-    u1 = 0u;
-    for (int r = r0; r < r0 + n && r < a.H; r++) {
-        const uint2 w = a.win[r];
-        if (w.y) {
-            u0 = min(u0, w.x);
-            u1 = max(u1, w.x + w.y);
-        }
-    }
-}
-
 // work[0, B): band lengths; work[B, 2B): band of each rank (longest first,
 // ties by band index).  One workgroup per ping.
 __device__ __forceinline__ void plain_rank_body(const ConvArgs& a, uint32_t* work)
@@ -119,10 +115,23 @@ __device__ __forceinline__ void plain_rank_body(const ConvArgs& a, uint32_t* wor
     __shared__ uint32_t lens[kRankLds];
     const int B = (int)plain2d_bands(a);
     const int t = threadIdx.x;
-    for (int b = t; b < B; b += kRankThreads) {
-        uint32_t u0, u1;
-        band_union(a, b * kTileRows, kTileRows, u0, u1);
-        work[b] = u1 > u0 ? u1 - u0 : 0u;
+    static_assert(64 % kTileRows == 0, "a band's rows are lanes of one wave");
+    // one row per thread; a band's union over its 16 lanes
+    for (int r0 = 0; r0 < B * kTileRows; r0 += kRankThreads) {
+        const int r = r0 + t;
+        uint32_t u0 = 0xFFFFFFFFu, u1 = 0u;
+        if (r < a.H) {
+            const uint2 w = a.win[r];
+            if (w.y) {
+                u0 = w.x;
+                u1 = w.x + w.y;
+            }
+        }
+        for (int m = 1; m < kTileRows; m <<= 1) {
+            u0 = min(u0, (uint32_t)__shfl_xor((int)u0, m, 64));
+            u1 = max(u1, (uint32_t)__shfl_xor((int)u1, m, 64));
+        }
+        if (r % kTileRows == 0 && r / kTileRows < B) work[r / kTileRows] = u1 > u0 ? u1 - u0 : 0u;
     }
     __threadfence_block();
     __syncthreads();
@@ -191,16 +200,53 @@ __device__ __forceinline__ float in_vgpr(float x)
     return v;
 }
 
+// floor(x) as an integer in one instruction (|x| < 2^31)
+__device__ __forceinline__ int floor_to_int(float x)
+{
+    int i;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
+// ---- bearing staging test ----------------------------------------------------
+// The bearing tap coordinate of a term is split into the sample's part and
+// the column's: with the origin o_b = the coordinate of the middle column of
+// the tile's strip (double-float),
+//   xB = (x_s - x_b) / pb + (Tb - 1) / 2 = xs + xc,
+//   xs = (x_s - o_b) / pb,   xc = (Tb - 1) / 2 - (x_b - o_b) / pb.
+// A sample is staged for a tile when xs lies in the strip's box: xB in
+// (-1, Tb) for one of its columns, widened by 1/64 tap (a term just outside is
+// clamped to the zero border and adds exactly 0).  The gather and the strip
+// lists evaluate these with the same expressions, so they agree bit for bit.
+constexpr float kBoxMargin = 1.0f / 64.0f;
+
+__device__ __forceinline__ float2 strip_origin(const ConvArgs& a, int j)
+{
+    return a.bear_c[min(j * (int)kStripCols + (int)kStripCols / 2, a.W - 1)];
+}
+__device__ __forceinline__ float column_xc(const ConvArgs& a, float2 o_b, float2 bx)
+{
+    return __builtin_fmaf(-((bx.x - o_b.x) + (bx.y - o_b.y)), a.inv_pb, a.half_b);
+}
+__device__ __forceinline__ float box_lo(float xc) { return -1.0f - xc - kBoxMargin; }
+__device__ __forceinline__ float box_hi(const ConvArgs& a, float xc) { return (float)a.Tb - xc + kBoxMargin; }
+__device__ __forceinline__ float sample_xs(const ConvArgs& a, float2 o_b, float2 x)
+{
+    return ((x.x - o_b.x) + (x.y - o_b.y)) * a.inv_pb;
+}
+
 template <bool kTapsLds>
 __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t blk)
 {
     extern __shared__ float4 lds4[];
-    float4* taps_lds = lds4;                                    // [Tr + 2][Tb + 2] cells when staged
-    const int ntaps = kTapsLds ? 2 * (a.Tr + 2) * (a.Tb + 2) : 0;
-    float4* cs_rec = lds4 + ntaps;                              // [kChunk] (re, im, t, index)
+    float4* taps_lds = lds4;                                    // 2 planes of kPlaneCells cells when staged
+    const int ncells = (a.Tr + 2) * (a.Tb + 2);                 // cells of a plane
+    float4* cs_rec = lds4 + (kTapsLds ? 2 * kPlaneCells : 0);   // [kChunk] (re, im, t, index)
     float* cs_x = reinterpret_cast<float*>(cs_rec + kChunk);    // [kChunk] xs
     __shared__ uint32_t counts[kChunkPer * kTileWaves];
     __shared__ float4 wave_box[kTileWaves];                     // (index lo, index hi, xs lo, xs hi)
+    __shared__ uint32_t strip_ws[kTileWaves];
+    __shared__ uint2 strip_range;
 
     // workgroup -> (band, tile column): XCD x = blk mod 8 runs the bands of
     // rank x, x + 8, ... in that order
@@ -219,26 +265,24 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     const int stride = a.Tb + 2;                                // cells per row
     const float4* taps = a.psf_cells;
     if constexpr (kTapsLds) {
-        for (int j = t; j < ntaps; j += kTileThreads) taps_lds[j] = a.psf_cells[j];
+        for (int j = t; j < ncells; j += kTileThreads) {
+            taps_lds[j] = a.psf_cells[j];
+            taps_lds[kPlaneCells + j] = a.psf_cells[ncells + j];
+        }
         taps = taps_lds;
     }
-    taps += 2 * (stride + 1);                                   // cell of tap (0, 0)
+    taps += stride + 1;                                         // cell of tap (0, 0)
+    const float4* taps_cd = taps + (kTapsLds ? kPlaneCells : ncells);   // its (C, D) plane
 
-    // The bearing tap coordinate of a term split into the sample's part and
-    // the lane's: with the tile's origin o_b = the coordinate of its middle
-    // column (double-float),
-    //   xB = (x_s - x_b) / pb + (Tb - 1) / 2 = xs + xc,
-    //   xs = (x_s - o_b) / pb,   xc = (Tb - 1) / 2 - (x_b - o_b) / pb.
-    // The staging computes xs once per sample and tile, each lane xc once, so
-    // a term's bearing coordinate costs one add.  |xs| and |xc| stay within
-    // the PSF's bearing span plus 8 grid columns, so their FP32 rounding is
-    // of the order of xB's own.  (The range coordinate keeps the exact
-    // difference t_s - r_r per term: a grid row can span many range taps, so
-    // a tile origin would put ys far from 0.)
-    const float2 o_b = a.bear_c[min(tc * kTileCols + kTileCols / 2, a.W - 1)];
-    const float inv_pb = a.inv_pb;
+    // xs is computed once per staged sample, each lane's xc once, so a term's
+    // bearing coordinate costs one add.  |xs| and |xc| stay within the PSF's
+    // bearing span plus 8 grid columns, so their FP32 rounding is of the order
+    // of xB's own.  (The range coordinate keeps the exact difference
+    // t_s - r_r per term: a grid row can span many range taps, so a tile
+    // origin would put a range offset far from 0.)
+    const float2 o_b = strip_origin(a, tc);
     const float inv_pr = in_vgpr(a.inv_pr), half_r = in_vgpr(a.half_r);
-    const float Tbf = (float)a.Tb, Trf = (float)a.Tr, strf = (float)stride;
+    const float Tbf = (float)a.Tb, Trf = (float)a.Tr;
 
     // this lane's cell: its row's window and range, its column's xc
     uint2 win = make_uint2(0u, 0u);
@@ -249,15 +293,15 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     }
     const bool col_ok = c < a.W;
     if (col_ok) bx = a.bear_c[c];
-    const float xc = __builtin_fmaf(-((bx.x - o_b.x) + (bx.y - o_b.y)), inv_pb, a.half_b);
-    // the block: index range of its rows' windows, and the xs for which a
-    // column's xB lies in (-1, Tb) (widened by 1/64 tap: a term just outside
-    // is clamped to the zero border and adds exactly 0)
-    constexpr float kBoxMargin = 1.0f / 64.0f;
+    const float xc = column_xc(a, o_b, bx);
+    // the range tap coordinate of a term: yB = (t_s - r_r) / pr + (Tr - 1) / 2
+    // = (t_s - r_hi) / pr + hr, hr = (Tr - 1) / 2 - r_lo / pr
+    const float hr = __builtin_fmaf(-rc.y, inv_pr, half_r);
+    // the block: index range of its rows' windows, and its columns' box
     const uint32_t wlo = wave_umin(win.y ? win.x : 0xFFFFFFFFu);
     const uint32_t whi = wave_umax(win.y ? win.x + win.y : 0u);
-    const float xlo = wave_min(col_ok ? -1.0f - xc - kBoxMargin : INFINITY);
-    const float xhi = wave_max(col_ok ? Tbf - xc + kBoxMargin : -INFINITY);
+    const float xlo = wave_min(col_ok ? box_lo(xc) : INFINITY);
+    const float xhi = wave_max(col_ok ? box_hi(a, xc) : -INFINITY);
     if (lane == 0) wave_box[w] = make_float4(__uint_as_float(wlo), __uint_as_float(whi), xlo, xhi);
     __syncthreads();
     uint32_t u0 = 0xFFFFFFFFu, u1 = 0u;
@@ -280,14 +324,13 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     auto term = [&](const float4& e, float xs) {
         const uint32_t s = __float_as_uint(e.w);
         const float xB = __builtin_amdgcn_fmed3f(xs + xc, -1.0f, Tbf);
-        const float dr = (e.z - rc.x) - rc.y;
-        const float y0 = __builtin_fmaf(dr, inv_pr, half_r);
+        const float y0 = __builtin_fmaf(e.z - rc.x, inv_pr, hr);
         const float yB = __builtin_amdgcn_fmed3f(s - win.x < win.y ? y0 : -1.0f, -1.0f, Trf);
-        const float fi = floorf(xB), fj = floorf(yB);
-        const float ax = xB - fi, ay = yB - fj;
-        // cell index fj stride + fi: small integers, exact in FP32
-        const float4* p = taps + 2 * (int)__builtin_fmaf(fj, strf, fi);
-        const float4 ab = p[0], cd = p[1];                      // (A, B), (C, D)
+        // the cell (floor(yB), floor(xB)) and the weights xB - floor(xB),
+        // yB - floor(yB) (fract; its result stays below 1)
+        const float ax = __builtin_amdgcn_fractf(xB), ay = __builtin_amdgcn_fractf(yB);
+        const int cell = __mul24(floor_to_int(yB), stride) + floor_to_int(xB);
+        const float4 ab = taps[cell], cd = taps_cd[cell];       // (A, B), (C, D)
         const float kr = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.z, cd.x), __builtin_fmaf(ax, ab.z, ab.x));
         const float ki = __builtin_fmaf(ay, __builtin_fmaf(ax, cd.w, cd.y), __builtin_fmaf(ax, ab.w, ab.y));
         acc_re = __builtin_fmaf(e.x, kr, acc_re);
@@ -296,23 +339,58 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         acc_im = __builtin_fmaf(e.y, kr, acc_im);
     };
 
-    for (uint32_t base = u0; base < u1; base += kChunk) {
+    // The staging source: the tile's strip list entries of its window when
+    // the lists are complete, else every sample of the window.  Both give the
+    // same staged samples in the same order.
+    uint32_t p0 = u0, p1 = u1;
+    bool from_list = false;
+    if (a.strip_ent != nullptr && u0 < u1) {
+        const uint32_t ns = plain_strips((uint32_t)a.W), nb = strip_blocks(a.n);
+        const uint32_t tv = (uint32_t)t < ns ? a.strip_cnt[t] : 0u;   // strip totals
+        uint32_t total;
+        const uint32_t first = sort_block_scan<kTileThreads>(tv, strip_ws, &total);
+        if (t == tc) {
+            const uint32_t* offs = a.strip_cnt + ns + (size_t)tc * nb;
+            const uint32_t blo = u0 / kStripBlock, bhi = (u1 - 1u) / kStripBlock;
+            strip_range = make_uint2(first + offs[blo], first + (bhi + 1 < nb ? offs[bhi + 1] : tv));
+        }
+        __syncthreads();
+        if (total <= a.strip_cap) {
+            from_list = true;
+            p0 = strip_range.x;
+            p1 = strip_range.y;
+        }
+    }
+
+    // relevant samples of the block so far: the k-th goes to wave k mod 4, so
+    // every lane adds the same terms in the same order however they are staged
+    uint32_t ranked = 0;
+    for (uint32_t base = p0; base < p1; base += kChunk) {
         // stage the chunk's samples that can reach the tile, in window order
         float4 rec[kChunkPer];
         float xsu[kChunkPer];
         uint64_t keep[kChunkPer];
 #pragma unroll
         for (int u = 0; u < kChunkPer; u++) {
-            const uint32_t s = base + (uint32_t)(u * kTileThreads + t);
-            const uint32_t sl = min(s, u1 - 1u);                // in bounds; lanes past u1 keep nothing
-            const float2 x = a.coord[sl];
-            xsu[u] = ((x.x - o_b.x) + (x.y - o_b.y)) * inv_pb;
-            const bool kp = s < u1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
-            keep[u] = __ballot(kp);
-            // few samples of the window reach the tile: only theirs are loaded
+            const uint32_t q = base + (uint32_t)(u * kTileThreads + t);
+            const uint32_t ql = min(q, p1 - 1u);                // in bounds; lanes past p1 keep nothing
             float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (kp) v = a.sorted[sl];
-            rec[u] = make_float4(v.x, v.y, v.z, __uint_as_float(s));
+            bool kp;
+            if (from_list) {
+                const uint2 en = a.strip_ent[ql];
+                xsu[u] = __uint_as_float(en.y);
+                kp = q < p1 && en.x >= u0 && en.x < u1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
+                if (kp) v = a.sorted[en.x];
+                v.w = __uint_as_float(en.x);
+            } else {
+                xsu[u] = sample_xs(a, o_b, a.coord[ql]);
+                kp = q < p1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
+                // few samples of the window reach the tile: only theirs are loaded
+                if (kp) v = a.sorted[ql];
+                v.w = __uint_as_float(q);
+            }
+            keep[u] = __ballot(kp);
+            rec[u] = v;
             if (lane == 0) counts[u * kTileWaves + w] = (uint32_t)__popcll(keep[u]);
         }
         __syncthreads();
@@ -334,9 +412,8 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         __syncthreads();
 
         // this wave's share of its block: a quarter of the relevant samples of
-        // every staged group of 64 (few samples of a chunk reach a tile, so a
-        // chunk often stages a single group: every wave takes part in it),
-        // two samples at a time
+        // every staged group of 64 (a chunk of the window often stages a
+        // single group: every wave takes part in it), two samples at a time
         for (uint32_t g = 0; g < n; g += 64) {
             bool rel = false;
             if (g + lane < n) {
@@ -346,7 +423,8 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
             }
             // this wave's share: the relevant samples of rank sub, sub + 4, ...
             const uint64_t all = __ballot(rel);
-            uint64_t m = __ballot(rel && (popc_below(all) & (kWavesPerBlock - 1)) == (uint32_t)sub);
+            uint64_t m = __ballot(rel && ((ranked + popc_below(all)) & (kWavesPerBlock - 1)) == (uint32_t)sub);
+            ranked += (uint32_t)__popcll(all);
             while (m & (m - 1)) {                               // two or more left
                 const uint32_t j1 = (uint32_t)__builtin_ctzll(m);
                 m &= m - 1;
@@ -436,6 +514,240 @@ void init_plain_attributes(int max_lds)
     set_dynamic_lds_limit((const void*)k_conv2d_plain<false>, max_lds);
     set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<true>, max_lds);
     set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<false>, max_lds);
+}
+
+// ---- strip lists ---------------------------------------------------------------
+// The tiles of one strip (one tile column) keep the samples whose xs lies in
+// the strip's box, about (16 + PSF bearing support) / W of their band's
+// window.  The lists give each strip those samples once per ping, in window
+// order as (sample index, xs), so a tile stages only them instead of
+// scanning the whole window: counts per block of kStripBlock sorted samples and strip, an
+// exclusive scan of each strip's counts (its totals at the front), then the
+// scatter of the entries.  Entry order is sample order (the scatter ranks
+// samples by ballots, not atomics), so the lists are deterministic.
+constexpr int kStripThreads = 1024;
+static_assert(kStripBlock == (uint32_t)kStripThreads, "one sample per thread");
+static_assert(kMaxStrips <= (uint32_t)kStripThreads && kMaxStrips <= (uint32_t)kTileThreads,
+              "one thread per strip");
+static_assert(kMaxStrips <= 1024, "a strip index fits 10 bits of a pair");
+
+// every strip's (origin hi, origin lo, xs lo, xs hi), as the gather's tiles
+// compute theirs (the minimum and maximum over the same 16 columns)
+__device__ __forceinline__ void strip_boxes(const ConvArgs& a, uint32_t ns, float4* box)
+{
+    const uint32_t t = threadIdx.x;
+    for (uint32_t c0 = 0; c0 < ns * kStripCols; c0 += kStripThreads) {
+        const uint32_t c = c0 + t, j = c / kStripCols;
+        float lo = INFINITY, hi = -INFINITY;
+        float2 o = make_float2(0.f, 0.f);
+        if (j < ns) {
+            o = strip_origin(a, (int)j);
+            if (c < (uint32_t)a.W) {
+                const float xc = column_xc(a, o, a.bear_c[c]);
+                lo = box_lo(xc);
+                hi = box_hi(a, xc);
+            }
+        }
+        for (int m = 1; m < (int)kStripCols; m <<= 1) {   // the strip's 16 lanes
+            lo = fminf(lo, __shfl_xor(lo, m, 64));
+            hi = fmaxf(hi, __shfl_xor(hi, m, 64));
+        }
+        if (j < ns && c % kStripCols == 0) box[j] = make_float4(o.x, o.y, lo, hi);
+    }
+}
+
+__device__ __forceinline__ bool in_strip(const ConvArgs& a, const float4& bx, float2 x, float& xs)
+{
+    xs = sample_xs(a, make_float2(bx.x, bx.y), x);
+    return xs >= bx.z && xs <= bx.w;
+}
+
+// the strips of the block's samples, four at a time (their boxes read from
+// LDS together): fn(j, member mask of the wave, xs) for every strip j
+template <typename Fn>
+__device__ __forceinline__ void for_each_strip(const ConvArgs& a, const float4* box, uint32_t ns, bool ok,
+                                               float2 x, Fn&& fn)
+{
+    for (uint32_t j0 = 0; j0 < ns; j0 += 4) {
+        float4 bx[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) bx[q] = box[min(j0 + q, ns - 1)];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            if (j0 + q >= ns) break;
+            float xs;
+            const bool in = ok && in_strip(a, bx[q], x, xs);
+            fn(j0 + q, in, __ballot(in), xs);
+        }
+    }
+}
+
+// block b: counts[j][b] = its samples in strip j
+__device__ __forceinline__ void strip_count_body(const ConvArgs& a, uint32_t b)
+{
+    constexpr int NW = kStripThreads / 64;
+    __shared__ float4 box[kMaxStrips];
+    __shared__ uint32_t wcnt[NW][kMaxStrips];
+    const uint32_t ns = plain_strips((uint32_t)a.W), nb = strip_blocks(a.n);
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t i = b * kStripBlock + t;
+    const bool ok = i < a.n;
+    const float2 x = a.coord[min(i, a.n - 1u)];
+    strip_boxes(a, ns, box);
+    __syncthreads();
+    for_each_strip(a, box, ns, ok, x, [&](uint32_t j, bool, uint64_t m, float) {
+        if (lane == 0) wcnt[w][j] = (uint32_t)__popcll(m);
+    });
+    __syncthreads();
+    if (t < ns) {
+        uint32_t c = 0;
+        for (int q = 0; q < NW; q++) c += wcnt[q][t];
+        a.strip_cnt[ns + (size_t)t * nb + b] = c;
+    }
+}
+
+// strip j: its counts become offsets within its list, its total goes first
+__device__ __forceinline__ void strip_scan_body(const ConvArgs& a, uint32_t j)
+{
+    __shared__ uint32_t ws[kStripThreads / 64];
+    const uint32_t ns = plain_strips((uint32_t)a.W), nb = strip_blocks(a.n);
+    uint32_t* row = a.strip_cnt + ns + (size_t)j * nb;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < nb; b0 += kStripThreads) {
+        const uint32_t b = b0 + threadIdx.x;
+        const uint32_t v = b < nb ? row[b] : 0u;
+        uint32_t total;
+        const uint32_t ex = sort_block_scan<kStripThreads>(v, ws, &total);
+        if (b < nb) row[b] = carry + ex;
+        carry += total;
+        __syncthreads();   // ws is reused by the next round
+    }
+    if (threadIdx.x == 0) a.strip_cnt[j] = carry;
+}
+
+// block b: its entries of every strip, at the strip's base + the block's
+// offset, in sample order (waves in order, lanes in order).  The membership
+// test runs once: each lane keeps its (strip, rank in its wave) pairs in LDS
+// (a sample reaches at most kMaxStripMult strips) and writes them once the
+// waves' positions are known.
+constexpr int kMaxStripMult = 8;
+
+__device__ __forceinline__ void strip_scatter_body(const ConvArgs& a, uint32_t b)
+{
+    constexpr int NW = kStripThreads / 64;
+    __shared__ float4 box[kMaxStrips];
+    __shared__ uint32_t wpos[NW][kMaxStrips];   // per wave: count, then first entry
+    __shared__ uint16_t mine[kMaxStripMult][kStripThreads];   // strip << 6 | rank
+    __shared__ uint32_t ws[NW];
+    const uint32_t ns = plain_strips((uint32_t)a.W), nb = strip_blocks(a.n);
+    const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const uint32_t i = b * kStripBlock + t;
+    const bool ok = i < a.n;
+    const float2 x = a.coord[min(i, a.n - 1u)];
+    const uint32_t tv = t < ns ? a.strip_cnt[t] : 0u;
+    uint32_t total;
+    const uint32_t first = sort_block_scan<kStripThreads>(tv, ws, &total);
+    if (total > a.strip_cap) return;   // incomplete lists: the gather scans the windows
+    strip_boxes(a, ns, box);
+    __syncthreads();
+    uint32_t k = 0;
+    for_each_strip(a, box, ns, ok, x, [&](uint32_t j, bool in, uint64_t m, float) {
+        if (lane == 0) wpos[w][j] = (uint32_t)__popcll(m);
+        if (in && k < (uint32_t)kMaxStripMult) mine[k++][t] = (uint16_t)(j << 6 | popc_below(m));
+    });
+    __syncthreads();
+    if (t < ns) {
+        uint32_t r = first + a.strip_cnt[ns + (size_t)t * nb + b];
+        for (int q = 0; q < NW; q++) {
+            const uint32_t c = wpos[q][t];
+            wpos[q][t] = r;
+            r += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t q = 0; q < k; q++) {
+        const uint32_t v = mine[q][t], j = v >> 6;
+        float xs;
+        (void)in_strip(a, box[j], x, xs);
+        a.strip_ent[wpos[w][j] + (v & 63u)] = make_uint2(i, __float_as_uint(xs));
+    }
+}
+
+__global__ __launch_bounds__(kStripThreads) void k_strip_count(ConvArgs a) { strip_count_body(a, blockIdx.x); }
+__global__ __launch_bounds__(kStripThreads) void k_strip_scan(ConvArgs a) { strip_scan_body(a, blockIdx.x); }
+__global__ __launch_bounds__(kStripThreads) void k_strip_scatter(ConvArgs a) { strip_scatter_body(a, blockIdx.x); }
+
+__global__ __launch_bounds__(kStripThreads) void k_strip_count_batch(const ConvArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    const ConvArgs a = args[p];
+    strip_count_body(a, blk);
+}
+__global__ __launch_bounds__(kStripThreads) void k_strip_scan_batch(const ConvArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    const ConvArgs a = args[p];
+    strip_scan_body(a, blk);
+}
+__global__ __launch_bounds__(kStripThreads) void k_strip_scatter_batch(const ConvArgs* __restrict__ args, BatchMap m)
+{
+    uint32_t blk, nblk;
+    const uint32_t p = batch_locate(m, blk, nblk);
+    const ConvArgs a = args[p];
+    strip_scatter_body(a, blk);
+}
+
+hipError_t launch_plain_strips(const ConvArgs& a, hipStream_t s)
+{
+    if (!a.strip_ent || a.n == 0 || a.W <= 0) return hipSuccess;
+    const uint32_t nb = strip_blocks(a.n), ns = plain_strips((uint32_t)a.W);
+    hipLaunchKernelGGL(k_strip_count, dim3(nb), dim3(kStripThreads), 0, s, a);
+    hipLaunchKernelGGL(k_strip_scan, dim3(ns), dim3(kStripThreads), 0, s, a);
+    hipLaunchKernelGGL(k_strip_scatter, dim3(nb), dim3(kStripThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_strip_count_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_strip_count_batch, dim3(blocks), dim3(kStripThreads), 0, s, args, m);
+    return hipGetLastError();
+}
+hipError_t launch_strip_scan_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_strip_scan_batch, dim3(blocks), dim3(kStripThreads), 0, s, args, m);
+    return hipGetLastError();
+}
+hipError_t launch_strip_scatter_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s)
+{
+    if (!blocks) return hipSuccess;
+    hipLaunchKernelGGL(k_strip_scatter_batch, dim3(blocks), dim3(kStripThreads), 0, s, args, m);
+    return hipGetLastError();
+}
+
+// Strip boxes in the bearing coordinate x: [x of the first column - supp,
+// x of the last + supp], widened by 2 tap pitches (more than the tiles'
+// 1/64 tap and FP32 rounding), so the count bounds what the device lists hold.
+uint32_t strip_multiplicity(const double* x, uint32_t W, double supp, double pitch)
+{
+    const uint32_t ns = plain_strips(W);
+    if (W == 0 || ns > kMaxStrips) return 0u;
+    for (uint32_t c = 1; c < W; c++)
+        if (!(x[c] > x[c - 1])) return 0u;   // the boxes are ordered only for ascending columns
+    const double wide = supp + 2.0 * pitch;
+    uint32_t mult = 0;
+    for (uint32_t j = 0, k = 0; j < ns; j++) {
+        // the point x_hi(j): boxes j .. k - 1 contain it (boxes start in order)
+        const double hi = x[std::min(W, (j + 1) * kStripCols) - 1] + wide;
+        k = std::max(k, j);
+        while (k < ns && x[k * kStripCols] - wide <= hi) k++;
+        mult = std::max(mult, k - j);
+    }
+    // lists only where they save most of the staging, at a bounded size
+    return mult <= (uint32_t)kMaxStripMult && 2 * mult <= ns ? mult : 0u;
 }
 
 // 1D: one wave per output range bin, FP32 partial sums reduced across the

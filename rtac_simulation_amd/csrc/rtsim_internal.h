@@ -230,8 +230,9 @@ struct ConvArgs {
     const float2* range_c;   // per output row: (hi, lo) of the range in metres
     const float2* bear_c;    // per output bearing: (hi, lo) of the axis coordinate
     const float2* psf;       // [Tr][Tb] complex
-    const float4* psf_cells; // 2D: bilinear cells [Tr + 2][Tb + 2] x 2 float4, j, i from -1:
-                             // (A, B), (C, D) with K(j + ay, i + ax) = A + ax B + ay (C + ax D),
+    const float4* psf_cells; // 2D: bilinear cells [Tr + 2][Tb + 2], j, i from -1, in two planes:
+                             // (A, B) of every cell, then (C, D) of every cell, with
+                             // K(j + ay, i + ax) = A + ax B + ay (C + ax D),
                              // taps T zero outside [0, Tr) x [0, Tb)
     float2* out;             // [H][W]
     int H, W, Tr, Tb;
@@ -240,8 +241,29 @@ struct ConvArgs {
     float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
     int psf_in_lds;          // 2D: stage the cells in LDS (else read them from global)
     uint32_t* work;          // 2D: band lengths and band order (plain2d_work_words)
+    // 2D strip lists (null: every tile stages its band's whole window):
+    // strip j's list = the sorted samples its tiles keep, in window order, as
+    // (sample index, bearing tap offset xs as bits)
+    uint32_t n;              // sorted samples
+    uint2* strip_ent;
+    uint32_t* strip_cnt;     // [strips] totals, then [strips][strip_blocks(n)] counts / offsets
+    size_t strip_cap;        // entries strip_ent holds
 };
 size_t plain2d_lds_bytes(const ConvArgs& a);
+// strip lists: a strip is the 16 bearing columns of one tile column; the
+// lists are built per block of kStripBlock sorted samples
+constexpr uint32_t kStripCols = 16, kStripBlock = 1024, kMaxStrips = 512;
+constexpr uint32_t plain_strips(uint32_t W) { return (W + kStripCols - 1) / kStripCols; }
+constexpr uint32_t strip_blocks(uint32_t n) { return (n + kStripBlock - 1) / kStripBlock; }
+constexpr size_t strip_cnt_words(uint32_t W, uint32_t n) { return (size_t)plain_strips(W) * (1 + strip_blocks(n)); }
+// the strips a sample can reach at most, for bearing-axis coordinates x
+// (ascending) and a PSF bearing support of `supp` (0: no lists: the strips
+// are too many, or a sample reaches too many of them to save staging)
+uint32_t strip_multiplicity(const double* x, uint32_t W, double supp, double pitch);
+hipError_t launch_plain_strips(const ConvArgs& a, hipStream_t s);
+hipError_t launch_strip_count_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
+hipError_t launch_strip_scan_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
+hipError_t launch_strip_scatter_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 // the 2D gather's bands of output rows, and its work words for `rows` rows
 constexpr uint32_t kPlainBandRows = 16;
 constexpr size_t plain2d_work_words(uint32_t rows) { return 2 * (((size_t)rows + kPlainBandRows - 1) / kPlainBandRows); }

@@ -63,6 +63,9 @@ struct PingBuffers {
     uint2* seg;              // [N]
     uint2* win;              // [N]
     uint32_t* work;          // plain 2D gather: band lengths and order
+    uint2* strip_ent;        // plain 2D gather: strip lists (null: none)
+    uint32_t* strip_cnt;
+    size_t strip_cap;        // entries
     float2* out;             // [N][W]
 };
 
@@ -121,6 +124,12 @@ struct Sensor {
     uint2* d_seg = nullptr;       // Binner::binIndexes_
     uint2* d_win = nullptr;       // bins_ as (start, size)
     uint32_t* d_work = nullptr;   // plain 2D gather: band lengths and order
+    // plain 2D gather: strip lists (ConvArgs::strip_*), sized for `capacity`
+    // samples reaching at most strip_mult strips each (0: no lists)
+    uint32_t strip_mult = 0;
+    uint2* d_strip_ent = nullptr;
+    uint32_t* d_strip_cnt = nullptr;
+    size_t strip_cap = 0, strip_cnt_cap = 0;
     float2* d_out = nullptr;      // ping [N][W]
     float2* d_psf = nullptr;
     float4* d_psf_cells = nullptr;  // plain 2D gather: bilinear cells of the taps (ConvArgs::psf_cells)
@@ -153,6 +162,8 @@ struct Sensor {
     int apply_waveform();           // recipe -> taps, spans (no-op without a recipe)
     int upload_psf();
     int check_plain_limits() const;
+    void free_strips();
+    int ensure_strips();             // strip lists for `capacity` samples (when they apply)
     int prepare(int form);          // form-specific state, no device work queued
     int ensure_grid();
     bool grid_ready() const;         // grid form prepared (no preparation is started)
