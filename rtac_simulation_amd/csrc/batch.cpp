@@ -83,7 +83,7 @@ struct Batch {
     // plain pings: the sort by travel, per pass
     Stage<HistArgs> hist[4];
     Stage<ScanArgs> scan[4];
-    Stage<ScatterArgs> scatter[4];
+    Stage<ScatterArgs> scatter[4], scatter_last[4];   // a ping's last pass: scatter_last
     // grid pings: the bucketing (counts, their scan, the scatter)
     Stage<BucketArgs> bucket_hist, bucket_scatter;
     Stage<ScanArgs> bucket_scan;
@@ -242,6 +242,7 @@ static int build(Batch* b)
         b->hist[k] = {};
         b->scan[k] = {};
         b->scatter[k] = {};
+        b->scatter_last[k] = {};
     }
     b->bucket_hist = {};
     b->bucket_scan = {};
@@ -294,7 +295,8 @@ static int build(Batch* b)
             for (int k = 0; k < p.sort.passes; k++) {
                 b->hist[k].add(p.sort.hist[k], p.sort.hist_blocks[k]);
                 b->scan[k].add(p.sort.scan[k], p.sort.pass_blocks[k] ? p.sort.hist[k].mask + 1 : 0u);
-                b->scatter[k].add(p.sort.scatter[k], p.sort.pass_blocks[k]);
+                (k == p.sort.passes - 1 ? b->scatter_last[k] : b->scatter[k])
+                    .add(p.sort.scatter[k], p.sort.pass_blocks[k]);
             }
             b->windows.add(p.bins, windows_blocks(p.bins));
             if (s->kind == RTSIM_SENSOR_2D) {
@@ -345,6 +347,7 @@ static int build(Batch* b)
         layout(b->hist[k], off);
         layout(b->scan[k], off);
         layout(b->scatter[k], off);
+        layout(b->scatter_last[k], off);
     }
     layout(b->bucket_hist, off);
     layout(b->bucket_scan, off);
@@ -374,6 +377,7 @@ static int build(Batch* b)
         pack(b->hist[k], host);
         pack(b->scan[k], host);
         pack(b->scatter[k], host);
+        pack(b->scatter_last[k], host);
     }
     pack(b->bucket_hist, host);
     pack(b->bucket_scan, host);
@@ -425,8 +429,10 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
         RT_TRY(launch_sort_hist_batch(k == 0, dargs(b, b->hist[k]), dmap(b, b->hist[k]), b->hist[k].blocks(),
                                       s));
         RT_TRY(launch_sort_scan_batch(dargs(b, b->scan[k]), dmap(b, b->scan[k]), b->scan[k].blocks(), s));
-        RT_TRY(launch_sort_scatter_batch(k == 0, k == 3, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
+        RT_TRY(launch_sort_scatter_batch(k == 0, false, dargs(b, b->scatter[k]), dmap(b, b->scatter[k]),
                                          b->scatter[k].blocks(), s));
+        RT_TRY(launch_sort_scatter_batch(k == 0, true, dargs(b, b->scatter_last[k]), dmap(b, b->scatter_last[k]),
+                                         b->scatter_last[k].blocks(), s));
     }
     RT_TRY(launch_bucket_hist_batch(dargs(b, b->bucket_hist), dmap(b, b->bucket_hist), b->bucket_hist.blocks(),
                                     b->bucket_hist.shm, s));

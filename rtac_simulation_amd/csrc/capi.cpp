@@ -461,6 +461,7 @@ static int plan_sort(Sensor* s, const PingBuffers& pb, bool introspect, PingPlan
     sb.out.key = s->key;
     sb.out.axis = s->axis;
     sb.out.seg = pb.seg;
+    sb.by_key = !introspect;   // the travel order only for the Binner introspection
     hipError_t e = make_sort_plan(pb.samples, pb.n, sb, &p->sort);
     if (e != hipSuccess) return hip_fail(e, "sort plan");
     BinArgs& ba = p->bins;

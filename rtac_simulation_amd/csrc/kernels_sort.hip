@@ -1,6 +1,10 @@
 // Stable sort of the receiver samples by travel (Receiver.cu:9-42): an LSD
 // radix sort of (travel image, sample index) in four 8-bit passes over tiles
-// of 1024 items (one workgroup of 256 lanes x 4 items per tile).  Per pass:
+// of 1024 items (one workgroup of 256 lanes x 4 items per tile).  A compute
+// that needs only the Binner segments and windows sorts by the Binner key
+// instead (out-of-range samples after every bin): the same segments and
+// windows, since the key is a non-decreasing function of travel, in one or
+// two passes (samples of one bin stay in input order).  Per pass:
 //
 //   k_sort_hist    : per-tile digit counts -> counts[digit][tile]
 //                    (pass 0 reads the records themselves)
@@ -32,17 +36,26 @@ constexpr int kSortThreads = 256;
 constexpr int kSortWaves = kSortThreads / 64;                  // 4
 constexpr int kSortItems = (int)kSortTileSmall / kSortThreads;  // 4 items per thread
 
+// the sort key of a record: its travel image, or its Binner key with the
+// out-of-range ones after every bin (k.n)
+__device__ __forceinline__ uint32_t sort_key(float t, bool by_key, const KeyParams& k)
+{
+    if (!by_key) return travel_image(t);
+    const uint32_t b = bin_key(k, t);
+    return b == kOutOfRange ? k.n : b;
+}
+
 template <bool FIRST>
 __device__ __forceinline__ void load_tile(const float4* __restrict__ rec,
                                           const uint2* __restrict__ kvin, uint32_t n,
                                           uint32_t wbase, uint32_t lane, uint32_t* key,
-                                          uint32_t* val)
+                                          uint32_t* val, bool by_key, const KeyParams& kp)
 {
 #pragma unroll
     for (int j = 0; j < kSortItems; j++) {
         const uint32_t i = min(wbase + j * 64 + lane, n - 1);
         if (FIRST) {
-            key[j] = travel_image(rec[i].z);
+            key[j] = sort_key(rec[i].z, by_key, kp);
             if (val) val[j] = i;
         } else {
             const uint2 kv = kvin[i];
@@ -65,7 +78,7 @@ __device__ __forceinline__ void sort_hist_body(const HistArgs& a, uint32_t blk, 
     if (a.n == 0 || blk >= a.tiles) return;
     const uint32_t wbase = blk * kSortTileSmall + w * (64 * kSortItems);
     uint32_t key[kSortItems];
-    load_tile<FIRST>(a.rec, a.kvin, a.n, wbase, lane, key, nullptr);
+    load_tile<FIRST>(a.rec, a.kvin, a.n, wbase, lane, key, nullptr, a.by_key != 0, a.key);
     __syncthreads();
 #pragma unroll
     for (int j = 0; j < kSortItems; j++)
@@ -181,7 +194,7 @@ __device__ __forceinline__ void sort_scatter_body(const ScatterArgs& a, uint32_t
 
     const uint32_t wbase = tb + w * (64 * LI);
     uint32_t key[LI], val[LI], lrank[LI];
-    load_tile<FIRST>(a.rec, a.kvin, a.n, wbase, lane, key, val);
+    load_tile<FIRST>(a.rec, a.kvin, a.n, wbase, lane, key, val, a.by_key != 0, a.out.key);
     __syncthreads();
 
     // stable ranks inside each wave
@@ -346,7 +359,9 @@ __global__ __launch_bounds__(256) void k_sort_scatter_batch(const ScatterArgs* _
 
 hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& b, SortPlan* p)
 {
-    constexpr int passes = 4;
+    // by Binner key: keys 0 .. N (N: out of range) in 8-bit digits
+    const bool by_key = b.by_key && b.out.key.n < 65535u;
+    const int passes = !by_key ? 4 : b.out.key.n < 255u ? 1 : 2;
     p->passes = passes;
     const uint32_t tiles = (n + kSortTileSmall - 1) / kSortTileSmall;
     if (tiles > b.tiles_cap || tiles > 256u * kScanPer) return hipErrorInvalidValue;
@@ -365,6 +380,8 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         h.tiles = tiles;
         h.seg_init = pass == 0 ? b.seg_init : nullptr;
         h.seg_count = pass == 0 ? b.seg_count : 0;
+        h.by_key = by_key;
+        h.key = b.out.key;
         p->scan[pass] = ScanArgs{b.counts, tiles, b.totals, nullptr, nullptr};
         ScatterArgs& c = p->scatter[pass];
         c.rec = records;
@@ -378,6 +395,7 @@ hipError_t make_sort_plan(const float4* records, uint32_t n, const SortBuffers& 
         c.totals = b.totals;
         c.tiles = tiles;
         c.out = b.out;
+        c.by_key = by_key;
         cur ^= 1;
     }
     p->order = b.order;

@@ -147,6 +147,10 @@ struct SortBuffers {
     uint2* seg_init;      // seg_init[0..seg_count) = (~0, 0) (Binner segments, min / max),
     size_t seg_count;     // prepared by the first histogram
     SortOutputs out;
+    // sort by Binner key (out-of-range last) instead of by travel: the same
+    // segments and windows in 1 or 2 passes instead of 4 (input order inside
+    // a bin); used when nothing asks for the travel order
+    bool by_key;
 };
 // One pass of the sort: per-tile digit counts, their scan, the scatter.
 struct HistArgs {
@@ -159,6 +163,8 @@ struct HistArgs {
     uint32_t tiles;
     uint2* seg_init;          // pass 0 only
     size_t seg_count;
+    int by_key;               // pass 0: the sort key is the Binner key (key.n: out of range)
+    KeyParams key;
 };
 // Exclusive scan of rows of tile counts: one workgroup per row (a digit, or a
 // bucket of the grid form), the row total in totals[row]; with tmax, row 0's
@@ -181,11 +187,12 @@ struct ScatterArgs {
     const uint32_t* counts;   // scanned [digit][tile]
     const uint32_t* totals;
     uint32_t tiles;
-    SortOutputs out;          // last pass only
+    SortOutputs out;          // last pass only (out.key: the Binner key of every pass 0 sort key)
+    int by_key;               // pass 0: as HistArgs::by_key
 };
 // Arguments and workgroup counts of the 3 launches per pass of one sort.
 struct SortPlan {
-    int passes;               // 4
+    int passes;               // 4 by travel, 1 or 2 by Binner key
     HistArgs hist[4];
     ScanArgs scan[4];
     ScatterArgs scatter[4];
