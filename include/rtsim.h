@@ -177,7 +177,10 @@ int rtsim_receiver_write_samples(rtsim_sensor h, const rtsim_hits* hits, size_t 
 
 /* SensorInstance::compute_output (SensorInstance.h:67; 2D: fill_ping
  * SensorInstance2D.h:84-93; 1D: reduce_samples SensorInstance1D.h:63-69).
- * Plain form: sort by travel -> Binner::compute -> the PSF gather in FP32.
+ * Plain form: sort -> Binner::compute -> the PSF gather in FP32 (the sort
+ * groups the samples by Binner key, input order inside a bin: the segments
+ * and windows of the sort by travel; rtsim_sensor_binner_debug sorts by
+ * travel).
  * MFMA forms (1D and 2D): the samples with a Binner key are bucketed by grid
  * row, deposited on the output grid (bilinear, exact fixed point) and
  * correlated with the taps on MFMA; rows whose Binner window is empty are 0.
@@ -238,8 +241,10 @@ int rtsim_sensor_output(rtsim_sensor h, const void** dev_ptr, uint32_t* height,
                         uint32_t* width);
 
 /* Sorted samples (receivedSamples_ after sort(), Receiver.cu:28) as the last
- * sort by travel left them: a plain-form compute or rtsim_sensor_binner_debug.
- * The MFMA forms do not sort the samples.                                         */
+ * sort left them: rtsim_sensor_binner_debug (by travel, the reference's order)
+ * or a plain-form compute (grouped by Binner key, input order inside a bin,
+ * out-of-range samples last; the reference's order whenever the samples of a
+ * bin arrive in travel order).  The MFMA forms do not sort the samples.          */
 int rtsim_sensor_sorted_samples(rtsim_sensor h, const void** dev_ptr);
 
 /* Binner internals (Binner.h:56-77) copied to host for bit-exact checks.  Any
@@ -266,11 +271,12 @@ int rtsim_sensor_download_output(rtsim_sensor h, float* host, void* stream);
 /* Per-stage timing with HIP events recorded on the compute stream.  When on,
  * each compute records events around its stages; stage_times returns the
  * mean milliseconds per stage over the computes since the last reset.
- * Stages: 0 stream, 1 binning (plain form: the sort by travel, whose last
+ * Stages: 0 stream, 1 binning (plain form: the sort by Binner key, whose last
  * pass also gathers the records and bearing coordinates and bounds the
  * segments; MFMA forms: the bucket counts' scan and the scatter), 2 windows
- * (plain form), 3 grid deposit with the windows' emptiness (MFMA forms),
- * 4 convolution (plain: band ordering + gather; MFMA: correlation).           */
+ * (plain form; 2D: with the strip lists), 3 grid deposit with the windows'
+ * emptiness (MFMA forms), 4 convolution (plain: band ordering + gather; MFMA:
+ * correlation).                                                               */
 int rtsim_sensor_set_profiling(rtsim_sensor h, int on);
 int rtsim_sensor_stage_times(rtsim_sensor h, float* ms, int nstages, int* ncomputes);
 
