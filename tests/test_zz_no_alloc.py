@@ -87,3 +87,26 @@ def test_noalloc_matches_oracle_and_rejects_what_it_cannot_do(gpu):
     # zero samples: a zero ping
     _noalloc(R, s, 0, rec_buf, 0, ping)
     assert not np.abs(ping.download(np.complex64, cfg.H * cfg.W)).any()
+
+
+def test_noalloc_plain_full_size_equals_compute(gpu):
+    """The 512 x 1024 ping from 1M samples in the plain form: the allocating
+    entry stages each tile's samples from its strip's sample list (32 strips of
+    16 bearings, a sample reaches at most 4), the no-allocation entry scans the
+    windows; every lane adds the same terms in the same order, so the pings
+    are equal bit for bit."""
+    R = gpu
+    cfg, scene, S, seed = configs.workload("fls_512x1024")
+    hits = R.generate_hits(scene, seed, S)
+    s = R.make_sensor(cfg)
+    s.set_sample_count(hits.n)
+    s.write_samples(hits.to_device())
+    s.compute_output(0)
+    want = s.output()
+    rec_buf = R.DeviceBuffer.from_array(s.samples())
+    H, W = s.shape()
+    ping = _poisoned_ping(R, H, W)
+    _noalloc(R, s, 0, rec_buf, hits.n, ping)
+    got = ping.download(np.complex64, H * W).reshape(H, W)
+    assert np.abs(want).max() > 0
+    assert _equal(got, want)
