@@ -34,6 +34,7 @@ import numpy as np  # noqa: E402
 
 PEAK_HBM_GBS = 8000.0
 PEAK_MFMA_TFLOPS = {"bf16": 2500.0, "fp16": 2500.0}
+PEAK_FP32_TFLOPS = 157.3
 
 
 def cpu_model() -> str:
@@ -189,6 +190,29 @@ def stage_roofline(stage, ms, S, cfg, dtype):
     ach = nbytes / (ms * 1e-3) / 1e9
     return {"stage": stage, "bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "algorithmic": f"{nbytes:.4g} B per ping"}
+
+
+def plain_stage_rooflines(st, S, cfg, workload):
+    """Rooflines of the plain form's stages: the stream and the sort against
+    HBM, the gather against both FP32 peak (one complex multiply-add per sample
+    and tap, 8 S Tr Tb flop) and HBM (16 B per sample + 8 B per cell; its
+    measured traffic from the PMC summary): it is bound by neither, but by LDS
+    cell reads and VALU issue."""
+    out = {k: stage_roofline(k, st[k], S, cfg, None) for k in ("stream", "sort") if st.get(k, 0) > 0}
+    ms = st.get("convolution", 0.0)
+    if ms > 0:
+        flops = 8.0 * S * cfg.Tr * cfg.Tb
+        nbytes = 16.0 * S + 8.0 * cfg.H * cfg.W
+        tf = flops / (ms * 1e-3) / 1e12
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out["convolution"] = {
+            "stage": "convolution", "bound": "lds cell reads + valu issue",
+            "achieved_tflops": tf, "peak_fp32_tflops": PEAK_FP32_TFLOPS, "frac_fp32": tf / PEAK_FP32_TFLOPS,
+            "achieved_gbs": gbs, "peak_gbs": PEAK_HBM_GBS, "frac_hbm": gbs / PEAK_HBM_GBS,
+            "algorithmic": f"8*S*Tr*Tb = {flops:.4g} flop, 16*S + 8*H*W = {nbytes:.4g} B per ping",
+            "traffic": measured_traffic(workload, "plain_fp32", "convolution"),
+            "includes": "band ordering + gather (k_plain_rank, k_conv2d_plain)"}
+    return out
 
 
 def copy_calibration(nbytes: int = 256 << 20, repeats: int = 3):
@@ -952,6 +976,7 @@ def main():
         "binning_bit_exact": exact,
         "roofline": roof,
         "stage_rooflines": per_stage,
+        "plain_stage_rooflines": plain_stage_rooflines(stages["plain_fp32"], S, cfg, args.workload),
         "measured_peaks": dict(peaks, how="rtsim_measure_peaks: best of 3 launches of a 512 MiB "
                                           "float4 copy (read + write bytes) and of dense "
                                           "v_mfma_f32_32x32x16 chains"),
