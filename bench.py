@@ -166,6 +166,7 @@ STAGE_BYTES = {
     "sort": lambda S, cfg, g: 32.0 * S,                      # one read + one write of records
                                                              # (grid forms: the bucketing)
     "bins": lambda S, cfg, g: 16.0 * cfg.H,                  # segments read, windows written
+                                                             # (plain 2D: + the strip lists)
     "deposit": lambda S, cfg, g: 16.0 * S + 8.0 * g,         # samples in, grid out
 }
 
@@ -524,7 +525,8 @@ CU_NUM, SIMD_NUM, XCC_NUM = 256, 1024, 8
 # they take the form of the next form-specific dispatch)
 _STAGE_OF = [("k_stream", "stream"), ("k_sort_hist", "sort"), ("k_sort_scan", "sort"),
              ("k_sort_scatter", "sort"), ("k_bucket_hist", "sort"), ("k_bucket_scatter", "sort"),
-             ("k_windows", "bins"), ("k_deposit", "deposit"), ("k_plain_rank", "convolution"),
+             ("k_windows", "bins"), ("k_strip_count", "bins"), ("k_strip_scan", "bins"),
+             ("k_strip_scatter", "bins"), ("k_deposit", "deposit"), ("k_plain_rank", "convolution"),
              ("k_conv2d_plain", "convolution"), ("k_conv1d_plain", "convolution"),
              ("k_correlate", "convolution")]
 
@@ -541,8 +543,8 @@ def _form_of(name):
         return "mfma_fp16"
     if "k_correlate<false" in name:
         return "mfma_bf16"
-    if any(k in name for k in ("k_sort_hist", "k_sort_scatter", "k_windows", "k_plain_rank", "k_conv2d_plain",
-                               "k_conv1d_plain")):
+    if any(k in name for k in ("k_sort_hist", "k_sort_scatter", "k_windows", "k_strip_", "k_plain_rank",
+                               "k_conv2d_plain", "k_conv1d_plain")):
         return "plain_fp32"
     if any(k in name for k in ("k_bucket_", "k_deposit")):
         return "mfma"
@@ -651,7 +653,9 @@ def _traffic_md(t, title, algorithmic):
               "`k_sort_scan`) count for the form of the next dispatch; the grid forms' correlation row is per "
               "launch (the autotuner adds launches of every tile shape).  Write-backs of one kernel's dirty "
               "lines can land in the next kernel's WRITE_SIZE.  The plain form's sort row also carries the "
-              "Binner-introspection sorts the run makes (they use the same kernels).", ""]
+              "Binner-introspection sorts the run makes (they use the same kernels).  The plain form's bins "
+              "row includes its strip lists (coordinates read twice, 8-B entries written: about 34 B per "
+              "sample at 512x1024).", ""]
     forms = [f for f in ("mfma_fp16", "mfma_bf16", "plain_fp32") if f in t["forms"]]
     lines.append("| stage | " + " | ".join(forms) + " | algorithmic |")
     lines.append("|---|" + "---:|" * (len(forms) + 1))
