@@ -535,9 +535,7 @@ int Sensor::launch_ping(const PingPlan& p, int form, hipStream_t s, bool skip_hi
         e = stage(1, [&] { return launch_sort_plan(p.sort, s); });
         if (e == hipSuccess)
             e = stage(2, [&] {
-                hipError_t r = launch_windows(p.bins, s);
-                if (r == hipSuccess && kind == RTSIM_SENSOR_2D) r = launch_plain_strips(p.conv, s);
-                return r;
+                return kind == RTSIM_SENSOR_2D ? launch_plain_bins(p.bins, p.conv, s) : launch_windows(p.bins, s);
             });
         if (e == hipSuccess)
             e = stage(4, [&] {

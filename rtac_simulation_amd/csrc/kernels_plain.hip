@@ -482,7 +482,8 @@ hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
 {
     const uint32_t blocks = conv2d_plain_blocks(a);
     if (!blocks) return hipSuccess;
-    hipLaunchKernelGGL(k_plain_rank, dim3(1), dim3(kRankThreads), 0, s, a, a.work);
+    // the band ranking ran with the lists when they are active
+    if (!plain_lists_active(a)) hipLaunchKernelGGL(k_plain_rank, dim3(1), dim3(kRankThreads), 0, s, a, a.work);
     if (a.psf_in_lds)
         hipLaunchKernelGGL(k_conv2d_plain<true>, dim3(blocks), dim3(kTileThreads), plain2d_lds_bytes(a), s, a);
     else
@@ -699,12 +700,30 @@ __global__ __launch_bounds__(kStripThreads) void k_strip_scatter_batch(const Con
     strip_scatter_body(a, blk);
 }
 
-hipError_t launch_plain_strips(const ConvArgs& a, hipStream_t s)
+// One ping's lists with two launches folded in: the Binner windows run as
+// extra workgroups of the counts (both only need the sort), the band ranking
+// as an extra workgroup of the scan (it needs the windows).
+__global__ __launch_bounds__(kStripThreads) void k_strip_count_windows(ConvArgs a, BinArgs w)
 {
-    if (!a.strip_ent || a.n == 0 || a.W <= 0) return hipSuccess;
+    const uint32_t nb = strip_blocks(a.n);
+    if (blockIdx.x < nb) strip_count_body(a, blockIdx.x);
+    else windows_body(w, blockIdx.x - nb);
+}
+__global__ __launch_bounds__(kStripThreads) void k_strip_scan_rank(ConvArgs a)
+{
+    if (blockIdx.x < plain_strips((uint32_t)a.W)) strip_scan_body(a, blockIdx.x);
+    else plain_rank_body(a, a.work);
+}
+
+bool plain_lists_active(const ConvArgs& a) { return a.strip_ent && a.n > 0 && a.W > 0 && a.H > 0; }
+
+hipError_t launch_plain_bins(const BinArgs& w, const ConvArgs& a, hipStream_t s)
+{
+    if (!plain_lists_active(a)) return launch_windows(w, s);
     const uint32_t nb = strip_blocks(a.n), ns = plain_strips((uint32_t)a.W);
-    hipLaunchKernelGGL(k_strip_count, dim3(nb), dim3(kStripThreads), 0, s, a);
-    hipLaunchKernelGGL(k_strip_scan, dim3(ns), dim3(kStripThreads), 0, s, a);
+    const uint32_t wb = (w.N + kStripThreads - 1) / kStripThreads;
+    hipLaunchKernelGGL(k_strip_count_windows, dim3(nb + wb), dim3(kStripThreads), 0, s, a, w);
+    hipLaunchKernelGGL(k_strip_scan_rank, dim3(ns + 1), dim3(kStripThreads), 0, s, a);
     hipLaunchKernelGGL(k_strip_scatter, dim3(nb), dim3(kStripThreads), 0, s, a);
     return hipGetLastError();
 }

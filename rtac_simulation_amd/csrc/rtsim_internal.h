@@ -267,7 +267,11 @@ constexpr size_t strip_cnt_words(uint32_t W, uint32_t n) { return (size_t)plain_
 // (ascending) and a PSF bearing support of `supp` (0: no lists: the strips
 // are too many, or a sample reaches too many of them to save staging)
 uint32_t strip_multiplicity(const double* x, uint32_t W, double supp, double pitch);
-hipError_t launch_plain_strips(const ConvArgs& a, hipStream_t s);
+// the plain form's bins stage: the windows, and for a 2D ping with active
+// strip lists the lists and the band ranking (launch_conv2d_plain then only
+// gathers)
+bool plain_lists_active(const ConvArgs& a);
+hipError_t launch_plain_bins(const BinArgs& w, const ConvArgs& a, hipStream_t s);
 hipError_t launch_strip_count_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_strip_scan_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_strip_scatter_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
