@@ -584,9 +584,9 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     c.inv_pb = (float)((double)Tb / (double)psf_bearing_span);
     c.half_b = 0.5f * (float)(Tb - 1);
     c.supp_b = (float)(0.5 * (double)(Tb + 1) * (double)psf_bearing_span / (double)Tb);
-    // cell planes of up to 1024 cells (16 KiB each) go to LDS; larger ones are
-    // read through L1/L2
-    c.psf_in_lds = ((size_t)Tr + 2) * ((size_t)Tb + 2) <= 1024;
+    // an (A, B) cell plane of up to kPlainLdsCells cells goes to LDS; a larger
+    // one is read through L1/L2 like the (C, D) plane
+    c.psf_in_lds = ((size_t)Tr + 2) * ((size_t)Tb + 2) <= kPlainLdsCells;
     return c;
 }
 

@@ -246,7 +246,7 @@ struct ConvArgs {
     float inv_pr, half_r;    // range tap coordinate yB = dr * inv_pr + half_r
     float inv_pb, half_b;    // bearing tap coordinate xB = dx * inv_pb + half_b
     float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
-    int psf_in_lds;          // 2D: stage the cells in LDS (else read them from global)
+    int psf_in_lds;          // 2D: stage the (A, B) cells in LDS (else read them from global)
     uint32_t* work;          // 2D: band lengths and band order (plain2d_work_words)
     // 2D strip lists (null: every tile stages its band's whole window):
     // strip j's list = the sorted samples its tiles keep, in window order, as
@@ -257,6 +257,9 @@ struct ConvArgs {
     size_t strip_cap;        // entries strip_ent holds
 };
 size_t plain2d_lds_bytes(const ConvArgs& a);
+// cells of the largest tap plane the 2D gather stages in LDS (two 1024-thread
+// workgroups per CU: 40 KiB of staged samples + the plane each)
+constexpr size_t kPlainLdsCells = 2500;
 // strip lists: a strip is the 16 bearing columns of one tile column; the
 // lists are built per block of kStripBlock sorted samples
 constexpr uint32_t kStripCols = 16, kStripBlock = 1024, kMaxStrips = 512;
