@@ -526,7 +526,7 @@ CU_NUM, SIMD_NUM, XCC_NUM = 256, 1024, 8
 _STAGE_OF = [("k_stream", "stream"), ("k_sort_hist", "sort"), ("k_sort_scan", "sort"),
              ("k_sort_scatter", "sort"), ("k_bucket_hist", "sort"), ("k_bucket_scatter", "sort"),
              ("k_windows", "bins"), ("k_strip_count", "bins"), ("k_strip_scan", "bins"),
-             ("k_strip_scatter", "bins"), ("k_deposit", "deposit"), ("k_plain_rank", "convolution"),
+             ("k_strip_scatter", "bins"), ("k_strip_count_windows", "bins"), ("k_strip_scan_rank", "bins"), ("k_deposit", "deposit"), ("k_plain_rank", "convolution"),
              ("k_conv2d_plain", "convolution"), ("k_conv1d_plain", "convolution"),
              ("k_correlate", "convolution")]
 
