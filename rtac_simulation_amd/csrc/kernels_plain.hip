@@ -48,24 +48,29 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
 // ---- 2D: register-accumulating gather over output tiles ---------------------
 // A workgroup owns a tile of 16 x 16 output cells made of four 8 x 8 blocks;
 // four waves share a block, and each lane of a wave owns one cell of it and
-// sums its terms in two registers.  The samples that can reach the tile (the
-// union of its rows' windows, those whose bearing coordinate lies within the
-// PSF's bearing support of its columns) are staged in LDS one chunk at a
-// time, in window order; each block's four waves then share the staged
-// samples that can reach the block (every fourth one), and every lane adds
-// v_s K(x_s - x_b, t_s - r_r) when sample s is in its row's window and inside
-// the PSF support (the reference's border mode makes every other term 0).
-// At the end the four partial sums of a cell are added in a fixed order, so
-// the ping is bit-reproducible.  The taps are read as bilinear cells with a
-// zero border (A, B, C, D: K = A + ax B + ay (C + ax D) inside the cell), in
-// two planes of cells, (A, B) and (C, D): a lookup is two 16-byte reads, the
-// first from the (A, B) plane staged in LDS, the second from the (C, D) plane
-// through the vector L1 cache, so the two go to different units (the 8 lanes
-// of a block row read 8 consecutive cells, 128 contiguous bytes: no LDS bank
-// conflict, one cache line), and three FMAs per component, and a term outside the support or the window reads only zeros,
-// so the lanes need no branch.  (Terms of a non-finite sample value
-// therefore reach every lane of the blocks it is staged for: 0 x inf is NaN,
-// as in the reference, whose gather multiplies every window sample.)
+// sums its terms in two registers.  The samples that can reach the tile (in
+// the union of its rows' windows, bearing coordinate within the PSF's bearing
+// support of its columns) are staged in LDS one chunk at a time, in window
+// order: from its strip's sample list when the ping has strip lists (below),
+// else from every sample of the window.  Each block's four waves share the
+// staged samples that can reach the block (the k-th of them goes to wave
+// k mod 4), and every lane adds v_s K(x_s - x_b, t_s - r_r) when sample s is
+// in its row's window and inside the PSF support (the reference's border
+// mode makes every other term 0).  At the end the four partial sums of a cell
+// are added in a fixed order, so the ping is bit-reproducible, and the same
+// whichever source staged the samples.
+//
+// The taps are read as bilinear cells with a zero border (A, B, C, D:
+// K = A + ax B + ay (C + ax D) inside the cell), in two planes of cells,
+// (A, B) and (C, D): a lookup is two 16-byte reads, the first from the (A, B)
+// plane staged in LDS, the second from the (C, D) plane through the vector L1
+// cache, so the two go to different units (the 8 lanes of a block row read 8
+// consecutive cells, 128 contiguous bytes: no LDS bank conflict, one cache
+// line), and three FMAs per component.  A term outside the support or the
+// window reads only zeros, so the lanes need no branch.  (Terms of a
+// non-finite sample value therefore reach every lane of the blocks it is
+// staged for: 0 x inf is NaN, as in the reference, whose gather multiplies
+// every window sample.)
 //
 // Work per tile follows the samples, which crowd into some ranges: a first
 // kernel orders the bands of 16 rows by the length of their union window,
