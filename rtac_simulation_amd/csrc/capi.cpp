@@ -198,10 +198,12 @@ int Sensor::ensure_strips()
     const size_t need = capacity * (size_t)strip_mult;
     const size_t words = strip_cnt_words(W, (uint32_t)capacity);
     if (d_strip_ent && strip_cap >= need && strip_cnt_cap >= words) return RTSIM_OK;
+    if (strip_failed && need >= strip_failed) return RTSIM_OK;   // that size did not fit: no lists
     free_strips();
     if (dev_alloc(&d_strip_ent, need) || dev_alloc(&d_strip_cnt, words)) {
         free_strips();
         (void)hipGetLastError();
+        strip_failed = need;
     } else {
         strip_cap = need;
         strip_cnt_cap = words;

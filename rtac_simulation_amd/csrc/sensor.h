@@ -130,6 +130,7 @@ struct Sensor {
     uint2* d_strip_ent = nullptr;
     uint32_t* d_strip_cnt = nullptr;
     size_t strip_cap = 0, strip_cnt_cap = 0;
+    size_t strip_failed = 0;      // entries of a list allocation that failed (0: none)
     float2* d_out = nullptr;      // ping [N][W]
     float2* d_psf = nullptr;
     float4* d_psf_cells = nullptr;  // plain 2D gather: bilinear cells of the taps (ConvArgs::psf_cells)
