@@ -513,6 +513,8 @@ class SensorInstance:
         return c.reshape(H, W) if self.kind == SENSOR_2D else c
 
     def samples(self, sorted: bool = False) -> np.ndarray:
+        """The resident records, or (sorted) as the last sort left them: grouped
+        by Binner key after a plain-form compute, by travel after binner_debug."""
         n = self.sample_count()
         out = np.empty((n, 4), np.float32)
         check(lib().rtsim_sensor_download_samples(self.handle, _ptr(out), n, int(sorted), None))
