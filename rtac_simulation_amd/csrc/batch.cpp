@@ -254,7 +254,7 @@ static int build(Batch* b)
     b->strip_scatter = {};
     b->plain_rank = {};
     b->plain2d = {};
-    b->plain2d.variant = 1;   // taps in LDS unless a ping's do not fit there
+    b->plain2d.variant = 3;   // bit 0: taps in LDS, bit 1: 3072-sample chunks (unless a ping's do not fit)
     b->plain1d = {};
     b->deposit = {};
     b->corr.clear();
@@ -265,6 +265,7 @@ static int build(Batch* b)
     b->persist_shm = 0;
     std::map<std::tuple<int, int, int>, size_t> corr_index;
     uint32_t nflags = 0;
+    size_t plain_taps = 0;   // the largest (A, B) tap plane of the batch's plain 2D pings
     for (auto& en : b->entries) {
         Sensor* s = en.sensor;
         PingPlan p;
@@ -307,8 +308,11 @@ static int build(Batch* b)
                 const uint32_t blocks = conv2d_plain_blocks(p.conv);
                 b->plain_rank.add(p.conv, blocks ? 1u : 0u);
                 b->plain2d.add(p.conv, blocks);
-                b->plain2d.shm = std::max(b->plain2d.shm, plain2d_lds_bytes(p.conv));
-                if (blocks && !p.conv.psf_in_lds) b->plain2d.variant = 0;
+                if (blocks) {
+                    if (!p.conv.psf_in_lds) b->plain2d.variant &= ~1;
+                    if (!p.conv.chunk3) b->plain2d.variant &= ~2;
+                    plain_taps = std::max(plain_taps, sizeof(float4) * (size_t)(p.conv.Tr + 2) * (size_t)(p.conv.Tb + 2));
+                }
             } else {
                 b->plain1d.add(p.conv, conv1d_plain_blocks(p.conv));
             }
@@ -339,6 +343,8 @@ static int build(Batch* b)
         en.generation = s->generation;
         en.pose_version = s->pose_version;
     }
+    b->plain2d.shm = plain2d_stage_bytes((b->plain2d.variant & 2) ? 3 : 2) +
+                     ((b->plain2d.variant & 1) ? plain_taps : 0);
     // one device buffer for every descriptor array and prefix table
     size_t off = 0;
     layout(b->stream, off);
@@ -447,7 +453,7 @@ static hipError_t enqueue(const Batch* b, hipStream_t s)
                                       b->strip_scatter.blocks(), s));
     RT_TRY(launch_plain_rank_batch(dargs(b, b->plain_rank), dmap(b, b->plain_rank), b->plain_rank.blocks(), s));
     RT_TRY(launch_conv2d_plain_batch(dargs(b, b->plain2d), dmap(b, b->plain2d), b->plain2d.blocks(),
-                                     b->plain2d.shm, b->plain2d.variant != 0, s));
+                                     b->plain2d.shm, b->plain2d.variant, s));
     RT_TRY(launch_conv1d_plain_batch(dargs(b, b->plain1d), dmap(b, b->plain1d), b->plain1d.blocks(), s));
     RT_TRY(launch_deposit_batch(dargs(b, b->deposit), dmap(b, b->deposit), b->deposit.blocks(),
                                 b->deposit.shm, s));

@@ -589,6 +589,11 @@ ConvArgs Sensor::conv_args(const PingBuffers& pb) const
     // an (A, B) cell plane of up to kPlainLdsCells cells goes to LDS; a larger
     // one is read through L1/L2 like the (C, D) plane
     c.psf_in_lds = ((size_t)Tr + 2) * ((size_t)Tb + 2) <= kPlainLdsCells;
+    // 3072-sample chunks when they and the staged plane still fit two
+    // workgroups per CU (measured at 512 x 1024 with the 31 x 17 PSF: the
+    // gather 7% faster than with 2048)
+    c.chunk3 = plain2d_stage_bytes(3) +
+                   (c.psf_in_lds ? sizeof(float4) * ((size_t)Tr + 2) * ((size_t)Tb + 2) : 0) <= kPlainLdsBudget;
     return c;
 }
 

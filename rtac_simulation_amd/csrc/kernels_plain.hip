@@ -85,21 +85,23 @@ constexpr int kTileBlocks = 4;
 constexpr int kWavesPerBlock = 4;
 constexpr int kTileWaves = kTileBlocks * kWavesPerBlock;
 constexpr int kTileThreads = 64 * kTileWaves;      // 1024
-constexpr int kChunkPer = 2;                       // samples per thread per staging pass
-constexpr int kChunk = kTileThreads * kChunkPer;
+// samples per thread per staging pass: 3 (chunks of 3072) when the staged
+// (A, B) plane leaves room for them at two workgroups per CU
+// (ConvArgs::chunk3), else 2; fewer chunks mean fewer barriers (the ping is
+// the same either way)
 constexpr int kRankThreads = 1024;
 constexpr int kRankLds = 4096;                     // band lengths staged per pass
-
-static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float)) * kChunk,
+static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float)) * 2 * kTileThreads,
               "partial sums reuse the staging space");
-static_assert(kChunkPer * kTileWaves <= 64, "one wave's count per lane");
+static_assert(3 * kTileWaves <= 64, "one wave's count per lane");
 
-// Staged taps: the (A, B) plane when ConvArgs::psf_in_lds (at most
-// kPlainLdsCells cells, so that two workgroups still fit a CU)
+size_t plain2d_stage_bytes(int chunk_per) { return (sizeof(float4) + sizeof(float)) * kTileThreads * (size_t)chunk_per; }
+
+// staged samples + the (A, B) plane when ConvArgs::psf_in_lds
 size_t plain2d_lds_bytes(const ConvArgs& a)
 {
     const size_t taps = a.psf_in_lds ? sizeof(float4) * (size_t)(a.Tr + 2) * (a.Tb + 2) : 0;
-    return (sizeof(float4) + sizeof(float)) * kChunk + taps;
+    return plain2d_stage_bytes(a.chunk3 ? 3 : 2) + taps;
 }
 
 __host__ __device__ inline uint32_t plain2d_bands(const ConvArgs& a) { return (uint32_t)(a.H + kTileRows - 1) / kTileRows; }
@@ -238,9 +240,10 @@ __device__ __forceinline__ float sample_xs(const ConvArgs& a, float2 o_b, float2
     return ((x.x - o_b.x) + (x.y - o_b.y)) * a.inv_pb;
 }
 
-template <bool kTapsLds>
+template <bool kTapsLds, int kChunkPer>
 __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t blk)
 {
+    constexpr int kChunk = kTileThreads * kChunkPer;
     extern __shared__ float4 lds4[];
     float4* taps_lds = lds4;                                    // the (A, B) plane when staged
     const int ncells = (a.Tr + 2) * (a.Tb + 2);                 // cells of a plane
@@ -464,20 +467,28 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
 }
 
 // 8 waves per SIMD (two workgroups per CU): at most 64 VGPRs
-template <bool kTapsLds>
+template <bool kTapsLds, int kChunkPer>
 __global__ __launch_bounds__(kTileThreads, 8) void k_conv2d_plain(ConvArgs a)
 {
-    conv2d_plain_body<kTapsLds>(a, blockIdx.x);
+    conv2d_plain_body<kTapsLds, kChunkPer>(a, blockIdx.x);
 }
 
-template <bool kTapsLds>
+template <bool kTapsLds, int kChunkPer>
 __global__ __launch_bounds__(kTileThreads, 8) void k_conv2d_plain_batch(const ConvArgs* __restrict__ args,
                                                                      BatchMap m)
 {
     uint32_t blk, nblk;
     const uint32_t p = batch_locate(m, blk, nblk);
     const ConvArgs a = args[p];   // a private copy: the body's stores cannot alias it
-    conv2d_plain_body<kTapsLds>(a, blk);
+    conv2d_plain_body<kTapsLds, kChunkPer>(a, blk);
+}
+
+template <bool kTapsLds>
+static void launch_gather(const ConvArgs& a, uint32_t blocks, bool chunk3, hipStream_t s)
+{
+    const size_t shm = plain2d_lds_bytes(a);
+    if (chunk3) hipLaunchKernelGGL((k_conv2d_plain<kTapsLds, 3>), dim3(blocks), dim3(kTileThreads), shm, s, a);
+    else hipLaunchKernelGGL((k_conv2d_plain<kTapsLds, 2>), dim3(blocks), dim3(kTileThreads), shm, s, a);
 }
 
 hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
@@ -486,10 +497,8 @@ hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s)
     if (!blocks) return hipSuccess;
     // the band ranking ran with the lists when they are active
     if (!plain_lists_active(a)) hipLaunchKernelGGL(k_plain_rank, dim3(1), dim3(kRankThreads), 0, s, a, a.work);
-    if (a.psf_in_lds)
-        hipLaunchKernelGGL(k_conv2d_plain<true>, dim3(blocks), dim3(kTileThreads), plain2d_lds_bytes(a), s, a);
-    else
-        hipLaunchKernelGGL(k_conv2d_plain<false>, dim3(blocks), dim3(kTileThreads), plain2d_lds_bytes(a), s, a);
+    if (a.psf_in_lds) launch_gather<true>(a, blocks, a.chunk3 != 0, s);
+    else launch_gather<false>(a, blocks, a.chunk3 != 0, s);
     return hipGetLastError();
 }
 
@@ -501,22 +510,29 @@ hipError_t launch_plain_rank_batch(const ConvArgs* args, BatchMap m, uint32_t bl
 }
 
 hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, size_t shm,
-                                     bool taps_lds, hipStream_t s)
+                                     int variant, hipStream_t s)
 {
     if (!blocks) return hipSuccess;
-    if (taps_lds)
-        hipLaunchKernelGGL(k_conv2d_plain_batch<true>, dim3(blocks), dim3(kTileThreads), shm, s, args, m);
-    else
-        hipLaunchKernelGGL(k_conv2d_plain_batch<false>, dim3(blocks), dim3(kTileThreads), shm, s, args, m);
+    const dim3 g(blocks), b(kTileThreads);
+    switch (variant & 3) {
+    case 3: hipLaunchKernelGGL((k_conv2d_plain_batch<true, 3>), g, b, shm, s, args, m); break;
+    case 2: hipLaunchKernelGGL((k_conv2d_plain_batch<false, 3>), g, b, shm, s, args, m); break;
+    case 1: hipLaunchKernelGGL((k_conv2d_plain_batch<true, 2>), g, b, shm, s, args, m); break;
+    default: hipLaunchKernelGGL((k_conv2d_plain_batch<false, 2>), g, b, shm, s, args, m); break;
+    }
     return hipGetLastError();
 }
 
 void init_plain_attributes(int max_lds)
 {
-    set_dynamic_lds_limit((const void*)k_conv2d_plain<true>, max_lds);
-    set_dynamic_lds_limit((const void*)k_conv2d_plain<false>, max_lds);
-    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<true>, max_lds);
-    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<false>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<true, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<false, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<true, 3>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain<false, 3>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<true, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<false, 2>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<true, 3>, max_lds);
+    set_dynamic_lds_limit((const void*)k_conv2d_plain_batch<false, 3>, max_lds);
 }
 
 // ---- strip lists ---------------------------------------------------------------

@@ -247,6 +247,7 @@ struct ConvArgs {
     float inv_pb, half_b;    // bearing tap coordinate xB = dx * inv_pb + half_b
     float supp_b;            // |dx| >= supp_b  =>  PSF weight is exactly 0
     int psf_in_lds;          // 2D: stage the (A, B) cells in LDS (else read them from global)
+    int chunk3;              // 2D: stage 3072 samples per chunk (else 2048)
     uint32_t* work;          // 2D: band lengths and band order (plain2d_work_words)
     // 2D strip lists (null: every tile stages its band's whole window):
     // strip j's list = the sorted samples its tiles keep, in window order, as
@@ -257,6 +258,9 @@ struct ConvArgs {
     size_t strip_cap;        // entries strip_ent holds
 };
 size_t plain2d_lds_bytes(const ConvArgs& a);
+size_t plain2d_stage_bytes(int chunk_per);   // staged samples of one chunk (chunk_per x 1024)
+// dynamic LDS of a 2D gather workgroup when two of them share a CU
+constexpr size_t kPlainLdsBudget = 81280;
 // cells of the largest tap plane the 2D gather stages in LDS (two 1024-thread
 // workgroups per CU: 40 KiB of staged samples + the plane each)
 constexpr size_t kPlainLdsCells = 2500;
@@ -299,8 +303,9 @@ hipError_t launch_conv2d_plain(const ConvArgs& a, hipStream_t s);
 hipError_t launch_conv1d_plain(const ConvArgs& a, hipStream_t s);
 uint32_t conv2d_plain_blocks(const ConvArgs& a);
 uint32_t conv1d_plain_blocks(const ConvArgs& a);
+// variant: bit 0 taps in LDS, bit 1 3072-sample chunks (for every ping of the launch)
 hipError_t launch_conv2d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
-                                     size_t shm, bool taps_lds, hipStream_t s);
+                                     size_t shm, int variant, hipStream_t s);
 // the band ordering that precedes the 2D gather: one workgroup per ping
 hipError_t launch_plain_rank_batch(const ConvArgs* args, BatchMap m, uint32_t blocks, hipStream_t s);
 hipError_t launch_conv1d_plain_batch(const ConvArgs* args, BatchMap m, uint32_t blocks,
