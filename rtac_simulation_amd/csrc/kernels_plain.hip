@@ -48,12 +48,13 @@ __device__ __forceinline__ float2 psf_bilinear(const float2* __restrict__ psf, i
 // ---- 2D: register-accumulating gather over output tiles ---------------------
 // A workgroup owns a tile of 16 x 16 output cells made of four 8 x 8 blocks;
 // four waves share a block, and each lane of a wave owns one cell of it and
-// sums its terms in two registers.  The samples that can reach the tile (in
-// the union of its rows' windows, bearing coordinate within the PSF's bearing
-// support of its columns) are staged in LDS one chunk at a time, in window
-// order: from its strip's sample list when the ping has strip lists (below),
-// else from every sample of the window.  Each block's four waves share the
-// staged samples that can reach the block (the k-th of them goes to wave
+// sums its terms in two registers.  The samples that can reach one of the
+// tile's blocks (in the union of the block's rows' windows, bearing coordinate
+// within the PSF's bearing support of its columns) are staged in LDS one chunk
+// at a time, in window order, with a bit per block they reach: from the
+// strip's sample list when the ping has strip lists (below), else from every
+// sample of the window.  Each block's four waves share the staged samples
+// that reach the block (the k-th of them goes to wave
 // k mod 4), and every lane adds v_s K(x_s - x_b, t_s - r_r) when sample s is
 // in its row's window and inside the PSF support (the reference's border
 // mode makes every other term 0).  At the end the four partial sums of a cell
@@ -95,7 +96,10 @@ static_assert(sizeof(float2) * kTileThreads <= (sizeof(float4) + sizeof(float)) 
               "partial sums reuse the staging space");
 static_assert(3 * kTileWaves <= 64, "one wave's count per lane");
 
-size_t plain2d_stage_bytes(int chunk_per) { return (sizeof(float4) + sizeof(float)) * kTileThreads * (size_t)chunk_per; }
+size_t plain2d_stage_bytes(int chunk_per)
+{
+    return (sizeof(float4) + sizeof(float) + sizeof(uint8_t)) * kTileThreads * (size_t)chunk_per;
+}
 
 // staged samples + the (A, B) plane when ConvArgs::psf_in_lds
 size_t plain2d_lds_bytes(const ConvArgs& a)
@@ -219,10 +223,12 @@ __device__ __forceinline__ int floor_to_int(float x)
 // the tile's strip (double-float),
 //   xB = (x_s - x_b) / pb + (Tb - 1) / 2 = xs + xc,
 //   xs = (x_s - o_b) / pb,   xc = (Tb - 1) / 2 - (x_b - o_b) / pb.
-// A sample is staged for a tile when xs lies in the strip's box: xB in
+// A sample goes into a strip's list when xs lies in the strip's box: xB in
 // (-1, Tb) for one of its columns, widened by 1/64 tap (a term just outside is
-// clamped to the zero border and adds exactly 0).  The gather and the strip
-// lists evaluate these with the same expressions, so they agree bit for bit.
+// clamped to the zero border and adds exactly 0), and reaches a block of a
+// tile when xs lies in the block's box of the same form.  The gather and the
+// strip lists evaluate these with the same expressions, so they agree bit for
+// bit (a block's box lies inside its strip's).
 constexpr float kBoxMargin = 1.0f / 64.0f;
 
 __device__ __forceinline__ float2 strip_origin(const ConvArgs& a, int j)
@@ -249,6 +255,7 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     const int ncells = (a.Tr + 2) * (a.Tb + 2);                 // cells of a plane
     float4* cs_rec = lds4 + (kTapsLds ? ncells : 0);            // [kChunk] (re, im, t, index)
     float* cs_x = reinterpret_cast<float*>(cs_rec + kChunk);    // [kChunk] xs
+    uint8_t* cs_m = reinterpret_cast<uint8_t*>(cs_x + kChunk);  // [kChunk] blocks it reaches (bit per block)
     __shared__ uint32_t counts[kChunkPer * kTileWaves];
     __shared__ float4 wave_box[kTileWaves];                     // (index lo, index hi, xs lo, xs hi)
     __shared__ uint32_t strip_ws[kTileWaves];
@@ -309,15 +316,21 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
     const float xhi = wave_max(col_ok ? box_hi(a, xc) : -INFINITY);
     if (lane == 0) wave_box[w] = make_float4(__uint_as_float(wlo), __uint_as_float(whi), xlo, xhi);
     __syncthreads();
+    // every block's box in scalar registers (uniform across the workgroup),
+    // and the union of the blocks' window index ranges
+    constexpr int kBlocks = kTileWaves / kWavesPerBlock;
+    uint32_t blo[kBlocks], bhi[kBlocks];
+    float bxlo[kBlocks], bxhi[kBlocks];
     uint32_t u0 = 0xFFFFFFFFu, u1 = 0u;
-    float gxlo = INFINITY, gxhi = -INFINITY;
 #pragma unroll
-    for (int j = 0; j < kTileWaves; j += kWavesPerBlock) {
-        const float4 b = wave_box[j];
-        u0 = min(u0, __float_as_uint(b.x));
-        u1 = max(u1, __float_as_uint(b.y));
-        gxlo = fminf(gxlo, b.z);
-        gxhi = fmaxf(gxhi, b.w);
+    for (int j = 0; j < kBlocks; j++) {
+        const float4 b = wave_box[j * kWavesPerBlock];
+        blo[j] = (uint32_t)__builtin_amdgcn_readfirstlane((int)__float_as_uint(b.x));
+        bhi[j] = (uint32_t)__builtin_amdgcn_readfirstlane((int)__float_as_uint(b.y));
+        bxlo[j] = __uint_as_float((uint32_t)__builtin_amdgcn_readfirstlane((int)__float_as_uint(b.z)));
+        bxhi[j] = __uint_as_float((uint32_t)__builtin_amdgcn_readfirstlane((int)__float_as_uint(b.w)));
+        u0 = min(u0, blo[j]);
+        u1 = max(u1, bhi[j]);
     }
 
     float acc_re = 0.f, acc_im = 0.f;
@@ -374,26 +387,37 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         // stage the chunk's samples that can reach the tile, in window order
         float4 rec[kChunkPer];
         float xsu[kChunkPer];
+        uint32_t reach = 0u;                                    // 4 bits per u
         uint64_t keep[kChunkPer];
 #pragma unroll
         for (int u = 0; u < kChunkPer; u++) {
             const uint32_t q = base + (uint32_t)(u * kTileThreads + t);
             const uint32_t ql = min(q, p1 - 1u);                // in bounds; lanes past p1 keep nothing
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            bool kp;
+            uint32_t si;
             if (from_list) {
                 const uint2 en = a.strip_ent[ql];
+                si = en.x;
                 xsu[u] = __uint_as_float(en.y);
-                kp = q < p1 && en.x >= u0 && en.x < u1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
-                if (kp) v = a.sorted[en.x];
-                v.w = __uint_as_float(en.x);
             } else {
+                si = ql;
                 xsu[u] = sample_xs(a, o_b, a.coord[ql]);
-                kp = q < p1 && xsu[u] >= gxlo && xsu[u] <= gxhi;
-                // few samples of the window reach the tile: only theirs are loaded
-                if (kp) v = a.sorted[ql];
-                v.w = __uint_as_float(q);
             }
+            // the blocks it reaches (window index range and bearing box of
+            // each block); only a sample that reaches one is staged, and few
+            // samples of a window reach the tile, so only theirs are loaded
+            uint32_t bits = 0u;
+            if (q < p1) {
+#pragma unroll
+                for (int b = 0; b < kBlocks; b++) {
+                    const bool in = si >= blo[b] && si < bhi[b] && xsu[u] >= bxlo[b] && xsu[u] <= bxhi[b];
+                    bits |= (uint32_t)in << b;
+                }
+            }
+            const bool kp = bits != 0u;
+            reach |= bits << (4 * u);
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (kp) v = a.sorted[si];
+            v.w = __uint_as_float(si);
             keep[u] = __ballot(kp);
             rec[u] = v;
             if (lane == 0) counts[u * kTileWaves + w] = (uint32_t)__popcll(keep[u]);
@@ -412,6 +436,7 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
                 const uint32_t pos = at + popc_below(keep[u]);
                 cs_rec[pos] = rec[u];
                 cs_x[pos] = xsu[u];
+                cs_m[pos] = (uint8_t)((reach >> (4 * u)) & 15u);
             }
         }
         __syncthreads();
@@ -420,12 +445,7 @@ __device__ __forceinline__ void conv2d_plain_body(const ConvArgs& a, uint32_t bl
         // every staged group of 64 (a chunk of the window often stages a
         // single group: every wave takes part in it), two samples at a time
         for (uint32_t g = 0; g < n; g += 64) {
-            bool rel = false;
-            if (g + lane < n) {
-                const uint32_t si = __float_as_uint(cs_rec[g + lane].w);
-                const float x = cs_x[g + lane];
-                rel = si >= wlo && si < whi && x >= xlo && x <= xhi;
-            }
+            const bool rel = g + lane < n && ((cs_m[g + lane] >> blkw) & 1u);
             // this wave's share: the relevant samples of rank sub, sub + 4, ...
             const uint64_t all = __ballot(rel);
             uint64_t m = __ballot(rel && ((ranked + popc_below(all)) & (kWavesPerBlock - 1)) == (uint32_t)sub);

@@ -262,8 +262,8 @@ size_t plain2d_stage_bytes(int chunk_per);   // staged samples of one chunk (chu
 // dynamic LDS of a 2D gather workgroup when two of them share a CU
 constexpr size_t kPlainLdsBudget = 81280;
 // cells of the largest tap plane the 2D gather stages in LDS (two 1024-thread
-// workgroups per CU: 40 KiB of staged samples + the plane each)
-constexpr size_t kPlainLdsCells = 2500;
+// workgroups per CU: 42 KiB of staged samples + the plane each)
+constexpr size_t kPlainLdsCells = 2380;
 // strip lists: a strip is the 16 bearing columns of one tile column; the
 // lists are built per block of kStripBlock sorted samples
 constexpr uint32_t kStripCols = 16, kStripBlock = 1024, kMaxStrips = 512;
